@@ -1,0 +1,278 @@
+/*
+ * les_hip.h - C ABI of the MI355X (gfx950) LES_Halo timestep library, libles_hip.so.
+ *
+ * It replaces the OpenCLIntegration layer under the reference's GPU host module
+ * (src/OpenCL/module_LES_combined_ocl_TEMPL_V2.f95, generated into
+ * src/OpenCL/Wrappers/module_LES_combined_ocl.f95) and exposes one entry point per
+ * kernel-level stage of the reference time step, with the reference's names.
+ *
+ * Conventions
+ *  - Every array keeps the reference's Fortran bounds and column-major layout (i fastest),
+ *    exactly as declared in src/Fortran/main_ocl.f95:40-167; les_field_bounds() gives them.
+ *    Uploads and downloads are plain copies of those arrays (FP32).
+ *  - Every call returns LES_OK (0) or an LES_E_* code; les_last_error() gives the message.
+ *    No call exits the process.
+ *  - One HIP stream per handle; calls are asynchronous on it except les_download,
+ *    les_get_scalars, les_sync and the les_exchange_halo_* host-buffer forms, which
+ *    synchronise.  A handle must not be used from two threads at once.
+ *  - The GPU code path is the only path: without a usable HIP device les_create fails with
+ *    LES_E_NODEV.
+ */
+#ifndef LES_HIP_H
+#define LES_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------- */
+enum {
+  LES_OK = 0,
+  LES_E_INVALID = 1, /* bad argument (size, field id, value) */
+  LES_E_HIP = 2,     /* a HIP runtime call failed */
+  LES_E_NOMEM = 3,   /* device allocation failed */
+  LES_E_STATE = 4,   /* call not valid in the handle's current state */
+  LES_E_NODEV = 5    /* no HIP device */
+};
+
+/* ---- fields: the arrays of main_ocl.f95:40-167 --------------------------------------- */
+typedef enum les_field {
+  LES_P = 0,   /* p(0:ip+2,0:jp+2,0:kp+1) */
+  LES_U,       /* u(0:ip+1,-1:jp+1,0:kp+1) */
+  LES_V,       /* v(0:ip+1,-1:jp+1,0:kp+1) */
+  LES_W,       /* w(0:ip+1,-1:jp+1,-1:kp+1) */
+  LES_USUM,    /* usum(0:ip,0:jp,0:kp) */
+  LES_VSUM,
+  LES_WSUM,
+  LES_F,       /* f(0:ip,0:jp,0:kp) */
+  LES_G,
+  LES_H,
+  LES_FOLD,    /* fold(ip,jp,kp) */
+  LES_GOLD,
+  LES_HOLD,
+  LES_RHS,     /* rhs(0:ip+1,0:jp+1,0:kp+1) */
+  LES_SM,      /* sm(-1:ip+1,-1:jp+1,0:kp+1) */
+  LES_DIU1,    /* diu1(-1:ip+2,0:jp+2,0:kp+2) */
+  LES_DIU2,    /* diu2..4,6..9(0:ip+2,0:jp+2,0:kp+2) */
+  LES_DIU3,
+  LES_DIU4,
+  LES_DIU5,    /* diu5(-1:ip+2,0:jp+2,0:kp+2) */
+  LES_DIU6,
+  LES_DIU7,
+  LES_DIU8,
+  LES_DIU9,
+  LES_AMASK1,  /* amask1(0:ip+1,0:jp+1,0:kp+1), values 0/1 */
+  LES_BMASK1,  /* bmask1(-1:ip+1,0:jp+1,0:kp+1), values 0/1 */
+  LES_CMASK1,  /* cmask1(0:ip+1,-1:jp+1,0:kp+1), values 0/1 */
+  LES_DMASK1,  /* dmask1(0:ip+1,0:jp+1,0:kp+1), values 0/1 */
+  LES_CN1,     /* cn1(ip,jp,kp) */
+  LES_CN2L,    /* cn2l(ip) */
+  LES_CN2S,
+  LES_CN3L,    /* cn3l(jp) */
+  LES_CN3S,
+  LES_CN4L,    /* cn4l(kp) */
+  LES_CN4S,
+  LES_DX1,     /* dx1(-1:ip+1) */
+  LES_DXS,     /* dxs(0:ip) */
+  LES_DY1,     /* dy1(0:jp+1) */
+  LES_DYS,     /* dys(0:jp) */
+  LES_DZN,     /* dzn(-1:kp+2) */
+  LES_DZS,     /* dzs(-1:kp+2) */
+  LES_Z2,      /* z2(kp+2) */
+  LES_DELX1,   /* delx1(kp) */
+  LES_WIND_PROFILE, /* wind_profile(jp,kp) (main_ocl.f95:162) */
+  LES_NFIELDS
+} les_field;
+
+/* Fortran bounds lo[3]..hi[3] of a field for interior sizes ip, jp, kp (1-D fields use only
+ * lo[0]/hi[0], 2-D wind_profile lo[0..1]/hi[0..1]); returns the element count, or -1. */
+int64_t les_field_bounds(int field, int ip, int jp, int kp, int lo[3], int hi[3]);
+
+/* ---- configuration ------------------------------------------------------------------- */
+enum { LES_PREC_FP32 = 0, LES_PREC_MIXED = 1 };
+
+typedef struct les_config {
+  int ip, jp, kp;   /* interior cells (params_common_sn.f95:3) */
+  float dt;         /* time step (set.f95:76) */
+  float ro;         /* density (set.f95:78) */
+  float vn;         /* kinematic viscosity (set.f95:79) */
+  float alpha;      /* feedback forcing gains (set.f95:81-82) */
+  float beta;
+  float cs0;        /* Smagorinsky constant (les.f95:31) */
+  float omega;      /* SOR relaxation (press.f95:46) */
+  float pjuge;      /* SOR early-exit threshold on the residual; 0 disables (press.f95:44,114) */
+  int nmaxp;        /* SOR iterations per step (press.f95:45) */
+  int precision;    /* LES_PREC_FP32: reference arithmetic; LES_PREC_MIXED: FP64 reductions and
+                       outer refinement of the pressure solve */
+  double refine_tol;  /* LES_PREC_MIXED: target L2 norm of the pressure residual */
+  int refine_max;     /* LES_PREC_MIXED: maximum outer refinement passes */
+  int device;         /* HIP device ordinal */
+  int sub_i, sub_j;   /* subdomains along i and j on the one device (1,1 = single domain) */
+} les_config;
+
+/* Fills the reference's defaults (set.f95, les.f95, press.f95) for an ip x jp x kp grid. */
+int les_default_config(les_config* cfg, int ip, int jp, int kp);
+
+typedef struct les_handle les_handle;
+
+/* Replaces oclInit + the oclMake*Buffer calls of initialise_LES_kernel
+ * (Wrappers/module_LES_combined_ocl.f95:224-306): allocates every device array, zeroed. */
+int les_create(const les_config* cfg, les_handle** out);
+int les_destroy(les_handle* h);
+const char* les_last_error(const les_handle* h);
+/* Waits for all work queued on the handle's stream. */
+int les_sync(les_handle* h);
+
+/* Replace the oclWrite* and oclRead* calls of one array (Wrappers/module_LES_combined_ocl.f95:359-418,
+ * TEMPL:1035-1056).  host points at the full Fortran-bounded array.  Mask fields must hold
+ * only 0 and 1.  Uploading a grid field (dx1..dzs) does not recompute coefficients: call
+ * les_init_coefficients afterwards, or upload cn1..cn4s and delx1 as well. */
+int les_upload(les_handle* h, int field, const float* host);
+int les_download(les_handle* h, int field, float* host);
+
+/* Device address and element count of a field (for callers that keep data on the device). */
+int les_device_ptr(les_handle* h, int field, void** dptr, int64_t* count);
+
+/* init.f95:82-107 (cn1, cn2l/s, cn3l/s, cn4l/s) and les.f95:33-37 (delx1) from the uploaded
+ * grid arrays, evaluated on the host in FP32 exactly as the reference does, then uploaded. */
+int les_init_coefficients(les_handle* h);
+
+/* ---- the kernel-level stages (reference names) ---------------------------------------- */
+/* velnw.f95:27-59 */
+int les_velnw(les_handle* h);
+/* bondv1_ext_wind_profile.f95:27-50: inflow planes i=0,1 from wind_profile, n==1 fill */
+int les_bondv1_init_uvw(les_handle* h, int n);
+/* state 1 of run_LES_kernel (TEMPL:459-497): velnw followed by the inflow/n==1 fill */
+int les_velnw__bondv1_init_uvw(les_handle* h, int n);
+/* bondv1.f95:67-75: uout = (max(0,max u(im,:,:)) + min(0,min u(im,:,:)))/2, kept on device */
+int les_bondv1_calc_uout(les_handle* h);
+/* bondv1.f95:81-137: outflow, periodic j, top/bottom of u, v, w */
+int les_bondv1_calc_uvw(les_handle* h);
+/* velFG.f95:56-105 with vel2.f95:48-188 (diu1..9 and their halo copies, f, g, h) */
+int les_velfg(les_handle* h);
+/* feedbf.f95:38-67 */
+int les_feedbf(les_handle* h);
+/* les.f95:33-60: Smagorinsky sm */
+int les_les_calc_sm(les_handle* h);
+/* state 4 (TEMPL:585-627): velfg + feedbf + les_calc_sm */
+int les_velfg__feedbf__les_calc_sm(les_handle* h);
+/* boundsm.f95:13-32 (state 5) */
+int les_les_bound_sm(les_handle* h);
+/* les.f95:78-182 */
+int les_les_calc_visc(les_handle* h);
+/* adam.f95:32-46 */
+int les_adam(les_handle* h);
+/* state 6 (TEMPL:668-706): les_calc_visc + adam */
+int les_les_calc_visc__adam(les_handle* h);
+/* les.f95 as a whole: calc_sm, boundsm, calc_visc */
+int les_les(les_handle* h);
+/* state 7: bondFG.f95:15-32, press.f95:51-82 (rhs, rhsav = sum(vol*rhs)/sum(vol), rhs -= rhsav);
+ * the volume sum is exact and order-independent, rounded once to FP32 */
+int les_press_rhsav(les_handle* h);
+/* state 8 (TEMPL:768-868): nrd = 0 or 1 -> one red-black colour sweep of press.f95:90-100
+ * followed by boundp1; nrd = 2 -> boundp2 */
+int les_press_sor(les_handle* h, int nrd);
+/* press.f95:87-115: the whole SOR loop, nmaxp iterations (early exit when pjuge > 0) */
+int les_press_sor_solve(les_handle* h);
+/* state 9: pav = sum(vol*p)/sum(vol) (press.f95:123-134), kept on device */
+int les_press_pav(les_handle* h);
+/* state 10: p -= pav (press.f95:135-141) */
+int les_press_adj(les_handle* h);
+/* state 11: boundp1 then boundp2 (press.f95:144-146, boundp.f95) */
+int les_press_boundp(les_handle* h);
+/* press.f95:49-146 as a whole */
+int les_press(les_handle* h);
+/* bondFG.f95, boundp.f95 on their own */
+int les_bondfg(les_handle* h);
+int les_boundp1(les_handle* h);
+int les_boundp2(les_handle* h);
+
+/* One whole time step n (main_ocl.f95:264-299 / run_LES_kernel states 1-11). */
+int les_step(les_handle* h, int n);
+/* Steps n0 .. n0+nsteps-1, replayed from a captured HIP graph of one step. */
+int les_run(les_handle* h, int n0, int nsteps);
+/* ifdata.f95:258-273 on the device: bondv1 (inflow from wind_profile, n=1), boundp1,
+ * boundp2, velfg, feedbf, les.  Fields must be zero (les_create) or uploaded. */
+int les_ifdata(les_handle* h);
+
+/* Scalars of the last stages (synchronises). */
+typedef struct les_scalars {
+  float uout;      /* bondv1.f95:75 */
+  float rhsav;     /* press.f95:75 (after the division) */
+  float area;      /* press.f95:71 */
+  float pav;       /* press.f95:134 (after the division) */
+  float pco;       /* press.f95:129 */
+  float sor;       /* press.f95:97 residual of the last SOR iteration (when computed) */
+  int sor_iters;   /* SOR iterations run in the last solve */
+  double resid_l2; /* LES_PREC_MIXED: final L2 norm of the pressure residual */
+  int refine_passes;
+} les_scalars;
+int les_get_scalars(les_handle* h, les_scalars* out);
+
+/* ---- halo exchange (src/OpenCL/Kernels/halo_exchange.cl, halo_exchange_correct.cl) --------
+ * Pack ("read") or unpack ("write") the h_w-wide N/S/W/E ring of one or more fields into a
+ * linear buffer.  The buffer holds v_dim consecutive blocks (one per field, in the order
+ * given), each laid out north|south|west|east as in halo_exchange.cl:3-51 and :156-185;
+ * "extent" arrays are the allocated (im, jm, km) of the fields, which must agree.
+ * correct = 0: halo_exchange.cl (ring rows/columns on the array edge, corners in N/S);
+ * correct = 1: halo_exchange_correct.cl read (inner ring, corners excluded). */
+int64_t les_halo_size(int v_dim, int im, int jm, int km, int h_w, int correct);
+int les_exchange_halo_read(les_handle* h, const int* fields, int v_dim, int h_w, int correct,
+                           float* dev_buffer);
+int les_exchange_halo_write(les_handle* h, const int* fields, int v_dim, int h_w,
+                            const float* dev_buffer);
+
+/* Exact, order-independent sum of n device floats, rounded once to FP32 (the reduction that
+ * rhsav, pav and the SOR residual use), result written to *out on the host. */
+int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n, float* out);
+
+/* ---- the reference's GPU host API (module_LES_combined_ocl) as single-handle drop-ins ---- */
+/* initialise_LES_kernel (Wrappers/module_LES_combined_ocl.f95:15-441), external wind profile
+ * form: creates the library's default handle and uploads every array. */
+int les_initialise_LES_kernel(
+    const float* p, const float* u, const float* v, const float* w, const float* usum,
+    const float* vsum, const float* wsum, const float* f, const float* g, const float* h,
+    const float* fold, const float* gold, const float* hold, const float* diu1,
+    const float* diu2, const float* diu3, const float* diu4, const float* diu5,
+    const float* diu6, const float* diu7, const float* diu8, const float* diu9,
+    const float* amask1, const float* bmask1, const float* cmask1, const float* dmask1,
+    const float* cn1, const float* cn2l, const float* cn2s, const float* cn3l,
+    const float* cn3s, const float* cn4l, const float* cn4s, const float* rhs,
+    const float* sm, const float* dxs, const float* dys, const float* dzs, const float* dx1,
+    const float* dy1, const float* dzn, const float* wind_profile, float dt, int im, int jm,
+    int km);
+/* run_LES_kernel(n, nmax, wind_profile) (TEMPL:219-1072): one step on the default handle. */
+int les_run_LES_kernel(int n, int nmax, const float* wind_profile);
+/* The default handle (for downloads/checkpoints), or NULL. */
+les_handle* les_default_handle(void);
+/* Releases the default handle. */
+int les_finalise_LES_kernel(void);
+
+/* ---- synthetic inputs for the benchmark configurations (host only, deterministic) ------- */
+/* Grid: dx1 = dx, dy1 = dy everywhere; dzn = dz0 while the cell top is <= z_flat, then each
+ * level is the previous one times ratio; halos, z2, dxs, dys, dzs as grid.f95:41-82. */
+int les_gen_grid(int ip, int jp, int kp, float dx, float dy, float dz0, float z_flat,
+                 float ratio, float* dx1, float* dxs, float* dy1, float* dys, float* dzn,
+                 float* dzs, float* z2);
+/* Terrain: zbm(-1:ip+1,-1:jp+1) of seeded rectangular blocks (widths in [wmin,wmax] m,
+ * heights in [0,hmax] m), max-overlap, until >= coverage of the (i,j) columns are covered. */
+int les_gen_terrain(int ip, int jp, float dx, float dy, uint64_t seed, float coverage,
+                    float wmin, float wmax, float hmax, float* zbm);
+/* feedbfm.f95:42-63: masks from zbm (amask1 = 0 where zbm(i,j) > z2(k)+0.5*dzn(k), then
+ * bmask1 = cmask1 = dmask1 = 1 there); halos are left 0. */
+int les_gen_masks(int ip, int jp, int kp, const float* zbm, const float* z2, const float* dzn,
+                  float* amask1, float* bmask1, float* cmask1, float* dmask1);
+/* Log-law inflow u(z) = (ustar/0.4) ln(z/z0) (1 + noise*xi_jk), z = z2(k)+0.5*dzn(k)
+ * (the inflow height of bondv1.f95:29), xi ~ N(0,1) seeded. */
+int les_gen_inflow(int jp, int kp, const float* z2, const float* dzn, float ustar, float z0,
+                   float noise, uint64_t seed, float* wind_profile);
+/* initial_wind_profile.f95:22-32 (the reference's power-law profile, kp >= 78). */
+int les_initial_wind_profile(int jp, int kp, const float* z2, const float* dzn,
+                             float* wind_profile);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LES_HIP_H */

@@ -1,0 +1,818 @@
+// C ABI of libles_hip.so (include/les_hip.h): handle, device arrays, uploads/downloads and
+// the stage orchestration that replaces run_LES_kernel's state machine
+// (src/OpenCL/module_LES_combined_ocl_TEMPL_V2.f95:219-1072).  Everything between the stages
+// stays on the device: scalars (uout, rhsav, pav, residual) live in device memory and the
+// SOR early exit is a device flag, so a whole step can be captured in one HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/les_hip.h"
+#include "les_kernels.h"
+
+using namespace les;
+
+struct les_handle {
+  les_config cfg;
+  Dims d;
+  Physics ph;
+  hipStream_t st = nullptr;
+  float* fld[LES_NFIELDS] = {};
+  int64_t cnt[LES_NFIELDS] = {};
+  uint8_t* mask = nullptr;  // interior bits: 1 b, 2 c, 4 d, 8 a
+  DevScalars* sc = nullptr;
+  RedBuf rb{};
+  std::vector<float> hgrid[LES_NFIELDS];  // host copies of the 1-D grid/coefficient arrays
+  std::string err;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  float* scratch_f = nullptr;  // single-float device scratch
+};
+
+namespace {
+
+les_handle* g_default = nullptr;
+
+int fail(les_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((h), LES_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+// Bounds table: lo/hi offsets relative to (0, ip, jp, kp) per dimension.
+struct BoundSpec {
+  int ndim;
+  int lo[3];
+  int hi_base[3];  // 0 = ip, 1 = jp, 2 = kp
+  int hi_off[3];
+};
+
+bool spec_of(int f, BoundSpec& s) {
+  auto three = [&](int l0, int h0, int l1, int h1, int l2, int h2) {
+    s.ndim = 3;
+    s.lo[0] = l0; s.hi_base[0] = 0; s.hi_off[0] = h0;
+    s.lo[1] = l1; s.hi_base[1] = 1; s.hi_off[1] = h1;
+    s.lo[2] = l2; s.hi_base[2] = 2; s.hi_off[2] = h2;
+    return true;
+  };
+  auto one = [&](int base, int l0, int h0) {
+    s.ndim = 1;
+    s.lo[0] = l0; s.hi_base[0] = base; s.hi_off[0] = h0;
+    return true;
+  };
+  switch (f) {
+    case LES_P: return three(0, 2, 0, 2, 0, 1);
+    case LES_U: case LES_V: return three(0, 1, -1, 1, 0, 1);
+    case LES_W: return three(0, 1, -1, 1, -1, 1);
+    case LES_USUM: case LES_VSUM: case LES_WSUM: case LES_F: case LES_G: case LES_H:
+      return three(0, 0, 0, 0, 0, 0);
+    case LES_FOLD: case LES_GOLD: case LES_HOLD: case LES_CN1: return three(1, 0, 1, 0, 1, 0);
+    case LES_RHS: return three(0, 1, 0, 1, 0, 1);
+    case LES_SM: return three(-1, 1, -1, 1, 0, 1);
+    case LES_DIU1: case LES_DIU5: return three(-1, 2, 0, 2, 0, 2);
+    case LES_DIU2: case LES_DIU3: case LES_DIU4: case LES_DIU6: case LES_DIU7: case LES_DIU8:
+    case LES_DIU9: return three(0, 2, 0, 2, 0, 2);
+    case LES_AMASK1: case LES_DMASK1: return three(0, 1, 0, 1, 0, 1);
+    case LES_BMASK1: return three(-1, 1, 0, 1, 0, 1);
+    case LES_CMASK1: return three(0, 1, -1, 1, 0, 1);
+    case LES_CN2L: case LES_CN2S: return one(0, 1, 0);
+    case LES_CN3L: case LES_CN3S: return one(1, 1, 0);
+    case LES_CN4L: case LES_CN4S: case LES_DELX1: return one(2, 1, 0);
+    case LES_DX1: return one(0, -1, 1);
+    case LES_DXS: return one(0, 0, 0);
+    case LES_DY1: return one(1, 0, 1);
+    case LES_DYS: return one(1, 0, 0);
+    case LES_DZN: case LES_DZS: return one(2, -1, 2);
+    case LES_Z2: return one(2, 1, 2);
+    case LES_WIND_PROFILE:
+      s.ndim = 2;
+      s.lo[0] = 1; s.hi_base[0] = 1; s.hi_off[0] = 0;
+      s.lo[1] = 1; s.hi_base[1] = 2; s.hi_off[1] = 0;
+      return true;
+    default: return false;
+  }
+}
+
+bool is_mask(int f) { return f >= LES_AMASK1 && f <= LES_DMASK1; }
+bool is_1d_host(int f) { return f >= LES_CN2L && f <= LES_WIND_PROFILE; }
+
+Grid1D grid_of(les_handle* h) {
+  Grid1D g;
+  g.dx1 = h->fld[LES_DX1];
+  g.dxs = h->fld[LES_DXS];
+  g.dy1 = h->fld[LES_DY1];
+  g.dys = h->fld[LES_DYS];
+  g.dzn = h->fld[LES_DZN];
+  g.dzs = h->fld[LES_DZS];
+  g.cn2l = h->fld[LES_CN2L];
+  g.cn2s = h->fld[LES_CN2S];
+  g.cn3l = h->fld[LES_CN3L];
+  g.cn3s = h->fld[LES_CN3S];
+  g.cn4l = h->fld[LES_CN4L];
+  g.cn4s = h->fld[LES_CN4S];
+  g.delx1 = h->fld[LES_DELX1];
+  return g;
+}
+
+Fields fields_of(les_handle* h) {
+  Fields F;
+  F.p = h->fld[LES_P];
+  F.u = h->fld[LES_U];
+  F.v = h->fld[LES_V];
+  F.w = h->fld[LES_W];
+  F.usum = h->fld[LES_USUM];
+  F.vsum = h->fld[LES_VSUM];
+  F.wsum = h->fld[LES_WSUM];
+  F.f = h->fld[LES_F];
+  F.g = h->fld[LES_G];
+  F.h = h->fld[LES_H];
+  F.fold = h->fld[LES_FOLD];
+  F.gold = h->fld[LES_GOLD];
+  F.hold = h->fld[LES_HOLD];
+  F.rhs = h->fld[LES_RHS];
+  F.sm = h->fld[LES_SM];
+  F.diu1 = h->fld[LES_DIU1];
+  F.diu2 = h->fld[LES_DIU2];
+  F.diu3 = h->fld[LES_DIU3];
+  F.diu4 = h->fld[LES_DIU4];
+  F.diu5 = h->fld[LES_DIU5];
+  F.diu6 = h->fld[LES_DIU6];
+  F.diu7 = h->fld[LES_DIU7];
+  F.diu8 = h->fld[LES_DIU8];
+  F.diu9 = h->fld[LES_DIU9];
+  F.mask = h->mask;
+  F.cn1 = h->fld[LES_CN1];
+  F.wind = h->fld[LES_WIND_PROFILE];
+  return F;
+}
+
+// interior mask bits from the four float masks (host arrays in their Fortran bounds)
+void build_mask_bits(const les_handle* h, const std::vector<float>* hm, std::vector<uint8_t>& out) {
+  const int ip = h->d.ip, jp = h->d.jp, kp = h->d.kp;
+  out.assign((size_t)ip * jp * kp, 0);
+  for (int k = 1; k <= kp; ++k)
+    for (int j = 1; j <= jp; ++j)
+      for (int i = 1; i <= ip; ++i) {
+        long ia = i + (long)(ip + 2) * (j + (long)(jp + 2) * k);        // a,d: (0:ip+1,0:jp+1,0:kp+1)
+        long ib = (i + 1) + (long)(ip + 3) * (j + (long)(jp + 2) * k);  // b: (-1:ip+1,0:jp+1,0:kp+1)
+        long ic = i + (long)(ip + 2) * ((j + 1) + (long)(jp + 3) * k);  // c: (0:ip+1,-1:jp+1,0:kp+1)
+        uint8_t m = 0;
+        if (hm[0].size() && hm[0][ia] != 0.0f) m |= 8;
+        if (hm[1].size() && hm[1][ib] != 0.0f) m |= 1;
+        if (hm[2].size() && hm[2][ic] != 0.0f) m |= 2;
+        if (hm[3].size() && hm[3][ia] != 0.0f) m |= 4;
+        out[(size_t)(i - 1) + (size_t)ip * ((j - 1) + (size_t)jp * (k - 1))] = m;
+      }
+}
+
+int upload_scalar_area(les_handle* h) {
+  // press.f95:67-74/125-133: sum of dx1(i)*dy1(j)*dzn(k) in the reference's sequential FP32
+  // order; a grid constant, evaluated once on the host.
+  const auto& dx1 = h->hgrid[LES_DX1];
+  const auto& dy1 = h->hgrid[LES_DY1];
+  const auto& dzn = h->hgrid[LES_DZN];
+  if (dx1.empty() || dy1.empty() || dzn.empty()) return LES_OK;
+  float area = 0.0f;
+  for (int k = 1; k <= h->d.kp; ++k)
+    for (int j = 1; j <= h->d.jp; ++j)
+      for (int i = 1; i <= h->d.ip; ++i) area = area + dx1[i + 1] * dy1[j] * dzn[k + 1];
+  HIPCHK(h, hipMemcpyAsync(&h->sc->area, &area, sizeof(float), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(&h->sc->pco, &area, sizeof(float), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LES_OK;
+}
+
+int check_launch(les_handle* h) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return LES_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+extern "C" int64_t les_field_bounds(int field, int ip, int jp, int kp, int lo[3], int hi[3]) {
+  BoundSpec s;
+  if (!spec_of(field, s)) return -1;
+  const int base[3] = {ip, jp, kp};
+  int64_t n = 1;
+  for (int q = 0; q < 3; ++q) {
+    if (q < s.ndim) {
+      lo[q] = s.lo[q];
+      hi[q] = base[s.hi_base[q]] + s.hi_off[q];
+      n *= (int64_t)(hi[q] - lo[q] + 1);
+    } else {
+      lo[q] = 0;
+      hi[q] = 0;
+    }
+  }
+  return n;
+}
+
+extern "C" int les_default_config(les_config* c, int ip, int jp, int kp) {
+  if (!c) return LES_E_INVALID;
+  std::memset(c, 0, sizeof(*c));
+  c->ip = ip;
+  c->jp = jp;
+  c->kp = kp;
+  c->dt = 0.2f;           // set.f95:76
+  c->ro = 1.1763f;        // set.f95:78
+  c->vn = 15.83f * powf(10.0f, -6.0f);  // set.f95:79
+  c->alpha = -10.0f;      // set.f95:81
+  c->beta = -1.0f;        // set.f95:82
+  c->cs0 = 0.1f;          // les.f95:31
+  c->omega = 1.0f;        // press.f95:46
+  c->pjuge = 0.0001f;     // press.f95:44
+  c->nmaxp = 50;          // press.f95:45
+  c->precision = LES_PREC_FP32;
+  c->refine_tol = 1e-7;
+  c->refine_max = 8;
+  c->device = 0;
+  c->sub_i = 1;
+  c->sub_j = 1;
+  return LES_OK;
+}
+
+extern "C" int les_create(const les_config* cfg, les_handle** out) {
+  if (!cfg || !out) return LES_E_INVALID;
+  *out = nullptr;
+  if (cfg->ip < 2 || cfg->jp < 2 || cfg->kp < 2 || cfg->ip > 16384 || cfg->jp > 16384 ||
+      cfg->kp > 4096 || cfg->nmaxp < 0)
+    return LES_E_INVALID;
+  if (cfg->sub_i != 1 || cfg->sub_j != 1) return LES_E_INVALID;  // one domain per handle
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LES_E_NODEV;
+  if (cfg->device < 0 || cfg->device >= ndev) return LES_E_INVALID;
+  les_handle* h = new les_handle();
+  h->cfg = *cfg;
+  h->d = Dims{cfg->ip, cfg->jp, cfg->kp};
+  h->ph = Physics{cfg->dt, cfg->ro, cfg->vn, cfg->alpha, cfg->beta, cfg->cs0, cfg->omega, cfg->pjuge};
+  auto bail = [&](int code) {
+    les_destroy(h);
+    return code;
+  };
+  if (hipSetDevice(cfg->device) != hipSuccess) return bail(LES_E_HIP);
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return bail(LES_E_HIP);
+  for (int f = 0; f < LES_NFIELDS; ++f) {
+    int lo[3], hi[3];
+    int64_t n = les_field_bounds(f, cfg->ip, cfg->jp, cfg->kp, lo, hi);
+    h->cnt[f] = n;
+    if (hipMalloc(&h->fld[f], (size_t)n * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
+    if (hipMemsetAsync(h->fld[f], 0, (size_t)n * sizeof(float), h->st) != hipSuccess)
+      return bail(LES_E_HIP);
+    if (is_1d_host(f) || f == LES_DX1 || f == LES_DXS || f == LES_DY1 || f == LES_DYS ||
+        f == LES_DZN || f == LES_DZS || f == LES_Z2)
+      h->hgrid[f].assign((size_t)n, 0.0f);
+  }
+  size_t nmask = (size_t)cfg->ip * cfg->jp * cfg->kp;
+  if (hipMalloc(&h->mask, nmask) != hipSuccess) return bail(LES_E_NOMEM);
+  if (hipMemsetAsync(h->mask, 0, nmask, h->st) != hipSuccess) return bail(LES_E_HIP);
+  if (hipMalloc(&h->sc, sizeof(DevScalars)) != hipSuccess) return bail(LES_E_NOMEM);
+  if (hipMemsetAsync(h->sc, 0, sizeof(DevScalars), h->st) != hipSuccess) return bail(LES_E_HIP);
+  h->rb.max_blocks = std::max(std::max(reduction_blocks(h->d), 2 * sor_resid_blocks(h->d)), 4096);
+  if (hipMalloc(&h->rb.partials, (size_t)h->rb.max_blocks * NLIMB * sizeof(int64_t)) != hipSuccess)
+    return bail(LES_E_NOMEM);
+  if (hipMalloc(&h->scratch_f, 256) != hipSuccess) return bail(LES_E_NOMEM);
+  if (hipStreamSynchronize(h->st) != hipSuccess) return bail(LES_E_HIP);
+  *out = h;
+  return LES_OK;
+}
+
+extern "C" int les_destroy(les_handle* h) {
+  if (!h) return LES_OK;
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  if (h->st) hipStreamSynchronize(h->st);
+  for (int f = 0; f < LES_NFIELDS; ++f)
+    if (h->fld[f]) hipFree(h->fld[f]);
+  if (h->mask) hipFree(h->mask);
+  if (h->sc) hipFree(h->sc);
+  if (h->rb.partials) hipFree(h->rb.partials);
+  if (h->scratch_f) hipFree(h->scratch_f);
+  if (h->st) hipStreamDestroy(h->st);
+  if (h == g_default) g_default = nullptr;
+  delete h;
+  return LES_OK;
+}
+
+extern "C" const char* les_last_error(const les_handle* h) { return h ? h->err.c_str() : ""; }
+
+extern "C" int les_sync(les_handle* h) {
+  if (!h) return LES_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LES_OK;
+}
+
+extern "C" int les_upload(les_handle* h, int field, const float* host) {
+  if (!h || !host || field < 0 || field >= LES_NFIELDS) return fail(h, LES_E_INVALID, "upload: bad argument");
+  const size_t n = (size_t)h->cnt[field];
+  if (is_mask(field)) {
+    for (size_t q = 0; q < n; ++q)
+      if (host[q] != 0.0f && host[q] != 1.0f)
+        return fail(h, LES_E_INVALID, "upload: mask values must be 0 or 1");
+  }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(h->fld[field], host, n * sizeof(float), hipMemcpyHostToDevice));
+  if (!h->hgrid[field].empty()) std::memcpy(h->hgrid[field].data(), host, n * sizeof(float));
+  if (is_mask(field)) {
+    // rebuild the interior bit mask from the four device masks
+    std::vector<float> hm[4];
+    for (int q = 0; q < 4; ++q) {
+      hm[q].resize((size_t)h->cnt[LES_AMASK1 + q]);
+      HIPCHK(h, hipMemcpy(hm[q].data(), h->fld[LES_AMASK1 + q], hm[q].size() * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    }
+    std::vector<uint8_t> bits;
+    build_mask_bits(h, hm, bits);
+    HIPCHK(h, hipMemcpy(h->mask, bits.data(), bits.size(), hipMemcpyHostToDevice));
+  }
+  if (field == LES_DX1 || field == LES_DY1 || field == LES_DZN) {
+    int r = upload_scalar_area(h);
+    if (r) return r;
+  }
+  return LES_OK;
+}
+
+extern "C" int les_download(les_handle* h, int field, float* host) {
+  if (!h || !host || field < 0 || field >= LES_NFIELDS) return fail(h, LES_E_INVALID, "download: bad argument");
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(host, h->fld[field], (size_t)h->cnt[field] * sizeof(float), hipMemcpyDeviceToHost));
+  return LES_OK;
+}
+
+extern "C" int les_device_ptr(les_handle* h, int field, void** dptr, int64_t* count) {
+  if (!h || field < 0 || field >= LES_NFIELDS) return LES_E_INVALID;
+  if (dptr) *dptr = h->fld[field];
+  if (count) *count = h->cnt[field];
+  return LES_OK;
+}
+
+// init.f95:82-107 and les.f95:33-37 on the host, FP32, reference operation order
+extern "C" int les_init_coefficients(les_handle* h) {
+  if (!h) return LES_E_INVALID;
+  const int ip = h->d.ip, jp = h->d.jp, kp = h->d.kp;
+  auto& dxs = h->hgrid[LES_DXS];
+  auto& dys = h->hgrid[LES_DYS];
+  auto& dzs = h->hgrid[LES_DZS];
+  auto& dx1 = h->hgrid[LES_DX1];
+  auto& dy1 = h->hgrid[LES_DY1];
+  auto& dzn = h->hgrid[LES_DZN];
+  std::vector<float> cn2s(ip), cn2l(ip), cn3s(jp), cn3l(jp), cn4s(kp), cn4l(kp), delx1(kp);
+  const float two = 2.0f, one = 1.0f;
+  for (int i = 1; i <= ip; ++i) {
+    cn2s[i - 1] = two / (dxs[i - 1] * (dxs[i - 1] + dxs[i]));
+    cn2l[i - 1] = two / (dxs[i] * (dxs[i - 1] + dxs[i]));
+  }
+  for (int j = 1; j <= jp; ++j) {
+    cn3s[j - 1] = two / (dys[j - 1] * (dys[j - 1] + dys[j]));
+    cn3l[j - 1] = two / (dys[j] * (dys[j - 1] + dys[j]));
+  }
+  std::vector<float> cn1((size_t)ip * jp * kp);
+  for (int k = 1; k <= kp; ++k) {
+    const float dz1 = dzs[k], dz2 = dzs[k + 1];  // dzs(k-1), dzs(k) with offset -1
+    cn4s[k - 1] = two / (dz1 * (dz1 + dz2));
+    cn4l[k - 1] = two / (dz2 * (dz1 + dz2));
+    for (int j = 1; j <= jp; ++j)
+      for (int i = 1; i <= ip; ++i) {
+        float c = two / (dxs[i - 1] * dxs[i]) + two / (dys[j - 1] * dys[j]) + two / (dz1 * dz2);
+        cn1[(size_t)(i - 1) + (size_t)ip * ((j - 1) + (size_t)jp * (k - 1))] = one / c;
+      }
+    // les.f95:36 (dx1(0)*dy1(0)*dzn(k))**(1./3.)
+    delx1[k - 1] = powf(dx1[1] * dy1[0] * dzn[k + 1], 1.0f / 3.0f);
+  }
+  int r;
+  if ((r = les_upload(h, LES_CN2S, cn2s.data()))) return r;
+  if ((r = les_upload(h, LES_CN2L, cn2l.data()))) return r;
+  if ((r = les_upload(h, LES_CN3S, cn3s.data()))) return r;
+  if ((r = les_upload(h, LES_CN3L, cn3l.data()))) return r;
+  if ((r = les_upload(h, LES_CN4S, cn4s.data()))) return r;
+  if ((r = les_upload(h, LES_CN4L, cn4l.data()))) return r;
+  if ((r = les_upload(h, LES_CN1, cn1.data()))) return r;
+  if ((r = les_upload(h, LES_DELX1, delx1.data()))) return r;
+  return upload_scalar_area(h);
+}
+
+// ------------------------------------------------------------------------------------------
+// stages
+// ------------------------------------------------------------------------------------------
+#define STAGE_BEGIN(h)                 \
+  if (!(h)) return LES_E_INVALID;      \
+  const Dims d = (h)->d;               \
+  const Grid1D g = grid_of(h);         \
+  const Fields F = fields_of(h);       \
+  (void)d; (void)g; (void)F;
+
+extern "C" int les_velnw(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_bondv1_init_uvw(les_handle* h, int n) {
+  STAGE_BEGIN(h);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 0, n < 1 ? 2 : n);
+  return check_launch(h);
+}
+
+extern "C" int les_velnw__bondv1_init_uvw(les_handle* h, int n) {
+  STAGE_BEGIN(h);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n);
+  return check_launch(h);
+}
+
+extern "C" int les_bondv1_calc_uout(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_uout(h->st, d, F, h->sc);
+  return check_launch(h);
+}
+
+extern "C" int les_bondv1_calc_uvw(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
+  return check_launch(h);
+}
+
+extern "C" int les_velfg(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_velfg(h->st, d, g, F, h->ph, 1, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_feedbf(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_velfg(h->st, d, g, F, h->ph, 0, 1);
+  return check_launch(h);
+}
+
+extern "C" int les_les_calc_sm(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_calc_sm(h->st, d, g, F, h->ph);
+  return check_launch(h);
+}
+
+extern "C" int les_velfg__feedbf__les_calc_sm(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_velfg(h->st, d, g, F, h->ph, 1, 1);
+  launch_calc_sm(h->st, d, g, F, h->ph);
+  return check_launch(h);
+}
+
+extern "C" int les_les_bound_sm(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_boundsm(h->st, d, F);
+  return check_launch(h);
+}
+
+extern "C" int les_les_calc_visc(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_adam(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_calc_visc_adam(h->st, d, g, F, 0, 1);
+  return check_launch(h);
+}
+
+extern "C" int les_les_calc_visc__adam(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 1);
+  return check_launch(h);
+}
+
+extern "C" int les_les(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_calc_sm(h->st, d, g, F, h->ph);
+  launch_boundsm(h->st, d, F);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_bondfg(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_bondfg(h->st, d, F);
+  return check_launch(h);
+}
+
+extern "C" int les_press_rhsav(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
+  launch_rhs_adjust(h->st, d, F, h->sc);
+  return check_launch(h);
+}
+
+extern "C" int les_boundp1(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_boundp1(h->st, d, F, h->sc, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_boundp2(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_boundp2(h->st, d, F, h->sc, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_press_sor(les_handle* h, int nrd) {
+  STAGE_BEGIN(h);
+  if (nrd < 0 || nrd > 2) return fail(h, LES_E_INVALID, "press_sor: nrd must be 0, 1 or 2");
+  if (nrd == 2) {
+    launch_boundp2(h->st, d, F, h->sc, 0);
+  } else {
+    launch_sor_colour(h->st, d, g, F, h->ph, nrd, 0, h->rb, 0, h->sc);
+    launch_boundp1(h->st, d, F, h->sc, 0);
+  }
+  return check_launch(h);
+}
+
+namespace {
+int enqueue_sor_solve(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_sor_reset(h->st, h->sc);
+  const bool resid = h->ph.pjuge > 0.0f;
+  const int nb = sor_resid_blocks(d);
+  for (int l = 1; l <= h->cfg.nmaxp; ++l) {
+    for (int nrd = 0; nrd <= 1; ++nrd) {
+      launch_sor_colour(h->st, d, g, F, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
+      launch_boundp1(h->st, d, F, h->sc, 1);
+    }
+    launch_boundp2(h->st, d, F, h->sc, 1);
+    if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, h->ph, h->sc);
+  }
+  return check_launch(h);
+}
+
+int enqueue_press(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
+  launch_rhs_adjust(h->st, d, F, h->sc);
+  int r = enqueue_sor_solve(h);
+  if (r) return r;
+  launch_pav(h->st, d, g, F, h->rb, h->sc);
+  launch_adj(h->st, d, F, h->sc, 1);
+  return check_launch(h);
+}
+
+int enqueue_step(les_handle* h, int n) {
+  STAGE_BEGIN(h);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n);
+  launch_uout(h->st, d, F, h->sc);
+  launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
+  launch_velfg(h->st, d, g, F, h->ph, 1, 1);
+  launch_calc_sm(h->st, d, g, F, h->ph);
+  launch_boundsm(h->st, d, F);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 1);
+  return enqueue_press(h);
+}
+}  // namespace
+
+extern "C" int les_press_sor_solve(les_handle* h) {
+  if (!h) return LES_E_INVALID;
+  return enqueue_sor_solve(h);
+}
+
+extern "C" int les_press_pav(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_pav(h->st, d, g, F, h->rb, h->sc);
+  return check_launch(h);
+}
+
+extern "C" int les_press_adj(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_adj(h->st, d, F, h->sc, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_press_boundp(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_boundp1(h->st, d, F, h->sc, 0);
+  launch_boundp2(h->st, d, F, h->sc, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_press(les_handle* h) {
+  if (!h) return LES_E_INVALID;
+  return enqueue_press(h);
+}
+
+extern "C" int les_step(les_handle* h, int n) {
+  if (!h) return LES_E_INVALID;
+  return enqueue_step(h, n);
+}
+
+extern "C" int les_run(les_handle* h, int n0, int nsteps) {
+  if (!h || nsteps < 0) return LES_E_INVALID;
+  int n = n0;
+  // the n == 1 step differs (interior fill): run it eagerly
+  while (nsteps > 0 && n <= 1) {
+    int r = enqueue_step(h, n);
+    if (r) return r;
+    ++n;
+    --nsteps;
+  }
+  if (nsteps == 0) return LES_OK;
+  if (!h->graph_valid) {
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    int r = enqueue_step(h, 2);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(h->st, &gr);
+    if (r) {
+      if (gr) hipGraphDestroy(gr);
+      return r;
+    }
+    if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    h->graph = gr;
+    HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->graph_valid = true;
+  }
+  for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->st));
+  return LES_OK;
+}
+
+extern "C" int les_ifdata(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 0, 1);
+  launch_uout(h->st, d, F, h->sc);
+  launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
+  launch_boundp1(h->st, d, F, h->sc, 0);
+  launch_boundp2(h->st, d, F, h->sc, 0);
+  launch_velfg(h->st, d, g, F, h->ph, 1, 1);
+  launch_calc_sm(h->st, d, g, F, h->ph);
+  launch_boundsm(h->st, d, F);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 0);
+  return check_launch(h);
+}
+
+extern "C" int les_get_scalars(les_handle* h, les_scalars* out) {
+  if (!h || !out) return LES_E_INVALID;
+  DevScalars s;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  out->uout = s.uout;
+  out->rhsav = s.rhsav;
+  out->area = s.area;
+  out->pav = s.pav;
+  out->pco = s.pco;
+  out->sor = s.sor;
+  out->sor_iters = (h->ph.pjuge > 0.0f) ? s.sor_iters : h->cfg.nmaxp;
+  out->resid_l2 = s.resid_l2;
+  out->refine_passes = s.refine_passes;
+  return LES_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// halo exchange
+// ------------------------------------------------------------------------------------------
+namespace {
+bool extent_of(const les_handle* h, int field, int& im, int& jm, int& km) {
+  int lo[3], hi[3];
+  if (les_field_bounds(field, h->d.ip, h->d.jp, h->d.kp, lo, hi) < 0) return false;
+  BoundSpec s;
+  spec_of(field, s);
+  if (s.ndim != 3) return false;
+  im = hi[0] - lo[0] + 1;
+  jm = hi[1] - lo[1] + 1;
+  km = hi[2] - lo[2] + 1;
+  return true;
+}
+}  // namespace
+
+extern "C" int64_t les_halo_size(int v_dim, int im, int jm, int km, int h_w, int correct) {
+  if (v_dim < 1 || im < 1 || jm < 1 || km < 1 || h_w < 1) return -1;
+  if (correct) return (int64_t)v_dim * 2 * km * (int64_t)(im + jm - 4 * h_w) * h_w;
+  return (int64_t)v_dim * (2LL * km * im * h_w + 2LL * km * (jm - 2 * h_w) * h_w);
+}
+
+static int halo_common(les_handle* h, const int* fields, int v_dim, int h_w, float** ptrs, int& im,
+                       int& jm, int& km) {
+  if (!h || !fields || v_dim < 1 || v_dim > 16 || h_w < 1) return LES_E_INVALID;
+  for (int v = 0; v < v_dim; ++v) {
+    int a, b, c;
+    if (fields[v] < 0 || fields[v] >= LES_NFIELDS || !extent_of(h, fields[v], a, b, c))
+      return fail(h, LES_E_INVALID, "halo: not a 3-D field");
+    if (v == 0) {
+      im = a;
+      jm = b;
+      km = c;
+    } else if (a != im || b != jm || c != km) {
+      return fail(h, LES_E_INVALID, "halo: fields of one exchange must have equal extents");
+    }
+    ptrs[v] = h->fld[fields[v]];
+  }
+  if (jm < 2 * h_w + 1 || im < 2 * h_w + 1) return fail(h, LES_E_INVALID, "halo: width too large");
+  return LES_OK;
+}
+
+extern "C" int les_exchange_halo_read(les_handle* h, const int* fields, int v_dim, int h_w,
+                                      int correct, float* dev_buffer) {
+  float* ptrs[16];
+  int im, jm, km;
+  int r = halo_common(h, fields, v_dim, h_w, ptrs, im, jm, km);
+  if (r) return r;
+  if (!dev_buffer) return fail(h, LES_E_INVALID, "halo: null buffer");
+  launch_halo_read(h->st, ptrs, v_dim, im, jm, km, h_w, correct ? 1 : 0, dev_buffer);
+  return check_launch(h);
+}
+
+extern "C" int les_exchange_halo_write(les_handle* h, const int* fields, int v_dim, int h_w,
+                                       const float* dev_buffer) {
+  float* ptrs[16];
+  int im, jm, km;
+  int r = halo_common(h, fields, v_dim, h_w, ptrs, im, jm, km);
+  if (r) return r;
+  if (!dev_buffer) return fail(h, LES_E_INVALID, "halo: null buffer");
+  launch_halo_write(h->st, ptrs, v_dim, im, jm, km, h_w, dev_buffer);
+  return check_launch(h);
+}
+
+// exact-sum utility on a device vector; result to host
+extern "C" int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n, float* out) {
+  if (!h || !dev_x || !out || n < 0) return LES_E_INVALID;
+  launch_exact_sum(h->st, dev_x, (long)n, h->rb, h->scratch_f);
+  HIPCHK(h, hipMemcpyAsync(out, h->scratch_f, sizeof(float), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return LES_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// the reference's GPU host API on a default handle
+// ------------------------------------------------------------------------------------------
+extern "C" int les_initialise_LES_kernel(
+    const float* p, const float* u, const float* v, const float* w, const float* usum,
+    const float* vsum, const float* wsum, const float* f, const float* g, const float* hh,
+    const float* fold, const float* gold, const float* hold, const float* diu1,
+    const float* diu2, const float* diu3, const float* diu4, const float* diu5,
+    const float* diu6, const float* diu7, const float* diu8, const float* diu9,
+    const float* amask1, const float* bmask1, const float* cmask1, const float* dmask1,
+    const float* cn1, const float* cn2l, const float* cn2s, const float* cn3l,
+    const float* cn3s, const float* cn4l, const float* cn4s, const float* rhs,
+    const float* sm, const float* dxs, const float* dys, const float* dzs, const float* dx1,
+    const float* dy1, const float* dzn, const float* wind_profile, float dt, int im, int jm,
+    int km) {
+  if (g_default) les_destroy(g_default);
+  les_config cfg;
+  les_default_config(&cfg, im, jm, km);
+  cfg.dt = dt;
+  les_handle* h = nullptr;
+  int r = les_create(&cfg, &h);
+  if (r) return r;
+  g_default = h;
+  const float* arrs[] = {p, u, v, w, usum, vsum, wsum, f, g, hh, fold, gold, hold, rhs, sm,
+                         diu1, diu2, diu3, diu4, diu5, diu6, diu7, diu8, diu9,
+                         amask1, bmask1, cmask1, dmask1, cn1, cn2l, cn2s, cn3l, cn3s, cn4l, cn4s,
+                         dx1, dxs, dy1, dys, dzn, dzs};
+  const int ids[] = {LES_P, LES_U, LES_V, LES_W, LES_USUM, LES_VSUM, LES_WSUM, LES_F, LES_G,
+                     LES_H, LES_FOLD, LES_GOLD, LES_HOLD, LES_RHS, LES_SM, LES_DIU1, LES_DIU2,
+                     LES_DIU3, LES_DIU4, LES_DIU5, LES_DIU6, LES_DIU7, LES_DIU8, LES_DIU9,
+                     LES_AMASK1, LES_BMASK1, LES_CMASK1, LES_DMASK1, LES_CN1, LES_CN2L, LES_CN2S,
+                     LES_CN3L, LES_CN3S, LES_CN4L, LES_CN4S, LES_DX1, LES_DXS, LES_DY1, LES_DYS,
+                     LES_DZN, LES_DZS};
+  for (size_t q = 0; q < sizeof(ids) / sizeof(ids[0]); ++q) {
+    if (!arrs[q]) continue;
+    if ((r = les_upload(h, ids[q], arrs[q]))) return r;
+  }
+  if (wind_profile && (r = les_upload(h, LES_WIND_PROFILE, wind_profile))) return r;
+  // delx1 (les.f95:36) from the uploaded grid
+  {
+    std::vector<float> delx1(km);
+    const auto& hdx1 = h->hgrid[LES_DX1];
+    const auto& hdy1 = h->hgrid[LES_DY1];
+    const auto& hdzn = h->hgrid[LES_DZN];
+    for (int k = 1; k <= km; ++k) delx1[k - 1] = powf(hdx1[1] * hdy1[0] * hdzn[k + 1], 1.0f / 3.0f);
+    if ((r = les_upload(h, LES_DELX1, delx1.data()))) return r;
+  }
+  return les_sync(h);
+}
+
+extern "C" int les_run_LES_kernel(int n, int nmax, const float* wind_profile) {
+  (void)nmax;
+  if (!g_default) return LES_E_STATE;
+  int r;
+  if (wind_profile && (r = les_upload(g_default, LES_WIND_PROFILE, wind_profile))) return r;
+  return les_step(g_default, n);
+}
+
+extern "C" les_handle* les_default_handle(void) { return g_default; }
+
+extern "C" int les_finalise_LES_kernel(void) {
+  if (g_default) les_destroy(g_default);
+  g_default = nullptr;
+  return LES_OK;
+}

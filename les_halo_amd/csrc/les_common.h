@@ -1,0 +1,246 @@
+// Device-side layout helpers and the exact reduction used by every LES_Halo kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace les {
+
+// Interior sizes; every array keeps the reference's Fortran bounds (main_ocl.f95:40-167),
+// column-major with i fastest.  Index helpers take Fortran indices.
+struct Dims {
+  int ip, jp, kp;
+  // p(0:ip+2,0:jp+2,0:kp+1)
+  __host__ __device__ long P(int i, int j, int k) const {
+    return (long)i + (long)(ip + 3) * ((long)j + (long)(jp + 3) * (long)k);
+  }
+  // u,v(0:ip+1,-1:jp+1,0:kp+1)
+  __host__ __device__ long U(int i, int j, int k) const {
+    return (long)i + (long)(ip + 2) * ((long)(j + 1) + (long)(jp + 3) * (long)k);
+  }
+  // w(0:ip+1,-1:jp+1,-1:kp+1)
+  __host__ __device__ long W(int i, int j, int k) const {
+    return (long)i + (long)(ip + 2) * ((long)(j + 1) + (long)(jp + 3) * (long)(k + 1));
+  }
+  // f,g,h,usum,vsum,wsum(0:ip,0:jp,0:kp)
+  __host__ __device__ long F(int i, int j, int k) const {
+    return (long)i + (long)(ip + 1) * ((long)j + (long)(jp + 1) * (long)k);
+  }
+  // fold,gold,hold,cn1(1:ip,1:jp,1:kp)
+  __host__ __device__ long O(int i, int j, int k) const {
+    return (long)(i - 1) + (long)ip * ((long)(j - 1) + (long)jp * (long)(k - 1));
+  }
+  // rhs(0:ip+1,0:jp+1,0:kp+1)
+  __host__ __device__ long R(int i, int j, int k) const {
+    return (long)i + (long)(ip + 2) * ((long)j + (long)(jp + 2) * (long)k);
+  }
+  // sm(-1:ip+1,-1:jp+1,0:kp+1)
+  __host__ __device__ long S(int i, int j, int k) const {
+    return (long)(i + 1) + (long)(ip + 3) * ((long)(j + 1) + (long)(jp + 3) * (long)k);
+  }
+  // diu1,diu5(-1:ip+2,0:jp+2,0:kp+2)
+  __host__ __device__ long D15(int i, int j, int k) const {
+    return (long)(i + 1) + (long)(ip + 4) * ((long)j + (long)(jp + 3) * (long)k);
+  }
+  // diu2..4,6..9(0:ip+2,0:jp+2,0:kp+2)
+  __host__ __device__ long D(int i, int j, int k) const {
+    return (long)i + (long)(ip + 3) * ((long)j + (long)(jp + 3) * (long)k);
+  }
+  // amask1,dmask1 / interior mask byte array (1:ip,1:jp,1:kp)
+  __host__ __device__ long M(int i, int j, int k) const { return O(i, j, k); }
+};
+
+// 1-D grid arrays, offsets of their lower bounds: dx1(-1:), dxs(0:), dy1(0:), dys(0:),
+// dzn(-1:), dzs(-1:), z2(1:), cn2*(1:), cn3*(1:), cn4*(1:), delx1(1:)
+struct Grid1D {
+  const float *dx1, *dxs, *dy1, *dys, *dzn, *dzs;
+  const float *cn2l, *cn2s, *cn3l, *cn3s, *cn4l, *cn4s, *delx1;
+  __device__ float DX1(int i) const { return dx1[i + 1]; }
+  __device__ float DXS(int i) const { return dxs[i]; }
+  __device__ float DY1(int j) const { return dy1[j]; }
+  __device__ float DYS(int j) const { return dys[j]; }
+  __device__ float DZN(int k) const { return dzn[k + 1]; }
+  __device__ float DZS(int k) const { return dzs[k + 1]; }
+};
+
+// ------------------------------------------------------------------------------------------
+// Exact order-independent summation of FP32 terms.
+//
+// A finite float is m * 2^(s-149) with a 24-bit integer m and s = max(E,1)-1 in [0,253].  The
+// sum is kept as a fixed-point integer with LSB 2^-149 in NLIMB signed 64-bit limbs holding
+// 32-bit digits in carry-save form (value = sum_q limb[q] * 2^(32q)).  Each term adds its low
+// and high 32-bit parts to two adjacent limbs, so no carry ever has to be propagated until
+// the final normalisation; with |digit| < 2^32 a limb can absorb 2^31 terms.  Any summation
+// order gives the same integer, hence the same correctly rounded result.
+// ------------------------------------------------------------------------------------------
+constexpr int NLIMB = 10;
+
+struct TermParts {
+  int limb;      // target limb of the low part (high part goes to limb + 1)
+  int64_t lo, hi;
+};
+
+__device__ __forceinline__ bool split_term(float x, TermParts& t) {
+  uint32_t b = __float_as_uint(x);
+  uint32_t e = (b >> 23) & 0xffu;
+  uint32_t m = b & 0x7fffffu;
+  if (e == 0xffu) return false;  // inf/nan: not representable, dropped
+  if (e) m |= 0x800000u;
+  if (!m) return false;
+  int s = (e ? (int)e : 1) - 1;
+  int off = s & 31;
+  uint64_t v = (uint64_t)m << off;  // < 2^56
+  int64_t lo = (int64_t)(v & 0xffffffffull), hi = (int64_t)(v >> 32);
+  if (b >> 31) {
+    lo = -lo;
+    hi = -hi;
+  }
+  t.limb = s >> 5;
+  t.lo = lo;
+  t.hi = hi;
+  return true;
+}
+
+// Per-thread accumulator: a 4-limb register window [base, base+4) plus a fall-back into the
+// block's LDS limbs for terms outside the window.
+struct ThreadAcc {
+  int64_t r0, r1, r2, r3;
+  int base;  // -1 until the first term
+  __device__ void init() {
+    r0 = r1 = r2 = r3 = 0;
+    base = -1;
+  }
+  __device__ __forceinline__ void add(float x, unsigned long long* lds) {
+    TermParts t;
+    if (!split_term(x, t)) return;
+    if (base < 0) base = min(max(t.limb - 1, 0), NLIMB - 4);
+    int d = t.limb - base;
+    if (d >= 0 && d <= 2) {
+      r0 += (d == 0) ? t.lo : 0;
+      r1 += (d == 0) ? t.hi : ((d == 1) ? t.lo : 0);
+      r2 += (d == 1) ? t.hi : ((d == 2) ? t.lo : 0);
+      r3 += (d == 2) ? t.hi : 0;
+    } else {
+      atomicAdd(&lds[t.limb], (unsigned long long)t.lo);
+      atomicAdd(&lds[t.limb + 1], (unsigned long long)t.hi);
+    }
+  }
+  __device__ void flush(unsigned long long* lds) {
+    if (base < 0) return;
+    if (r0) atomicAdd(&lds[base + 0], (unsigned long long)r0);
+    if (r1) atomicAdd(&lds[base + 1], (unsigned long long)r1);
+    if (r2) atomicAdd(&lds[base + 2], (unsigned long long)r2);
+    if (r3) atomicAdd(&lds[base + 3], (unsigned long long)r3);
+  }
+};
+
+// Normalises NLIMB carry-save limbs and rounds the exact value (times 2^-149) to the
+// nearest float, ties to even.  An exactly zero sum gives +0.
+__device__ inline float limbs_to_float(const int64_t* in) {
+  // carry-propagate into 32-bit digits; the final carry is the sign extension
+  uint32_t dig[NLIMB + 2];
+  int64_t carry = 0;
+  for (int q = 0; q < NLIMB; ++q) {
+    int64_t v = in[q] + carry;
+    dig[q] = (uint32_t)(v & 0xffffffffll);
+    carry = v >> 32;  // arithmetic shift
+  }
+  dig[NLIMB] = (uint32_t)(carry & 0xffffffffll);
+  dig[NLIMB + 1] = (uint32_t)((carry >> 32) & 0xffffffffll);
+  const int ND = NLIMB + 2;
+  bool neg = (dig[ND - 1] >> 31) != 0;
+  if (neg) {  // magnitude = two's complement negation
+    uint64_t c = 1;
+    for (int q = 0; q < ND; ++q) {
+      uint64_t s = (uint64_t)(uint32_t)(~dig[q]) + c;
+      dig[q] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  int top = -1;
+  for (int q = ND - 1; q >= 0; --q) {
+    if (dig[q]) {
+      top = q * 32 + 31 - __clz(dig[q]);
+      break;
+    }
+  }
+  if (top < 0) return 0.0f;
+  uint32_t out;
+  if (top < 24) {  // exact: integer below 2^24 times 2^-149 (subnormal, or exponent field 1)
+    out = dig[0];
+  } else {
+    int drop = top - 23;
+    // 24 kept bits starting at bit 'drop'
+    uint64_t window = 0;
+    int q0 = drop >> 5, o0 = drop & 31;
+    uint64_t lo = dig[q0], mid = (q0 + 1 < ND) ? dig[q0 + 1] : 0u;
+    window = ((mid << 32) | lo) >> o0;
+    uint32_t keep = (uint32_t)(window & 0xffffffu);
+    // round bit and sticky bits below it
+    int rpos = drop - 1;
+    uint32_t rnd = (dig[rpos >> 5] >> (rpos & 31)) & 1u;
+    bool sticky = false;
+    for (int q = 0; q < (rpos >> 5); ++q) sticky |= dig[q] != 0;
+    sticky |= (dig[rpos >> 5] & ((1u << (rpos & 31)) - 1u)) != 0;
+    if (rnd && (sticky || (keep & 1u))) keep += 1u;
+    uint32_t ef = (uint32_t)drop + 1u;
+    if (keep == (1u << 24)) {
+      keep >>= 1;
+      ef += 1u;
+    }
+    out = (ef >= 255u) ? 0x7f800000u : ((ef << 23) | (keep & 0x7fffffu));
+  }
+  if (neg) out |= 0x80000000u;
+  return __uint_as_float(out);
+}
+
+// Same exact value rounded to double (used by the mixed-precision mode).
+__device__ inline double limbs_to_double(const int64_t* in) {
+  uint32_t dig[NLIMB + 2];
+  int64_t carry = 0;
+  for (int q = 0; q < NLIMB; ++q) {
+    int64_t v = in[q] + carry;
+    dig[q] = (uint32_t)(v & 0xffffffffll);
+    carry = v >> 32;
+  }
+  dig[NLIMB] = (uint32_t)(carry & 0xffffffffll);
+  dig[NLIMB + 1] = (uint32_t)((carry >> 32) & 0xffffffffll);
+  const int ND = NLIMB + 2;
+  bool neg = (dig[ND - 1] >> 31) != 0;
+  if (neg) {
+    uint64_t c = 1;
+    for (int q = 0; q < ND; ++q) {
+      uint64_t s = (uint64_t)(uint32_t)(~dig[q]) + c;
+      dig[q] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  int top = -1;
+  for (int q = ND - 1; q >= 0; --q) {
+    if (dig[q]) {
+      top = q * 32 + 31 - __clz(dig[q]);
+      break;
+    }
+  }
+  if (top < 0) return 0.0;
+  // hi63 = bits [scale, top] aligned so that its leading bit is bit 62; sticky = bits below
+  int scale = top - 62;
+  uint64_t hi63 = 0;
+  bool sticky = false;
+  int lo = scale < 0 ? 0 : scale;
+  for (int bpos = top; bpos >= lo; --bpos)
+    hi63 = (hi63 << 1) | ((dig[bpos >> 5] >> (bpos & 31)) & 1u);
+  if (scale < 0) {
+    hi63 <<= -scale;
+  } else {
+    for (int q = 0; q < (scale >> 5); ++q) sticky |= dig[q] != 0;
+    sticky |= (dig[scale >> 5] & ((1u << (scale & 31)) - 1u)) != 0;
+  }
+  uint64_t mant = hi63 >> 10;  // 53 significant bits
+  uint64_t rem = hi63 & 0x3ffull;
+  bool up = (rem > 0x200ull) || (rem == 0x200ull && (sticky || (mant & 1ull)));
+  if (up) mant += 1;
+  double r = ldexp((double)mant, scale + 10 - 149);
+  return neg ? -r : r;
+}
+
+}  // namespace les

@@ -1,0 +1,1083 @@
+// HIP kernels (gfx950) for the LES_Halo time step.  Each kernel restates one reference stage
+// (file:line cited per kernel) in the reference's Fortran array layouts, with the reference's
+// FP32 operation order, compiled without FP contraction, so that every element is bit-identical
+// to the sequential reference.  Threads run along i (the contiguous axis) so every wavefront
+// reads and writes 64 consecutive floats.  Boundary ("halo") values are written by the thread
+// that owns their source, or by a dedicated boundary kernel whose every read is of a cell that
+// the kernel never writes; no kernel relies on inter-workgroup ordering.
+#include "les_kernels.h"
+
+namespace les {
+
+namespace {
+
+constexpr int BX = 64;  // one wavefront along i
+constexpr int BY = 4;   // rows of j per workgroup
+
+inline dim3 interior_grid(Dims d, int kcount) {
+  return dim3((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, kcount);
+}
+
+__device__ __forceinline__ float mbit(uint8_t m, int b) { return (m >> b) & 1u ? 1.0f : 0.0f; }
+
+}  // namespace
+
+// ==========================================================================================
+// velnw.f95:27-59 fused with the inflow of bondv1_ext_wind_profile.f95:27-50
+// (state 1 of run_LES_kernel).  Threads cover i = 0..imax, j = 1..jm, k = 1..km.
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F, Physics ph,
+                                                      int do_velnw, int inflow_n, int imax) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > imax || j > d.jp) return;
+  const long iu = d.U(i, j, k);
+  const long iw = d.W(i, j, k);
+  if (inflow_n > 0 && (i <= 1 || inflow_n == 1)) {
+    // inflow planes i = 0,1 (and, at n == 1, the fill u(i,:,:) = u(1,:,:) for i = 2..im)
+    F.u[iu] = F.wind[(j - 1) + d.jp * (k - 1)];
+    F.v[iu] = 0.0f;
+    F.w[iw] = 0.0f;
+    return;
+  }
+  if (!do_velnw || i < 1) return;
+  const float ro = ph.ro, dt = ph.dt;
+  const float p0 = F.p[d.P(i, j, k)];
+  const long ifg = d.F(i, j, k);
+  float pz = (-p0 + F.p[d.P(i + 1, j, k)]) / ro / g.DXS(i);
+  F.u[iu] = F.u[iu] + dt * (F.f[ifg] - pz);
+  pz = (-p0 + F.p[d.P(i, j + 1, k)]) / ro / g.DYS(j);
+  F.v[iu] = F.v[iu] + dt * (F.g[ifg] - pz);
+  if (k <= d.kp - 1) {
+    pz = (-p0 + F.p[d.P(i, j, k + 1)]) / ro / g.DZS(k);
+    F.w[iw] = F.w[iw] + dt * (F.h[ifg] - pz);
+  }
+}
+
+void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
+                         int inflow_n) {
+  int imax = (do_velnw || inflow_n == 1) ? d.ip : 1;
+  dim3 grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+  hipLaunchKernelGGL(k_velnw_inflow, grid, dim3(BX, BY), 0, st, d, g, F, ph, do_velnw, inflow_n,
+                     imax);
+}
+
+// ==========================================================================================
+// bondv1.f95:67-75: uout = (max(0, max u(im,:,:)) + min(0, min u(im,:,:))) / 2.
+// max/min are order-independent, so one workgroup of 1024 threads suffices.
+// ==========================================================================================
+__global__ __launch_bounds__(1024) void k_uout(Dims d, Fields F, DevScalars* sc) {
+  __shared__ float smax[16], smin[16];
+  float mx = 0.0f, mn = 0.0f;
+  const int n = d.jp * d.kp;
+  for (int t = threadIdx.x; t < n; t += 1024) {
+    int j = 1 + t % d.jp, k = 1 + t / d.jp;
+    float x = F.u[d.U(d.ip, j, k)];
+    mx = fmaxf(mx, x);
+    mn = fminf(mn, x);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, o));
+    mn = fminf(mn, __shfl_xor(mn, o));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    smax[wv] = mx;
+    smin[wv] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a = smax[0], b = smin[0];
+    for (int q = 1; q < 16; ++q) {
+      a = fmaxf(a, smax[q]);
+      b = fminf(b, smin[q]);
+    }
+    sc->uout = (a + b) / 2.0f;
+  }
+}
+
+void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc) {
+  hipLaunchKernelGGL(k_uout, dim3(1), dim3(1024), 0, st, d, F, sc);
+}
+
+// ==========================================================================================
+// bondv1.f95:81-97: advective outflow at i = im (u) and i = im+1 (v, w)
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_outflow(Dims d, Grid1D g, Fields F, Physics ph,
+                                                 const DevScalars* sc) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= d.jp * d.kp) return;
+  const int j = 1 + t % d.jp, k = 1 + t / d.jp;
+  const int im = d.ip;
+  const float dt = ph.dt, uout = sc->uout, dxs = g.DXS(im);
+  long a = d.U(im, j, k), b = d.U(im - 1, j, k);
+  F.u[a] = F.u[a] - dt * uout * (F.u[a] - F.u[b]) / dxs;
+  a = d.U(im + 1, j, k);
+  b = d.U(im, j, k);
+  F.v[a] = F.v[a] - dt * uout * (F.v[a] - F.v[b]) / dxs;
+  a = d.W(im + 1, j, k);
+  b = d.W(im, j, k);
+  F.w[a] = F.w[a] - dt * uout * (F.w[a] - F.w[b]) / dxs;
+}
+
+// bondv1.f95:99-137: periodic j, then top/bottom; each halo cell gets its final value from
+// interior rows (which this kernel never writes).  The reference's w loop runs i = -1..im+1
+// although w starts at i = 0; in the column-major layout w(-1,j,k) is w(ip+1,j-1,k), so the
+// only cells it reaches beyond the i = im+1 column are w(ip+1,jp+1,-1) and w(ip+1,jp+1,km-1),
+// both zeroed here as well.
+__global__ __launch_bounds__(256) void k_bondv1_halo(Dims d, Fields F) {
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const int ni = im + 2;
+  const long n1 = (long)ni * km * 2;               // j faces, k = 1..km, two faces
+  const long n2 = (long)ni * (jm + 3) * 2;         // k faces, j = -1..jm+1, two planes
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n1) {
+    const int i = (int)(t % ni);
+    const int k = 1 + (int)((t / ni) % km);
+    const int face = (int)(t / ((long)ni * km));
+    const int jdst = face ? jm + 1 : 0, jsrc = face ? 1 : jm;
+    F.u[d.U(i, jdst, k)] = F.u[d.U(i, jsrc, k)];
+    F.v[d.U(i, jdst, k)] = F.v[d.U(i, jsrc, k)];
+    if (k <= km - 1) {
+      float wv = F.w[d.W(i, jsrc, k)];
+      if (i == im + 1 && face == 1 && k == km - 1) wv = 0.0f;  // the w(-1,-1,km) write
+      F.w[d.W(i, jdst, k)] = wv;
+    }
+    return;
+  }
+  t -= n1;
+  if (t >= n2) return;
+  const int i = (int)(t % ni);
+  const int j = -1 + (int)((t / ni) % (jm + 3));
+  const int top = (int)(t / ((long)ni * (jm + 3)));
+  // w: zero planes k = 0 and k = km (j = -1..jm+1)
+  F.w[d.W(i, j, top ? km : 0)] = 0.0f;
+  if (!top && i == im + 1 && j == jm + 1) F.w[d.W(im + 1, jm + 1, -1)] = 0.0f;  // w(-1,-1,0)
+  if (j < 0) return;
+  // u, v: k = 0 antisymmetric, k = km+1 copy, from the periodic-completed rows
+  const int js = (j == 0) ? jm : ((j == jm + 1) ? 1 : j);
+  if (top) {
+    F.u[d.U(i, j, km + 1)] = F.u[d.U(i, js, km)];
+    F.v[d.U(i, j, km + 1)] = F.v[d.U(i, js, km)];
+  } else {
+    F.u[d.U(i, j, 0)] = -F.u[d.U(i, js, 1)];
+    F.v[d.U(i, j, 0)] = -F.v[d.U(i, js, 1)];
+  }
+}
+
+void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc) {
+  int n = d.jp * d.kp;
+  hipLaunchKernelGGL(k_outflow, dim3((n + 255) / 256), dim3(256), 0, st, d, g, F, ph, sc);
+  long nh = (long)(d.ip + 2) * d.kp * 2 + (long)(d.ip + 2) * (d.jp + 3) * 2;
+  hipLaunchKernelGGL(k_bondv1_halo, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, d, F);
+}
+
+// ==========================================================================================
+// vel2.f95:48-123 formulas at one point; velFG.f95:59-105; feedbf.f95:38-67.
+// ==========================================================================================
+namespace {
+
+struct VelCtx {
+  Dims d;
+  Grid1D g;
+  const float *u, *v, *w;
+  __device__ float U(int i, int j, int k) const { return u[d.U(i, j, k)]; }
+  __device__ float V(int i, int j, int k) const { return v[d.U(i, j, k)]; }
+  __device__ float W(int i, int j, int k) const { return w[d.W(i, j, k)]; }
+  __device__ float dx1(int i) const { return g.DX1(i); }
+  __device__ float dy1(int j) const { return g.DY1(j); }
+  __device__ float dzn(int k) const { return g.DZN(k); }
+  __device__ float dzs(int k) const { return g.DZS(k); }
+
+  // vel2.f95:51-60
+  __device__ float diu1(int i, int j, int k) const { return (-U(i - 1, j, k) + U(i, j, k)) / dx1(i); }
+  __device__ float cov1(int i, int j, int k) const {
+    return ((U(i - 1, j, k) + U(i, j, k)) / 2.0f) * diu1(i, j, k);
+  }
+  __device__ float diu5(int i, int j, int k) const { return (-V(i, j - 1, k) + V(i, j, k)) / dy1(j); }
+  __device__ float cov5(int i, int j, int k) const {
+    return ((V(i, j - 1, k) + V(i, j, k)) / 2.0f) * diu5(i, j, k);
+  }
+  __device__ float diu9(int i, int j, int k) const { return (-W(i, j, k - 1) + W(i, j, k)) / dzn(k); }
+  __device__ float cov9(int i, int j, int k) const {
+    return ((W(i, j, k - 1) + W(i, j, k)) / 2.0f) * diu9(i, j, k);
+  }
+  // vel2.f95:68-70
+  __device__ float diu2(int i, int j, int k) const {
+    return 2.0f * (-U(i, j - 1, k) + U(i, j, k)) / (dy1(j - 1) + dy1(j));
+  }
+  __device__ float cov2(int i, int j, int k) const {
+    float nou = (dx1(i + 1) * V(i, j - 1, k) + dx1(i) * V(i + 1, j - 1, k)) / (dx1(i) + dx1(i + 1));
+    return nou * diu2(i, j, k);
+  }
+  // vel2.f95:78-80
+  __device__ float diu3(int i, int j, int k) const {
+    return (-U(i, j, k - 1) + U(i, j, k)) / dzs(k - 1);
+  }
+  __device__ float cov3(int i, int j, int k) const {
+    float nou = (dx1(i + 1) * W(i, j, k - 1) + dx1(i) * W(i + 1, j, k - 1)) / (dx1(i) + dx1(i + 1));
+    return nou * diu3(i, j, k);
+  }
+  // vel2.f95:88-90 (the "- u0" with u0 = 0 is exact)
+  __device__ float diu4(int i, int j, int k) const {
+    return 2.0f * (-V(i - 1, j, k) + V(i, j, k)) / (dx1(i - 1) + dx1(i));
+  }
+  __device__ float cov4(int i, int j, int k) const {
+    float nou = (dy1(j + 1) * U(i - 1, j, k) + dy1(j) * U(i - 1, j + 1, k)) / (dy1(j) + dy1(j + 1));
+    return nou * diu4(i, j, k);
+  }
+  // vel2.f95:98-100
+  __device__ float diu6(int i, int j, int k) const {
+    return (-V(i, j, k - 1) + V(i, j, k)) / dzs(k - 1);
+  }
+  __device__ float cov6(int i, int j, int k) const {
+    float nou = (dy1(j + 1) * W(i, j, k - 1) + dy1(j) * W(i, j + 1, k - 1)) / (dy1(j) + dy1(j + 1));
+    return nou * diu6(i, j, k);
+  }
+  // vel2.f95:108-110
+  __device__ float diu7(int i, int j, int k) const {
+    return 2.0f * (-W(i - 1, j, k) + W(i, j, k)) / (dx1(i - 1) + dx1(i));
+  }
+  __device__ float cov7(int i, int j, int k) const {
+    float nou = (dzn(k + 1) * U(i - 1, j, k) + dzn(k) * U(i - 1, j, k + 1)) / (dzn(k) + dzn(k + 1));
+    return nou * diu7(i, j, k);
+  }
+  // vel2.f95:118-120
+  __device__ float diu8(int i, int j, int k) const {
+    return 2.0f * (-W(i, j - 1, k) + W(i, j, k)) / (dy1(j - 1) + dy1(j));
+  }
+  __device__ float cov8(int i, int j, int k) const {
+    float nou = (dzn(k + 1) * V(i, j - 1, k) + dzn(k) * V(i, j - 1, k + 1)) / (dzn(k) + dzn(k + 1));
+    return nou * diu8(i, j, k);
+  }
+};
+
+}  // namespace
+
+// One thread per interior cell.  Neighbour values of cov/diu at i+1 = im+1 use the vel2 halo
+// copy (value at im), at j+1 = jm+1 the periodic copy (value at j = 1); k+1 values are
+// computed (vel2 computes them up to km+1).  diu1..9 are stored with every halo copy of
+// vel2.f95:125-188, each written by the thread owning its source.
+__global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F, Physics ph,
+                                                      int do_velfg, int do_feedbf) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  if (i > im || j > jm) return;
+  VelCtx c{d, g, F.u, F.v, F.w};
+  const int ip1 = (i == im) ? im : i + 1;  // i+1 with the im+1 -> im copy
+  const int jp1 = (j == jm) ? 1 : j + 1;   // j+1 with the jm+1 -> 1 copy
+  const long ifg = d.F(i, j, k);
+  float fv = 0.0f, gv = 0.0f, hv = 0.0f;
+  const float two = 2.0f;
+
+  // ---- diu values at this cell (also stored) ----
+  const float d1 = c.diu1(i, j, k), d2 = c.diu2(i, j, k), d3 = c.diu3(i, j, k);
+  const float d4 = c.diu4(i, j, k), d5 = c.diu5(i, j, k), d6 = c.diu6(i, j, k);
+  const float d9 = c.diu9(i, j, k);
+  const bool hk = k <= km - 1;
+  const float d7 = hk ? c.diu7(i, j, k) : 0.0f, d8 = hk ? c.diu8(i, j, k) : 0.0f;
+
+  if (do_velfg) {
+    const float dx1i = c.dx1(i), dx1ip = c.dx1(i + 1), dy1j = c.dy1(j), dy1jp = c.dy1(j + 1);
+    const float dznk = c.dzn(k), dznkp = c.dzn(k + 1);
+    // u momentum (velFG.f95:59-72)
+    {
+      float covx1 = (dx1ip * c.cov1(i, j, k) + dx1i * c.cov1(ip1, j, k)) / (dx1i + dx1ip);
+      float covy1 = (c.cov2(i, j, k) + c.cov2(i, jp1, k)) / two;
+      float covz1 = (c.cov3(i, j, k) + c.cov3(i, j, k + 1)) / two;
+      float covc = covx1 + covy1 + covz1;
+      float dfu1 = two * (-d1 + c.diu1(ip1, j, k)) / (dx1i + dx1ip) + (-d2 + c.diu2(i, jp1, k)) / dy1j +
+                   (-d3 + c.diu3(i, j, k + 1)) / dznk;
+      float df = ph.vn * dfu1;
+      fv = (-covc + df);
+    }
+    // v momentum (velFG.f95:75-88)
+    {
+      float covx1 = (c.cov4(i, j, k) + c.cov4(ip1, j, k)) / two;
+      float covy1 = (dy1jp * c.cov5(i, j, k) + dy1j * c.cov5(i, jp1, k)) / (dy1j + dy1jp);
+      float covz1 = (c.cov6(i, j, k) + c.cov6(i, j, k + 1)) / two;
+      float covc = covx1 + covy1 + covz1;
+      float dfv1 = (-d4 + c.diu4(ip1, j, k)) / dx1i + two * (-d5 + c.diu5(i, jp1, k)) / (dy1j + dy1jp) +
+                   (-d6 + c.diu6(i, j, k + 1)) / dznk;
+      float df = ph.vn * dfv1;
+      gv = (-covc + df);
+    }
+    // w momentum (velFG.f95:92-105), k <= km-1
+    if (hk) {
+      float covx1 = (c.cov7(i, j, k) + c.cov7(ip1, j, k)) / two;
+      float covy1 = (c.cov8(i, j, k) + c.cov8(i, jp1, k)) / two;
+      float covz1 = (dznkp * c.cov9(i, j, k) + dznk * c.cov9(i, j, k + 1)) / (dznk + dznkp);
+      float covc = covx1 + covy1 + covz1;
+      float dfw1 = (-d7 + c.diu7(ip1, j, k)) / dx1i + (-d8 + c.diu8(i, jp1, k)) / dy1j +
+                   (-d9 + c.diu9(i, j, k + 1)) / c.dzs(k);
+      float df = ph.vn * dfw1;
+      hv = (-covc + df);
+    } else {
+      hv = F.h[ifg];  // velfg leaves h(:,:,km) untouched
+    }
+  } else {
+    fv = F.f[ifg];
+    gv = F.g[ifg];
+    hv = F.h[ifg];
+  }
+
+  if (do_feedbf) {  // feedbf.f95:41-65
+    const uint8_t m = F.mask[d.M(i, j, k)];
+    const float bm = mbit(m, 0), cm = mbit(m, 1), dm = mbit(m, 2);
+    const float uu = c.U(i, j, k), vv = c.V(i, j, k), ww = c.W(i, j, k);
+    const float us = (F.usum[ifg] + uu) * bm;
+    const float vs = (F.vsum[ifg] + vv) * cm;
+    const float ws = (F.wsum[ifg] + ww) * dm;
+    F.usum[ifg] = us;
+    F.vsum[ifg] = vs;
+    F.wsum[ifg] = ws;
+    const float f1x = ph.alpha * us * ph.dt, f1y = ph.alpha * vs * ph.dt, f1z = ph.alpha * ws * ph.dt;
+    const float f2x = ph.beta * uu * bm, f2y = ph.beta * vv * cm, f2z = ph.beta * ww * dm;
+    fv = fv + (f1x + f2x);
+    gv = gv + (f1y + f2y);
+    hv = hv + (f1z + f2z);
+  }
+  F.f[ifg] = fv;
+  F.g[ifg] = gv;
+  F.h[ifg] = hv;
+
+  if (!do_velfg) return;
+  // ---- diu stores with the halo copies of vel2.f95:125-188 ----
+  F.diu1[d.D15(i, j, k)] = d1;
+  if (i == im) F.diu1[d.D15(im + 1, j, k)] = d1;
+  F.diu5[d.D15(i, j, k)] = d5;
+  if (j == jm) F.diu5[d.D15(i, 0, k)] = d5;
+  if (j == 1) F.diu5[d.D15(i, jm + 1, k)] = d5;
+  F.diu9[d.D(i, j, k)] = d9;
+  // diu2: periodic j rows (k <= km) and the i = im+1 column for j = 1..jm+1, k = 1..km+1
+  F.diu2[d.D(i, j, k)] = d2;
+  if (j == jm) F.diu2[d.D(i, 0, k)] = d2;
+  if (j == 1) F.diu2[d.D(i, jm + 1, k)] = d2;
+  if (i == im) {
+    F.diu2[d.D(im + 1, j, k)] = d2;
+    if (j == 1) F.diu2[d.D(im + 1, jm + 1, k)] = d2;
+    if (k == km) {  // diu2(im, :, km+1) is never computed: plain copy of memory
+      F.diu2[d.D(im + 1, j, km + 1)] = F.diu2[d.D(im, j, km + 1)];
+      if (j == jm) F.diu2[d.D(im + 1, jm + 1, km + 1)] = F.diu2[d.D(im, jm + 1, km + 1)];
+    }
+  }
+  // diu3: k = 1..km+1; i = im+1 column for j = 1..jm+1
+  F.diu3[d.D(i, j, k)] = d3;
+  float d3top = 0.0f;
+  if (k == km) {
+    d3top = c.diu3(i, j, km + 1);
+    F.diu3[d.D(i, j, km + 1)] = d3top;
+  }
+  if (i == im) {
+    F.diu3[d.D(im + 1, j, k)] = d3;
+    if (k == km) F.diu3[d.D(im + 1, j, km + 1)] = d3top;
+    if (j == jm) {  // diu3(im, jm+1, :) is never computed
+      F.diu3[d.D(im + 1, jm + 1, k)] = F.diu3[d.D(im, jm + 1, k)];
+      if (k == km) F.diu3[d.D(im + 1, jm + 1, km + 1)] = F.diu3[d.D(im, jm + 1, km + 1)];
+    }
+  }
+  // diu4: i = im+1 column (k <= km), then row j = 0 for i = 1..im+1, k = 1..km+1
+  F.diu4[d.D(i, j, k)] = d4;
+  if (i == im) F.diu4[d.D(im + 1, j, k)] = d4;
+  if (j == jm) {
+    F.diu4[d.D(i, 0, k)] = d4;
+    if (i == im) F.diu4[d.D(im + 1, 0, k)] = d4;
+    if (k == km) {  // diu4(:, jm, km+1) is never computed
+      F.diu4[d.D(i, 0, km + 1)] = F.diu4[d.D(i, jm, km + 1)];
+      if (i == im) F.diu4[d.D(im + 1, 0, km + 1)] = F.diu4[d.D(im + 1, jm, km + 1)];
+    }
+  }
+  // diu6: k = 1..km+1; row j = 0 for i = 1..im+1
+  F.diu6[d.D(i, j, k)] = d6;
+  float d6top = 0.0f;
+  if (k == km) {
+    d6top = c.diu6(i, j, km + 1);
+    F.diu6[d.D(i, j, km + 1)] = d6top;
+  }
+  if (j == jm) {
+    F.diu6[d.D(i, 0, k)] = d6;
+    if (k == km) F.diu6[d.D(i, 0, km + 1)] = d6top;
+    if (i == im) {  // diu6(im+1, jm, :) is never computed
+      F.diu6[d.D(im + 1, 0, k)] = F.diu6[d.D(im + 1, jm, k)];
+      if (k == km) F.diu6[d.D(im + 1, 0, km + 1)] = F.diu6[d.D(im + 1, jm, km + 1)];
+    }
+  }
+  if (hk) {
+    F.diu7[d.D(i, j, k)] = d7;
+    if (i == im) F.diu7[d.D(im + 1, j, k)] = d7;
+    F.diu8[d.D(i, j, k)] = d8;
+    if (j == jm) F.diu8[d.D(i, 0, k)] = d8;
+    if (j == 1) F.diu8[d.D(i, jm + 1, k)] = d8;
+  }
+}
+
+void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
+                  int do_feedbf) {
+  hipLaunchKernelGGL(k_velfg_feedbf, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F, ph,
+                     do_velfg, do_feedbf);
+}
+
+// ==========================================================================================
+// les.f95:40-60: Smagorinsky eddy viscosity sm from the stored diu fields
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const float q = 0.25f, two = 2.0f;
+  const float* diu2 = F.diu2;
+  const float* diu3 = F.diu3;
+  const float* diu4 = F.diu4;
+  const float* diu6 = F.diu6;
+  const float* diu7 = F.diu7;
+  const float* diu8 = F.diu8;
+  float dudxx1 = F.diu1[d.D15(i, j, k)];
+  float dudyx1 = (diu2[d.D(i - 1, j, k)] + diu2[d.D(i - 1, j + 1, k)] + diu2[d.D(i, j, k)] + diu2[d.D(i, j + 1, k)]) * q;
+  float dudzx1 = (diu3[d.D(i - 1, j, k)] + diu3[d.D(i - 1, j, k + 1)] + diu3[d.D(i, j, k)] + diu3[d.D(i, j, k + 1)]) * q;
+  float dvdxx1 = (diu4[d.D(i, j, k)] + diu4[d.D(i, j - 1, k)] + diu4[d.D(i + 1, j, k)] + diu4[d.D(i + 1, j - 1, k)]) * q;
+  float dvdyx1 = F.diu5[d.D15(i, j, k)];
+  float dvdzx1 = (diu6[d.D(i, j - 1, k)] + diu6[d.D(i, j - 1, k + 1)] + diu6[d.D(i, j, k)] + diu6[d.D(i, j, k + 1)]) * q;
+  float dwdxx1 = (diu7[d.D(i, j, k)] + diu7[d.D(i, j, k - 1)] + diu7[d.D(i + 1, j, k)] + diu7[d.D(i + 1, j, k - 1)]) * q;
+  float dwdyx1 = (diu8[d.D(i, j, k)] + diu8[d.D(i, j, k - 1)] + diu8[d.D(i, j + 1, k)] + diu8[d.D(i, j + 1, k - 1)]) * q;
+  float dwdzx1 = F.diu9[d.D(i, j, k)];
+  float cd = ph.cs0 * g.delx1[k - 1];
+  float s = two * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
+            (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
+            (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
+  F.sm[d.S(i, j, k)] = (cd * cd) * sqrtf(s);
+}
+
+void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
+  hipLaunchKernelGGL(k_calc_sm, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F, ph);
+}
+
+// ==========================================================================================
+// boundsm.f95:13-32.  The three loops (i faces, j faces, k planes) are composed: every halo
+// cell (a,b,c) takes its final value from cells the kernel never writes.
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_boundsm(Dims d, Fields F) {
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const int na = im + 2, nb = jm + 3, nc = km + 2;  // a = 0..im+1, b = -1..jm+1, c = 0..km+1
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)na * nb * nc) return;
+  const int a = (int)(t % na);
+  const int b = -1 + (int)((t / na) % nb);
+  const int c = (int)(t / ((long)na * nb));
+  const bool ah = (a == 0 || a == im + 1), bh = (b == 0 || b == jm + 1), ch = (c == 0 || c == km + 1);
+  if (!(ah || bh || ch)) return;
+  // V1: after the i-face loop; V2: after the j-face loop
+  auto V1 = [&](int x, int y, int z) -> float {
+    int xs = (x == 0) ? 1 : ((x == im + 1) ? im : x);
+    return F.sm[d.S(xs, y, z)];
+  };
+  auto V2 = [&](int x, int y, int z) -> float {
+    if (y == 0) return V1(x, 1, z);
+    if (y == jm + 1) return V1(x, jm, z);
+    return V1(x, y, z);
+  };
+  float val;
+  if (c == 0)
+    val = -V2(a, b, 1);
+  else if (c == km + 1)
+    val = V2(a, b, km);
+  else
+    val = V2(a, b, c);
+  F.sm[d.S(a, b, c)] = val;
+}
+
+void launch_boundsm(hipStream_t st, Dims d, Fields F) {
+  long n = (long)(d.ip + 2) * (d.jp + 3) * (d.kp + 2);
+  hipLaunchKernelGGL(k_boundsm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F);
+}
+
+// ==========================================================================================
+// les.f95:78-182 (viscous terms) fused with adam.f95:32-46 (Adams-Bashforth)
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
+                                                   int do_adam) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const long ifg = d.F(i, j, k);
+  float fv = F.f[ifg], gv = F.g[ifg], hv = F.h[ifg];
+  if (do_visc) {
+    const float two = 2.0f;
+    auto sm = [&](int x, int y, int z) -> float { return F.sm[d.S(x, y, z)]; };
+    auto D1 = [&](int x, int y, int z) -> float { return F.diu1[d.D15(x, y, z)]; };
+    auto D5 = [&](int x, int y, int z) -> float { return F.diu5[d.D15(x, y, z)]; };
+    auto D2 = [&](int x, int y, int z) -> float { return F.diu2[d.D(x, y, z)]; };
+    auto D3 = [&](int x, int y, int z) -> float { return F.diu3[d.D(x, y, z)]; };
+    auto D4 = [&](int x, int y, int z) -> float { return F.diu4[d.D(x, y, z)]; };
+    auto D6 = [&](int x, int y, int z) -> float { return F.diu6[d.D(x, y, z)]; };
+    auto D7 = [&](int x, int y, int z) -> float { return F.diu7[d.D(x, y, z)]; };
+    auto D8 = [&](int x, int y, int z) -> float { return F.diu8[d.D(x, y, z)]; };
+    auto D9 = [&](int x, int y, int z) -> float { return F.diu9[d.D(x, y, z)]; };
+    const float dx1i = g.DX1(i), dx1ip = g.DX1(i + 1), dx1im = g.DX1(i - 1);
+    const float dy1j = g.DY1(j), dy1jp = g.DY1(j + 1), dy1jm = g.DY1(j - 1);
+    const float dznk = g.DZN(k), dznkp = g.DZN(k + 1), dznkm = g.DZN(k - 1);
+    const float s000 = sm(i, j, k), s100 = sm(i + 1, j, k);
+    // x-momentum (les.f95:78-109)
+    {
+      float evsx2 = s100;
+      float evsx1 = s000;
+      float evsy2 = (dy1jp * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip)) +
+                     dy1j * ((dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k)) / (dx1i + dx1ip))) /
+                    (dy1j + dy1jp);
+      float evsy1 = (dy1jp * ((dx1ip * sm(i, j - 1, k) + dx1i * sm(i + 1, j - 1, k)) / (dx1i + dx1ip)) +
+                     dy1j * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip))) /
+                    (dy1j + dy1jp);
+      float evsz2 = (dznkp * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip)) +
+                     dznk * ((dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1)) / (dx1i + dx1ip))) /
+                    (dznk + dznkp);
+      float evsz1 = (dznk * ((dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1)) / (dx1i + dx1ip)) +
+                     dznkm * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip))) /
+                    (dznkm + dznk);
+      float visux2 = (evsx2) * two * D1(i + 1, j, k);
+      float visux1 = (evsx1) * two * D1(i, j, k);
+      float visuy2 = (evsy2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
+      float visuy1 = (evsy1) * (D2(i, j, k) + D4(i + 1, j - 1, k));
+      float visuz2 = (evsz2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
+      float visuz1 = (evsz1) * (D3(i, j, k) + D7(i + 1, j, k - 1));
+      float vfu = (visux2 - visux1) / dx1i + (visuy2 - visuy1) / dy1j + (visuz2 - visuz1) / dznk;
+      fv = (fv + vfu);
+    }
+    // y-momentum (les.f95:112-143)
+    {
+      float evsy2 = sm(i, j + 1, k);
+      float evsy1 = s000;
+      float evsx2 = (dy1jp * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip)) +
+                     dy1j * ((dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k)) / (dx1i + dx1ip))) /
+                    (dy1j + dy1jp);
+      float evsx1 = (dy1jp * ((dx1i * sm(i - 1, j, k) + dx1im * s000) / (dx1im + dx1i)) +
+                     dy1j * ((dx1i * sm(i - 1, j + 1, k) + dx1im * sm(i, j + 1, k)) / (dx1im + dx1i))) /
+                    (dy1j + dy1jp);
+      float evsz2 = (dznkp * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip)) +
+                     dznk * ((dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1)) / (dx1i + dx1ip))) /
+                    (dznk + dznkp);
+      float evsz1 = (dznk * ((dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1)) / (dx1i + dx1ip)) +
+                     dznkm * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip))) /
+                    (dznkm + dznk);
+      float visvx2 = (evsx2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
+      float visvx1 = (evsx1) * (D2(i - 1, j + 1, k) + D4(i, j, k));
+      float visvy2 = (evsy2) * two * D5(i, j + 1, k);
+      float visvy1 = (evsy1) * two * D5(i, j, k);
+      float visvz2 = (evsz2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
+      float visvz1 = (evsz1) * (D6(i, j, k) + D8(i, j + 1, k - 1));
+      float vfv = (visvx2 - visvx1) / dx1i + (visvy2 - visvy1) / dy1j + (visvz2 - visvz1) / dznk;
+      gv = (gv + vfv);
+    }
+    // z-momentum (les.f95:146-180)
+    {
+      float evsz2 = sm(i, j, k + 1);
+      float evsz1 = s000;
+      float evsx2 = (dznkp * ((dx1ip * s000 + dx1i * s100) / (dx1i + dx1ip)) +
+                     dznk * ((dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1)) / (dx1i + dx1ip))) /
+                    (dznk + dznkp);
+      float evsx1 = (dznkp * ((dx1i * sm(i - 1, j, k) + dx1im * s000) / (dx1im + dx1i)) +
+                     dznk * ((dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1)) / (dx1im + dx1i))) /
+                    (dznk + dznkp);
+      float evsy2 = (dznkp * ((dy1jp * s000 + dy1j * sm(i, j + 1, k)) / (dy1j + dy1jp)) +
+                     dznk * ((dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1)) / (dy1j + dy1jp))) /
+                    (dznk + dznkp);
+      float evsy1 = (dznkp * ((dy1j * sm(i, j - 1, k) + dy1jm * s000) / (dy1jm + dy1j)) +
+                     dznk * ((dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1)) / (dy1jm + dy1j))) /
+                    (dznk + dznkp);
+      float viswx2 = (evsx2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
+      float viswx1 = (evsx1) * (D3(i - 1, j, k + 1) + D7(i, j, k));
+      float viswy2 = (evsy2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
+      float viswy1 = (evsy1) * (D6(i, j - 1, k + 1) + D8(i, j, k));
+      float viswz2 = (evsz2) * two * D9(i, j, k + 1);
+      float viswz1 = (evsz1) * two * D9(i, j, k);
+      float vfw = (viswx2 - viswx1) / dx1i + (viswy2 - viswy1) / dy1j + (viswz2 - viswz1) / dznk;
+      hv = (hv + vfw);
+    }
+  }
+  if (do_adam) {  // adam.f95:35-43
+    const long io = d.O(i, j, k);
+    const float fo = F.fold[io], go = F.gold[io], ho = F.hold[io];
+    F.fold[io] = fv;
+    F.gold[io] = gv;
+    F.hold[io] = hv;
+    fv = 1.5f * fv - 0.5f * fo;
+    gv = 1.5f * gv - 0.5f * go;
+    hv = 1.5f * hv - 0.5f * ho;
+  }
+  F.f[ifg] = fv;
+  F.g[ifg] = gv;
+  F.h[ifg] = hv;
+}
+
+void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam) {
+  hipLaunchKernelGGL(k_visc_adam, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F, do_visc,
+                     do_adam);
+}
+
+// ==========================================================================================
+// Reductions: blocks accumulate exact limbs in LDS and write them to partials[block]; one
+// finalisation workgroup adds all partials (integer, exact) and rounds once.
+// ==========================================================================================
+namespace {
+
+constexpr int RED_KC = 6;  // k-chunks of the reduction grids
+
+__device__ void block_limbs_init(unsigned long long* lds) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid < NLIMB) lds[tid] = 0ull;
+  __syncthreads();
+}
+
+__device__ void block_limbs_store(unsigned long long* lds, int64_t* out) {
+  __syncthreads();
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid < NLIMB) out[tid] = (int64_t)lds[tid];
+}
+
+// Sum partials[0..nb) into limbs (one workgroup of 256 threads).
+__device__ void sum_partials(const int64_t* partials, int nb, int64_t* limbs_out) {
+  __shared__ unsigned long long acc[NLIMB];
+  if (threadIdx.x < NLIMB) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  int64_t r[NLIMB];
+  for (int q = 0; q < NLIMB; ++q) r[q] = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int q = 0; q < NLIMB; ++q) r[q] += partials[(long)b * NLIMB + q];
+  for (int q = 0; q < NLIMB; ++q)
+    if (r[q]) atomicAdd(&acc[q], (unsigned long long)r[q]);
+  __syncthreads();
+  if (threadIdx.x < NLIMB) limbs_out[threadIdx.x] = (int64_t)acc[threadIdx.x];
+  __syncthreads();
+}
+
+}  // namespace
+
+int reduction_blocks(Dims d) {
+  return ((d.ip + BX - 1) / BX) * ((d.jp + BY - 1) / BY) * RED_KC;
+}
+
+// ---- bondFG.f95:15-32 on its own ----
+__global__ __launch_bounds__(256) void k_bondfg(Dims d, Fields F) {
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const long n1 = (long)jm * km, n2 = (long)im * km, n3 = (long)im * jm;
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n1) {
+    int j = 1 + (int)(t % jm), k = 1 + (int)(t / jm);
+    F.f[d.F(0, j, k)] = F.f[d.F(1, j, k)];
+    return;
+  }
+  t -= n1;
+  if (t < n2) {
+    int i = 1 + (int)(t % im), k = 1 + (int)(t / im);
+    F.g[d.F(i, 0, k)] = F.g[d.F(i, jm, k)];
+    return;
+  }
+  t -= n2;
+  if (t < n3) {
+    int i = 1 + (int)(t % im), j = 1 + (int)(t / im);
+    F.h[d.F(i, j, 0)] = 0.0f;
+    F.h[d.F(i, j, km)] = 0.0f;
+  }
+}
+
+void launch_bondfg(hipStream_t st, Dims d, Fields F) {
+  long n = (long)d.jp * d.kp + (long)d.ip * d.kp + (long)d.ip * d.jp;
+  hipLaunchKernelGGL(k_bondfg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F);
+}
+
+// ---- press.f95:49-75: bondfg + rhs + exact partial sums of dx1*dy1*dzn*rhs ----
+__global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics ph,
+                                             int64_t* partials) {
+  __shared__ unsigned long long lds[NLIMB];
+  block_limbs_init(lds);
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const int kchunk = (km + RED_KC - 1) / RED_KC;
+  const int k0 = 1 + blockIdx.z * kchunk, k1 = min(km, k0 + kchunk - 1);
+  ThreadAcc acc;
+  acc.init();
+  if (i <= im && j <= jm) {
+    const float dx1i = g.DX1(i), dy1j = g.DY1(j), dt = ph.dt;
+    const int jm1 = (j == 1) ? jm : j - 1;  // g(i,0,k) = g(i,jm,k) (bondFG)
+    for (int k = k0; k <= k1; ++k) {
+      const float dznk = g.DZN(k);
+      const long ifg = d.F(i, j, k);
+      // bondFG: f(0,j,k) = f(1,j,k); g(i,0,k) = g(i,jm,k); h(i,j,0) = h(i,j,km) = 0
+      if (i == 1) F.f[d.F(0, j, k)] = F.f[ifg];
+      if (j == jm) F.g[d.F(i, 0, k)] = F.g[ifg];
+      if (k == 1) F.h[d.F(i, j, 0)] = 0.0f;
+      if (k == km) F.h[ifg] = 0.0f;
+      const float fim = (i == 1) ? F.f[ifg] : F.f[d.F(i - 1, j, k)];
+      const float gjm = F.g[d.F(i, jm1, k)];
+      const float hk = (k == km) ? 0.0f : F.h[ifg];
+      const float hkm = (k == 1) ? 0.0f : F.h[d.F(i, j, k - 1)];
+      float r = (-F.u[d.U(i - 1, j, k)] + F.u[d.U(i, j, k)]) / dx1i +
+                (-F.v[d.U(i, j - 1, k)] + F.v[d.U(i, j, k)]) / dy1j +
+                (-F.w[d.W(i, j, k - 1)] + F.w[d.W(i, j, k)]) / dznk;
+      r = (F.f[ifg] - fim) / dx1i + (F.g[ifg] - gjm) / dy1j + (hk - hkm) / dznk + r / dt;
+      F.rhs[d.R(i, j, k)] = r;
+      acc.add(dx1i * dy1j * dznk * r, lds);
+    }
+  }
+  acc.flush(lds);
+  const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  block_limbs_store(lds, partials + (long)blk * NLIMB);
+}
+
+__global__ __launch_bounds__(256) void k_rhsav_finalize(const int64_t* partials, int nb,
+                                                        DevScalars* sc) {
+  __shared__ int64_t limbs[NLIMB];
+  sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) sc->rhsav = limbs_to_float(limbs) / sc->area;
+}
+
+__global__ __launch_bounds__(256) void k_rhs_adjust(Dims d, Fields F, const DevScalars* sc) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const long ir = d.R(i, j, k);
+  F.rhs[ir] = F.rhs[ir] - sc->rhsav;
+}
+
+void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
+                DevScalars* sc) {
+  dim3 grid = interior_grid(d, RED_KC);
+  int nb = (int)(grid.x * grid.y * grid.z);
+  hipLaunchKernelGGL(k_rhs, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
+  hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
+}
+
+void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc) {
+  hipLaunchKernelGGL(k_rhs_adjust, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, F, sc);
+}
+
+// ==========================================================================================
+// press.f95:90-100: one red-black colour sweep (in place; a colour reads only the other
+// colour and its own cell, so the parallel sweep equals the sequential one bit for bit).
+// Threads: x -> every second i of the row's colour, y -> j, z -> k.
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_sor_colour(Dims d, Grid1D g, Fields F, Physics ph,
+                                                    int nrd, int want_resid, int64_t* partials,
+                                                    const DevScalars* sc) {
+  __shared__ unsigned long long lds[NLIMB];
+  if (sc->sor_done) return;
+  if (want_resid) block_limbs_init(lds);
+  const int t = blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int i = 1 + ((k + j + nrd) & 1) + 2 * t;
+  ThreadAcc acc;
+  acc.init();
+  if (i <= d.ip && j <= d.jp) {
+    const long ip = d.P(i, j, k);
+    const long sj = d.ip + 3, sk = (long)(d.ip + 3) * (d.jp + 3);
+    const float pc = F.p[ip];
+    const float reltmp =
+        ph.omega * (F.cn1[d.O(i, j, k)] *
+                        (g.cn2l[i - 1] * F.p[ip + 1] + g.cn2s[i - 1] * F.p[ip - 1] +
+                         g.cn3l[j - 1] * F.p[ip + sj] + g.cn3s[j - 1] * F.p[ip - sj] +
+                         g.cn4l[k - 1] * F.p[ip + sk] + g.cn4s[k - 1] * F.p[ip - sk] -
+                         F.rhs[d.R(i, j, k)]) -
+                    pc);
+    F.p[ip] = pc + reltmp;
+    if (want_resid) acc.add(reltmp * reltmp, lds);
+  }
+  if (want_resid) {
+    acc.flush(lds);
+    const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_limbs_store(lds, partials + (long)blk * NLIMB);
+  }
+}
+
+int sor_resid_blocks(Dims d) {
+  int half = (d.ip + 1) / 2;
+  return ((half + BX - 1) / BX) * ((d.jp + BY - 1) / BY) * d.kp;
+}
+
+void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
+                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc) {
+  int half = (d.ip + 1) / 2;
+  dim3 grid((half + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+  hipLaunchKernelGGL(k_sor_colour, grid, dim3(BX, BY), 0, st, d, g, F, ph, nrd, want_resid,
+                     rb.partials + (long)blk_offset * NLIMB, sc);
+}
+
+// residual of one SOR iteration (both colours) and the early exit of press.f95:114
+__global__ __launch_bounds__(256) void k_sor_resid_finalize(const int64_t* partials, int nb,
+                                                            Physics ph, DevScalars* sc) {
+  __shared__ int64_t limbs[NLIMB];
+  if (sc->sor_done) return;
+  sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) {
+    float s = limbs_to_float(limbs);
+    sc->sor = s;
+    sc->sor_iters += 1;
+    if (s < ph.pjuge) sc->sor_done = 1;
+  }
+}
+
+void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, int nblocks, Physics ph,
+                               DevScalars* sc) {
+  hipLaunchKernelGGL(k_sor_resid_finalize, dim3(1), dim3(256), 0, st, rb.partials, nblocks, ph,
+                     sc);
+}
+
+__global__ void k_sor_reset(DevScalars* sc) {
+  sc->sor_done = 0;
+  sc->sor_iters = 0;
+  sc->sor = 0.0f;
+}
+
+void launch_sor_reset(hipStream_t st, DevScalars* sc) {
+  hipLaunchKernelGGL(k_sor_reset, dim3(1), dim3(1), 0, st, sc);
+}
+
+// boundp.f95:23-44 (boundp1): i faces for j = 0..jm+1 then j faces for i = 0..im+1, at every
+// k = 0..km+1; composed so that every source is an interior (i,j) column.
+__global__ __launch_bounds__(256) void k_boundp1(Dims d, Fields F, const DevScalars* sc,
+                                                 int check_done) {
+  if (check_done && sc->sor_done) return;
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const long nface_i = (long)jm * (km + 2) * 2;       // a in {0,im+1}, b = 1..jm
+  const long nface_j = (long)(im + 2) * (km + 2) * 2; // b in {0,jm+1}, a = 0..im+1
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < nface_i) {
+    const int b = 1 + (int)(t % jm);
+    const int c = (int)((t / jm) % (km + 2));
+    const int side = (int)(t / ((long)jm * (km + 2)));
+    const int a = side ? im + 1 : 0, as = side ? im : 1;
+    F.p[d.P(a, b, c)] = F.p[d.P(as, b, c)];
+    return;
+  }
+  t -= nface_i;
+  if (t >= nface_j) return;
+  const int a = (int)(t % (im + 2));
+  const int c = (int)((t / (im + 2)) % (km + 2));
+  const int side = (int)(t / ((long)(im + 2) * (km + 2)));
+  const int b = side ? jm + 1 : 0, bs = side ? 1 : jm;
+  const int as = (a == 0) ? 1 : ((a == im + 1) ? im : a);
+  F.p[d.P(a, b, c)] = F.p[d.P(as, bs, c)];
+}
+
+// boundp.f95:5-21 (boundp2): k planes from k = 1 and k = km for i = 0..im+1, j = 0..jm+1
+__global__ __launch_bounds__(256) void k_boundp2(Dims d, Fields F, const DevScalars* sc,
+                                                 int check_done) {
+  if (check_done && sc->sor_done) return;
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const long n = (long)(im + 2) * (jm + 2);
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= 2 * n) return;
+  const int top = (int)(t / n);
+  t %= n;
+  const int a = (int)(t % (im + 2)), b = (int)(t / (im + 2));
+  if (top)
+    F.p[d.P(a, b, km + 1)] = F.p[d.P(a, b, km)];
+  else
+    F.p[d.P(a, b, 0)] = F.p[d.P(a, b, 1)];
+}
+
+void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done) {
+  long n = (long)d.jp * (d.kp + 2) * 2 + (long)(d.ip + 2) * (d.kp + 2) * 2;
+  hipLaunchKernelGGL(k_boundp1, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F, sc,
+                     check_done);
+}
+
+void launch_boundp2(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done) {
+  long n = 2L * (d.ip + 2) * (d.jp + 2);
+  hipLaunchKernelGGL(k_boundp2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F, sc,
+                     check_done);
+}
+
+// ==========================================================================================
+// press.f95:123-141: pav = sum(p*dx1*dy1*dzn)/pco; p -= pav; then boundp1 + boundp2 which,
+// composed, set every halo cell of p to its mirror interior cell.
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_pav(Dims d, Grid1D g, Fields F, int64_t* partials) {
+  __shared__ unsigned long long lds[NLIMB];
+  block_limbs_init(lds);
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int km = d.kp;
+  const int kchunk = (km + RED_KC - 1) / RED_KC;
+  const int k0 = 1 + blockIdx.z * kchunk, k1 = min(km, k0 + kchunk - 1);
+  ThreadAcc acc;
+  acc.init();
+  if (i <= d.ip && j <= d.jp) {
+    const float dx1i = g.DX1(i), dy1j = g.DY1(j);
+    for (int k = k0; k <= k1; ++k) acc.add(F.p[d.P(i, j, k)] * dx1i * dy1j * g.DZN(k), lds);
+  }
+  acc.flush(lds);
+  const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  block_limbs_store(lds, partials + (long)blk * NLIMB);
+}
+
+__global__ __launch_bounds__(256) void k_pav_finalize(const int64_t* partials, int nb,
+                                                      DevScalars* sc) {
+  __shared__ int64_t limbs[NLIMB];
+  sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) sc->pav = limbs_to_float(limbs) / sc->pco;
+}
+
+void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalars* sc) {
+  dim3 grid = interior_grid(d, RED_KC);
+  int nb = (int)(grid.x * grid.y * grid.z);
+  hipLaunchKernelGGL(k_pav, grid, dim3(BX, BY), 0, st, d, g, F, rb.partials);
+  hipLaunchKernelGGL(k_pav_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
+}
+
+// p -= pav on the interior; with_boundp also writes every mirror image of the cell
+// (boundp1 then boundp2 leave p(a,b,c) = p(M(a),M(b),M(c)) with M(0)=1, M(im+1)=im in i,
+// M(0)=jm, M(jm+1)=1 in j, M(0)=1, M(km+1)=km in k).
+__global__ __launch_bounds__(256) void k_adj(Dims d, Fields F, const DevScalars* sc,
+                                             int with_boundp) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  if (i > im || j > jm) return;
+  const long ip = d.P(i, j, k);
+  const float val = F.p[ip] - sc->pav;
+  F.p[ip] = val;
+  if (!with_boundp) return;
+  int ai[2], bj[2], ck[2];
+  int na = 0, nb = 0, nc = 0;
+  ai[na++] = i;
+  if (i == 1) ai[na++] = 0;
+  if (i == im) ai[na++] = im + 1;  // (im >= 2: at most one of the two)
+  bj[nb++] = j;
+  if (j == jm) bj[nb++] = 0;
+  if (j == 1) bj[nb++] = jm + 1;
+  ck[nc++] = k;
+  if (k == 1) ck[nc++] = 0;
+  if (k == km) ck[nc++] = km + 1;
+  for (int x = 0; x < na; ++x)
+    for (int y = 0; y < nb; ++y)
+      for (int z = 0; z < nc; ++z)
+        if (x | y | z) F.p[d.P(ai[x], bj[y], ck[z])] = val;
+}
+
+void launch_adj(hipStream_t st, Dims d, Fields F, DevScalars* sc, int with_boundp) {
+  hipLaunchKernelGGL(k_adj, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, F, sc, with_boundp);
+}
+
+// ==========================================================================================
+// Exact sum of a device vector (the reduction on its own, for testing)
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_exact_sum_part(const float* x, long n, int64_t* partials) {
+  __shared__ unsigned long long lds[NLIMB];
+  block_limbs_init(lds);
+  ThreadAcc acc;
+  acc.init();
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256)
+    acc.add(x[t], lds);
+  acc.flush(lds);
+  block_limbs_store(lds, partials + (long)blockIdx.x * NLIMB);
+}
+
+__global__ __launch_bounds__(256) void k_exact_sum_fin(const int64_t* partials, int nb, float* out) {
+  __shared__ int64_t limbs[NLIMB];
+  sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) *out = limbs_to_float(limbs);
+}
+
+void launch_exact_sum(hipStream_t st, const float* x, long n, RedBuf rb, float* out) {
+  int nb = (int)min((long)rb.max_blocks, max(1L, (n + 255) / 256));
+  hipLaunchKernelGGL(k_exact_sum_part, dim3(nb), dim3(256), 0, st, x, n, rb.partials);
+  hipLaunchKernelGGL(k_exact_sum_fin, dim3(1), dim3(256), 0, st, rb.partials, nb, out);
+}
+
+// ==========================================================================================
+// Halo exchange pack/unpack: halo_exchange.cl:3-51 (write), :156-185 (read) and
+// halo_exchange_correct.cl:156-185 (read, inner ring without corners), with the OpenCL
+// vector components replaced by separate fields (buffer block v = field v).
+// Thread t -> (x, k) with x along the row/column, i.e. the reference's gl_id mapping.
+// ==========================================================================================
+struct HaloPtrs {
+  float* a[16];
+};
+
+__global__ __launch_bounds__(256) void k_halo(HaloPtrs arr, int v_dim, int im, int jm, int km,
+                                              int hw, int correct, int write, float* buffer) {
+  const long L = (long)max(im, jm);
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= L * km) return;
+  const int k = (int)(t / L);
+  const int x = (int)(t % L);
+  const long plane = (long)im * jm;
+  if (!correct) {
+    const long v_sz = 2L * km * im * hw + 2L * km * (jm - 2 * hw) * hw;
+    for (int v = 0; v < v_dim; ++v) {
+      float* A = arr.a[v];
+      float* B = buffer + v * v_sz;
+      for (int w = 0; w < hw; ++w) {
+        if (x < im) {
+          long an = x + (long)im * w + k * plane, bn = x + (long)im * w + (long)k * im * hw;
+          long as = an + (long)im * (jm - hw), bs = (long)km * im * hw + bn;
+          if (write) {
+            A[an] = B[bn];
+            A[as] = B[bs];
+          } else {
+            B[bn] = A[an];
+            B[bs] = A[as];
+          }
+        }
+        if (x < jm - hw && x > hw - 1) {
+          long aw = (long)x * im + w + k * plane;
+          long bw = 2L * km * im * hw + (long)(x - hw) * hw + w + (long)k * (jm - 2 * hw) * hw;
+          long ae = aw + im - hw;
+          long be = bw + (long)km * (jm - 2 * hw) * hw;
+          if (write) {
+            A[aw] = B[bw];
+            A[ae] = B[be];
+          } else {
+            B[bw] = A[aw];
+            B[be] = A[ae];
+          }
+        }
+      }
+    }
+  } else {  // read only: inner ring, corners excluded
+    const long v_sz = 2L * km * (im + jm - 4 * hw) * hw;
+    for (int v = 0; v < v_dim; ++v) {
+      const float* A = arr.a[v];
+      float* B = buffer + v * v_sz;
+      for (int w = 0; w < hw; ++w) {
+        if (x < im - hw && x > hw - 1) {
+          long bn = (x - hw) + (long)(im - 2 * hw) * w + (long)k * (im - 2 * hw) * hw;
+          B[bn] = A[(long)im * hw + x + (long)im * w + k * plane];
+          B[(long)km * (im - 2 * hw) * hw + bn] = A[x + k * plane + (long)im * (jm - 2 * hw) + (long)im * w];
+        }
+        if (x < jm - hw && x > hw - 1) {
+          long bw = 2L * km * (im - 2 * hw) * hw + (long)(x - hw) * hw + w + (long)k * (jm - 2 * hw) * hw;
+          B[bw] = A[(long)x * im + w + hw + k * plane];
+          B[bw + (long)km * (jm - 2 * hw) * hw] = A[(long)x * im + w + k * plane + im - 2 * hw];
+        }
+      }
+    }
+  }
+}
+
+void launch_halo_read(hipStream_t st, const float* const* arrays, int v_dim, int im, int jm,
+                      int km, int h_w, int correct, float* buffer) {
+  HaloPtrs hp;
+  for (int v = 0; v < v_dim && v < 16; ++v) hp.a[v] = const_cast<float*>(arrays[v]);
+  long n = (long)max(im, jm) * km;
+  hipLaunchKernelGGL(k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, hp, v_dim, im,
+                     jm, km, h_w, correct, 0, buffer);
+}
+
+void launch_halo_write(hipStream_t st, float* const* arrays, int v_dim, int im, int jm, int km,
+                       int h_w, const float* buffer) {
+  HaloPtrs hp;
+  for (int v = 0; v < v_dim && v < 16; ++v) hp.a[v] = arrays[v];
+  long n = (long)max(im, jm) * km;
+  hipLaunchKernelGGL(k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, hp, v_dim, im,
+                     jm, km, h_w, 0, 1, const_cast<float*>(buffer));
+}
+
+}  // namespace les

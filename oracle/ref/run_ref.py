@@ -1,0 +1,109 @@
+"""Runs the reference Fortran physics (built by oracle/ref/build_ref.sh into oracle/_ref/) on
+inputs written in ref_harness.f95's raw stream format - test infrastructure only, used in this
+container to produce the golden vectors under tests/golden/."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from les_halo_amd import fields as fl
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_OUT = os.path.join(os.path.dirname(HERE), "_ref")
+REF_SRC = os.environ.get("LES_REFERENCE_SRC", "/root/reference/src/Fortran")
+
+DUMP_ORDER = (["p", "u", "v", "w", "usum", "vsum", "wsum", "f", "g", "h", "fold", "gold", "hold",
+               "rhs", "sm"] + ["diu%d" % q for q in range(1, 10)]
+              + ["cn1", "cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s", "delx1"])
+
+
+def tag_of(ip, jp, kp):
+    return "ref150" if (ip, jp, kp) == (150, 150, 90) else "%dx%dx%d" % (ip, jp, kp)
+
+
+def available():
+    return os.path.isdir(REF_SRC)
+
+
+def binary(ip, jp, kp):
+    tag = tag_of(ip, jp, kp)
+    exe = os.path.join(REF_OUT, tag, "ref_harness")
+    if not os.path.exists(exe):
+        subprocess.run([os.path.join(HERE, "build_ref.sh"), tag], check=True,
+                       stdout=subprocess.DEVNULL)
+    return exe
+
+
+def _f32(x):
+    return np.asarray(x, dtype=np.float32).reshape(-1).tobytes()
+
+
+def _i32(*xs):
+    return np.asarray(xs, dtype=np.int32).tobytes()
+
+
+def _header(mode, ip, jp, kp, phys, case):
+    b = _i32(mode, ip, jp, kp)
+    b += _f32([phys["dt"], phys["ro"], phys["vn"], phys["alpha"], phys["beta"]])
+    b += b"".join(_f32(case[n]) for n in ["dx1", "dy1", "dzn", "dzs", "dxs", "dys", "z2"])
+    b += b"".join(_f32(case[n]) for n in ["amask1", "bmask1", "cmask1", "dmask1", "wind_profile"])
+    return b
+
+
+def _parse(raw, ip, jp, kp, ndumps):
+    sizes = [fl.size(n, ip, jp, kp) for n in DUMP_ORDER]
+    per = sum(sizes)
+    arr = np.frombuffer(raw, dtype=np.float32)
+    assert arr.size == per * ndumps, (arr.size, per, ndumps)
+    out = []
+    off = 0
+    for _ in range(ndumps):
+        d = {}
+        for n, s in zip(DUMP_ORDER, sizes):
+            d[n] = arr[off:off + s].copy()
+            off += s
+        out.append(d)
+    return out
+
+
+def _run(exe, payload, cwd=None):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(payload)
+        subprocess.run([exe, fin, fout], check=True, cwd=cwd, stdout=subprocess.DEVNULL)
+        with open(fout, "rb") as f:
+            return f.read()
+
+
+def trajectory(case, phys, nsteps, dump_steps):
+    """Mode 0: init + ifdata sequence + nsteps steps; returns one state dict per dump step."""
+    ip, jp, kp = case["dims"]
+    payload = _header(0, ip, jp, kp, phys, case) + _i32(nsteps, len(dump_steps)) + _i32(*dump_steps)
+    return _parse(_run(binary(ip, jp, kp), payload), ip, jp, kp, len(dump_steps))
+
+
+def stage(case, phys, state, stage_id, n):
+    """Mode 1: one reference stage on a full state (dict of flat float32 arrays)."""
+    ip, jp, kp = case["dims"]
+    payload = _header(1, ip, jp, kp, phys, case) + _i32(stage_id, n)
+    payload += b"".join(_f32(state[nm]) for nm in DUMP_ORDER)
+    return _parse(_run(binary(ip, jp, kp), payload), ip, jp, kp, 1)[0]
+
+
+STAGE_IDS = {"velnw": 1, "bondv1": 2, "velfg": 3, "feedbf": 4, "les": 5, "adam": 6, "press": 7,
+             "boundp1": 8, "boundp2": 9, "bondfg": 10, "boundsm": 11, "step": 12}
+
+
+def tokyo_masks(ip, jp, kp, dzn, z2, zbm):
+    """Mode 2: feedbfm.f95 with its own GIS read (needs the 150x150 reference build)."""
+    b = _i32(2, ip, jp, kp) + _f32(dzn) + _f32(z2) + _f32(zbm)
+    raw = _run(binary(ip, jp, kp), b, cwd=REF_SRC)
+    arr = np.frombuffer(raw, dtype=np.float32)
+    sizes = [(ip + 3) * (jp + 3)] + [fl.size(n, ip, jp, kp) for n in fl.MASKS]
+    out, off = {}, 0
+    for n, s in zip(["zbm"] + fl.MASKS, sizes):
+        out[n] = arr[off:off + s].copy()
+        off += s
+    return out
