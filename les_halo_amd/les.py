@@ -1,0 +1,224 @@
+"""Host-side mirror of the reference's GPU host API for the LES_Halo time step.
+
+`LES` owns one device domain through libles_hip.so and exposes the reference's kernel-level
+names (run_LES_kernel's states, src/OpenCL/module_LES_combined_ocl_TEMPL_V2.f95:453-1058, and
+the CPU subroutines of src/Fortran/main_ocl.f95:264-299).  Arrays cross the boundary in the
+reference's Fortran bounds and layouts (les_halo_amd.fields).  Every call raises LESError on a
+non-zero status, with the library's message.
+"""
+import ctypes
+
+import numpy as np
+
+from . import fields as fl
+from ._lib import (ERR_NAMES, LES_PREC_FP32, LES_PREC_MIXED, LesConfig, LesScalars, lib, ptr)
+
+
+class LESError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERR_NAMES.get(code, code), msg))
+        self.code = code
+
+
+class LES:
+    """One ip x jp x kp domain on the GPU."""
+
+    def __init__(self, ip, jp, kp, dt=0.2, ro=1.1763, vn=None, alpha=-10.0, beta=-1.0, cs0=0.1,
+                 omega=1.0, pjuge=1e-4, nmaxp=50, precision=LES_PREC_FP32, refine_tol=1e-7,
+                 refine_max=8, device=0):
+        L = lib()
+        cfg = LesConfig()
+        L.les_default_config(ctypes.byref(cfg), ip, jp, kp)
+        cfg.dt, cfg.ro, cfg.alpha, cfg.beta, cfg.cs0 = dt, ro, alpha, beta, cs0
+        if vn is not None:
+            cfg.vn = vn
+        cfg.omega, cfg.pjuge, cfg.nmaxp, cfg.precision = omega, pjuge, nmaxp, precision
+        cfg.refine_tol, cfg.refine_max, cfg.device = refine_tol, refine_max, device
+        self.cfg = cfg
+        self.dims = (ip, jp, kp)
+        h = ctypes.c_void_p()
+        r = L.les_create(ctypes.byref(cfg), ctypes.byref(h))
+        if r != 0:
+            raise LESError(r, "les_create failed (%dx%dx%d)" % (ip, jp, kp))
+        self.h = h
+
+    # ---- lifetime ----
+    def close(self):
+        if getattr(self, "h", None):
+            lib().les_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, r, what):
+        if r != 0:
+            msg = lib().les_last_error(self.h)
+            raise LESError(r, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    # ---- data ----
+    def upload(self, name, data):
+        ip, jp, kp = self.dims
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float32).reshape(-1))
+        if a.size != fl.size(name, ip, jp, kp):
+            raise ValueError("%s: expected %d values, got %d" % (name, fl.size(name, ip, jp, kp), a.size))
+        self._chk(lib().les_upload(self.h, fl.field_id(name), ptr(a)), "upload " + name)
+
+    def download(self, name):
+        ip, jp, kp = self.dims
+        a = np.empty(fl.size(name, ip, jp, kp), np.float32)
+        self._chk(lib().les_download(self.h, fl.field_id(name), ptr(a)), "download " + name)
+        return a
+
+    def device_ptr(self, name):
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        self._chk(lib().les_device_ptr(self.h, fl.field_id(name), ctypes.byref(p), ctypes.byref(n)),
+                  "device_ptr")
+        return p.value, n.value
+
+    def load_case(self, case, coefficients=True):
+        """Uploads grid, masks and inflow of a synthetic case; computes cn*/delx1."""
+        for n in fl.GRID + fl.MASKS + ["wind_profile"]:
+            self.upload(n, case[n])
+        if coefficients:
+            self.init_coefficients()
+
+    def init_coefficients(self):
+        self._chk(lib().les_init_coefficients(self.h), "init_coefficients")
+
+    def state(self, names=None):
+        return {n: self.download(n) for n in (names or fl.STATE_3D)}
+
+    def sync(self):
+        self._chk(lib().les_sync(self.h), "sync")
+
+    def scalars(self):
+        s = LesScalars()
+        self._chk(lib().les_get_scalars(self.h, ctypes.byref(s)), "get_scalars")
+        return {k: getattr(s, k) for k, _ in LesScalars._fields_}
+
+    # ---- reference stages (names of run_LES_kernel's states and of main_ocl's calls) ----
+    def _call(self, name, *args):
+        self._chk(getattr(lib(), "les_" + name)(self.h, *args), name)
+
+    def velnw(self):
+        self._call("velnw")
+
+    def bondv1_init_uvw(self, n):
+        self._call("bondv1_init_uvw", n)
+
+    def velnw__bondv1_init_uvw(self, n):
+        self._call("velnw__bondv1_init_uvw", n)
+
+    def bondv1_calc_uout(self):
+        self._call("bondv1_calc_uout")
+
+    def bondv1_calc_uvw(self):
+        self._call("bondv1_calc_uvw")
+
+    def bondv1(self, n):
+        """bondv1_ext_wind_profile.f95 as a whole."""
+        self.bondv1_init_uvw(n)
+        self.bondv1_calc_uout()
+        self.bondv1_calc_uvw()
+
+    def velfg(self):
+        self._call("velfg")
+
+    def feedbf(self):
+        self._call("feedbf")
+
+    def les_calc_sm(self):
+        self._call("les_calc_sm")
+
+    def velfg__feedbf__les_calc_sm(self):
+        self._call("velfg__feedbf__les_calc_sm")
+
+    def les_bound_sm(self):
+        self._call("les_bound_sm")
+
+    def les_calc_visc(self):
+        self._call("les_calc_visc")
+
+    def adam(self):
+        self._call("adam")
+
+    def les_calc_visc__adam(self):
+        self._call("les_calc_visc__adam")
+
+    def les(self):
+        self._call("les")
+
+    def bondfg(self):
+        self._call("bondfg")
+
+    def boundp1(self):
+        self._call("boundp1")
+
+    def boundp2(self):
+        self._call("boundp2")
+
+    def press_rhsav(self):
+        self._call("press_rhsav")
+
+    def press_sor(self, nrd):
+        self._call("press_sor", nrd)
+
+    def press_sor_solve(self):
+        self._call("press_sor_solve")
+
+    def press_pav(self):
+        self._call("press_pav")
+
+    def press_adj(self):
+        self._call("press_adj")
+
+    def press_boundp(self):
+        self._call("press_boundp")
+
+    def press(self):
+        self._call("press")
+
+    def step(self, n):
+        self._call("step", n)
+
+    def run(self, n0, nsteps):
+        self._chk(lib().les_run(self.h, n0, nsteps), "run")
+
+    def ifdata(self):
+        self._call("ifdata")
+
+    # ---- halo exchange ----
+    def halo_size(self, names, h_w=1, correct=0):
+        ip, jp, kp = self.dims
+        ext = fl.shape(names[0], ip, jp, kp)
+        return int(lib().les_halo_size(len(names), ext[0], ext[1], ext[2], h_w, correct))
+
+    def exchange_halo_read(self, names, dev_buffer, h_w=1, correct=0):
+        ids = (ctypes.c_int * len(names))(*[fl.field_id(n) for n in names])
+        self._chk(lib().les_exchange_halo_read(self.h, ids, len(names), h_w, correct,
+                                               ctypes.c_void_p(dev_buffer)), "exchange_halo_read")
+
+    def exchange_halo_write(self, names, dev_buffer, h_w=1):
+        ids = (ctypes.c_int * len(names))(*[fl.field_id(n) for n in names])
+        self._chk(lib().les_exchange_halo_write(self.h, ids, len(names), h_w,
+                                                ctypes.c_void_p(dev_buffer)), "exchange_halo_write")
+
+    def reduce_sum_exact(self, dev_ptr, n):
+        out = ctypes.c_float()
+        self._chk(lib().les_reduce_sum_exact(self.h, ctypes.c_void_p(dev_ptr), n,
+                                             ctypes.byref(out)), "reduce_sum_exact")
+        return np.float32(out.value)
+
+
+__all__ = ["LES", "LESError", "LES_PREC_FP32", "LES_PREC_MIXED"]

@@ -1,0 +1,220 @@
+"""HIP path against the oracle, bit for bit.
+
+Each kernel-level entry point of libles_hip.so runs on a seeded random state (every array,
+halos included) and must reproduce the oracle's FP32 restatement of the same reference stage
+exactly; the oracle runs with exact (order-independent) reductions, the semantics of the HIP
+reductions.  The oracle itself is pinned to the reference Fortran by test_oracle_reference.py.
+"""
+import numpy as np
+import pytest
+
+import common
+from les_halo_amd import fields as fl
+from oracle.oracle import SUM_EXACT
+
+pytestmark = pytest.mark.gpu
+
+STATE = [n for n in common.DUMP_ORDER]
+
+
+def _gpu(case, **sor):
+    from les_halo_amd import LES
+    ip, jp, kp = case["dims"]
+    L = LES(ip, jp, kp, **common.REF_PHYS, **sor)
+    L.load_case(case)
+    return L
+
+
+def _pair(case, seed=None, **sor):
+    L = _gpu(case, **sor)
+    o = common.oracle_from_case(case, **common.REF_PHYS, **sor, sum_mode=SUM_EXACT)
+    if seed is not None:
+        ip, jp, kp = case["dims"]
+        for name, arr in common.random_state(ip, jp, kp, seed).items():
+            L.upload(name, arr)
+            o.set(name, arr)
+    return L, o
+
+
+def _compare(L, o, tag, names=STATE):
+    bad = []
+    for n in names:
+        g = L.download(n)
+        if not common.bits_equal(g, o.a[n]):
+            bad.append(common.mismatch_report(g, o.a[n], n))
+    assert not bad, "%s:\n  %s" % (tag, "\n  ".join(bad))
+
+
+@pytest.fixture(scope="module")
+def case():
+    return common.tiny_case()
+
+
+@pytest.fixture(scope="module")
+def case_odd():
+    return common.tiny_case(14, 11, 8, seed=9)
+
+
+def test_coefficients_match_init(case):
+    L, o = _pair(case)
+    _compare(L, o, "init.f95 coefficients", ["cn1", "cn2l", "cn2s", "cn3l", "cn3s", "cn4l",
+                                             "cn4s", "delx1"])
+
+
+# (gpu calls, oracle calls) per stage; oracle calls are (stage, n)
+STAGES = {
+    "velnw": (lambda L: L.velnw(), [("velnw", 0)]),
+    "bondv1_n1": (lambda L: L.bondv1(1), [("bondv1", 1)]),
+    "bondv1_n3": (lambda L: L.bondv1(3), [("bondv1", 3)]),
+    "state1_n2": (lambda L: (L.velnw__bondv1_init_uvw(2), L.bondv1_calc_uout(), L.bondv1_calc_uvw()),
+                  [("velnw", 0), ("bondv1", 2)]),
+    "velfg": (lambda L: L.velfg(), [("velfg", 0)]),
+    "feedbf": (lambda L: L.feedbf(), [("feedbf", 0)]),
+    "state4": (lambda L: L.velfg__feedbf__les_calc_sm(),
+               [("velfg", 0), ("feedbf", 0), ("les_calc_sm", 0)]),
+    "les_calc_sm": (lambda L: L.les_calc_sm(), [("les_calc_sm", 0)]),
+    "les_bound_sm": (lambda L: L.les_bound_sm(), [("boundsm", 0)]),
+    "les_calc_visc": (lambda L: L.les_calc_visc(), [("les_calc_visc", 0)]),
+    "les": (lambda L: L.les(), [("les", 0)]),
+    "adam": (lambda L: L.adam(), [("adam", 0)]),
+    "state6": (lambda L: L.les_calc_visc__adam(), [("les_calc_visc", 0), ("adam", 0)]),
+    "bondfg": (lambda L: L.bondfg(), [("bondfg", 0)]),
+    "boundp1": (lambda L: L.boundp1(), [("boundp1", 0)]),
+    "boundp2": (lambda L: L.boundp2(), [("boundp2", 0)]),
+    "press_rhsav": (lambda L: L.press_rhsav(), [("press_rhsav", 0)]),
+    "press_sor_red": (lambda L: L.press_sor(0), [("sor_colour", 0), ("boundp1", 0)]),
+    "press_sor_black": (lambda L: L.press_sor(1), [("sor_colour", 1), ("boundp1", 0)]),
+    "press_sor_k": (lambda L: L.press_sor(2), [("boundp2", 0)]),
+    "press_sor_solve": (lambda L: L.press_sor_solve(), [("press_sor", 0)]),
+    "press_pav_adj_boundp": (lambda L: (L.press_pav(), L.press_adj(), L.press_boundp()),
+                             [("press_pav_adj", 0)]),
+    "press": (lambda L: L.press(), [("press", 0)]),
+    "step_n1": (lambda L: L.step(1), [("step", 1)]),
+    "step_n5": (lambda L: L.step(5), [("step", 5)]),
+}
+
+
+@pytest.mark.parametrize("sor", ["ref", "cfg"])
+@pytest.mark.parametrize("stage", sorted(STAGES))
+def test_stage_bitwise(case, stage, sor):
+    sorp = common.REF_SOR if sor == "ref" else common.CFG_SOR
+    if sor == "cfg" and "press" not in stage and "step" not in stage:
+        pytest.skip("SOR parameters only matter for press")
+    gpu_call, orc_calls = STAGES[stage]
+    L, o = _pair(case, seed=101 + sorted(STAGES).index(stage), **sorp)
+    print("stage", stage, sor, flush=True)
+    gpu_call(L)
+    for name, n in orc_calls:
+        o.call(name, n)
+    _compare(L, o, stage)
+    s = L.scalars()
+    if stage.startswith("bondv1") or stage.startswith("state1") or stage.startswith("step"):
+        assert np.float32(s["uout"]) == np.float32(o.scalars.uout) or stage.startswith("step")
+    if stage == "press_rhsav":
+        assert np.float32(s["rhsav"]) == np.float32(o.scalars.rhsav)
+        assert np.float32(s["area"]) == np.float32(o.scalars.area)
+    if stage == "press_sor_solve" and sor == "ref":
+        assert s["sor_iters"] == o.scalars.sor_iters
+        assert np.float32(s["sor"]) == np.float32(o.scalars.sor)
+    L.close()
+
+
+def test_ifdata_from_zero(case):
+    L, o = _pair(case)
+    L.ifdata()
+    o.ifdata()
+    _compare(L, o, "ifdata")
+
+
+@pytest.mark.parametrize("sor", ["ref", "cfg"])
+def test_trajectory_eager(case, sor):
+    sorp = common.REF_SOR if sor == "ref" else common.CFG_SOR
+    L, o = _pair(case, **sorp)
+    L.ifdata()
+    o.ifdata()
+    for n in range(1, 13):
+        L.step(n)
+        o.step(n)
+        if n in (1, 2, 5, 12):
+            _compare(L, o, "eager step %d" % n)
+
+
+def test_trajectory_odd_jm(case_odd):
+    L, o = _pair(case_odd, **common.REF_SOR)
+    L.ifdata()
+    o.ifdata()
+    for n in range(1, 4):
+        L.step(n)
+        o.step(n)
+    _compare(L, o, "odd jm step 3")
+
+
+def test_graph_replay_equals_eager(case):
+    """les_run replays a captured HIP graph; several replays with eager work in between."""
+    A, o = _pair(case, **common.CFG_SOR)
+    B = _gpu(case, **common.CFG_SOR)
+    A.ifdata()
+    B.ifdata()
+    o.ifdata()
+    for n in range(1, 4):
+        A.step(n)
+        o.step(n)
+    B.run(1, 3)
+    _compare(B, o, "graph after 3")
+    for rep in range(3):
+        n0 = 4 + 2 * rep
+        B.run(n0, 2)
+        B.sync()
+        _ = B.download("u")
+        for n in (n0, n0 + 1):
+            o.step(n)
+        _compare(B, o, "graph replay %d" % rep, ["u", "v", "w", "p", "f", "g", "h", "sm", "rhs"])
+
+
+def test_halo_size_matches_reference_buffers():
+    """TEMPL:111-119 halo buffer sizes (h_w = 1) from les_halo_size."""
+    from les_halo_amd._lib import lib
+    ip, jp, kp = 300, 300, 90
+    L = lib()
+    assert L.les_halo_size(2, ip + 3, jp + 3, kp + 2, 1, 0) == 4 * (ip + jp + 6 - 2) * (kp + 2)
+
+
+def test_exact_sum_gpu_vs_oracle():
+    from les_halo_amd import LES
+    from oracle.oracle import exact_sum_f32
+    ip, jp, kp = 40, 30, 20
+    L = LES(ip, jp, kp)
+    n = fl.size("p", ip, jp, kp)
+    ptr, cnt = L.device_ptr("p")
+    rng = np.random.default_rng(5)
+    vecs = [
+        rng.standard_normal(n).astype(np.float32),
+        np.concatenate([np.float32([1e30, 1.0, -1e30]), np.zeros(n - 3, np.float32)]),
+        (rng.standard_normal(n) * np.exp(rng.uniform(-80, 80, n))).astype(np.float32),
+        np.full(n, np.float32(1e-45)),
+    ]
+    half = rng.standard_normal(n // 2).astype(np.float32)
+    cancel = np.zeros(n, np.float32)
+    cancel[:n // 2] = half
+    cancel[n // 2:n // 2 * 2] = -half[::-1]
+    vecs.append(cancel)  # exactly zero
+    for q, x in enumerate(vecs):
+        x = np.ascontiguousarray(x.astype(np.float32))
+        L.upload("p", x)
+        got = L.reduce_sum_exact(ptr, n)
+        want = exact_sum_f32(x)
+        assert got.view(np.uint32) == want.view(np.uint32), (q, got, want)
+
+
+@pytest.mark.slow
+def test_full_size_one_step():
+    """300x300x90 (the benchmark grid): init and one step, every array bit for bit."""
+    from les_halo_amd import inputs
+    case = inputs.make_case(300, 300, 90)
+    L, o = _pair(case, **common.CFG_SOR)
+    L.ifdata()
+    o.ifdata()
+    L.step(1)
+    o.step(1)
+    _compare(L, o, "300x300x90 step 1", ["u", "v", "w", "p", "f", "g", "h", "sm", "rhs", "usum",
+                                         "fold", "diu1", "diu5", "diu9"])
