@@ -1,0 +1,129 @@
+#!/usr/bin/env python3
+"""Benchmark of the LES_Halo time step on one MI355X.
+
+Workload: the plain LES step at 300x300x90 FP32 (dx = dy = 4 m, dz = 2 m stretched x1.05 above
+40 m, seeded synthetic buildings covering 30 % of the ground, log-law inflow u* = 0.2 m/s with
+1 % noise, dt = 0.05 s, Smagorinsky Cs = 0.1, exactly 50 red-black SOR iterations at omega 1.5).
+A step is one pass of the whole time step (velnw .. press) over the domain, replayed from a
+captured HIP graph with every input already resident in HBM.
+
+Prints one JSON line: value = M cell-steps/s of the whole job; roofline = the dominant kernel
+(one SOR colour sweep) timed live with HIP events on the library's stream, against the HBM
+peak; cpu_baseline = the reference Fortran time loop (built from the reference sources by
+oracle/ref/build_ref.sh, serial) on a bounded sample of the same workload, or the FP32 CPU
+oracle when the reference build is absent.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+B_STEP = 1057                  # algorithmic FP32 bytes per interior cell-step
+SOR_SWEEP_B_PER_CELL = 8       # one colour sweep: r p 4 + r rhs 2 + w p 2
+METRIC = "M cell-steps/s (50 SOR iters/step) + % HBM peak, 1 GPU; batched vs plain"
+GRID = (300, 300, 90)
+PHYS = dict(dt=0.05, ro=1.1763, vn=1.583e-5, alpha=-10.0, beta=-1.0, cs0=0.1, omega=1.5,
+            pjuge=0.0, nmaxp=50)
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the SOR sweep from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(HERE, "profiles", "sor_sweep_300x300x90_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return float(json.load(open(path))["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(case, nsteps):
+    """The reference's serial time loop on nsteps steps of the same workload."""
+    from oracle.ref import run_ref
+    ip, jp, kp = case["dims"]
+    exe = os.path.join(HERE, "oracle", "_ref", run_ref.tag_of(ip, jp, kp), "ref_harness")
+    phys = {k: PHYS[k] for k in ("dt", "ro", "vn", "alpha", "beta")}
+    if os.path.exists(exe):
+        secs = run_ref.timed_steps(exe, case, phys, nsteps)
+        kind = "reference"
+        sample = ("reference Fortran time loop (flang -O2, omega 1, early exit at 1e-4 as "
+                  "hard-coded in press.f95), %d steps after init at %dx%dx%d" % (nsteps, ip, jp, kp))
+    else:
+        from oracle.oracle import OracleState
+        o = OracleState(ip, jp, kp, **PHYS)
+        o.load_case(case)
+        o.init()
+        o.ifdata()
+        t0 = time.perf_counter()
+        for n in range(1, nsteps + 1):
+            o.step(n)
+        secs = time.perf_counter() - t0
+        kind = "port"
+        sample = "FP32 C++ oracle, %d steps at %dx%dx%d" % (nsteps, ip, jp, kp)
+    return {"value": ip * jp * kp * nsteps / secs / 1e6, "unit": "M cell-steps/s", "cores": 1,
+            "kind": kind, "sample": sample, "seconds": round(secs, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the LES step on one GPU only (no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from les_halo_amd import LES, inputs
+    ip, jp, kp = GRID
+    case = inputs.make_case(ip, jp, kp)
+    L = LES(ip, jp, kp, **PHYS)
+    L.load_case(case)
+    L.ifdata()
+    L.run(1, args.warmup)
+    L.sync()
+    t0 = time.perf_counter()
+    L.run(1 + args.warmup, args.steps)
+    L.sync()
+    secs = time.perf_counter() - t0
+    cells = ip * jp * kp
+    ms_per_step = secs / args.steps * 1e3
+    value = cells * args.steps / secs / 1e6
+
+    # dominant kernel, timed live with HIP events on the library's stream
+    sweep_ms = L.time_kernel("sor_sweep", 200)
+    bytes_launch = SOR_SWEEP_B_PER_CELL * cells
+    achieved = bytes_launch / (sweep_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(), "kernel": "k_sor_colour",
+                "bytes_per_launch": bytes_launch, "avg_us": round(sweep_ms * 1e3, 3)}
+    step_hbm_frac = B_STEP * cells / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+
+    line = {"metric": METRIC, "value": round(value, 2), "unit": "M cell-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded terrain and inflow)",
+            "config": {"workload": "LES_Halo plain step, 300x300x90 FP32, 50 red-black SOR "
+                                   "iterations (omega 1.5), single domain",
+                       "grid": list(GRID), "sor_iters": 50, "omega": 1.5, "dt": 0.05,
+                       "parallelism": "single domain, 1 GPU"},
+            "step_hbm_frac": round(step_hbm_frac, 4), "step_bytes_per_cell": B_STEP,
+            "roofline": roofline}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

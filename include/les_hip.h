@@ -228,6 +228,21 @@ int les_exchange_halo_write(les_handle* h, const int* fields, int v_dim, int h_w
  * rhsav, pav and the SOR residual use), result written to *out on the host. */
 int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n, float* out);
 
+/* ---- measurement (replaces the per-launch exectime of runOcl, TEMPL:366-370,1061-1071) ----
+ * Launches one kernel of the step 'reps' times on the handle's stream between two HIP events
+ * and returns the average duration per launch in milliseconds.  The kernel runs on the
+ * current device state (it changes it, as the step would). */
+enum {
+  LES_KERNEL_SOR_SWEEP = 0,  /* one red-black colour sweep (the dominant kernel) */
+  LES_KERNEL_VELFG = 1,      /* velfg + feedbf */
+  LES_KERNEL_VISC_ADAM = 2,  /* les viscous terms + adam */
+  LES_KERNEL_CALC_SM = 3,
+  LES_KERNEL_RHS = 4,
+  LES_KERNEL_VELNW = 5,
+  LES_KERNEL_STEP = 6        /* one whole step replayed from the captured graph */
+};
+int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms);
+
 /* ---- the reference's GPU host API (module_LES_combined_ocl) as single-handle drop-ins ---- */
 /* initialise_LES_kernel (Wrappers/module_LES_combined_ocl.f95:15-441), external wind profile
  * form: creates the library's default handle and uploads every array. */

@@ -76,6 +76,7 @@ PROTOS = {
     "les_exchange_halo_read": (c_int, [H, ctypes.POINTER(c_int), c_int, c_int, c_int, c_void_p]),
     "les_exchange_halo_write": (c_int, [H, ctypes.POINTER(c_int), c_int, c_int, c_void_p]),
     "les_reduce_sum_exact": (c_int, [H, c_void_p, c_int64, P_float]),
+    "les_time_kernel": (c_int, [H, c_int, c_int, P_float]),
     "les_initialise_LES_kernel": (c_int, [c_void_p] * 42 + [c_float, c_int, c_int, c_int]),
     "les_run_LES_kernel": (c_int, [c_int, c_int, c_void_p]),
     "les_default_handle": (c_void_p, []),

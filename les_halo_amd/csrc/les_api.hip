@@ -751,6 +751,44 @@ extern "C" int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n
   return LES_OK;
 }
 
+extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms) {
+  if (!h || !avg_ms || reps < 1) return LES_E_INVALID;
+  STAGE_BEGIN(h);
+  if (which == LES_KERNEL_STEP && !h->graph_valid) {
+    int r = les_run(h, 2, 1);  // captures the graph
+    if (r) return r;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  HIPCHK(h, hipEventRecord(e0, h->st));
+  for (int q = 0; q < reps; ++q) {
+    switch (which) {
+      case LES_KERNEL_SOR_SWEEP:
+        launch_sor_colour(h->st, d, g, F, h->ph, q & 1, 0, h->rb, 0, h->sc);
+        break;
+      case LES_KERNEL_VELFG: launch_velfg(h->st, d, g, F, h->ph, 1, 1); break;
+      case LES_KERNEL_VISC_ADAM: launch_calc_visc_adam(h->st, d, g, F, 1, 1); break;
+      case LES_KERNEL_CALC_SM: launch_calc_sm(h->st, d, g, F, h->ph); break;
+      case LES_KERNEL_RHS: launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc); break;
+      case LES_KERNEL_VELNW: launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 2); break;
+      case LES_KERNEL_STEP: HIPCHK(h, hipGraphLaunch(h->gexec, h->st)); break;
+      default:
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return fail(h, LES_E_INVALID, "time_kernel: unknown kernel");
+    }
+  }
+  HIPCHK(h, hipEventRecord(e1, h->st));
+  HIPCHK(h, hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *avg_ms = ms / (float)reps;
+  return check_launch(h);
+}
+
 // ------------------------------------------------------------------------------------------
 // the reference's GPU host API on a default handle
 // ------------------------------------------------------------------------------------------

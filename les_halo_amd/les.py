@@ -214,6 +214,16 @@ class LES:
         self._chk(lib().les_exchange_halo_write(self.h, ids, len(names), h_w,
                                                 ctypes.c_void_p(dev_buffer)), "exchange_halo_write")
 
+    KERNELS = {"sor_sweep": 0, "velfg": 1, "visc_adam": 2, "calc_sm": 3, "rhs": 4, "velnw": 5,
+               "step": 6}
+
+    def time_kernel(self, which, reps=20):
+        """Average milliseconds per launch of one kernel (HIP events on the handle's stream)."""
+        out = ctypes.c_float()
+        self._chk(lib().les_time_kernel(self.h, self.KERNELS[which], reps, ctypes.byref(out)),
+                  "time_kernel")
+        return float(out.value)
+
     def reduce_sum_exact(self, dev_ptr, n):
         out = ctypes.c_float()
         self._chk(lib().les_reduce_sum_exact(self.h, ctypes.c_void_p(dev_ptr), n,

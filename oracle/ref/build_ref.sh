@@ -1,7 +1,7 @@
 #!/bin/bash
 # Builds the reference CPU physics from the reference's own Fortran sources, where they lie,
 # plus the test driver ref_harness.f95, into oracle/_ref/<tag>/ (never committed).
-#   build_ref.sh ref150        -> grid from the reference's params_common_sn_150x150x90.f95
+#   build_ref.sh ref150|ref300 -> grid from the reference's params_common_sn_{150x150,300x300}x90.f95
 #   build_ref.sh IPxJPxKP      -> grid chosen through a params_common_sn module with those sizes
 #                                 (the reference selects its grid the same way: one module of three
 #                                 integer parameters, params_common_sn_*.f95)
@@ -19,6 +19,8 @@ mkdir -p "$OUT"
 cd "$OUT"
 if [ "$TAG" = ref150 ]; then
   PARAMS="$REF/params_common_sn_150x150x90.f95"
+elif [ "$TAG" = ref300 ]; then
+  PARAMS="$REF/params_common_sn_300x300x90.f95"
 else
   IFS=x read -r IP JP KP <<< "$TAG"
   PARAMS="$OUT/params_common_sn.f95"

@@ -66,6 +66,7 @@ program ref_harness
   real(kind=4) :: dt, ro, vn, alpha, beta
   integer :: mode, nip, njp, nkp, nsteps, ndump, stage, nstage, n, idump, im, jm, km
   integer, allocatable :: dump_steps(:)
+  integer(kind=8) :: clk0, clk1, clkrate
   character(len=512) :: fin, fout
   character(len=70) :: data20, data21
 
@@ -125,12 +126,15 @@ program ref_harness
     do idump = 1, ndump
       if (dump_steps(idump) == 0) call dump_state()
     end do
+    call system_clock(clk0, clkrate)
     do n = 1, nsteps
       call one_step(n)
       do idump = 1, ndump
         if (dump_steps(idump) == n) call dump_state()
       end do
     end do
+    call system_clock(clk1)
+    print '(A,F12.6)', 'steps_seconds=', real(clk1 - clk0, 8) / real(clkrate, 8)
   else if (mode == 1) then
     read(10) stage, nstage
     call read_state()

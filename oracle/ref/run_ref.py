@@ -19,7 +19,11 @@ DUMP_ORDER = (["p", "u", "v", "w", "usum", "vsum", "wsum", "f", "g", "h", "fold"
 
 
 def tag_of(ip, jp, kp):
-    return "ref150" if (ip, jp, kp) == (150, 150, 90) else "%dx%dx%d" % (ip, jp, kp)
+    if (ip, jp, kp) == (150, 150, 90):
+        return "ref150"
+    if (ip, jp, kp) == (300, 300, 90):
+        return "ref300"
+    return "%dx%dx%d" % (ip, jp, kp)
 
 
 def available():
@@ -67,14 +71,27 @@ def _parse(raw, ip, jp, kp, ndumps):
     return out
 
 
-def _run(exe, payload, cwd=None):
+def _run(exe, payload, cwd=None, want_stdout=False):
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
             f.write(payload)
-        subprocess.run([exe, fin, fout], check=True, cwd=cwd, stdout=subprocess.DEVNULL)
+        r = subprocess.run([exe, fin, fout], check=True, cwd=cwd, stdout=subprocess.PIPE, text=True)
         with open(fout, "rb") as f:
-            return f.read()
+            raw = f.read()
+        return (raw, r.stdout) if want_stdout else raw
+
+
+def timed_steps(exe, case, phys, nsteps):
+    """Runs init + nsteps steps of the reference without dumps; returns the seconds the
+    reference's time loop took (system_clock around the loop, ref_harness.f95)."""
+    ip, jp, kp = case["dims"]
+    payload = _header(0, ip, jp, kp, phys, case) + _i32(nsteps, 0)
+    _, out = _run(exe, payload, want_stdout=True)
+    for line in out.splitlines():
+        if "steps_seconds=" in line:
+            return float(line.split("=")[1])
+    raise RuntimeError("no timing in reference output: %r" % out[-500:])
 
 
 def trajectory(case, phys, nsteps, dump_steps):
