@@ -33,6 +33,12 @@ struct les_handle {
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
   float* scratch_f = nullptr;  // single-float device scratch
+  // colour-split SOR copy and the 1-D factors of cn1
+  SplitSOR sp{};
+  float* split_mem = nullptr;
+  std::vector<float> hcn1;     // host copy of cn1 (to know whether it is the separable form)
+  bool cn1_separable = false;  // cn1 == 1/((ax(i)+by(j))+cz(k)) bit for bit
+  bool p_consistent = true;    // every halo of p equals its mirror interior cell
 };
 
 namespace {
@@ -194,6 +200,40 @@ int upload_scalar_area(les_handle* h) {
   return LES_OK;
 }
 
+// The 1-D factors of cn1 (init.f95:98-103) for the split SOR, and whether the stored cn1 is
+// exactly 1/((ax(i)+by(j))+cz(k)) (it is whenever it came from init.f95's formula).
+int refresh_split_factors(les_handle* h) {
+  const int ip = h->d.ip, jp = h->d.jp, kp = h->d.kp;
+  const auto& dxs = h->hgrid[LES_DXS];
+  const auto& dys = h->hgrid[LES_DYS];
+  const auto& dzs = h->hgrid[LES_DZS];
+  std::vector<float> ax(ip + 2, 0.0f), by(jp + 2, 0.0f), cz(kp + 2, 0.0f);
+  for (int i = 1; i <= ip; ++i) ax[i] = 2.0f / (dxs[i - 1] * dxs[i]);
+  for (int j = 1; j <= jp; ++j) by[j] = 2.0f / (dys[j - 1] * dys[j]);
+  for (int k = 1; k <= kp; ++k) cz[k] = 2.0f / (dzs[k] * dzs[k + 1]);  // dzs(k-1)*dzs(k)
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(const_cast<float*>(h->sp.ax), ax.data(), ax.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(const_cast<float*>(h->sp.by), by.data(), by.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(const_cast<float*>(h->sp.cz), cz.data(), cz.size() * sizeof(float), hipMemcpyHostToDevice));
+  bool sep = true;
+  for (int k = 1; k <= kp && sep; ++k)
+    for (int j = 1; j <= jp && sep; ++j)
+      for (int i = 1; i <= ip; ++i) {
+        const float c = 1.0f / (ax[i] + by[j] + cz[k]);
+        const float s = h->hcn1[(size_t)(i - 1) + (size_t)ip * ((j - 1) + (size_t)jp * (k - 1))];
+        uint32_t a, b;
+        std::memcpy(&a, &c, 4);
+        std::memcpy(&b, &s, 4);
+        if (a != b) {
+          sep = false;
+          break;
+        }
+      }
+  h->cn1_separable = sep;
+  h->graph_valid = false;
+  return LES_OK;
+}
+
 int check_launch(les_handle* h) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -282,9 +322,29 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   if (hipMalloc(&h->sc, sizeof(DevScalars)) != hipSuccess) return bail(LES_E_NOMEM);
   if (hipMemsetAsync(h->sc, 0, sizeof(DevScalars), h->st) != hipSuccess) return bail(LES_E_HIP);
   h->rb.max_blocks = std::max(std::max(reduction_blocks(h->d), 2 * sor_resid_blocks(h->d)), 4096);
+  h->rb.max_blocks = std::max(h->rb.max_blocks, 2 * sor_split_blocks(h->d));
   if (hipMalloc(&h->rb.partials, (size_t)h->rb.max_blocks * NLIMB * sizeof(int64_t)) != hipSuccess)
     return bail(LES_E_NOMEM);
   if (hipMalloc(&h->scratch_f, 256) != hipSuccess) return bail(LES_E_NOMEM);
+  {  // split SOR copy: 2 colours x (p, rhs), plus the cn1 factors ax(0:ip+1), by(0:jp+1), cz(0:kp+1)
+    SplitSOR& sp = h->sp;
+    sp.nh = (cfg->ip + 1) / 2 + 1;
+    sp.nj = cfg->jp + 2;
+    sp.nk = cfg->kp + 2;
+    const size_t ns = (size_t)sp.nh * sp.nj * sp.nk;
+    const size_t nf = 4 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + (cfg->kp + 2);
+    if (hipMalloc(&h->split_mem, nf * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
+    if (hipMemsetAsync(h->split_mem, 0, nf * sizeof(float), h->st) != hipSuccess) return bail(LES_E_HIP);
+    sp.p[0] = h->split_mem;
+    sp.p[1] = sp.p[0] + ns;
+    sp.r[0] = sp.p[1] + ns;
+    sp.r[1] = sp.r[0] + ns;
+    float* fac = sp.r[1] + ns;
+    sp.ax = fac;
+    sp.by = fac + (cfg->ip + 2);
+    sp.cz = fac + (cfg->ip + 2) + (cfg->jp + 2);
+    h->hcn1.assign((size_t)cfg->ip * cfg->jp * cfg->kp, 0.0f);
+  }
   if (hipStreamSynchronize(h->st) != hipSuccess) return bail(LES_E_HIP);
   *out = h;
   return LES_OK;
@@ -301,6 +361,7 @@ extern "C" int les_destroy(les_handle* h) {
   if (h->sc) hipFree(h->sc);
   if (h->rb.partials) hipFree(h->rb.partials);
   if (h->scratch_f) hipFree(h->scratch_f);
+  if (h->split_mem) hipFree(h->split_mem);
   if (h->st) hipStreamDestroy(h->st);
   if (h == g_default) g_default = nullptr;
   delete h;
@@ -340,6 +401,12 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
   }
   if (field == LES_DX1 || field == LES_DY1 || field == LES_DZN) {
     int r = upload_scalar_area(h);
+    if (r) return r;
+  }
+  if (field == LES_P) h->p_consistent = false;
+  if (field == LES_CN1) std::memcpy(h->hcn1.data(), host, n * sizeof(float));
+  if (field == LES_CN1 || field == LES_DXS || field == LES_DYS || field == LES_DZS) {
+    int r = refresh_split_factors(h);
     if (r) return r;
   }
   return LES_OK;
@@ -535,11 +602,20 @@ extern "C" int les_press_sor(les_handle* h, int nrd) {
     launch_sor_colour(h->st, d, g, F, h->ph, nrd, 0, h->rb, 0, h->sc);
     launch_boundp1(h->st, d, F, h->sc, 0);
   }
+  h->p_consistent = false;  // a lone colour or boundp2 leaves an intermediate halo state
   return check_launch(h);
 }
 
 namespace {
-int enqueue_sor_solve(les_handle* h) {
+// The colour-split solve (les_sor.hip) applies when its equivalence to the reference's
+// in-place sweep holds: even jp, mirror-consistent p halos, cn1 of the separable form.
+bool use_split(const les_handle* h) {
+  return (h->d.jp % 2 == 0) && h->p_consistent && h->cn1_separable;
+}
+
+// press.f95:87-115 on the Fortran arrays: colour sweep, boundp1 per colour, boundp2 per
+// iteration (exact for any input state)
+int enqueue_sor_inplace(les_handle* h) {
   STAGE_BEGIN(h);
   launch_sor_reset(h->st, h->sc);
   const bool resid = h->ph.pjuge > 0.0f;
@@ -555,14 +631,51 @@ int enqueue_sor_solve(les_handle* h) {
   return check_launch(h);
 }
 
+// the same iterations on the colour-split copy (already scattered)
+int enqueue_sor_split_loop(les_handle* h) {
+  STAGE_BEGIN(h);
+  launch_sor_reset(h->st, h->sc);
+  const bool resid = h->ph.pjuge > 0.0f;
+  const int nb = sor_split_blocks(d);
+  for (int l = 1; l <= h->cfg.nmaxp; ++l) {
+    for (int nrd = 0; nrd <= 1; ++nrd)
+      launch_sor_split(h->st, d, g, h->sp, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
+    if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, h->ph, h->sc);
+  }
+  return check_launch(h);
+}
+
+int enqueue_sor_solve(les_handle* h) {
+  STAGE_BEGIN(h);
+  int r;
+  if (use_split(h)) {
+    launch_split_scatter(h->st, d, F, h->sp, h->sc, 0);
+    if ((r = enqueue_sor_split_loop(h))) return r;
+    launch_split_gather_adj(h->st, d, F, h->sp, h->sc, 0);
+  } else if ((r = enqueue_sor_inplace(h))) {
+    return r;
+  }
+  if (h->cfg.nmaxp > 0) h->p_consistent = true;
+  return check_launch(h);
+}
+
+// press.f95:49-146
 int enqueue_press(les_handle* h) {
   STAGE_BEGIN(h);
+  int r;
   launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
-  launch_rhs_adjust(h->st, d, F, h->sc);
-  int r = enqueue_sor_solve(h);
-  if (r) return r;
-  launch_pav(h->st, d, g, F, h->rb, h->sc);
-  launch_adj(h->st, d, F, h->sc, 1);
+  if (use_split(h)) {
+    launch_split_scatter(h->st, d, F, h->sp, h->sc, 1);  // rhs -= rhsav on the way
+    if ((r = enqueue_sor_split_loop(h))) return r;
+    launch_pav_split(h->st, d, g, h->sp, h->rb, h->sc);
+    launch_split_gather_adj(h->st, d, F, h->sp, h->sc, 1);  // p - pav, halos included
+  } else {
+    launch_rhs_adjust(h->st, d, F, h->sc);
+    if ((r = enqueue_sor_inplace(h))) return r;
+    launch_pav(h->st, d, g, F, h->rb, h->sc);
+    launch_adj(h->st, d, F, h->sc, 1);
+  }
+  h->p_consistent = true;
   return check_launch(h);
 }
 
@@ -593,6 +706,7 @@ extern "C" int les_press_pav(les_handle* h) {
 extern "C" int les_press_adj(les_handle* h) {
   STAGE_BEGIN(h);
   launch_adj(h->st, d, F, h->sc, 0);
+  h->p_consistent = false;  // interior shifted, halos not yet
   return check_launch(h);
 }
 
@@ -600,6 +714,7 @@ extern "C" int les_press_boundp(les_handle* h) {
   STAGE_BEGIN(h);
   launch_boundp1(h->st, d, F, h->sc, 0);
   launch_boundp2(h->st, d, F, h->sc, 0);
+  h->p_consistent = true;
   return check_launch(h);
 }
 
@@ -618,6 +733,14 @@ extern "C" int les_run(les_handle* h, int n0, int nsteps) {
   int n = n0;
   // the n == 1 step differs (interior fill): run it eagerly
   while (nsteps > 0 && n <= 1) {
+    int r = enqueue_step(h, n);
+    if (r) return r;
+    ++n;
+    --nsteps;
+  }
+  // a step from a state with arbitrary p halos takes the in-place solve once; every step leaves
+  // the halos mirror-consistent, the state the captured graph assumes
+  while (nsteps > 0 && !h->p_consistent) {
     int r = enqueue_step(h, n);
     if (r) return r;
     ++n;
@@ -653,6 +776,7 @@ extern "C" int les_ifdata(les_handle* h) {
   launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
   launch_boundp1(h->st, d, F, h->sc, 0);
   launch_boundp2(h->st, d, F, h->sc, 0);
+  h->p_consistent = true;
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
   launch_calc_sm(h->st, d, g, F, h->ph);
   launch_boundsm(h->st, d, F);
@@ -765,7 +889,10 @@ extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms
   for (int q = 0; q < reps; ++q) {
     switch (which) {
       case LES_KERNEL_SOR_SWEEP:
-        launch_sor_colour(h->st, d, g, F, h->ph, q & 1, 0, h->rb, 0, h->sc);
+        if (use_split(h))
+          launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, 0, h->sc);
+        else
+          launch_sor_colour(h->st, d, g, F, h->ph, q & 1, 0, h->rb, 0, h->sc);
         break;
       case LES_KERNEL_VELFG: launch_velfg(h->st, d, g, F, h->ph, 1, 1); break;
       case LES_KERNEL_VISC_ADAM: launch_calc_visc_adam(h->st, d, g, F, 1, 1); break;

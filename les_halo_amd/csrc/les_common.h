@@ -243,4 +243,48 @@ __device__ inline double limbs_to_double(const int64_t* in) {
   return neg ? -r : r;
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Block-level pieces of the exact reductions: every block accumulates its limbs in LDS and
+// writes them to partials[block]; a one-workgroup finaliser adds all partials (integers,
+// exact) and rounds once.
+// ------------------------------------------------------------------------------------------
+constexpr int RED_KC = 6;  // k-chunks of the reduction grids
+
+__device__ inline void block_limbs_init(unsigned long long* lds) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid < NLIMB) lds[tid] = 0ull;
+  __syncthreads();
+}
+
+__device__ inline void block_limbs_store(unsigned long long* lds, int64_t* out) {
+  __syncthreads();
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid < NLIMB) out[tid] = (int64_t)lds[tid];
+}
+
+// Sum partials[0..nb) into limbs_out (one workgroup).
+__device__ inline void sum_partials(const int64_t* partials, int nb, int64_t* limbs_out) {
+  __shared__ unsigned long long acc[NLIMB];
+  if (threadIdx.x < NLIMB) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  int64_t r[NLIMB];
+  for (int q = 0; q < NLIMB; ++q) r[q] = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int q = 0; q < NLIMB; ++q) r[q] += partials[(long)b * NLIMB + q];
+  for (int q = 0; q < NLIMB; ++q)
+    if (r[q]) atomicAdd(&acc[q], (unsigned long long)r[q]);
+  __syncthreads();
+  if (threadIdx.x < NLIMB) limbs_out[threadIdx.x] = (int64_t)acc[threadIdx.x];
+  __syncthreads();
+}
+
+// Workgroups are handed to the 8 XCDs (each with its own L2) round-robin.  With a grid padded
+// to a multiple of 8, this maps dispatch id w to a logical tile so that every XCD works on one
+// contiguous range of tiles: spatial neighbours then share an L2.
+__device__ __forceinline__ int xcd_tile(int w, int nwg_padded) {
+  const int per = nwg_padded >> 3;
+  return (w & 7) * per + (w >> 3);
+}
+
 }  // namespace les

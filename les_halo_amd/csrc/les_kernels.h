@@ -26,6 +26,20 @@ struct Physics {
   float dt, ro, vn, alpha, beta, cs0, omega, pjuge;
 };
 
+// Red/black split copy of p and rhs used inside the SOR solve.  Cell (i,j,k) of the box
+// [0,ip+1]x[0,jp+1]x[0,kp+1] lives in array (i+j+k)&1 at (i>>1, j, k); within one (j,k) row
+// every cell of one colour is contiguous, so the sweeps read and write whole cache lines.
+// cfac = 1-D factors of cn1: cn1(i,j,k) = 1/((ax(i)+by(j))+cz(k)) (init.f95:98-104).
+struct SplitSOR {
+  float* p[2];
+  float* r[2];
+  const float *ax, *by, *cz;
+  int nh, nj, nk;  // (ip+1)/2+1, jp+2, kp+2
+  __host__ __device__ long S(int ii, int j, int k) const {
+    return (long)ii + (long)nh * ((long)j + (long)nj * (long)k);
+  }
+};
+
 // Reduction scratch: per-block carry-save limbs.
 struct RedBuf {
   int64_t* partials;  // [max_blocks][NLIMB]
@@ -58,6 +72,16 @@ void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalar
 void launch_adj(hipStream_t st, Dims d, Fields F, DevScalars* sc, int with_boundp);
 void launch_boundp_mirror(hipStream_t st, Dims d, Fields F);
 int sor_resid_blocks(Dims d);
+
+// split-layout SOR (jp even; p halos mirror-consistent on entry)
+void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
+                          int adjust_rhs);
+void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
+                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc);
+int sor_split_blocks(Dims d);
+void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc);
+void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
+                             int subtract_pav);
 int reduction_blocks(Dims d);
 
 // exact sum of n device floats into *out (device), for tests of the reduction itself

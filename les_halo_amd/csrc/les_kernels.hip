@@ -621,39 +621,6 @@ void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_vi
 // Reductions: blocks accumulate exact limbs in LDS and write them to partials[block]; one
 // finalisation workgroup adds all partials (integer, exact) and rounds once.
 // ==========================================================================================
-namespace {
-
-constexpr int RED_KC = 6;  // k-chunks of the reduction grids
-
-__device__ void block_limbs_init(unsigned long long* lds) {
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  if (tid < NLIMB) lds[tid] = 0ull;
-  __syncthreads();
-}
-
-__device__ void block_limbs_store(unsigned long long* lds, int64_t* out) {
-  __syncthreads();
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  if (tid < NLIMB) out[tid] = (int64_t)lds[tid];
-}
-
-// Sum partials[0..nb) into limbs (one workgroup of 256 threads).
-__device__ void sum_partials(const int64_t* partials, int nb, int64_t* limbs_out) {
-  __shared__ unsigned long long acc[NLIMB];
-  if (threadIdx.x < NLIMB) acc[threadIdx.x] = 0ull;
-  __syncthreads();
-  int64_t r[NLIMB];
-  for (int q = 0; q < NLIMB; ++q) r[q] = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    for (int q = 0; q < NLIMB; ++q) r[q] += partials[(long)b * NLIMB + q];
-  for (int q = 0; q < NLIMB; ++q)
-    if (r[q]) atomicAdd(&acc[q], (unsigned long long)r[q]);
-  __syncthreads();
-  if (threadIdx.x < NLIMB) limbs_out[threadIdx.x] = (int64_t)acc[threadIdx.x];
-  __syncthreads();
-}
-
-}  // namespace
 
 int reduction_blocks(Dims d) {
   return ((d.ip + BX - 1) / BX) * ((d.jp + BY - 1) / BY) * RED_KC;

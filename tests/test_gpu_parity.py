@@ -218,3 +218,62 @@ def test_full_size_one_step():
     o.step(1)
     _compare(L, o, "300x300x90 step 1", ["u", "v", "w", "p", "f", "g", "h", "sm", "rhs", "usum",
                                          "fold", "diu1", "diu5", "diu9"])
+
+
+def _consistent_pair(case, seed, **sor):
+    """Random state whose p halos are mirror-consistent (as every press leaves them)."""
+    L, o = _pair(case, seed=seed, **sor)
+    L.press_boundp()
+    o.call("boundp1")
+    o.call("boundp2")
+    return L, o
+
+
+@pytest.mark.parametrize("sor", ["ref", "cfg"])
+def test_split_press_bitwise(case, sor):
+    """press on the colour-split solve (consistent halos, even jp, separable cn1)."""
+    sorp = common.REF_SOR if sor == "ref" else common.CFG_SOR
+    L, o = _consistent_pair(case, 77, **sorp)
+    L.press()
+    o.call("press")
+    _compare(L, o, "split press")
+    s = L.scalars()
+    assert np.float32(s["pav"]) == np.float32(o.scalars.pav)
+    assert np.float32(s["rhsav"]) == np.float32(o.scalars.rhsav)
+
+
+def test_split_sor_solve_bitwise(case):
+    L, o = _consistent_pair(case, 78, **common.CFG_SOR)
+    L.press_sor_solve()
+    o.call("press_sor")
+    _compare(L, o, "split sor solve")
+
+
+def test_nonseparable_cn1_falls_back(case):
+    """A cn1 that is not 1/((ax+by)+cz) runs the in-place solve and stays exact."""
+    L, o = _consistent_pair(case, 79, **common.CFG_SOR)
+    cn1 = o.a["cn1"].copy()
+    cn1[5] *= np.float32(1.25)
+    L.upload("cn1", cn1)
+    o.set("cn1", cn1)
+    L.press()
+    o.call("press")
+    _compare(L, o, "non-separable cn1 press")
+
+
+def test_run_after_inconsistent_upload(case):
+    """les_run from arbitrary p halos: first step in place, then the captured graph."""
+    L, o = _pair(case, **common.CFG_SOR)
+    L.ifdata()
+    o.ifdata()
+    for n in range(1, 3):
+        L.step(n)
+        o.step(n)
+    p = o.a["p"].copy()
+    p[:7] = np.float32(3.5)  # break some halo mirrors
+    L.upload("p", p)
+    o.set("p", p)
+    L.run(3, 4)
+    for n in range(3, 7):
+        o.step(n)
+    _compare(L, o, "run after inconsistent upload")
