@@ -106,7 +106,7 @@ def main():
     achieved = bytes_launch / (sweep_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(), "kernel": "k_sor_colour",
+                "traffic": traffic_from_profiles(), "kernel": "k_sor_split",
                 "bytes_per_launch": bytes_launch, "avg_us": round(sweep_ms * 1e3, 3)}
     step_hbm_frac = B_STEP * cells / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 

@@ -230,6 +230,23 @@ int refresh_split_factors(les_handle* h) {
         }
       }
   h->cn1_separable = sep;
+  // uniform horizontal grid: every ax, by, cn2l, cn2s, cn3l, cn3s equal (bitwise); the sweep
+  // then takes them as scalars and cn1 from a per-k table, the same values bit for bit
+  auto all_equal = [](const float* a, int n) {
+    for (int q = 1; q < n; ++q)
+      if (std::memcmp(&a[q], &a[0], 4) != 0) return false;
+    return true;
+  };
+  const bool uni = all_equal(ax.data() + 1, ip) && all_equal(by.data() + 1, jp) &&
+                   all_equal(h->hgrid[LES_CN2L].data(), ip) &&
+                   all_equal(h->hgrid[LES_CN2S].data(), ip) &&
+                   all_equal(h->hgrid[LES_CN3L].data(), jp) &&
+                   all_equal(h->hgrid[LES_CN3S].data(), jp);
+  std::vector<float> cn1k(kp + 2, 0.0f);
+  for (int k = 1; k <= kp; ++k) cn1k[k] = 1.0f / (ax[1] + by[1] + cz[k]);
+  HIPCHK(h, hipMemcpy(const_cast<float*>(h->sp.cn1k), cn1k.data(), cn1k.size() * sizeof(float),
+                      hipMemcpyHostToDevice));
+  h->sp.uniform = uni ? 1 : 0;
   h->graph_valid = false;
   return LES_OK;
 }
@@ -326,13 +343,13 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   if (hipMalloc(&h->rb.partials, (size_t)h->rb.max_blocks * NLIMB * sizeof(int64_t)) != hipSuccess)
     return bail(LES_E_NOMEM);
   if (hipMalloc(&h->scratch_f, 256) != hipSuccess) return bail(LES_E_NOMEM);
-  {  // split SOR copy: 2 colours x (p, rhs), plus the cn1 factors ax(0:ip+1), by(0:jp+1), cz(0:kp+1)
+  {  // split SOR copy: 2 colours x (p, rhs) of interior rows padded to nr floats, plus the
+     // cn1 factors ax(0:ip+1), by(0:jp+1), cz(0:kp+1) and the table cn1k(0:kp+1)
     SplitSOR& sp = h->sp;
-    sp.nh = (cfg->ip + 1) / 2 + 1;
-    sp.nj = cfg->jp + 2;
-    sp.nk = cfg->kp + 2;
-    const size_t ns = (size_t)sp.nh * sp.nj * sp.nk;
-    const size_t nf = 4 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + (cfg->kp + 2);
+    sp.nr = ((cfg->ip + 1) / 2 + 3) / 4 * 4;
+    sp.nj = cfg->jp;
+    const size_t ns = (size_t)sp.nr * cfg->jp * cfg->kp;
+    const size_t nf = 4 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + 2 * (cfg->kp + 2);
     if (hipMalloc(&h->split_mem, nf * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
     if (hipMemsetAsync(h->split_mem, 0, nf * sizeof(float), h->st) != hipSuccess) return bail(LES_E_HIP);
     sp.p[0] = h->split_mem;
@@ -343,6 +360,8 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     sp.ax = fac;
     sp.by = fac + (cfg->ip + 2);
     sp.cz = fac + (cfg->ip + 2) + (cfg->jp + 2);
+    sp.cn1k = fac + (cfg->ip + 2) + (cfg->jp + 2) + (cfg->kp + 2);
+    sp.uniform = 0;
     h->hcn1.assign((size_t)cfg->ip * cfg->jp * cfg->kp, 0.0f);
   }
   if (hipStreamSynchronize(h->st) != hipSuccess) return bail(LES_E_HIP);
@@ -405,7 +424,8 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
   }
   if (field == LES_P) h->p_consistent = false;
   if (field == LES_CN1) std::memcpy(h->hcn1.data(), host, n * sizeof(float));
-  if (field == LES_CN1 || field == LES_DXS || field == LES_DYS || field == LES_DZS) {
+  if (field == LES_CN1 || field == LES_DXS || field == LES_DYS || field == LES_DZS ||
+      field == LES_CN2L || field == LES_CN2S || field == LES_CN3L || field == LES_CN3S) {
     int r = refresh_split_factors(h);
     if (r) return r;
   }
@@ -501,7 +521,7 @@ extern "C" int les_velnw__bondv1_init_uvw(les_handle* h, int n) {
 
 extern "C" int les_bondv1_calc_uout(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_uout(h->st, d, F, h->sc);
+  launch_uout(h->st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
   return check_launch(h);
 }
 
@@ -682,7 +702,7 @@ int enqueue_press(les_handle* h) {
 int enqueue_step(les_handle* h, int n) {
   STAGE_BEGIN(h);
   launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n);
-  launch_uout(h->st, d, F, h->sc);
+  launch_uout(h->st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
   launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
   launch_calc_sm(h->st, d, g, F, h->ph);
@@ -772,7 +792,7 @@ extern "C" int les_run(les_handle* h, int n0, int nsteps) {
 extern "C" int les_ifdata(les_handle* h) {
   STAGE_BEGIN(h);
   launch_velnw_inflow(h->st, d, g, F, h->ph, 0, 1);
-  launch_uout(h->st, d, F, h->sc);
+  launch_uout(h->st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
   launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
   launch_boundp1(h->st, d, F, h->sc, 0);
   launch_boundp2(h->st, d, F, h->sc, 0);

@@ -19,25 +19,24 @@ struct DevScalars {
   int sor_iters, sor_done;
   double resid_l2;
   int refine_passes;
-  int pad;
+  unsigned int uout_count;  // workgroups of k_uout done (re-armed to 0 by the last one)
 };
 
 struct Physics {
   float dt, ro, vn, alpha, beta, cs0, omega, pjuge;
 };
 
-// Red/black split copy of p and rhs used inside the SOR solve.  Cell (i,j,k) of the box
-// [0,ip+1]x[0,jp+1]x[0,kp+1] lives in array (i+j+k)&1 at (i>>1, j, k); within one (j,k) row
-// every cell of one colour is contiguous, so the sweeps read and write whole cache lines.
-// cfac = 1-D factors of cn1: cn1(i,j,k) = 1/((ax(i)+by(j))+cz(k)) (init.f95:98-104).
+// Red/black split copy of p and rhs used inside the SOR solve: interior cell (i,j,k) lives in
+// array (i+j+k)&1 at ((i-1)>>1) + nr*((j-1) + nj*(k-1)); each (j,k) row of one colour is
+// contiguous from element 0, rows padded to nr (a multiple of 4) floats.
+// cn1(i,j,k) = 1/((ax(i)+by(j))+cz(k)) (init.f95:98-104); on a uniform horizontal grid
+// (uniform != 0) the sweep uses cn2l(1), cn2s(1), cn3l(1), cn3s(1) and cn1k(k) instead.
 struct SplitSOR {
   float* p[2];
   float* r[2];
-  const float *ax, *by, *cz;
-  int nh, nj, nk;  // (ip+1)/2+1, jp+2, kp+2
-  __host__ __device__ long S(int ii, int j, int k) const {
-    return (long)ii + (long)nh * ((long)j + (long)nj * (long)k);
-  }
+  const float *ax, *by, *cz, *cn1k;
+  int nr, nj;  // padded colour row length, jp
+  int uniform;
 };
 
 // Reduction scratch: per-block carry-save limbs.
@@ -49,7 +48,7 @@ struct RedBuf {
 // ---- launchers (les_kernels.hip); all asynchronous on 'st' ----
 void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
                          int inflow_n);
-void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc);
+void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc, float* part);
 void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc);
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf);
@@ -70,7 +69,6 @@ void launch_boundp2(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_
 void launch_sor_reset(hipStream_t st, DevScalars* sc);
 void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalars* sc);
 void launch_adj(hipStream_t st, Dims d, Fields F, DevScalars* sc, int with_boundp);
-void launch_boundp_mirror(hipStream_t st, Dims d, Fields F);
 int sor_resid_blocks(Dims d);
 
 // split-layout SOR (jp even; p halos mirror-consistent on entry)

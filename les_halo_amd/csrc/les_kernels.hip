@@ -6,6 +6,7 @@
 // that owns their source, or by a dedicated boundary kernel whose every read is of a cell that
 // the kernel never writes; no kernel relies on inter-workgroup ordering.
 #include "les_kernels.h"
+#include "les_vel.h"
 
 namespace les {
 
@@ -65,18 +66,12 @@ void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph,
 
 // ==========================================================================================
 // bondv1.f95:67-75: uout = (max(0, max u(im,:,:)) + min(0, min u(im,:,:))) / 2.
-// max/min are order-independent, so one workgroup of 1024 threads suffices.
+// One workgroup per level k; max/min are order-independent, so the last workgroup to finish
+// folds the per-level partials (seeded with 0 as amax1/amin1 are) into the exact result and
+// re-arms the counter for the next launch.
 // ==========================================================================================
-__global__ __launch_bounds__(1024) void k_uout(Dims d, Fields F, DevScalars* sc) {
-  __shared__ float smax[16], smin[16];
-  float mx = 0.0f, mn = 0.0f;
-  const int n = d.jp * d.kp;
-  for (int t = threadIdx.x; t < n; t += 1024) {
-    int j = 1 + t % d.jp, k = 1 + t / d.jp;
-    float x = F.u[d.U(d.ip, j, k)];
-    mx = fmaxf(mx, x);
-    mn = fminf(mn, x);
-  }
+namespace {
+__device__ __forceinline__ void block_maxmin(float& mx, float& mn, float* smax, float* smin) {
   for (int o = 32; o > 0; o >>= 1) {
     mx = fmaxf(mx, __shfl_xor(mx, o));
     mn = fminf(mn, __shfl_xor(mn, o));
@@ -87,18 +82,52 @@ __global__ __launch_bounds__(1024) void k_uout(Dims d, Fields F, DevScalars* sc)
     smin[wv] = mn;
   }
   __syncthreads();
+  mx = smax[0];
+  mn = smin[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+    mx = fmaxf(mx, smax[q]);
+    mn = fminf(mn, smin[q]);
+  }
+  __syncthreads();
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_uout(Dims d, Fields F, DevScalars* sc, float* part) {
+  __shared__ float smax[4], smin[4];
+  __shared__ int last;
+  const int k = 1 + blockIdx.x;
+  float mx = 0.0f, mn = 0.0f;
+  for (int j = 1 + threadIdx.x; j <= d.jp; j += 256) {
+    const float x = F.u[d.U(d.ip, j, k)];
+    mx = fmaxf(mx, x);
+    mn = fminf(mn, x);
+  }
+  block_maxmin(mx, mn, smax, smin);
   if (threadIdx.x == 0) {
-    float a = smax[0], b = smin[0];
-    for (int q = 1; q < 16; ++q) {
-      a = fmaxf(a, smax[q]);
-      b = fminf(b, smin[q]);
-    }
-    sc->uout = (a + b) / 2.0f;
+    part[2 * blockIdx.x] = mx;
+    part[2 * blockIdx.x + 1] = mn;
+    __threadfence();
+    const unsigned prev = atomicAdd(&sc->uout_count, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  mx = 0.0f;
+  mn = 0.0f;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
+    mx = fmaxf(mx, __hip_atomic_load(&part[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    mn = fminf(mn, __hip_atomic_load(&part[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  block_maxmin(mx, mn, smax, smin);
+  if (threadIdx.x == 0) {
+    sc->uout = (mx + mn) / 2.0f;
+    sc->uout_count = 0u;
   }
 }
 
-void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc) {
-  hipLaunchKernelGGL(k_uout, dim3(1), dim3(1024), 0, st, d, F, sc);
+void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc, float* part) {
+  hipLaunchKernelGGL(k_uout, dim3(d.kp), dim3(256), 0, st, d, F, sc, part);
 }
 
 // ==========================================================================================
@@ -176,84 +205,6 @@ void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, D
 // ==========================================================================================
 // vel2.f95:48-123 formulas at one point; velFG.f95:59-105; feedbf.f95:38-67.
 // ==========================================================================================
-namespace {
-
-struct VelCtx {
-  Dims d;
-  Grid1D g;
-  const float *u, *v, *w;
-  __device__ float U(int i, int j, int k) const { return u[d.U(i, j, k)]; }
-  __device__ float V(int i, int j, int k) const { return v[d.U(i, j, k)]; }
-  __device__ float W(int i, int j, int k) const { return w[d.W(i, j, k)]; }
-  __device__ float dx1(int i) const { return g.DX1(i); }
-  __device__ float dy1(int j) const { return g.DY1(j); }
-  __device__ float dzn(int k) const { return g.DZN(k); }
-  __device__ float dzs(int k) const { return g.DZS(k); }
-
-  // vel2.f95:51-60
-  __device__ float diu1(int i, int j, int k) const { return (-U(i - 1, j, k) + U(i, j, k)) / dx1(i); }
-  __device__ float cov1(int i, int j, int k) const {
-    return ((U(i - 1, j, k) + U(i, j, k)) / 2.0f) * diu1(i, j, k);
-  }
-  __device__ float diu5(int i, int j, int k) const { return (-V(i, j - 1, k) + V(i, j, k)) / dy1(j); }
-  __device__ float cov5(int i, int j, int k) const {
-    return ((V(i, j - 1, k) + V(i, j, k)) / 2.0f) * diu5(i, j, k);
-  }
-  __device__ float diu9(int i, int j, int k) const { return (-W(i, j, k - 1) + W(i, j, k)) / dzn(k); }
-  __device__ float cov9(int i, int j, int k) const {
-    return ((W(i, j, k - 1) + W(i, j, k)) / 2.0f) * diu9(i, j, k);
-  }
-  // vel2.f95:68-70
-  __device__ float diu2(int i, int j, int k) const {
-    return 2.0f * (-U(i, j - 1, k) + U(i, j, k)) / (dy1(j - 1) + dy1(j));
-  }
-  __device__ float cov2(int i, int j, int k) const {
-    float nou = (dx1(i + 1) * V(i, j - 1, k) + dx1(i) * V(i + 1, j - 1, k)) / (dx1(i) + dx1(i + 1));
-    return nou * diu2(i, j, k);
-  }
-  // vel2.f95:78-80
-  __device__ float diu3(int i, int j, int k) const {
-    return (-U(i, j, k - 1) + U(i, j, k)) / dzs(k - 1);
-  }
-  __device__ float cov3(int i, int j, int k) const {
-    float nou = (dx1(i + 1) * W(i, j, k - 1) + dx1(i) * W(i + 1, j, k - 1)) / (dx1(i) + dx1(i + 1));
-    return nou * diu3(i, j, k);
-  }
-  // vel2.f95:88-90 (the "- u0" with u0 = 0 is exact)
-  __device__ float diu4(int i, int j, int k) const {
-    return 2.0f * (-V(i - 1, j, k) + V(i, j, k)) / (dx1(i - 1) + dx1(i));
-  }
-  __device__ float cov4(int i, int j, int k) const {
-    float nou = (dy1(j + 1) * U(i - 1, j, k) + dy1(j) * U(i - 1, j + 1, k)) / (dy1(j) + dy1(j + 1));
-    return nou * diu4(i, j, k);
-  }
-  // vel2.f95:98-100
-  __device__ float diu6(int i, int j, int k) const {
-    return (-V(i, j, k - 1) + V(i, j, k)) / dzs(k - 1);
-  }
-  __device__ float cov6(int i, int j, int k) const {
-    float nou = (dy1(j + 1) * W(i, j, k - 1) + dy1(j) * W(i, j + 1, k - 1)) / (dy1(j) + dy1(j + 1));
-    return nou * diu6(i, j, k);
-  }
-  // vel2.f95:108-110
-  __device__ float diu7(int i, int j, int k) const {
-    return 2.0f * (-W(i - 1, j, k) + W(i, j, k)) / (dx1(i - 1) + dx1(i));
-  }
-  __device__ float cov7(int i, int j, int k) const {
-    float nou = (dzn(k + 1) * U(i - 1, j, k) + dzn(k) * U(i - 1, j, k + 1)) / (dzn(k) + dzn(k + 1));
-    return nou * diu7(i, j, k);
-  }
-  // vel2.f95:118-120
-  __device__ float diu8(int i, int j, int k) const {
-    return 2.0f * (-W(i, j - 1, k) + W(i, j, k)) / (dy1(j - 1) + dy1(j));
-  }
-  __device__ float cov8(int i, int j, int k) const {
-    float nou = (dzn(k + 1) * V(i, j - 1, k) + dzn(k) * V(i, j - 1, k + 1)) / (dzn(k) + dzn(k + 1));
-    return nou * diu8(i, j, k);
-  }
-};
-
-}  // namespace
 
 // One thread per interior cell.  Neighbour values of cov/diu at i+1 = im+1 use the vel2 halo
 // copy (value at im), at j+1 = jm+1 the periodic copy (value at j = 1); k+1 values are
@@ -461,14 +412,29 @@ void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
 // ==========================================================================================
 __global__ __launch_bounds__(256) void k_boundsm(Dims d, Fields F) {
   const int im = d.ip, jm = d.jp, km = d.kp;
-  const int na = im + 2, nb = jm + 3, nc = km + 2;  // a = 0..im+1, b = -1..jm+1, c = 0..km+1
+  // the written cells (a = 0..im+1, b = -1..jm+1, c = 0..km+1 with a, b or c on a face) as
+  // three disjoint sets: the k planes, the j faces between them, the i faces of the rest
+  const long n1 = 2L * (im + 2) * (jm + 3);
+  const long n2 = 2L * (im + 2) * km;
+  const long n3 = 2L * (jm + 1) * km;
   long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)na * nb * nc) return;
-  const int a = (int)(t % na);
-  const int b = -1 + (int)((t / na) % nb);
-  const int c = (int)(t / ((long)na * nb));
-  const bool ah = (a == 0 || a == im + 1), bh = (b == 0 || b == jm + 1), ch = (c == 0 || c == km + 1);
-  if (!(ah || bh || ch)) return;
+  int a, b, c;
+  if (t < n1) {
+    a = (int)(t % (im + 2));
+    b = -1 + (int)((t / (im + 2)) % (jm + 3));
+    c = (t / ((long)(im + 2) * (jm + 3))) ? km + 1 : 0;
+  } else if ((t -= n1) < n2) {
+    a = (int)(t % (im + 2));
+    c = 1 + (int)((t / (im + 2)) % km);
+    b = (t / ((long)(im + 2) * km)) ? jm + 1 : 0;
+  } else if ((t -= n2) < n3) {
+    const int bb = (int)(t % (jm + 1));  // b in {-1} U [1, jm]
+    b = (bb == 0) ? -1 : bb;
+    c = 1 + (int)((t / (jm + 1)) % km);
+    a = (t / ((long)(jm + 1) * km)) ? im + 1 : 0;
+  } else {
+    return;
+  }
   // V1: after the i-face loop; V2: after the j-face loop
   auto V1 = [&](int x, int y, int z) -> float {
     int xs = (x == 0) ? 1 : ((x == im + 1) ? im : x);
@@ -490,7 +456,7 @@ __global__ __launch_bounds__(256) void k_boundsm(Dims d, Fields F) {
 }
 
 void launch_boundsm(hipStream_t st, Dims d, Fields F) {
-  long n = (long)(d.ip + 2) * (d.jp + 3) * (d.kp + 2);
+  long n = 2L * (d.ip + 2) * (d.jp + 3) + 2L * (d.ip + 2) * d.kp + 2L * (d.jp + 1) * d.kp;
   hipLaunchKernelGGL(k_boundsm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F);
 }
 

@@ -1,113 +1,158 @@
 // Red-black SOR of press.f95:87-115 on a colour-split copy of p and rhs.
 //
-// The reference sweeps the Fortran array in place with stride 2 along i.  Here, for the
-// duration of the solve, cell (i,j,k) of the box [0,ip+1]x[0,jp+1]x[0,kp+1] is stored in array
-// (i+j+k)&1 at position (i>>1, j, k).  In every (j,k) row the cells of one colour are then
-// contiguous, and their six neighbours sit in the other array at (i>>1 or i>>1 +- 1, j, k),
-// (i>>1, j+-1, k), (i>>1, j, k+-1): a sweep reads one colour's p, rhs and the other colour's
-// p and writes one colour's p - 8 bytes per cell of the grid, all in whole cache lines.
+// The reference sweeps the Fortran array in place with stride 2 along i and refreshes the
+// boundary images of p after every colour (boundp1) and every iteration (boundp2).  Here, for
+// the duration of the solve, interior cell (i,j,k) is stored in array c = (i+j+k)&1 at
+// position ((i-1)>>1, j-1, k-1) of a padded row-major box (rows of nr floats, nr a multiple of
+// 4): every row of one colour starts at element 0 and is contiguous, so a sweep reads and
+// writes whole cache lines, 4 cells per thread as float4 - 8 bytes per grid cell per sweep.
 //
-// cn1 is recomputed from its 1-D factors (init.f95:98-104: 2/(dxs(i-1)dxs(i)) +
-// 2/(dys(j-1)dys(j)) + 2/(dzs(k-1)dzs(k)), then the reciprocal) with the reference's order of
-// operations, so it is bit-identical to the stored array and costs no memory traffic.
+// Boundary images.  When the halos of p are mirror images of the interior on entry (the state
+// boundp1 + boundp2 leave, i.e. after every press), the image a cell reads across an i face or
+// a k face is, at the time of the read, the cell's own current value (it was refreshed after
+// the cell's last update and the cell has not changed since); the image across the periodic j
+// face is the current value of the cell at the other end of the row, which for even jp has the
+// other colour and was refreshed after the other colour's last sweep.  The sweep therefore
+// reads the cell itself for i/k faces and the wrapped row for j faces, keeps no halos, and the
+// final gather writes every image from the interior - the reference's state after the last
+// boundp2.  Odd jp (same colour across the seam) and inconsistent halos use the in-place path.
 //
-// boundp1 (after each colour) and boundp2 (after each iteration) are fused: the thread that
-// updates a cell also writes every mirror image of it (i faces, periodic j faces, k faces).
-// Each mirror of a cell is read only by that cell, except the periodic j images, which are
-// read by the cell across the seam; for even jp that cell has the other colour and is read in
-// the other sweep.  So, for even jp and mirror-consistent halos on entry (the state every
-// press leaves), every read returns the value the reference's sequential sweep reads, and the
-// state after each full iteration is the reference's state after boundp2.
+// cn1 is recomputed from its 1-D factors (init.f95:98-104) in the reference's operation order,
+// bit-identical to the stored array; on a uniform horizontal grid (every ax, by, cn2*, cn3*
+// equal) the factors reduce to scalars and a per-k table of cn1.
 #include "les_kernels.h"
 
 namespace les {
 
 namespace {
-constexpr int SX = 32;  // colour cells along i per workgroup row (a wavefront = 2 rows)
-constexpr int SY = 8;   // rows of j per workgroup
 
-__device__ __forceinline__ void put_split(const SplitSOR& sp, int a, int b, int c, float v) {
-  sp.p[(a + b + c) & 1][sp.S(a >> 1, b, c)] = v;
+// colour of cell (i,j,k) and first interior i of colour c in row (j,k)
+__device__ __forceinline__ int first_i(int c, int j, int k) { return ((1 + j + k) & 1) == c ? 1 : 2; }
+
+__device__ __forceinline__ int split_index(const SplitSOR& sp, int i, int j, int k) {
+  return ((i - 1) >> 1) + sp.nr * ((j - 1) + sp.nj * (k - 1));
 }
+
 }  // namespace
 
-// ---- p (Fortran layout) -> split copy; rhs -= rhsav (press.f95:76-82) into both layouts ----
+// ---- p, rhs (Fortran layouts) -> split copy; optionally rhs -= rhsav (press.f95:76-82) ----
 __global__ __launch_bounds__(256) void k_split_scatter(Dims d, Fields F, SplitSOR sp,
                                                        const DevScalars* sc, int adjust_rhs) {
-  const int i = blockIdx.x * 64 + threadIdx.x;  // 0..ip+1
-  const int j = blockIdx.y * 4 + threadIdx.y;   // 0..jp+1
-  const int k = blockIdx.z;                     // 0..kp+1
-  if (i > d.ip + 1 || j > d.jp + 1) return;
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
   const int par = (i + j + k) & 1;
-  const long is = sp.S(i >> 1, j, k);
+  const int is = split_index(sp, i, j, k);
   sp.p[par][is] = F.p[d.P(i, j, k)];
-  if (i >= 1 && i <= d.ip && j >= 1 && j <= d.jp && k >= 1 && k <= d.kp) {
-    const long ir = d.R(i, j, k);
-    float r = F.rhs[ir];
-    if (adjust_rhs) {
-      r = r - sc->rhsav;
-      F.rhs[ir] = r;
-    }
-    sp.r[par][is] = r;
+  const long ir = d.R(i, j, k);
+  float r = F.rhs[ir];
+  if (adjust_rhs) {
+    r = r - sc->rhsav;
+    F.rhs[ir] = r;
   }
+  sp.r[par][is] = r;
 }
 
 void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                           int adjust_rhs) {
-  dim3 grid((d.ip + 2 + 63) / 64, (d.jp + 2 + 3) / 4, d.kp + 2);
+  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
   hipLaunchKernelGGL(k_split_scatter, grid, dim3(64, 4), 0, st, d, F, sp, sc, adjust_rhs);
 }
 
-// ---- one colour sweep ----
+// ---- one colour sweep: 4 consecutive cells of one colour row per thread ----
+template <bool UNIFORM>
 __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp, Physics ph,
                                                    int nrd, int want_resid, int64_t* partials,
-                                                   const DevScalars* sc, int nbx, int nby,
-                                                   int ntiles) {
+                                                   const DevScalars* sc, int ng, int nthreads) {
   __shared__ unsigned long long lds[NLIMB];
   if (sc->sor_done) return;
   if (want_resid) block_limbs_init(lds);
   ThreadAcc acc;
   acc.init();
   const int L = xcd_tile(blockIdx.x, gridDim.x);
-  if (L < ntiles) {
-    const int bx = L % nbx;
-    const int by = (L / nbx) % nby;
-    const int k = 1 + L / (nbx * nby);
-    const int j = 1 + by * SY + threadIdx.y;
-    const int i = 1 + ((k + j + nrd) & 1) + 2 * (bx * SX + threadIdx.x);
-    if (i <= d.ip && j <= d.jp) {
-      const int c = (i + j + k) & 1, o = c ^ 1;
-      const float* po = sp.p[o];
-      const int ii = i >> 1;
-      const long s0 = sp.S(ii, j, k);
-      const long sj = sp.nh, sk = (long)sp.nh * sp.nj;
-      const float pc = sp.p[c][s0];
-      const float pe = po[sp.S((i + 1) >> 1, j, k)];
-      const float pw = po[sp.S((i - 1) >> 1, j, k)];
-      const float cn1 = 1.0f / (sp.ax[i] + sp.by[j] + sp.cz[k]);
-      const float reltmp =
-          ph.omega * (cn1 * (g.cn2l[i - 1] * pe + g.cn2s[i - 1] * pw + g.cn3l[j - 1] * po[s0 + sj] +
-                             g.cn3s[j - 1] * po[s0 - sj] + g.cn4l[k - 1] * po[s0 + sk] +
-                             g.cn4s[k - 1] * po[s0 - sk] - sp.r[c][s0]) -
-                      pc);
-      const float pn = pc + reltmp;
-      sp.p[c][s0] = pn;
-      if (want_resid) acc.add(reltmp * reltmp, &lds[0]);
-      // mirror images (boundp1 i/j faces, boundp2 k faces)
-      const bool mi = (i == 1) || (i == d.ip), mj = (j == 1) || (j == d.jp),
-                 mk = (k == 1) || (k == d.kp);
-      if (mi || mj || mk) {
-        const int ia = (i == 1) ? 0 : d.ip + 1;
-        const int jb = (j == d.jp) ? 0 : d.jp + 1;
-        const int kc = (k == 1) ? 0 : d.kp + 1;
-        if (mi) put_split(sp, ia, j, k, pn);
-        if (mj) put_split(sp, i, jb, k, pn);
-        if (mk) put_split(sp, i, j, kc, pn);
-        if (mi && mj) put_split(sp, ia, jb, k, pn);
-        if (mi && mk) put_split(sp, ia, j, kc, pn);
-        if (mj && mk) put_split(sp, i, jb, kc, pn);
-        if (mi && mj && mk) put_split(sp, ia, jb, kc, pn);
+  const int t = L * 256 + threadIdx.x;
+  if (t < nthreads) {
+    const int grp = t % ng;
+    const int row = t / ng;
+    const int j = 1 + row % d.jp;
+    const int k = 1 + row / d.jp;
+    const int c = (1 + nrd) & 1;  // nrd = 0 updates i = 1 + mod(k+j,2): odd i+j+k
+    const int i0 = first_i(c, j, k) + 8 * grp;  // i of the group's first cell
+    if (i0 <= d.ip) {
+      const int rs = sp.nr * ((j - 1) + sp.nj * (k - 1));
+      const int base = rs + 4 * grp;
+      const float* po = sp.p[c ^ 1];
+      // neighbour rows: periodic in j, the cell itself across k faces
+      const int rn = (j == d.jp) ? -sp.nr * (d.jp - 1) : sp.nr;  // j+1
+      const int rsj = (j == 1) ? sp.nr * (d.jp - 1) : -sp.nr;    // j-1
+      const int pl = sp.nr * sp.nj;
+      const float4 own = *reinterpret_cast<const float4*>(sp.p[c] + base);
+      const float4 rhs = *reinterpret_cast<const float4*>(sp.r[c] + base);
+      const float4 on = *reinterpret_cast<const float4*>(po + base + rn);
+      const float4 os = *reinterpret_cast<const float4*>(po + base + rsj);
+      float4 ou, od;
+      if (k < d.kp) ou = *reinterpret_cast<const float4*>(po + base + pl);
+      else ou = own;
+      if (k > 1) od = *reinterpret_cast<const float4*>(po + base - pl);
+      else od = own;
+      // i neighbours: odd i -> east o[ii], west o[ii-1]; even i -> east o[ii+1], west o[ii]
+      const float4 oc = *reinterpret_cast<const float4*>(po + base);
+      float xe[4], xw[4];
+      const float ov[4] = {oc.x, oc.y, oc.z, oc.w};
+      if (i0 & 1) {
+        const float prev = (grp > 0) ? po[base - 1] : 0.0f;
+        xe[0] = ov[0]; xe[1] = ov[1]; xe[2] = ov[2]; xe[3] = ov[3];
+        xw[0] = prev; xw[1] = ov[0]; xw[2] = ov[1]; xw[3] = ov[2];
+      } else {
+        const float next = (i0 + 7 <= d.ip) ? po[base + 4] : 0.0f;
+        xw[0] = ov[0]; xw[1] = ov[1]; xw[2] = ov[2]; xw[3] = ov[3];
+        xe[0] = ov[1]; xe[1] = ov[2]; xe[2] = ov[3]; xe[3] = next;
       }
+      const float pv[4] = {own.x, own.y, own.z, own.w};
+      const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
+      const float nv[4] = {on.x, on.y, on.z, on.w};
+      const float sv[4] = {os.x, os.y, os.z, os.w};
+      const float uv[4] = {ou.x, ou.y, ou.z, ou.w};
+      const float dv[4] = {od.x, od.y, od.z, od.w};
+      float out[4];
+      float c2l = 0.0f, c2s = 0.0f, c3l, c3s, cn1k = 0.0f;
+      const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+      if (UNIFORM) {
+        c2l = g.cn2l[0];
+        c2s = g.cn2s[0];
+        c3l = g.cn3l[0];
+        c3s = g.cn3s[0];
+        cn1k = sp.cn1k[k];
+      } else {
+        c3l = g.cn3l[j - 1];
+        c3s = g.cn3s[j - 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + 2 * q;
+        float pc = pv[q];
+        if (i <= d.ip) {
+          const float pe = (i == d.ip) ? pc : xe[q];
+          const float pw = (i == 1) ? pc : xw[q];
+          float cn1, a2l = c2l, a2s = c2s;
+          if (UNIFORM) {
+            cn1 = cn1k;
+          } else {
+            cn1 = 1.0f / (sp.ax[i] + sp.by[j] + sp.cz[k]);
+            a2l = g.cn2l[i - 1];
+            a2s = g.cn2s[i - 1];
+          }
+          const float reltmp =
+              ph.omega * (cn1 * (a2l * pe + a2s * pw + c3l * nv[q] + c3s * sv[q] + c4l * uv[q] +
+                                 c4s * dv[q] - rv[q]) -
+                          pc);
+          pc = pc + reltmp;
+          if (want_resid) acc.add(reltmp * reltmp, lds);
+        }
+        out[q] = pc;
+      }
+      *reinterpret_cast<float4*>(sp.p[c] + base) = make_float4(out[0], out[1], out[2], out[3]);
     }
   }
   if (want_resid) {
@@ -117,27 +162,32 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
 }
 
 namespace {
-void split_tiles(Dims d, int& nbx, int& nby, int& ntiles, int& nwg) {
-  const int half = (d.ip + 1) / 2;  // colour cells per row (upper bound)
-  nbx = (half + SX - 1) / SX;
-  nby = (d.jp + SY - 1) / SY;
-  ntiles = nbx * nby * d.kp;
-  nwg = (ntiles + 7) / 8 * 8;
+void split_threads(Dims d, const SplitSOR& sp, int& ng, int& nthreads, int& nwg) {
+  ng = sp.nr / 4;
+  nthreads = ng * d.jp * d.kp;
+  const int nb = (nthreads + 255) / 256;
+  nwg = (nb + 7) / 8 * 8;
 }
 }  // namespace
 
 int sor_split_blocks(Dims d) {
-  int nbx, nby, nt, nwg;
-  split_tiles(d, nbx, nby, nt, nwg);
+  SplitSOR sp{};
+  sp.nr = ((d.ip + 1) / 2 + 3) / 4 * 4;
+  int ng, nt, nwg;
+  split_threads(d, sp, ng, nt, nwg);
   return nwg;
 }
 
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc) {
-  int nbx, nby, nt, nwg;
-  split_tiles(d, nbx, nby, nt, nwg);
-  hipLaunchKernelGGL(k_sor_split, dim3(nwg), dim3(SX, SY), 0, st, d, g, sp, ph, nrd, want_resid,
-                     rb.partials + (long)blk_offset * NLIMB, sc, nbx, nby, nt);
+  int ng, nt, nwg;
+  split_threads(d, sp, ng, nt, nwg);
+  if (sp.uniform)
+    hipLaunchKernelGGL(k_sor_split<true>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, nrd,
+                       want_resid, rb.partials + (long)blk_offset * NLIMB, sc, ng, nt);
+  else
+    hipLaunchKernelGGL(k_sor_split<false>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, nrd,
+                       want_resid, rb.partials + (long)blk_offset * NLIMB, sc, ng, nt);
 }
 
 // ---- pav partial sums (press.f95:125-132) from the split copy ----
@@ -154,7 +204,7 @@ __global__ __launch_bounds__(256) void k_pav_split(Dims d, Grid1D g, SplitSOR sp
   if (i <= d.ip && j <= d.jp) {
     const float dx1i = g.DX1(i), dy1j = g.DY1(j);
     for (int k = k0; k <= k1; ++k)
-      acc.add(sp.p[(i + j + k) & 1][sp.S(i >> 1, j, k)] * dx1i * dy1j * g.DZN(k), lds);
+      acc.add(sp.p[(i + j + k) & 1][split_index(sp, i, j, k)] * dx1i * dy1j * g.DZN(k), lds);
   }
   acc.flush(lds);
   const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -175,16 +225,20 @@ void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, 
   hipLaunchKernelGGL(k_pav_split_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
 }
 
-// ---- split copy -> p, minus pav (press.f95:135-146): every box cell, the halos being the
-// mirror images of the adjusted interior exactly as boundp1 + boundp2 leave them ----
+// ---- split copy -> p over the box [0,ip+1]x[0,jp+1]x[0,kp+1], optionally minus pav
+// (press.f95:135-146): every halo cell takes its mirror interior cell, M(0)=1, M(ip+1)=ip in i,
+// M(0)=jp, M(jp+1)=1 in j, M(0)=1, M(kp+1)=kp in k, as boundp1 then boundp2 leave them ----
 __global__ __launch_bounds__(256) void k_split_gather_adj(Dims d, Fields F, SplitSOR sp,
                                                           const DevScalars* sc, int subtract_pav) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i > d.ip + 1 || j > d.jp + 1) return;
-  const float v = sp.p[(i + j + k) & 1][sp.S(i >> 1, j, k)];
-  F.p[d.P(i, j, k)] = subtract_pav ? v - sc->pav : v;
+  const int a = blockIdx.x * 64 + threadIdx.x;
+  const int b = blockIdx.y * 4 + threadIdx.y;
+  const int c = blockIdx.z;
+  if (a > d.ip + 1 || b > d.jp + 1) return;
+  const int i = (a == 0) ? 1 : ((a == d.ip + 1) ? d.ip : a);
+  const int j = (b == 0) ? d.jp : ((b == d.jp + 1) ? 1 : b);
+  const int k = (c == 0) ? 1 : ((c == d.kp + 1) ? d.kp : c);
+  const float v = sp.p[(i + j + k) & 1][split_index(sp, i, j, k)];
+  F.p[d.P(a, b, c)] = subtract_pav ? v - sc->pav : v;
 }
 
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
