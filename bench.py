@@ -71,6 +71,23 @@ def cpu_baseline(case, nsteps):
             "kind": kind, "sample": sample, "seconds": round(secs, 3)}
 
 
+def profile_sor(L, reps):
+    """For rocprofv3 --pmc runs: REPS sweeps (k_sor_split) and one copy kernel that reads and
+    writes SOR_SWEEP_B_PER_CELL/2 bytes per cell each, i.e. the sweep's algorithmic bytes."""
+    import torch
+    ip, jp, kp = GRID
+    n = SOR_SWEEP_B_PER_CELL // 2 * ip * jp * kp // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    ms = L.time_kernel("sor_sweep", reps)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"sor_sweeps": reps, "sweep_bytes_per_launch": SOR_SWEEP_B_PER_CELL * ip * jp * kp,
+                      "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+                      "sweep_avg_us": round(ms * 1e3, 3)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,12 +95,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-sor", type=int, default=0, metavar="REPS",
+                    help="counter runs: set up the workload, launch REPS SOR colour sweeps and "
+                         "one device copy of the same algorithmic bytes (calibration), no timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the LES step on one GPU only (no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import torch
     from les_halo_amd import LES, inputs
     ip, jp, kp = GRID
     case = inputs.make_case(ip, jp, kp)
@@ -92,9 +113,14 @@ def main():
     L.ifdata()
     L.run(1, args.warmup)
     L.sync()
+    if args.profile_sor:
+        profile_sor(L, args.profile_sor)
+        return
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     L.run(1 + args.warmup, args.steps)
     L.sync()
+    torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     cells = ip * jp * kp
     ms_per_step = secs / args.steps * 1e3

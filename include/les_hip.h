@@ -223,6 +223,15 @@ int les_exchange_halo_read(les_handle* h, const int* fields, int v_dim, int h_w,
                            float* dev_buffer);
 int les_exchange_halo_write(les_handle* h, const int* fields, int v_dim, int h_w,
                             const float* dev_buffer);
+/* The same pack/unpack on caller-owned device arrays of allocated extent (im, jm, km), one per
+ * vector component: exchange_{1,2,4,16}_halo_read(array, buffer, im, jm, km, h_w)
+ * (halo_exchange.cl:156-290, halo_exchange_correct.cl:156-290) and
+ * exchange_{1,2,4,16}_halo_write (halo_exchange.cl:3-155), with the OpenCL vector's
+ * components as separate arrays (v_dim = 1..16).  Enqueued on the handle's stream. */
+int les_halo_read_arrays(les_handle* h, float* const* dev_arrays, int v_dim, int im, int jm,
+                         int km, int h_w, int correct, float* dev_buffer);
+int les_halo_write_arrays(les_handle* h, float* const* dev_arrays, int v_dim, int im, int jm,
+                          int km, int h_w, const float* dev_buffer);
 
 /* Exact, order-independent sum of n device floats, rounded once to FP32 (the reduction that
  * rhsav, pav and the SOR residual use), result written to *out on the host. */

@@ -75,6 +75,10 @@ PROTOS = {
     "les_halo_size": (c_int64, [c_int, c_int, c_int, c_int, c_int, c_int]),
     "les_exchange_halo_read": (c_int, [H, ctypes.POINTER(c_int), c_int, c_int, c_int, c_void_p]),
     "les_exchange_halo_write": (c_int, [H, ctypes.POINTER(c_int), c_int, c_int, c_void_p]),
+    "les_halo_read_arrays": (c_int, [H, ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int,
+                                     c_int, c_void_p]),
+    "les_halo_write_arrays": (c_int, [H, ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int,
+                                      c_void_p]),
     "les_reduce_sum_exact": (c_int, [H, c_void_p, c_int64, P_float]),
     "les_time_kernel": (c_int, [H, c_int, c_int, P_float]),
     "les_initialise_LES_kernel": (c_int, [c_void_p] * 42 + [c_float, c_int, c_int, c_int]),

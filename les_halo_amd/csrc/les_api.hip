@@ -886,6 +886,33 @@ extern "C" int les_exchange_halo_write(les_handle* h, const int* fields, int v_d
   return check_launch(h);
 }
 
+static int halo_arrays_check(les_handle* h, float* const* dev_arrays, int v_dim, int im, int jm,
+                             int km, int h_w, const float* dev_buffer) {
+  if (!h) return LES_E_INVALID;
+  if (!dev_arrays || !dev_buffer || v_dim < 1 || v_dim > 16 || km < 1 || h_w < 1 ||
+      im < 2 * h_w + 1 || jm < 2 * h_w + 1)
+    return fail(h, LES_E_INVALID, "halo: bad arrays, sizes or width");
+  for (int v = 0; v < v_dim; ++v)
+    if (!dev_arrays[v]) return fail(h, LES_E_INVALID, "halo: null array");
+  return LES_OK;
+}
+
+extern "C" int les_halo_read_arrays(les_handle* h, float* const* dev_arrays, int v_dim, int im,
+                                    int jm, int km, int h_w, int correct, float* dev_buffer) {
+  int r = halo_arrays_check(h, dev_arrays, v_dim, im, jm, km, h_w, dev_buffer);
+  if (r) return r;
+  launch_halo_read(h->st, dev_arrays, v_dim, im, jm, km, h_w, correct ? 1 : 0, dev_buffer);
+  return check_launch(h);
+}
+
+extern "C" int les_halo_write_arrays(les_handle* h, float* const* dev_arrays, int v_dim, int im,
+                                     int jm, int km, int h_w, const float* dev_buffer) {
+  int r = halo_arrays_check(h, dev_arrays, v_dim, im, jm, km, h_w, dev_buffer);
+  if (r) return r;
+  launch_halo_write(h->st, dev_arrays, v_dim, im, jm, km, h_w, dev_buffer);
+  return check_launch(h);
+}
+
 // exact-sum utility on a device vector; result to host
 extern "C" int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n, float* out) {
   if (!h || !dev_x || !out || n < 0) return LES_E_INVALID;

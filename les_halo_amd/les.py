@@ -214,6 +214,20 @@ class LES:
         self._chk(lib().les_exchange_halo_write(self.h, ids, len(names), h_w,
                                                 ctypes.c_void_p(dev_buffer)), "exchange_halo_write")
 
+    def halo_read_arrays(self, dev_arrays, im, jm, km, dev_buffer, h_w=1, correct=0):
+        """exchange_<v_dim>_halo_read on caller-owned device arrays (addresses), enqueued on
+        the handle's stream."""
+        ptrs = (ctypes.c_void_p * len(dev_arrays))(*dev_arrays)
+        self._chk(lib().les_halo_read_arrays(self.h, ptrs, len(dev_arrays), im, jm, km, h_w,
+                                             correct, ctypes.c_void_p(dev_buffer)),
+                  "halo_read_arrays")
+
+    def halo_write_arrays(self, dev_arrays, im, jm, km, dev_buffer, h_w=1):
+        """exchange_<v_dim>_halo_write on caller-owned device arrays (addresses)."""
+        ptrs = (ctypes.c_void_p * len(dev_arrays))(*dev_arrays)
+        self._chk(lib().les_halo_write_arrays(self.h, ptrs, len(dev_arrays), im, jm, km, h_w,
+                                              ctypes.c_void_p(dev_buffer)), "halo_write_arrays")
+
     KERNELS = {"sor_sweep": 0, "velfg": 1, "visc_adam": 2, "calc_sm": 3, "rhs": 4, "velnw": 5,
                "step": 6}
 
