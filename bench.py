@@ -27,20 +27,23 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 B_STEP = 1057                  # algorithmic FP32 bytes per interior cell-step
 SOR_SWEEP_B_PER_CELL = 8       # one colour sweep: r p 4 + r rhs 2 + w p 2
+SOR_ITER_B_PER_CELL = 12       # one fused iteration (both colours): r p 4 + r rhs 4 + w p 4
 METRIC = "M cell-steps/s (50 SOR iters/step) + % HBM peak, 1 GPU; batched vs plain"
 GRID = (300, 300, 90)
 PHYS = dict(dt=0.05, ro=1.1763, vn=1.583e-5, alpha=-10.0, beta=-1.0, cs0=0.1, omega=1.5,
             pjuge=0.0, nmaxp=50)
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the SOR sweep from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(HERE, "profiles", "sor_sweep_300x300x90_traffic.json")
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
+    (profiles/sor_300x300x90_traffic.json, written from FETCH_SIZE and WRITE_SIZE runs)."""
+    path = os.path.join(HERE, "profiles", "sor_300x300x90_traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        return float(json.load(open(path))["hbm_bytes_per_launch"])
-    except Exception:
+        ent = json.load(open(path)).get(kernel.split("<")[0])
+        return float(ent["hbm_bytes_per_launch"]) if ent else None
+    except (ValueError, KeyError, TypeError):
         return None
 
 
@@ -81,11 +84,15 @@ def profile_sor(L, reps):
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     ms = L.time_kernel("sor_sweep", reps)
+    ms_it = L.time_kernel("sor_iter", reps)
     dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"sor_sweeps": reps, "sweep_bytes_per_launch": SOR_SWEEP_B_PER_CELL * ip * jp * kp,
+    print(json.dumps({"launches": reps,
+                      "sweep_bytes_per_launch": SOR_SWEEP_B_PER_CELL * ip * jp * kp,
+                      "iter_bytes_per_launch": SOR_ITER_B_PER_CELL * ip * jp * kp,
                       "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
-                      "sweep_avg_us": round(ms * 1e3, 3)}), flush=True)
+                      "sweep_avg_us": round(ms * 1e3, 3), "iter_avg_us": round(ms_it * 1e3, 3)}),
+          flush=True)
 
 
 def main():
@@ -111,6 +118,7 @@ def main():
     L = LES(ip, jp, kp, **PHYS)
     L.load_case(case)
     L.ifdata()
+    tuning = L.autotune(20)  # picks the fused SOR iteration's tile; outside the timed region
     L.run(1, args.warmup)
     L.sync()
     if args.profile_sor:
@@ -126,14 +134,22 @@ def main():
     ms_per_step = secs / args.steps * 1e3
     value = cells * args.steps / secs / 1e6
 
-    # dominant kernel, timed live with HIP events on the library's stream
-    sweep_ms = L.time_kernel("sor_sweep", 200)
-    bytes_launch = SOR_SWEEP_B_PER_CELL * cells
-    achieved = bytes_launch / (sweep_ms * 1e-3) / 1e9
+    # dominant kernel (the fused SOR iteration when the grid allows it, else the colour
+    # sweep), timed live with HIP events on the library's stream
+    if tuning["sor_tile_j"] > 0:
+        kern_ms = L.time_kernel("sor_iter", 100)
+        bytes_launch = SOR_ITER_B_PER_CELL * cells
+        kname = "k_sor_iter<%d>" % tuning["sor_tile_j"]
+    else:
+        kern_ms = L.time_kernel("sor_sweep", 200)
+        bytes_launch = SOR_SWEEP_B_PER_CELL * cells
+        kname = "k_sor_split<true>"
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(), "kernel": "k_sor_split",
-                "bytes_per_launch": bytes_launch, "avg_us": round(sweep_ms * 1e3, 3)}
+                "traffic": traffic_from_profiles(kname), "kernel": kname,
+                "bytes_per_launch": bytes_launch, "avg_us": round(kern_ms * 1e3, 3),
+                "tile": [tuning["sor_tile_j"], tuning["sor_tile_k"]]}
     step_hbm_frac = B_STEP * cells / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "M cell-steps/s", "n_gpus": 1,

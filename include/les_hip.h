@@ -248,9 +248,20 @@ enum {
   LES_KERNEL_CALC_SM = 3,
   LES_KERNEL_RHS = 4,
   LES_KERNEL_VELNW = 5,
-  LES_KERNEL_STEP = 6        /* one whole step replayed from the captured graph */
+  LES_KERNEL_STEP = 6,       /* one whole step replayed from the captured graph */
+  LES_KERNEL_SOR_ITER = 7    /* one fused SOR iteration (both colours; uniform grids, ip <= 320) */
 };
 int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms);
+
+/* Autotuner (no counterpart in the reference, whose OpenCL launch shapes are fixed): times
+ * every workgroup tile (rows of j x levels of k) of the fused SOR iteration on this handle's
+ * grid, 'reps' launches each, and keeps the fastest for later solves.  Changes no simulation
+ * state.  *out (may be NULL) receives the choice. */
+typedef struct les_tuning {
+  int sor_tile_j, sor_tile_k;  /* 0 when the fused iteration does not apply */
+  float sor_iter_ms;           /* its average time per iteration, or -1 */
+} les_tuning;
+int les_autotune(les_handle* h, int reps, les_tuning* out);
 
 /* ---- the reference's GPU host API (module_LES_combined_ocl) as single-handle drop-ins ---- */
 /* initialise_LES_kernel (Wrappers/module_LES_combined_ocl.f95:15-441), external wind profile

@@ -31,6 +31,10 @@ class LesScalars(ctypes.Structure):
                 ("resid_l2", c_double), ("refine_passes", c_int)]
 
 
+class LesTuning(ctypes.Structure):
+    _fields_ = [("sor_tile_j", c_int), ("sor_tile_k", c_int), ("sor_iter_ms", c_float)]
+
+
 # every symbol declared in include/les_hip.h: name -> (restype, argtypes)
 H = c_void_p
 PROTOS = {
@@ -81,6 +85,7 @@ PROTOS = {
                                       c_void_p]),
     "les_reduce_sum_exact": (c_int, [H, c_void_p, c_int64, P_float]),
     "les_time_kernel": (c_int, [H, c_int, c_int, P_float]),
+    "les_autotune": (c_int, [H, c_int, ctypes.POINTER(LesTuning)]),
     "les_initialise_LES_kernel": (c_int, [c_void_p] * 42 + [c_float, c_int, c_int, c_int]),
     "les_run_LES_kernel": (c_int, [c_int, c_int, c_void_p]),
     "les_default_handle": (c_void_p, []),
@@ -99,6 +104,18 @@ PROTOS = {
 _lib = None
 
 
+def _bind_to_torch_runtime():
+    """PyTorch-ROCm wheels ship their own HIP runtime under the same SONAME
+    (libamdhip64.so.7).  Loading torch first makes libles_hip.so bind to that copy, so torch
+    tensors and this library share one runtime and one device context; loaded the other way
+    round, the process would hold two HIP runtimes and the second to initialise finds no device.
+    Without torch installed, libles_hip.so uses the system ROCm runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Loads libles_hip.so (built in-tree); raises if it is missing."""
     global _lib
@@ -106,6 +123,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libles_hip.so is not built (%s); run `python -m les_halo_amd.build`"
                                % LIB_PATH)
+        _bind_to_torch_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in PROTOS.items():
             fn = getattr(L, name)

@@ -14,6 +14,7 @@
 
 #include "../../include/les_hip.h"
 #include "les_kernels.h"
+#include "les_mixed.h"
 
 using namespace les;
 
@@ -39,6 +40,13 @@ struct les_handle {
   std::vector<float> hcn1;     // host copy of cn1 (to know whether it is the separable form)
   bool cn1_separable = false;  // cn1 == 1/((ax(i)+by(j))+cz(k)) bit for bit
   bool p_consistent = true;    // every halo of p equals its mirror interior cell
+  // mixed precision: FP64 iterate and rhs in the split layout, block partials
+  Mixed mx{};
+  double* mx_mem = nullptr;
+  // second iterate of the fused SOR iteration (ping-pong with sp.p) and its workgroup tile
+  float* palt[2] = {nullptr, nullptr};
+  SorIterShape iter_shape{8, 10};
+  bool tuned = false;
 };
 
 namespace {
@@ -194,8 +202,14 @@ int upload_scalar_area(les_handle* h) {
   for (int k = 1; k <= h->d.kp; ++k)
     for (int j = 1; j <= h->d.jp; ++j)
       for (int i = 1; i <= h->d.ip; ++i) area = area + dx1[i + 1] * dy1[j] * dzn[k + 1];
+  // the same volumes summed in FP64 for the mixed-precision means
+  double area64 = 0.0;
+  for (int k = 1; k <= h->d.kp; ++k)
+    for (int j = 1; j <= h->d.jp; ++j)
+      for (int i = 1; i <= h->d.ip; ++i) area64 += (double)(dx1[i + 1] * dy1[j] * dzn[k + 1]);
   HIPCHK(h, hipMemcpyAsync(&h->sc->area, &area, sizeof(float), hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipMemcpyAsync(&h->sc->pco, &area, sizeof(float), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipMemcpyAsync(&h->sc->area64, &area64, sizeof(double), hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   return LES_OK;
 }
@@ -309,6 +323,9 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
       cfg->kp > 4096 || cfg->nmaxp < 0)
     return LES_E_INVALID;
   if (cfg->sub_i != 1 || cfg->sub_j != 1) return LES_E_INVALID;  // one domain per handle
+  if (cfg->precision != LES_PREC_FP32 && cfg->precision != LES_PREC_MIXED) return LES_E_INVALID;
+  if (cfg->precision == LES_PREC_MIXED && (!(cfg->refine_tol > 0.0) || cfg->refine_max < 0))
+    return LES_E_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LES_E_NODEV;
   if (cfg->device < 0 || cfg->device >= ndev) return LES_E_INVALID;
@@ -349,20 +366,36 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     sp.nr = ((cfg->ip + 1) / 2 + 3) / 4 * 4;
     sp.nj = cfg->jp;
     const size_t ns = (size_t)sp.nr * cfg->jp * cfg->kp;
-    const size_t nf = 4 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + 2 * (cfg->kp + 2);
+    const size_t nf = 6 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + 2 * (cfg->kp + 2);
     if (hipMalloc(&h->split_mem, nf * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
     if (hipMemsetAsync(h->split_mem, 0, nf * sizeof(float), h->st) != hipSuccess) return bail(LES_E_HIP);
     sp.p[0] = h->split_mem;
     sp.p[1] = sp.p[0] + ns;
     sp.r[0] = sp.p[1] + ns;
     sp.r[1] = sp.r[0] + ns;
-    float* fac = sp.r[1] + ns;
+    h->palt[0] = sp.r[1] + ns;  // float4-aligned like the four arrays before it
+    h->palt[1] = h->palt[0] + ns;
+    float* fac = h->palt[1] + ns;
     sp.ax = fac;
     sp.by = fac + (cfg->ip + 2);
     sp.cz = fac + (cfg->ip + 2) + (cfg->jp + 2);
     sp.cn1k = fac + (cfg->ip + 2) + (cfg->jp + 2) + (cfg->kp + 2);
     sp.uniform = 0;
     h->hcn1.assign((size_t)cfg->ip * cfg->jp * cfg->kp, 0.0f);
+    if (cfg->precision == LES_PREC_MIXED) {
+      Mixed& mx = h->mx;
+      mx.nblk = mixed_blocks(h->d);
+      const size_t nd = 4 * ns + 2 * (size_t)mx.nblk;
+      if (hipMalloc(&h->mx_mem, nd * sizeof(double)) != hipSuccess) return bail(LES_E_NOMEM);
+      if (hipMemsetAsync(h->mx_mem, 0, nd * sizeof(double), h->st) != hipSuccess)
+        return bail(LES_E_HIP);
+      mx.p64[0] = h->mx_mem;
+      mx.p64[1] = mx.p64[0] + ns;
+      mx.r64[0] = mx.p64[1] + ns;
+      mx.r64[1] = mx.r64[0] + ns;
+      mx.part = mx.r64[1] + ns;
+      mx.cn1 = h->fld[LES_CN1];
+    }
   }
   if (hipStreamSynchronize(h->st) != hipSuccess) return bail(LES_E_HIP);
   *out = h;
@@ -381,6 +414,7 @@ extern "C" int les_destroy(les_handle* h) {
   if (h->rb.partials) hipFree(h->rb.partials);
   if (h->scratch_f) hipFree(h->scratch_f);
   if (h->split_mem) hipFree(h->split_mem);
+  if (h->mx_mem) hipFree(h->mx_mem);
   if (h->st) hipStreamDestroy(h->st);
   if (h == g_default) g_default = nullptr;
   delete h;
@@ -651,13 +685,24 @@ int enqueue_sor_inplace(les_handle* h) {
   return check_launch(h);
 }
 
-// the same iterations on the colour-split copy (already scattered)
-int enqueue_sor_split_loop(les_handle* h) {
+// the same iterations on the colour-split copy (already scattered); early exit only when
+// pjuge > 0 and allowed.  Without a residual, pairs of iterations run as fused launches
+// (k_sor_iter) from sp.p to the second iterate and back, so the result ends in sp.p.
+int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true) {
   STAGE_BEGIN(h);
   launch_sor_reset(h->st, h->sc);
-  const bool resid = h->ph.pjuge > 0.0f;
+  const bool resid = allow_exit && h->ph.pjuge > 0.0f;
   const int nb = sor_split_blocks(d);
-  for (int l = 1; l <= h->cfg.nmaxp; ++l) {
+  int l0 = 1;
+  if (!resid && sor_iter_supported(d, h->sp)) {
+    const SorPair fwd{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
+    const SorPair back{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}};
+    for (; l0 + 1 <= h->cfg.nmaxp; l0 += 2) {
+      launch_sor_iter(h->st, d, g, h->sp, h->ph, fwd, h->sc, h->iter_shape);
+      launch_sor_iter(h->st, d, g, h->sp, h->ph, back, h->sc, h->iter_shape);
+    }
+  }
+  for (int l = l0; l <= h->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd)
       launch_sor_split(h->st, d, g, h->sp, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
     if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, h->ph, h->sc);
@@ -679,9 +724,44 @@ int enqueue_sor_solve(les_handle* h) {
   return check_launch(h);
 }
 
+// press.f95:49-146 with FP64 means and outer iterative refinement (les_mixed.hip).  Runs
+// eagerly: the host reads the residual after each pass.
+int enqueue_press_mixed(les_handle* h) {
+  STAGE_BEGIN(h);
+  if (!(h->d.jp % 2 == 0 && h->cn1_separable))
+    return fail(h, LES_E_STATE, "mixed precision needs an even jp and cn1 of the init.f95 form");
+  int r;
+  if (!h->p_consistent) {  // the refinement works on mirror-consistent boundary images
+    launch_boundp1(h->st, d, F, h->sc, 0);
+    launch_boundp2(h->st, d, F, h->sc, 0);
+  }
+  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
+  launch_mx_rhsav(h->st, d, h->rb, h->sc);
+  launch_mx_scatter(h->st, d, F, h->sp, h->mx, h->sc);
+  if ((r = enqueue_sor_split_loop(h, false))) return r;
+  launch_mx_promote(h->st, d, h->sp, h->mx);
+  int passes = 0;
+  for (;;) {
+    launch_mx_resid(h->st, d, g, h->sp, h->mx, passes == 0 ? 1 : 0, h->cfg.refine_tol, h->sc);
+    int done = 0;
+    HIPCHK(h, hipMemcpyAsync(&done, &h->sc->refine_done, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (done || passes >= h->cfg.refine_max) break;
+    if ((r = enqueue_sor_split_loop(h, false))) return r;
+    launch_mx_accumulate(h->st, d, h->sp, h->mx);
+    ++passes;
+  }
+  HIPCHK(h, hipMemcpyAsync(&h->sc->refine_passes, &passes, sizeof(int), hipMemcpyHostToDevice, h->st));
+  launch_mx_pav_gather(h->st, d, g, F, h->sp, h->mx, h->sc);
+  HIPCHK(h, hipStreamSynchronize(h->st));  // 'passes' lives on this stack frame
+  h->p_consistent = true;
+  return check_launch(h);
+}
+
 // press.f95:49-146
 int enqueue_press(les_handle* h) {
   STAGE_BEGIN(h);
+  if (h->cfg.precision == LES_PREC_MIXED) return enqueue_press_mixed(h);
   int r;
   launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
   if (use_split(h)) {
@@ -759,8 +839,9 @@ extern "C" int les_run(les_handle* h, int n0, int nsteps) {
     --nsteps;
   }
   // a step from a state with arbitrary p halos takes the in-place solve once; every step leaves
-  // the halos mirror-consistent, the state the captured graph assumes
-  while (nsteps > 0 && !h->p_consistent) {
+  // the halos mirror-consistent, the state the captured graph assumes.  Mixed precision reads
+  // the residual on the host between passes, so its steps are not captured.
+  while (nsteps > 0 && (!h->p_consistent || h->cfg.precision == LES_PREC_MIXED)) {
     int r = enqueue_step(h, n);
     if (r) return r;
     ++n;
@@ -947,6 +1028,17 @@ extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms
       case LES_KERNEL_RHS: launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc); break;
       case LES_KERNEL_VELNW: launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 2); break;
       case LES_KERNEL_STEP: HIPCHK(h, hipGraphLaunch(h->gexec, h->st)); break;
+      case LES_KERNEL_SOR_ITER: {
+        if (!sor_iter_supported(d, h->sp)) {
+          hipEventDestroy(e0);
+          hipEventDestroy(e1);
+          return fail(h, LES_E_STATE, "time_kernel: the fused SOR iteration does not apply");
+        }
+        const SorPair io = (q & 1) ? SorPair{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}}
+                                   : SorPair{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
+        launch_sor_iter(h->st, d, g, h->sp, h->ph, io, h->sc, h->iter_shape);
+        break;
+      }
       default:
         hipEventDestroy(e0);
         hipEventDestroy(e1);
@@ -960,6 +1052,56 @@ extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   *avg_ms = ms / (float)reps;
+  return check_launch(h);
+}
+
+// Autotuner: times every workgroup tile of the fused SOR iteration on this handle's grid (the
+// split copy is scratch between solves, so the trial launches change no simulation state) and
+// keeps the fastest; the captured step graph is rebuilt with it.
+extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
+  if (!h || reps < 1) return LES_E_INVALID;
+  STAGE_BEGIN(h);
+  les_tuning best{};
+  best.sor_iter_ms = -1.0f;
+  if (sor_iter_supported(d, h->sp)) {
+    const int tjs[] = {4, 8, 16};
+    const int kcs[] = {5, 10, 15, 23, 30, 45};
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    for (int tj : tjs)
+      for (int kc : kcs) {
+        if (kc > h->d.kp) continue;
+        const SorIterShape shp{tj, kc};
+        for (int q = 0; q < 4; ++q) {  // warm-up
+          const SorPair io = (q & 1) ? SorPair{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}}
+                                     : SorPair{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
+          launch_sor_iter(h->st, d, g, h->sp, h->ph, io, h->sc, shp);
+        }
+        HIPCHK(h, hipEventRecord(e0, h->st));
+        for (int q = 0; q < reps; ++q) {
+          const SorPair io = (q & 1) ? SorPair{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}}
+                                     : SorPair{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
+          launch_sor_iter(h->st, d, g, h->sp, h->ph, io, h->sc, shp);
+        }
+        HIPCHK(h, hipEventRecord(e1, h->st));
+        HIPCHK(h, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        if (best.sor_iter_ms < 0.0f || ms < best.sor_iter_ms) {
+          best.sor_iter_ms = ms;
+          best.sor_tile_j = tj;
+          best.sor_tile_k = kc;
+        }
+      }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    h->iter_shape = SorIterShape{best.sor_tile_j, best.sor_tile_k};
+    h->graph_valid = false;
+  }
+  h->tuned = true;
+  if (out) *out = best;
   return check_launch(h);
 }
 

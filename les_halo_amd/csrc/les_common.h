@@ -62,6 +62,20 @@ struct Grid1D {
   __device__ float DZS(int k) const { return dzs[k + 1]; }
 };
 
+// a / b with IEEE rounding.  When b is a positive normal power of two whose reciprocal is
+// normal (biased exponent 1..253), 1/b is exact and a * (1/b) is the same real number as a / b,
+// so both round to the same float (subnormal results included): one multiply instead of the
+// division sequence (scale, reciprocal, four FMAs, fixup).  Grid spacings here are mostly
+// powers of two (4 m horizontal, 2 m vertical below the stretching), so the fast path is the
+// common one and, for a wavefront-uniform b, a uniform branch.
+__device__ __forceinline__ float qdiv(float a, float b) {
+  const uint32_t bb = __float_as_uint(b);
+  const uint32_t e = bb & 0x7f800000u;
+  if ((bb & 0x807fffffu) == 0u && e - 0x00800000u <= 0x7e000000u)
+    return a * __uint_as_float(0x7f000000u - e);
+  return a / b;
+}
+
 // ------------------------------------------------------------------------------------------
 // Exact order-independent summation of FP32 terms.
 //

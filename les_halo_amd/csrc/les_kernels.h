@@ -17,9 +17,12 @@ struct Fields {
 struct DevScalars {
   float uout, rhsav, area, pav, pco, sor;
   int sor_iters, sor_done;
-  double resid_l2;
+  double resid_l2;          // mixed precision: |rhs - L p|_2 / |rhs|_2 of the last check
   int refine_passes;
   unsigned int uout_count;  // workgroups of k_uout done (re-armed to 0 by the last one)
+  // mixed precision (FP64 reductions and iterate)
+  double rhsav64, area64, pav64, rhs_l2;
+  int refine_done;
 };
 
 struct Physics {
@@ -38,6 +41,17 @@ struct SplitSOR {
   int nr, nj;  // padded colour row length, jp
   int uniform;
 };
+
+// Fused SOR iteration (k_sor_iter): previous and next iterate, colour 0 and colour 1 each
+struct SorPair {
+  const float* pin[2];
+  float* pout[2];
+};
+// workgroup tile of k_sor_iter: tj rows of j (4, 8 or 16), kc levels of k
+struct SorIterShape {
+  int tj, kc;
+};
+constexpr int SOR_ITER_NRMAX = 160;  // longest colour row (ip <= 320) the fused kernel stages in LDS
 
 // Reduction scratch: per-block carry-save limbs.
 struct RedBuf {
@@ -77,6 +91,9 @@ void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScal
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc);
 int sor_split_blocks(Dims d);
+int sor_iter_supported(Dims d, const SplitSOR& sp);
+void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
+                     DevScalars* sc, SorIterShape shape);
 void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc);
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                              int subtract_pav);

@@ -1,0 +1,26 @@
+// Device state and launchers of the mixed-precision pressure solve (les_mixed.hip).
+#pragma once
+#include "les_kernels.h"
+
+namespace les {
+
+// FP64 companions of the colour-split SOR copy (same indexing as SplitSOR::p / SplitSOR::r)
+struct Mixed {
+  double* p64[2];     // pressure iterate
+  double* r64[2];     // rhs - rhsav64
+  double* part;       // per-block partial sums, 2 * nblk
+  const float* cn1;   // cn1(1:ip,1:jp,1:kp)
+  int nblk;
+};
+
+int mixed_blocks(Dims d);
+void launch_mx_rhsav(hipStream_t st, Dims d, RedBuf rb, DevScalars* sc);
+void launch_mx_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, Mixed mx, DevScalars* sc);
+void launch_mx_promote(hipStream_t st, Dims d, SplitSOR sp, Mixed mx);
+void launch_mx_resid(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int first,
+                     double tol, DevScalars* sc);
+void launch_mx_accumulate(hipStream_t st, Dims d, SplitSOR sp, Mixed mx);
+void launch_mx_pav_gather(hipStream_t st, Dims d, Grid1D g, Fields F, SplitSOR sp, Mixed mx,
+                          DevScalars* sc);
+
+}  // namespace les

@@ -1,0 +1,235 @@
+// Mixed-precision pressure solve (LES_PREC_MIXED): the stencils and the SOR sweeps stay FP32;
+// the rhs mean, the residual norm, the pressure iterate and its mean run in FP64.
+//
+// press.f95:49-146 restated as iterative refinement of the discrete Poisson problem that the
+// SOR of press.f95:91-100 relaxes.  Its fixed point p = cn1*(sum_nb c_nb*p_nb - rhs) is
+// L p = rhs with L p = sum_nb c_nb*p_nb - p/cn1 (cn2l/cn2s along i, cn3l/cn3s along j,
+// cn4l/cn4s along k, init.f95:82-107), mirror images across the i and k faces and periodic j
+// (boundp.f95).  The solve:
+//   rhs64 = rhs - rhsav64                 (FP64 mean of vol*rhs, exact sum rounded to FP64)
+//   p32   = SOR_nmaxp(p, rhs64)           (the FP32 colour-split sweeps)
+//   p64   = p32
+//   repeat: r64 = rhs64 - L p64 (FP64);  stop when |r64|_2 <= tol * |rhs64|_2
+//           e32 = SOR_nmaxp(0, r64)       (FP32 sweeps on the correction)
+//           p64 = p64 + e32
+//   p = p64 - pav64                        (FP64 mean of vol*p64), with every boundary image.
+// In exact arithmetic SOR from p on rhs equals p plus SOR from 0 on rhs - L p, so each pass
+// continues the reference iteration while FP64 carries the accumulated solution.  All state is
+// in the colour-split layout of les_sor.hip (interior cells, colour (i+j+k)&1).
+#include "les_mixed.h"
+
+namespace les {
+
+namespace {
+
+__device__ __forceinline__ int split_index(const SplitSOR& sp, int i, int j, int k) {
+  return ((i - 1) >> 1) + sp.nr * ((j - 1) + sp.nj * (k - 1));
+}
+
+__device__ __forceinline__ double p64_at(const SplitSOR& sp, const Mixed& mx, int i, int j, int k) {
+  return mx.p64[(i + j + k) & 1][split_index(sp, i, j, k)];
+}
+
+__device__ __forceinline__ int tid() { return threadIdx.x + blockDim.x * threadIdx.y; }
+
+// Block sum of one double per thread (256 threads, 4 wavefronts), in a fixed order; the total
+// is valid in thread 0.  Every thread of the block must call it.
+__device__ double block_sum(double v, double* lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = tid() & 63, wave = tid() >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (tid() == 0) s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  return s;
+}
+
+constexpr int MX_BX = 64, MX_BY = 4;
+
+inline dim3 mx_grid(Dims d) { return dim3((d.ip + MX_BX - 1) / MX_BX, (d.jp + MX_BY - 1) / MX_BY, d.kp); }
+inline int mx_blocks(Dims d) {
+  dim3 g = mx_grid(d);
+  return (int)(g.x * g.y * g.z);
+}
+
+__device__ __forceinline__ int flat_block() {
+  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+}  // namespace
+
+// ---- rhsav in FP64 from the exact partial sums of k_rhs (press.f95:65-75) ----
+__global__ __launch_bounds__(256) void k_mx_rhsav(const int64_t* partials, int nb, DevScalars* sc) {
+  __shared__ int64_t limbs[NLIMB];
+  sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) {
+    sc->rhsav64 = limbs_to_double(limbs) / sc->area64;
+    sc->rhsav = (float)sc->rhsav64;
+  }
+}
+
+// ---- p -> split FP32; rhs64 = rhs - rhsav64 (split FP64 and its FP32 rounding for the sweeps) ----
+__global__ __launch_bounds__(256) void k_mx_scatter(Dims d, Fields F, SplitSOR sp, Mixed mx,
+                                                    const DevScalars* sc) {
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int par = (i + j + k) & 1;
+  const int is = split_index(sp, i, j, k);
+  sp.p[par][is] = F.p[d.P(i, j, k)];
+  const long ir = d.R(i, j, k);
+  const double r = (double)F.rhs[ir] - sc->rhsav64;
+  mx.r64[par][is] = r;
+  sp.r[par][is] = (float)r;
+  F.rhs[ir] = (float)r;
+}
+
+// ---- p64 = p32 ----
+__global__ __launch_bounds__(256) void k_mx_promote(Dims d, SplitSOR sp, Mixed mx) {
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+  mx.p64[par][is] = (double)sp.p[par][is];
+}
+
+// ---- r = rhs64 - L p64 in FP64; sweeps get (float)r and a zero correction; block sums of
+// r^2 (and of rhs64^2 when first != 0) ----
+__global__ __launch_bounds__(256) void k_mx_resid(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
+                                                  int first) {
+  __shared__ double lds[4];
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double rr = 0.0, bb = 0.0;
+  if (i <= d.ip && j <= d.jp) {
+    const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+    const double pc = mx.p64[par][is];
+    const double pe = (i == d.ip) ? pc : p64_at(sp, mx, i + 1, j, k);
+    const double pw = (i == 1) ? pc : p64_at(sp, mx, i - 1, j, k);
+    const double pn = p64_at(sp, mx, i, j == d.jp ? 1 : j + 1, k);
+    const double ps = p64_at(sp, mx, i, j == 1 ? d.jp : j - 1, k);
+    const double pu = (k == d.kp) ? pc : p64_at(sp, mx, i, j, k + 1);
+    const double pd = (k == 1) ? pc : p64_at(sp, mx, i, j, k - 1);
+    const double cn1 = (double)mx.cn1[d.O(i, j, k)];
+    const double lp = (double)g.cn2l[i - 1] * pe + (double)g.cn2s[i - 1] * pw +
+                      (double)g.cn3l[j - 1] * pn + (double)g.cn3s[j - 1] * ps +
+                      (double)g.cn4l[k - 1] * pu + (double)g.cn4s[k - 1] * pd - pc / cn1;
+    const double b = mx.r64[par][is];
+    const double r = b - lp;
+    sp.r[par][is] = (float)r;
+    sp.p[par][is] = 0.0f;
+    rr = r * r;
+    bb = b * b;
+  }
+  const double s = block_sum(rr, lds);
+  if (tid() == 0) mx.part[flat_block()] = s;
+  if (first) {
+    __syncthreads();
+    const double t = block_sum(bb, lds);
+    if (tid() == 0) mx.part[mx.nblk + flat_block()] = t;
+  }
+}
+
+// ---- fixed-order sum of the block partials; relative residual and the stop flag ----
+__global__ __launch_bounds__(256) void k_mx_resid_finalize(Mixed mx, int first, double tol,
+                                                           DevScalars* sc) {
+  __shared__ double lds[4];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < mx.nblk; q += 256) {
+    a += mx.part[q];
+    if (first) b += mx.part[mx.nblk + q];
+  }
+  const double sa = block_sum(a, lds);
+  __syncthreads();
+  const double sb = block_sum(b, lds);
+  if (threadIdx.x == 0) {
+    if (first) sc->rhs_l2 = sqrt(sb);
+    const double rn = sqrt(sa);
+    sc->resid_l2 = (sc->rhs_l2 > 0.0) ? rn / sc->rhs_l2 : rn;
+    sc->refine_done = (sc->resid_l2 <= tol) ? 1 : 0;
+  }
+}
+
+// ---- p64 += e32 ----
+__global__ __launch_bounds__(256) void k_mx_accumulate(Dims d, SplitSOR sp, Mixed mx) {
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+  mx.p64[par][is] = mx.p64[par][is] + (double)sp.p[par][is];
+}
+
+// ---- pav64 = sum(vol * p64) / sum(vol) (press.f95:123-134 in FP64) ----
+__global__ __launch_bounds__(256) void k_mx_pav(Dims d, Grid1D g, SplitSOR sp, Mixed mx) {
+  __shared__ double lds[4];
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double v = 0.0;
+  if (i <= d.ip && j <= d.jp)
+    v = p64_at(sp, mx, i, j, k) * ((double)g.DX1(i) * (double)g.DY1(j) * (double)g.DZN(k));
+  const double s = block_sum(v, lds);
+  if (tid() == 0) mx.part[flat_block()] = s;
+}
+
+__global__ __launch_bounds__(256) void k_mx_pav_finalize(Mixed mx, DevScalars* sc) {
+  __shared__ double lds[4];
+  double a = 0.0;
+  for (int q = threadIdx.x; q < mx.nblk; q += 256) a += mx.part[q];
+  const double s = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    sc->pav64 = s / sc->area64;
+    sc->pav = (float)sc->pav64;
+  }
+}
+
+// ---- p(0:ip+1,0:jp+1,0:kp+1) = p64(mirror cell) - pav64, rounded once to FP32 ----
+__global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp, Mixed mx,
+                                                   const DevScalars* sc) {
+  const int a = blockIdx.x * 64 + threadIdx.x;
+  const int b = blockIdx.y * 4 + threadIdx.y;
+  const int c = blockIdx.z;
+  if (a > d.ip + 1 || b > d.jp + 1) return;
+  const int i = (a == 0) ? 1 : ((a == d.ip + 1) ? d.ip : a);
+  const int j = (b == 0) ? d.jp : ((b == d.jp + 1) ? 1 : b);
+  const int k = (c == 0) ? 1 : ((c == d.kp + 1) ? d.kp : c);
+  F.p[d.P(a, b, c)] = (float)(p64_at(sp, mx, i, j, k) - sc->pav64);
+}
+
+int mixed_blocks(Dims d) { return mx_blocks(d); }
+
+void launch_mx_rhsav(hipStream_t st, Dims d, RedBuf rb, DevScalars* sc) {
+  hipLaunchKernelGGL(k_mx_rhsav, dim3(1), dim3(256), 0, st, rb.partials, reduction_blocks(d), sc);
+}
+
+void launch_mx_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, Mixed mx, DevScalars* sc) {
+  hipLaunchKernelGGL(k_mx_scatter, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, F, sp, mx, sc);
+}
+
+void launch_mx_promote(hipStream_t st, Dims d, SplitSOR sp, Mixed mx) {
+  hipLaunchKernelGGL(k_mx_promote, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, sp, mx);
+}
+
+void launch_mx_resid(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int first,
+                     double tol, DevScalars* sc) {
+  hipLaunchKernelGGL(k_mx_resid, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, g, sp, mx, first);
+  hipLaunchKernelGGL(k_mx_resid_finalize, dim3(1), dim3(256), 0, st, mx, first, tol, sc);
+}
+
+void launch_mx_accumulate(hipStream_t st, Dims d, SplitSOR sp, Mixed mx) {
+  hipLaunchKernelGGL(k_mx_accumulate, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, sp, mx);
+}
+
+void launch_mx_pav_gather(hipStream_t st, Dims d, Grid1D g, Fields F, SplitSOR sp, Mixed mx,
+                          DevScalars* sc) {
+  hipLaunchKernelGGL(k_mx_pav, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, g, sp, mx);
+  hipLaunchKernelGGL(k_mx_pav_finalize, dim3(1), dim3(256), 0, st, mx, sc);
+  dim3 grid((d.ip + 2 + 63) / 64, (d.jp + 2 + 3) / 4, d.kp + 2);
+  hipLaunchKernelGGL(k_mx_gather, grid, dim3(64, 4), 0, st, d, F, sp, mx, sc);
+}
+
+}  // namespace les

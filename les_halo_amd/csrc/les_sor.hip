@@ -161,6 +161,169 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
   }
 }
 
+// ---- one whole SOR iteration (colour 1 then colour 0, press.f95:90-100 with nrd = 0, 1) per
+// launch, on a uniform horizontal grid without residual.
+//
+// The iteration reads the previous iterate (pin) and writes the next (pout), so workgroups
+// never see each other's results.  A workgroup owns TJ rows of j (all of i) and a chunk of k
+// levels and marches up in k with two rings of three levels in LDS:
+//   sB: the old colour-0 values, rows j0-2 .. j0+TJ+1 (TJ+4 rows);
+//   sA: the new colour-1 values, rows j0-1 .. j0+TJ   (TJ+2 rows), recomputed on the two
+//       rows next to the tile so that colour 0 at the tile's edge rows sees final colour-1
+//       neighbours.
+// At level k it loads old colour 0 at k+2, updates colour 1 at k+1 from it, then updates
+// colour 0 at k from the colour-1 ring and writes both colours of level k.  Every cell gets the
+// same FP32 expression as in k_sor_split with the same operand values, so the result is
+// bit-identical to the two colour sweeps.  Traffic per iteration: p read once, rhs read once,
+// p written once (12 B per cell) instead of 16 B for the two sweeps.
+template <int TJ>
+__global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
+                                                  SorPair io, const DevScalars* sc, int kc,
+                                                  int njt) {
+  __shared__ float sB[3][TJ + 4][SOR_ITER_NRMAX];
+  __shared__ float sA[3][TJ + 2][SOR_ITER_NRMAX];
+  if (sc->sor_done) return;
+  const int L = xcd_tile(blockIdx.x, gridDim.x);
+  const int jt = L % njt, kt = L / njt;
+  const int j0 = 1 + jt * TJ;
+  const int k0 = 1 + kt * kc;
+  if (k0 > d.kp) return;  // padding workgroups of the XCD remap (uniform per workgroup)
+  const int k1 = min(d.kp, k0 + kc - 1);
+  const int tid = threadIdx.x;
+  const int ng = sp.nr >> 2;
+  const int jp = d.jp, kp = d.kp, ip = d.ip;
+  const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
+  const float om = ph.omega;
+  auto wrapj = [&](int j) { return ((j - 1) % jp + jp) % jp + 1; };  // periodic j
+  auto rowbase = [&](int j, int k) { return sp.nr * ((j - 1) + jp * (k - 1)); };
+
+  // old colour 0 of level k (1 <= k <= kp) -> sB ring
+  auto load_b = [&](int k) {
+    float(*dst)[SOR_ITER_NRMAX] = sB[k % 3];
+    for (int w = tid; w < (TJ + 4) * ng; w += 256) {
+      const int r = w / ng, q = w - r * ng;
+      const float4 v =
+          *reinterpret_cast<const float4*>(io.pin[0] + rowbase(wrapj(j0 - 2 + r), k) + 4 * q);
+      *reinterpret_cast<float4*>(&dst[r][4 * q]) = v;
+    }
+  };
+  // new colour 1 of level k (1 <= k <= kp) on rows j0-1 .. j0+TJ -> sA ring
+  auto update_a = [&](int k) {
+    const float cn1 = sp.cn1k[k], c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+    const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
+    const float(*bu)[SOR_ITER_NRMAX] = sB[(k + 1) % 3];
+    const float(*bd)[SOR_ITER_NRMAX] = sB[(k + 2) % 3];  // == level k-1
+    float(*dst)[SOR_ITER_NRMAX] = sA[k % 3];
+    for (int w = tid; w < (TJ + 2) * ng; w += 256) {
+      const int r = w / ng, q = w - r * ng;
+      const int j = wrapj(j0 - 1 + r);
+      const int f = ((1 + j + k) & 1) == 1 ? 1 : 2;  // first i of colour 1 in this row
+      const int base = rowbase(j, k) + 4 * q;
+      const float4 own = *reinterpret_cast<const float4*>(io.pin[1] + base);
+      const float4 rhs = *reinterpret_cast<const float4*>(sp.r[1] + base);
+      const float pv[4] = {own.x, own.y, own.z, own.w};
+      const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ii = 4 * q + e;
+        const int i = f + 2 * ii;
+        float pc = pv[e];
+        if (i <= ip) {
+          const float pe = (i == ip) ? pc : bm[r + 1][ii + f - 1];
+          const float pw = (i == 1) ? pc : bm[r + 1][ii + f - 2];
+          const float pn = bm[r + 2][ii], ps = bm[r][ii];
+          const float pu = (k < kp) ? bu[r + 1][ii] : pc;
+          const float pd = (k > 1) ? bd[r + 1][ii] : pc;
+          const float reltmp =
+              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd - rv[e]) -
+                    pc);
+          pc = pc + reltmp;
+        }
+        dst[r][ii] = pc;
+      }
+    }
+  };
+  // new colour 0 of level k on the tile rows; writes both colours of level k
+  auto update_b = [&](int k) {
+    const float cn1 = sp.cn1k[k], c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+    const float(*am)[SOR_ITER_NRMAX] = sA[k % 3];
+    const float(*au)[SOR_ITER_NRMAX] = sA[(k + 1) % 3];
+    const float(*ad)[SOR_ITER_NRMAX] = sA[(k + 2) % 3];  // == level k-1
+    const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
+    for (int w = tid; w < TJ * ng; w += 256) {
+      const int r = w / ng, q = w - r * ng;
+      const int j = j0 + r;
+      if (j > jp) continue;
+      const int f = ((1 + j + k) & 1) == 0 ? 1 : 2;  // first i of colour 0 in this row
+      const int base = rowbase(j, k) + 4 * q;
+      const float4 rhs = *reinterpret_cast<const float4*>(sp.r[0] + base);
+      const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
+      float out[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ii = 4 * q + e;
+        const int i = f + 2 * ii;
+        float pc = bm[r + 2][ii];
+        if (i <= ip) {
+          const float pe = (i == ip) ? pc : am[r + 1][ii + f - 1];
+          const float pw = (i == 1) ? pc : am[r + 1][ii + f - 2];
+          const float pn = am[r + 2][ii], ps = am[r][ii];
+          const float pu = (k < kp) ? au[r + 1][ii] : pc;
+          const float pd = (k > 1) ? ad[r + 1][ii] : pc;
+          const float reltmp =
+              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd - rv[e]) -
+                    pc);
+          pc = pc + reltmp;
+        }
+        out[e] = pc;
+      }
+      *reinterpret_cast<float4*>(io.pout[0] + base) = make_float4(out[0], out[1], out[2], out[3]);
+      *reinterpret_cast<float4*>(io.pout[1] + base) =
+          make_float4(am[r + 1][4 * q], am[r + 1][4 * q + 1], am[r + 1][4 * q + 2], am[r + 1][4 * q + 3]);
+    }
+  };
+
+  // prologue: colour 1 at k0-1 and k0 (colour 0 at k0 needs both)
+  if (k0 - 2 >= 1) load_b(k0 - 2);
+  if (k0 - 1 >= 1) load_b(k0 - 1);
+  load_b(k0);
+  __syncthreads();
+  if (k0 - 1 >= 1) update_a(k0 - 1);
+  __syncthreads();
+  if (k0 + 1 <= kp) load_b(k0 + 1);
+  __syncthreads();
+  update_a(k0);
+  for (int k = k0; k <= k1; ++k) {
+    __syncthreads();  // sA(k) complete; sB slot of level k-1 free
+    if (k + 2 <= kp) load_b(k + 2);
+    __syncthreads();
+    if (k + 1 <= kp) update_a(k + 1);
+    __syncthreads();
+    update_b(k);
+  }
+}
+
+int sor_iter_supported(Dims d, const SplitSOR& sp) { return sp.uniform && sp.nr <= SOR_ITER_NRMAX; }
+
+void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
+                     DevScalars* sc, SorIterShape shape) {
+  const int kc = shape.kc > 0 ? shape.kc : 10;
+  const int nkt = (d.kp + kc - 1) / kc;
+  if (shape.tj == 4) {
+    const int njt = (d.jp + 3) / 4;
+    const int nwg = (njt * nkt + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_sor_iter<4>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
+  } else if (shape.tj == 16) {
+    const int njt = (d.jp + 15) / 16;
+    const int nwg = (njt * nkt + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_sor_iter<16>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
+  } else {
+    const int njt = (d.jp + 7) / 8;
+    const int nwg = (njt * nkt + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_sor_iter<8>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
+  }
+}
+
 namespace {
 void split_threads(Dims d, const SplitSOR& sp, int& ng, int& nthreads, int& nwg) {
   ng = sp.nr / 4;

@@ -229,7 +229,16 @@ class LES:
                                               ctypes.c_void_p(dev_buffer)), "halo_write_arrays")
 
     KERNELS = {"sor_sweep": 0, "velfg": 1, "visc_adam": 2, "calc_sm": 3, "rhs": 4, "velnw": 5,
-               "step": 6}
+               "step": 6, "sor_iter": 7}
+
+    def autotune(self, reps=20):
+        """Picks the fused SOR iteration's workgroup tile by timing every candidate on this
+        grid (les_autotune); returns the choice."""
+        from ._lib import LesTuning
+        t = LesTuning()
+        self._chk(lib().les_autotune(self.h, reps, ctypes.byref(t)), "autotune")
+        return {"sor_tile_j": t.sor_tile_j, "sor_tile_k": t.sor_tile_k,
+                "sor_iter_ms": float(t.sor_iter_ms)}
 
     def time_kernel(self, which, reps=20):
         """Average milliseconds per launch of one kernel (HIP events on the handle's stream)."""

@@ -70,11 +70,15 @@ typedef struct {
   double dt, ro, vn, alpha, beta, cs0, omega, pjuge;
   int nmaxp;
   int sum_mode;  // 0 = sequential, 1 = exact
+  double refine_tol;  // > 0: after the SOR, more blocks of nmaxp iterations until the relative
+  int refine_max;     //      residual |rhs - L p|_2/|rhs|_2 <= refine_tol (at most refine_max)
 } orc_params;
 
 typedef struct {
   double uout, rhsav, area, pav, pco, sor;
   int sor_iters;
+  double resid_l2;
+  int refine_passes;
 } orc_scalars;
 
 }  // extern "C"
@@ -849,11 +853,42 @@ T press_pav_adj(F<T>& a, int sum_mode, T* pco_out) {
   return pav;
 }
 
+// Relative L2 residual |rhs - L p|_2 / |rhs|_2 of the Poisson problem whose fixed point the SOR
+// of press.f95:91-100 relaxes: L p = sum_nb c_nb p_nb - p/cn1 with the current boundary images
+// of p (mirror in i and k, periodic in j after boundp), accumulated in FP64.
+template <typename T>
+double rel_residual(F<T>& a) {
+  double rr = 0.0, bb = 0.0;
+  for (int k = 1; k <= a.km; ++k)
+    for (int j = 1; j <= a.jm; ++j)
+      for (int i = 1; i <= a.im; ++i) {
+        const double lp = (double)a.cn2l(i) * a.p(i + 1, j, k) + (double)a.cn2s(i) * a.p(i - 1, j, k) +
+                          (double)a.cn3l(j) * a.p(i, j + 1, k) + (double)a.cn3s(j) * a.p(i, j - 1, k) +
+                          (double)a.cn4l(k) * a.p(i, j, k + 1) + (double)a.cn4s(k) * a.p(i, j, k - 1) -
+                          (double)a.p(i, j, k) / (double)a.cn1(i, j, k);
+        const double b = a.rhs(i, j, k), r = b - lp;
+        rr += r * r;
+        bb += b * b;
+      }
+  return bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
+}
+
 template <typename T>
 void press(F<T>& a, const orc_params& q, orc_scalars* out) {
   T area, pco, sorv;
   T rhsav = press_rhsav(a, (T)q.dt, q.sum_mode, &area);
   int it = press_sor(a, (T)q.omega, (T)q.pjuge, q.nmaxp, q.sum_mode, &sorv);
+  int passes = 0;
+  double res = 0.0;
+  if (q.refine_tol > 0.0) {  // converged solve: further blocks of nmaxp iterations
+    res = rel_residual(a);
+    while (res > q.refine_tol && passes < q.refine_max) {
+      T s2;
+      press_sor(a, (T)q.omega, T(0), q.nmaxp, q.sum_mode, &s2);
+      ++passes;
+      res = rel_residual(a);
+    }
+  }
   T pav = press_pav_adj(a, q.sum_mode, &pco);
   if (out) {
     out->rhsav = rhsav;
@@ -862,6 +897,8 @@ void press(F<T>& a, const orc_params& q, orc_scalars* out) {
     out->sor_iters = it;
     out->pav = pav;
     out->pco = pco;
+    out->resid_l2 = res;
+    out->refine_passes = passes;
   }
 }
 

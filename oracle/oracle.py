@@ -33,12 +33,14 @@ class OrcFields(ctypes.Structure):
 class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ["dt", "ro", "vn", "alpha", "beta", "cs0", "omega", "pjuge"]] + \
-               [("nmaxp", ctypes.c_int), ("sum_mode", ctypes.c_int)]
+               [("nmaxp", ctypes.c_int), ("sum_mode", ctypes.c_int),
+                ("refine_tol", ctypes.c_double), ("refine_max", ctypes.c_int)]
 
 
 class OrcScalars(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ["uout", "rhsav", "area", "pav", "pco", "sor"]] + \
-               [("sor_iters", ctypes.c_int)]
+               [("sor_iters", ctypes.c_int), ("resid_l2", ctypes.c_double),
+                ("refine_passes", ctypes.c_int)]
 
 
 SUM_SEQ, SUM_EXACT = 0, 1
@@ -119,7 +121,8 @@ class OracleState:
         for n in ("fx", "fy", "fz", "dfu1", "dfv1", "dfw1"):
             self.a[n] = np.zeros(_scratch_size(n, ip, jp, kp), self.dtype)
         self.params = dict(dt=0.2, ro=1.1763, vn=1.583e-5, alpha=-10.0, beta=-1.0, cs0=0.1,
-                           omega=1.0, pjuge=1e-4, nmaxp=50, sum_mode=SUM_SEQ)
+                           omega=1.0, pjuge=1e-4, nmaxp=50, sum_mode=SUM_SEQ,
+                           refine_tol=0.0, refine_max=0)
         self.params.update(physics)
         self.scalars = OrcScalars()
 
@@ -163,6 +166,8 @@ class OracleState:
             setattr(q, k, v)
         q.nmaxp = int(p["nmaxp"])
         q.sum_mode = int(p["sum_mode"])
+        q.refine_tol = float(p["refine_tol"])
+        q.refine_max = int(p["refine_max"])
         return q
 
     def call(self, stage, n=0, **over):

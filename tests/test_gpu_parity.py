@@ -277,3 +277,24 @@ def test_run_after_inconsistent_upload(case):
     for n in range(3, 7):
         o.step(n)
     _compare(L, o, "run after inconsistent upload")
+
+
+@pytest.mark.parametrize("dims", [(40, 32, 20), (37, 18, 23)])
+def test_fused_sor_iteration_after_autotune(dims):
+    """The fused iteration (both colours per launch, ping-pong iterates) with the tile the
+    autotuner picks reproduces the oracle's trajectory bit for bit (50 iterations at omega 1.5,
+    no early exit)."""
+    case = common.tiny_case(*dims, seed=21)
+    L, o = _pair(case, **common.CFG_SOR)
+    t = L.autotune(2)
+    assert t["sor_tile_j"] in (4, 8, 16) and t["sor_tile_k"] > 0, t
+    L.ifdata()
+    o.ifdata()
+    for n in range(1, 4):
+        L.step(n)
+        o.step(n)
+    _compare(L, o, "fused SOR after autotune %s" % (t,), ["u", "v", "w", "p", "rhs", "sm"])
+    L.run(4, 2)
+    for n in (4, 5):
+        o.step(n)
+    _compare(L, o, "fused SOR, graph replay", ["u", "v", "w", "p"])

@@ -226,3 +226,63 @@ def test_fp64_oracle_tracks_fp32():
     for n in ("u", "v", "w"):
         assert common.rel_l2(o32.a[n], o64.a[n]) < 1e-4, n
     assert common.rel_l2(o32.a["p"], o64.a["p"]) < 1e-3
+
+
+def _press_residual(o, **over):
+    o.call("press", **over)
+    return o.scalars.resid_l2, o.scalars.refine_passes
+
+
+def test_poisson_residual_pins():
+    """The FP64 residual |rhs - L p|/|rhs| of the converged-solve mode: 1 for p = 0 and for any
+    constant p (L annihilates constants, which a dropped term, a wrong sign or a wrong
+    neighbour index breaks), and scale invariant."""
+    case = common.tiny_case()
+    o = common.oracle_from_case(case, np.float64, **common.REF_PHYS, **common.CFG_SOR)
+    o.ifdata()
+    o.step(1)
+    base = o.copy()
+    chk = dict(nmaxp=0, refine_tol=1e-300, refine_max=0)
+    o.a["p"][:] = 0.0
+    r0, _ = _press_residual(o, **chk)
+    assert abs(r0 - 1.0) < 1e-14
+    o = base.copy()
+    o.a["p"][:] = 3.7
+    r1, _ = _press_residual(o, **chk)
+    assert abs(r1 - 1.0) < 1e-12
+    o = base.copy()
+    rng = np.random.default_rng(2)
+    o.a["p"][:] = rng.standard_normal(o.a["p"].size)
+    o2 = o.copy()
+    o2.a["p"][:] *= 2.0
+    o2.a["u"][:] *= 2.0
+    o2.a["v"][:] *= 2.0
+    o2.a["w"][:] *= 2.0
+    o2.a["f"][:] *= 2.0
+    o2.a["g"][:] *= 2.0
+    o2.a["h"][:] *= 2.0
+    ra, _ = _press_residual(o, **chk)
+    rb, _ = _press_residual(o2, **chk)
+    assert abs(ra - rb) < 1e-12 * max(ra, 1.0) and ra > 1e-3
+
+
+def test_refinement_converges_and_continues_sor():
+    """Refinement passes are further blocks of nmaxp SOR iterations, run until the FP64
+    residual reaches the tolerance; the residual decreases with every block."""
+    case = common.tiny_case()
+    o = common.oracle_from_case(case, np.float64, **common.REF_PHYS, **common.CFG_SOR)
+    o.ifdata()
+    o.step(1)
+    res = []
+    for nm in (10, 20, 40):
+        q = o.copy()
+        r, _ = _press_residual(q, nmaxp=nm, refine_tol=1e-300, refine_max=0)
+        res.append(r)
+    assert res[0] > res[1] > res[2]
+    q = o.copy()
+    r, passes = _press_residual(q, nmaxp=10, refine_tol=1e-9, refine_max=500)
+    assert r <= 1e-9 and passes > 0
+    # the same as 10 * (passes + 1) plain iterations
+    q2 = o.copy()
+    r2, _ = _press_residual(q2, nmaxp=10 * (passes + 1), refine_tol=1e-300, refine_max=0)
+    assert abs(r2 - r) <= 1e-6 * r
