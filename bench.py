@@ -95,6 +95,46 @@ def profile_sor(L, reps):
           flush=True)
 
 
+def ensemble_bench(nmembers, steps, warmup):
+    """The batched-ensemble configuration: nmembers members of uneven sizes (ip, jp in
+    96..320, kp 60 or 90, own terrain seed and u*), the same physics as the plain step.
+    Aggregate cell-steps/s of the members advanced together (les_ensemble_run) against the
+    same members run one after another (les_run each)."""
+    import torch
+    from les_halo_amd import LES, Ensemble, inputs
+    members, cells = [], 0
+    for spec in inputs.ensemble_specs(nmembers):
+        ip, jp, kp = spec["dims"]
+        L = LES(ip, jp, kp, **PHYS)
+        L.load_case(inputs.make_case(ip, jp, kp, seed=spec["seed"], ustar=spec["ustar"]))
+        L.ifdata()
+        L.autotune(5)
+        L.run(1, warmup)
+        members.append(L)
+        cells += ip * jp * kp
+    for L in members:
+        L.sync()
+    n0 = 1 + warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for L in members:
+        L.run(n0, steps)
+        L.sync()
+    t_seq = time.perf_counter() - t0
+    E = Ensemble(members)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E.run(n0 + steps, steps)
+    E.sync()
+    t_ens = time.perf_counter() - t0
+    E.close()
+    agg = lambda t: cells * steps / t / 1e6  # noqa: E731
+    return {"members": nmembers, "cells": cells, "steps": steps,
+            "value": round(agg(t_ens), 2), "unit": "M cell-steps/s",
+            "sequential_value": round(agg(t_seq), 2), "speedup_vs_sequential": round(t_seq / t_ens, 3),
+            "ms_per_step_all_members": round(t_ens / steps * 1e3, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +142,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="also run the batched-ensemble configuration with N members")
     ap.add_argument("--profile-sor", type=int, default=0, metavar="REPS",
                     help="counter runs: set up the workload, launch REPS SOR colour sweeps and "
                          "one device copy of the same algorithmic bytes (calibration), no timing")
@@ -162,6 +204,9 @@ def main():
                        "parallelism": "single domain, 1 GPU"},
             "step_hbm_frac": round(step_hbm_frac, 4), "step_bytes_per_cell": B_STEP,
             "roofline": roofline}
+    if args.ensemble:
+        del L
+        line["ensemble"] = ensemble_bench(args.ensemble, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(line), flush=True)

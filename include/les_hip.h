@@ -122,6 +122,8 @@ typedef struct les_handle les_handle;
 int les_create(const les_config* cfg, les_handle** out);
 int les_destroy(les_handle* h);
 const char* les_last_error(const les_handle* h);
+/* Interior sizes of the handle's grid. */
+int les_get_dims(const les_handle* h, int* ip, int* jp, int* kp);
 /* Waits for all work queued on the handle's stream. */
 int les_sync(les_handle* h);
 
@@ -193,9 +195,31 @@ int les_boundp2(les_handle* h);
 int les_step(les_handle* h, int n);
 /* Steps n0 .. n0+nsteps-1, replayed from a captured HIP graph of one step. */
 int les_run(les_handle* h, int n0, int nsteps);
+/* Ensemble of independent members (handles of any sizes, each with its own state and
+ * stream), advanced together: every step enqueues one graph launch of each member, step-major,
+ * so the members' kernels run concurrently on the device.  No counterpart in the reference
+ * (one domain per run); les_ensemble_run(e, n0, k) leaves every member as les_run(member, n0, k)
+ * would.  Members keep their ownership (les_ensemble_destroy does not destroy them); mixed
+ * precision members are not supported. */
+typedef struct les_ensemble les_ensemble;
+int les_ensemble_create(les_handle* const* members, int n, les_ensemble** out);
+int les_ensemble_run(les_ensemble* e, int n0, int nsteps);
+int les_ensemble_sync(les_ensemble* e);
+int les_ensemble_destroy(les_ensemble* e);
 /* ifdata.f95:258-273 on the device: bondv1 (inflow from wind_profile, n=1), boundp1,
  * boundp2, velfg, feedbf, les.  Fields must be zero (les_create) or uploaded. */
 int les_ifdata(les_handle* h);
+
+/* ---- restart and output files (Fortran unformatted sequential records) ---------------
+ * les_write_restart writes data20 (press.f95:192-199: u, v, w, p, usum, vsum, wsum over the
+ * interior, one record each) and data21 (adam.f95:24-29: fold, gold, hold, fghold = 0); with
+ * with_header != 0 data20 starts with the record (n, time), which makes it the data30 file of
+ * the ICAL = 1 start.  les_read_restart reads data30/data31 as ifdata.f95:227-247 does into the
+ * interiors (halos untouched; run les_ifdata afterwards, as the reference's init does) and
+ * returns n and time.  Both synchronise. */
+int les_write_restart(les_handle* h, const char* data20, const char* data21, int n, float time,
+                      int with_header);
+int les_read_restart(les_handle* h, const char* data30, const char* data31, int* n, float* time);
 
 /* Scalars of the last stages (synchronises). */
 typedef struct les_scalars {
@@ -299,6 +323,13 @@ int les_gen_terrain(int ip, int jp, float dx, float dy, uint64_t seed, float cov
  * bmask1 = cmask1 = dmask1 = 1 there); halos are left 0. */
 int les_gen_masks(int ip, int jp, int kp, const float* zbm, const float* z2, const float* dzn,
                   float* amask1, float* bmask1, float* cmask1, float* dmask1);
+/* feedbfm.f95:31-37: reads the GIS building-height file (the format of GIS/Tokyo_20mgrid.txt:
+ * 100 x 100 heights, one per line, rows j = 100..1) into zbm(-1:ipmax+1,-1:jpmax+1) at
+ * (i+25, j+25); other entries are left as they are.  Needs ipmax, jpmax >= 125. */
+int les_gis_read_zbm(const char* path, int ipmax, int jpmax, float* zbm);
+/* feedbfm.f95:42-63 on the device: the four masks (interiors; halos cleared) and the stencils'
+ * bit mask from zbm(-1:ip+1,-1:jp+1); z2 and dzn must be uploaded. */
+int les_set_terrain(les_handle* h, const float* zbm);
 /* Log-law inflow u(z) = (ustar/0.4) ln(z/z0) (1 + noise*xi_jk), z = z2(k)+0.5*dzn(k)
  * (the inflow height of bondv1.f95:29), xi ~ N(0,1) seeded. */
 int les_gen_inflow(int jp, int kp, const float* z2, const float* dzn, float ustar, float z0,

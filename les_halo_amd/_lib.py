@@ -44,6 +44,10 @@ PROTOS = {
     "les_destroy": (c_int, [H]),
     "les_last_error": (ctypes.c_char_p, [H]),
     "les_sync": (c_int, [H]),
+    "les_get_dims": (c_int, [H, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "les_write_restart": (c_int, [H, ctypes.c_char_p, ctypes.c_char_p, c_int, c_float, c_int]),
+    "les_read_restart": (c_int, [H, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(c_int),
+                                 ctypes.POINTER(c_float)]),
     "les_upload": (c_int, [H, c_int, c_void_p]),
     "les_download": (c_int, [H, c_int, c_void_p]),
     "les_device_ptr": (c_int, [H, c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_int64)]),
@@ -86,6 +90,10 @@ PROTOS = {
     "les_reduce_sum_exact": (c_int, [H, c_void_p, c_int64, P_float]),
     "les_time_kernel": (c_int, [H, c_int, c_int, P_float]),
     "les_autotune": (c_int, [H, c_int, ctypes.POINTER(LesTuning)]),
+    "les_ensemble_create": (c_int, [ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
+    "les_ensemble_run": (c_int, [c_void_p, c_int, c_int]),
+    "les_ensemble_sync": (c_int, [c_void_p]),
+    "les_ensemble_destroy": (c_int, [c_void_p]),
     "les_initialise_LES_kernel": (c_int, [c_void_p] * 42 + [c_float, c_int, c_int, c_int]),
     "les_run_LES_kernel": (c_int, [c_int, c_int, c_void_p]),
     "les_default_handle": (c_void_p, []),
@@ -99,6 +107,8 @@ PROTOS = {
     "les_gen_inflow": (c_int, [c_int, c_int, c_void_p, c_void_p, c_float, c_float, c_float,
                                c_uint64, c_void_p]),
     "les_initial_wind_profile": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "les_gis_read_zbm": (c_int, [ctypes.c_char_p, c_int, c_int, c_void_p]),
+    "les_set_terrain": (c_int, [H, c_void_p]),
 }
 
 _lib = None

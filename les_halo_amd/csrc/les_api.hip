@@ -423,6 +423,14 @@ extern "C" int les_destroy(les_handle* h) {
 
 extern "C" const char* les_last_error(const les_handle* h) { return h ? h->err.c_str() : ""; }
 
+extern "C" int les_get_dims(const les_handle* h, int* ip, int* jp, int* kp) {
+  if (!h || !ip || !jp || !kp) return LES_E_INVALID;
+  *ip = h->d.ip;
+  *jp = h->d.jp;
+  *kp = h->d.kp;
+  return LES_OK;
+}
+
 extern "C" int les_sync(les_handle* h) {
   if (!h) return LES_E_INVALID;
   HIPCHK(h, hipStreamSynchronize(h->st));
@@ -463,6 +471,29 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
     int r = refresh_split_factors(h);
     if (r) return r;
   }
+  return LES_OK;
+}
+
+// feedbfm.f95:42-63 on the device from a host zbm(-1:ip+1,-1:jp+1) (ipmax = ip, jpmax = jp);
+// needs z2 and dzn uploaded
+extern "C" int les_set_terrain(les_handle* h, const float* zbm) {
+  if (!h || !zbm) return LES_E_INVALID;
+  const size_t nz = (size_t)(h->d.ip + 3) * (h->d.jp + 3);
+  float* dz = nullptr;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMalloc(&dz, nz * sizeof(float)));
+  hipError_t e = hipMemcpy(dz, zbm, nz * sizeof(float), hipMemcpyHostToDevice);
+  for (int f = LES_AMASK1; f <= LES_DMASK1 && e == hipSuccess; ++f)
+    e = hipMemsetAsync(h->fld[f], 0, (size_t)h->cnt[f] * sizeof(float), h->st);
+  if (e == hipSuccess) {
+    launch_feedbfm(h->st, h->d, grid_of(h), h->fld[LES_Z2], dz, h->fld[LES_AMASK1],
+                   h->fld[LES_BMASK1], h->fld[LES_CMASK1], h->fld[LES_DMASK1], h->mask);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  hipFree(dz);
+  if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("set_terrain: ") + hipGetErrorString(e));
+  h->graph_valid = false;
   return LES_OK;
 }
 
@@ -828,13 +859,14 @@ extern "C" int les_step(les_handle* h, int n) {
   return enqueue_step(h, n);
 }
 
-extern "C" int les_run(les_handle* h, int n0, int nsteps) {
-  if (!h || nsteps < 0) return LES_E_INVALID;
-  int n = n0;
-  // the n == 1 step differs (interior fill): run it eagerly
+namespace {
+// Steps that cannot be replayed from the graph (n <= 1, a state with arbitrary p halos, mixed
+// precision) run eagerly; returns the first step number left for the graph.
+int run_eager_prefix(les_handle* h, int n, int& nsteps, int& r) {
+  r = LES_OK;
+  // the n == 1 step differs (interior fill)
   while (nsteps > 0 && n <= 1) {
-    int r = enqueue_step(h, n);
-    if (r) return r;
+    if ((r = enqueue_step(h, n))) return n;
     ++n;
     --nsteps;
   }
@@ -842,31 +874,99 @@ extern "C" int les_run(les_handle* h, int n0, int nsteps) {
   // the halos mirror-consistent, the state the captured graph assumes.  Mixed precision reads
   // the residual on the host between passes, so its steps are not captured.
   while (nsteps > 0 && (!h->p_consistent || h->cfg.precision == LES_PREC_MIXED)) {
-    int r = enqueue_step(h, n);
-    if (r) return r;
+    if ((r = enqueue_step(h, n))) return n;
     ++n;
     --nsteps;
   }
-  if (nsteps == 0) return LES_OK;
-  if (!h->graph_valid) {
-    if (h->gexec) hipGraphExecDestroy(h->gexec);
-    if (h->graph) hipGraphDestroy(h->graph);
-    h->gexec = nullptr;
-    h->graph = nullptr;
-    HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-    int r = enqueue_step(h, 2);
-    hipGraph_t gr = nullptr;
-    hipError_t e = hipStreamEndCapture(h->st, &gr);
-    if (r) {
-      if (gr) hipGraphDestroy(gr);
-      return r;
-    }
-    if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    h->graph = gr;
-    HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
-    h->graph_valid = true;
+  return n;
+}
+
+// captures one step (n >= 2) of the handle into its graph, if not valid yet
+int ensure_graph(les_handle* h) {
+  if (h->graph_valid) return LES_OK;
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  int r = enqueue_step(h, 2);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(h->st, &gr);
+  if (r) {
+    if (gr) hipGraphDestroy(gr);
+    return r;
   }
+  if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  h->graph = gr;
+  HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  h->graph_valid = true;
+  return LES_OK;
+}
+}  // namespace
+
+extern "C" int les_run(les_handle* h, int n0, int nsteps) {
+  if (!h || nsteps < 0) return LES_E_INVALID;
+  int r;
+  run_eager_prefix(h, n0, nsteps, r);
+  if (r) return r;
+  if (nsteps == 0) return LES_OK;
+  if ((r = ensure_graph(h))) return r;
   for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->st));
+  return LES_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// ensemble: independent members of any sizes advanced together
+// ------------------------------------------------------------------------------------------
+struct les_ensemble {
+  std::vector<les_handle*> m;
+  std::string err;
+};
+
+extern "C" int les_ensemble_create(les_handle* const* members, int n, les_ensemble** out) {
+  if (!members || n < 1 || !out) return LES_E_INVALID;
+  for (int q = 0; q < n; ++q)
+    if (!members[q]) return LES_E_INVALID;
+  les_ensemble* e = new les_ensemble();
+  e->m.assign(members, members + n);
+  *out = e;
+  return LES_OK;
+}
+
+extern "C" int les_ensemble_destroy(les_ensemble* e) {
+  delete e;
+  return LES_OK;
+}
+
+// Every member's stream gets one graph launch per step, step-major, so the members' kernels
+// are in flight together and small members fill the device between the large ones.
+extern "C" int les_ensemble_run(les_ensemble* e, int n0, int nsteps) {
+  if (!e || nsteps < 0) return LES_E_INVALID;
+  const int nm = (int)e->m.size();
+  std::vector<int> left(nm, nsteps);
+  for (int q = 0; q < nm; ++q) {
+    les_handle* h = e->m[q];
+    int r;
+    run_eager_prefix(h, n0, left[q], r);
+    if (r) return r;
+    if (left[q] > 0 && h->cfg.precision == LES_PREC_MIXED) return LES_E_STATE;
+    if (left[q] > 0 && (r = ensure_graph(h))) return r;
+  }
+  for (int s = 0; s < nsteps; ++s)
+    for (int q = 0; q < nm; ++q) {
+      les_handle* h = e->m[q];
+      if (s < nsteps - left[q]) continue;  // this member ran its first steps eagerly
+      HIPCHK(h, hipGraphLaunch(h->gexec, h->st));
+    }
+  return LES_OK;
+}
+
+extern "C" int les_ensemble_sync(les_ensemble* e) {
+  if (!e) return LES_E_INVALID;
+  for (les_handle* h : e->m) {
+    int r = les_sync(h);
+    if (r) return r;
+  }
   return LES_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "../../include/les_hip.h"
@@ -134,6 +135,29 @@ extern "C" int les_gen_masks(int ip, int jp, int kp, const float* zbm, const flo
           A(dmask1, i, j, k) = 1.0f;
         }
   return LES_OK;
+}
+
+// feedbfm.f95:31-37: the GIS height file (GIS/Tokyo_20mgrid.txt) holds 100 x 100 building
+// heights, one list-directed record per line, rows j = 100 down to 1, i = 1..100 within a row,
+// stored at zbm(i+25, j+25) of zbm(-1:ipmax+1,-1:jpmax+1).  A record contributes its first value.
+extern "C" int les_gis_read_zbm(const char* path, int ipmax, int jpmax, float* zbm) {
+  if (!path || !zbm || ipmax < 125 || jpmax < 125) return LES_E_INVALID;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return LES_E_INVALID;
+  const long ni = ipmax + 3;
+  char line[256];
+  int status = LES_OK;
+  for (int j = 100; j >= 1 && status == LES_OK; --j)
+    for (int i = 1; i <= 100; ++i) {
+      float v;
+      if (!std::fgets(line, sizeof line, f) || std::sscanf(line, "%f", &v) != 1) {
+        status = LES_E_INVALID;
+        break;
+      }
+      zbm[(i + 25 + 1) + ni * (long)(j + 25 + 1)] = v;
+    }
+  std::fclose(f);
+  return status;
 }
 
 extern "C" int les_gen_inflow(int jp, int kp, const float* z2, const float* dzn, float ustar,

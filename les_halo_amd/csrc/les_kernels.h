@@ -102,6 +102,10 @@ int reduction_blocks(Dims d);
 // exact sum of n device floats into *out (device), for tests of the reduction itself
 void launch_exact_sum(hipStream_t st, const float* x, long n, RedBuf rb, float* out);
 
+// feedbfm.f95:42-63 masks from building heights
+void launch_feedbfm(hipStream_t st, Dims d, Grid1D g, const float* z2, const float* zbm,
+                    float* am, float* bm, float* cm, float* dm, uint8_t* bits);
+
 // halo exchange pack/unpack (halo_exchange.cl semantics, SoA over the fields)
 void launch_halo_read(hipStream_t st, const float* const* arrays, int v_dim, int im, int jm,
                       int km, int h_w, int correct, float* buffer);

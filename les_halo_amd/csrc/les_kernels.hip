@@ -933,6 +933,36 @@ void launch_exact_sum(hipStream_t st, const float* x, long n, RedBuf rb, float* 
 }
 
 // ==========================================================================================
+// feedbfm.f95:42-63 on the device: amask1 = 0 where zbm(i,j) > z2(k)+0.5*dzn(k), else 1;
+// bmask1 = cmask1 = dmask1 = 1 - amask1 over the interior (halos are cleared by the caller),
+// and the interior bit mask the stencils read.  zbm(-1:ip+1,-1:jp+1), z2(1:kp+2).
+// ==========================================================================================
+__global__ __launch_bounds__(256) void k_feedbfm(Dims d, Grid1D g, const float* z2,
+                                                 const float* zbm, float* am, float* bm, float* cm,
+                                                 float* dm, uint8_t* bits) {
+  const int i = 1 + blockIdx.x * BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int ip = d.ip, jp = d.jp;
+  const float hb = zbm[(i + 1) + (long)(ip + 3) * (j + 1)];
+  const bool solid = hb > z2[k - 1] + 0.5f * g.DZN(k);
+  const float a = solid ? 0.0f : 1.0f, o = solid ? 1.0f : 0.0f;
+  const long ia = i + (long)(ip + 2) * (j + (long)(jp + 2) * k);
+  am[ia] = a;
+  dm[ia] = o;
+  bm[(i + 1) + (long)(ip + 3) * (j + (long)(jp + 2) * k)] = o;
+  cm[i + (long)(ip + 2) * ((j + 1) + (long)(jp + 3) * k)] = o;
+  bits[d.M(i, j, k)] = solid ? 7 : 8;  // b|c|d or a
+}
+
+void launch_feedbfm(hipStream_t st, Dims d, Grid1D g, const float* z2, const float* zbm,
+                    float* am, float* bm, float* cm, float* dm, uint8_t* bits) {
+  hipLaunchKernelGGL(k_feedbfm, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, z2, zbm, am,
+                     bm, cm, dm, bits);
+}
+
+// ==========================================================================================
 // Halo exchange pack/unpack: halo_exchange.cl:3-51 (write), :156-185 (read) and
 // halo_exchange_correct.cl:156-185 (read, inner ring without corners), with the OpenCL
 // vector components replaced by separate fields (buffer block v = field v).

@@ -56,3 +56,28 @@ def make_case(ip, jp, kp, seed=0x4C455331, ustar=0.2, dx=4.0, dy=4.0, dz0=2.0, z
 # constants (set.f95:78-82)
 CONFIG_PHYSICS = dict(dt=0.05, ro=1.1763, vn=1.583e-5, alpha=-10.0, beta=-1.0, cs0=0.1,
                       omega=1.5, pjuge=0.0, nmaxp=50)
+
+
+def ensemble_specs(n=64, seed=0x454E5331):
+    """Member sizes and inflow friction velocities of the batched-ensemble configuration: ip
+    and jp uniform over {96, 128, ..., 320}, kp over {60, 90}, u* uniform in [0.1, 0.5] m/s,
+    one terrain seed per member (numpy PCG64 stream from 'seed')."""
+    rng = np.random.default_rng(seed)
+    sizes = np.arange(96, 321, 32)
+    out = []
+    for q in range(n):
+        ip, jp = (int(x) for x in rng.choice(sizes, 2))
+        kp = int(rng.choice([60, 90]))
+        out.append(dict(dims=(ip, jp, kp), seed=int(rng.integers(1, 2 ** 63)),
+                        ustar=float(rng.uniform(0.1, 0.5))))
+    return out
+
+
+def gis_zbm(path, ipmax, jpmax):
+    """zbm(-1:ipmax+1,-1:jpmax+1), zero except the 100 x 100 block of a GIS height file placed
+    as feedbfm.f95:31-37 does (les_gis_read_zbm)."""
+    zbm = np.zeros((ipmax + 3) * (jpmax + 3), np.float32)
+    r = lib().les_gis_read_zbm(path.encode(), ipmax, jpmax, zbm.ctypes.data)
+    if r != 0:
+        raise ValueError("les_gis_read_zbm(%s) failed (%d)" % (path, r))
+    return zbm

@@ -60,6 +60,10 @@ class LES:
     def __exit__(self, *exc):
         self.close()
 
+    def last_error(self):
+        msg = lib().les_last_error(self.h)
+        return msg.decode() if msg else ""
+
     def _chk(self, r, what):
         if r != 0:
             msg = lib().les_last_error(self.h)
@@ -93,8 +97,26 @@ class LES:
         if coefficients:
             self.init_coefficients()
 
+    def set_terrain(self, zbm):
+        """feedbfm.f95:42-63 on the device from zbm(-1:ip+1,-1:jp+1) (z2, dzn uploaded)."""
+        z = np.ascontiguousarray(zbm, dtype=np.float32)
+        self._chk(lib().les_set_terrain(self.h, z.ctypes.data), "set_terrain")
+
     def init_coefficients(self):
         self._chk(lib().les_init_coefficients(self.h), "init_coefficients")
+
+    def write_restart(self, data20, data21, n=0, time=0.0, with_header=False):
+        """data20/data21 in the reference's unformatted records (press.f95:192-199,
+        adam.f95:24-29); with_header makes data20 a data30 restart file."""
+        self._chk(lib().les_write_restart(self.h, data20.encode(), data21.encode(), int(n),
+                                          float(time), 1 if with_header else 0), "write_restart")
+
+    def read_restart(self, data30, data31):
+        """ifdata.f95:227-247 (ICAL = 1): interiors of u..wsum, fold..hold; returns (n, time)."""
+        n, t = ctypes.c_int(), ctypes.c_float()
+        self._chk(lib().les_read_restart(self.h, data30.encode(), data31.encode(), ctypes.byref(n),
+                                         ctypes.byref(t)), "read_restart")
+        return n.value, t.value
 
     def state(self, names=None):
         return {n: self.download(n) for n in (names or fl.STATE_3D)}
@@ -254,4 +276,41 @@ class LES:
         return np.float32(out.value)
 
 
-__all__ = ["LES", "LESError", "LES_PREC_FP32", "LES_PREC_MIXED"]
+class Ensemble:
+    """Independent LES members of any sizes advanced together (les_ensemble_*): each step
+    enqueues one graph launch per member on its own stream, so the members run concurrently.
+    The members stay owned by the caller."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h for m in self.members])
+        e = ctypes.c_void_p()
+        r = lib().les_ensemble_create(arr, len(self.members), ctypes.byref(e))
+        if r != 0:
+            raise LESError(r, "les_ensemble_create failed")
+        self.e = e
+
+    def run(self, n0, nsteps):
+        r = lib().les_ensemble_run(self.e, n0, nsteps)
+        if r != 0:
+            msgs = "; ".join(m.last_error() for m in self.members if m.last_error())
+            raise LESError(r, "les_ensemble_run: %s" % msgs)
+
+    def sync(self):
+        r = lib().les_ensemble_sync(self.e)
+        if r != 0:
+            raise LESError(r, "les_ensemble_sync")
+
+    def close(self):
+        if getattr(self, "e", None):
+            lib().les_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["LES", "Ensemble", "LESError", "LES_PREC_FP32", "LES_PREC_MIXED"]
