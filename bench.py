@@ -74,12 +74,16 @@ def cpu_baseline(case, nsteps):
             "kind": kind, "sample": sample, "seconds": round(secs, 3)}
 
 
+CALIB_COPY_BYTES = 256 << 20  # calibration copy: 256 MiB, far larger than the 32 MiB of L2
+
+
 def profile_sor(L, reps):
-    """For rocprofv3 --pmc runs: REPS sweeps (k_sor_split) and one copy kernel that reads and
-    writes SOR_SWEEP_B_PER_CELL/2 bytes per cell each, i.e. the sweep's algorithmic bytes."""
+    """For rocprofv3 --pmc runs: REPS colour sweeps (k_sor_split), REPS fused iterations
+    (k_sor_iter) and one device copy of CALIB_COPY_BYTES (read and written once each, its source
+    evicted from L2 by the copy itself), whose FETCH_SIZE/WRITE_SIZE calibrate the counters."""
     import torch
     ip, jp, kp = GRID
-    n = SOR_SWEEP_B_PER_CELL // 2 * ip * jp * kp // 4
+    n = CALIB_COPY_BYTES // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()

@@ -210,6 +210,21 @@ int les_ensemble_destroy(les_ensemble* e);
  * boundp2, velfg, feedbf, les.  Fields must be zero (les_create) or uploaded. */
 int les_ifdata(les_handle* h);
 
+/* ---- aveflow statistics (aveflow.f95:42-157) -------------------------------------------
+ * les_aveflow(h, n, n1, nmax) is the reference's call after step n: zeroes the statistics at
+ * n == n1, accumulates for n >= n1, and at n == nmax turns them into means, RMS values and
+ * spanwise means.  les_aveflow_download copies one array: 'which' in the order below;
+ * (ip,jp,kp) arrays for 0..10, (ip,kp) arrays for 11..17, flat with i fastest.  The uw-flux
+ * correction at k = 1 reads aveu/avew at level 0, before the arrays; those values read as 0. */
+enum {
+  LES_AVE_AVEU = 0, LES_AVE_AVEV, LES_AVE_AVEW, LES_AVE_AVEP, LES_AVE_AVEL, LES_AVE_AVEUU,
+  LES_AVE_AVEVV, LES_AVE_AVEWW, LES_AVE_AVESM, LES_AVE_AVESMSM, LES_AVE_UWFX,
+  LES_AVE_AVESU, LES_AVE_AVESV, LES_AVE_AVESW, LES_AVE_AVESUU, LES_AVE_AVESVV, LES_AVE_AVESWW,
+  LES_AVE_UWFXS, LES_AVE_COUNT
+};
+int les_aveflow(les_handle* h, int n, int n1, int nmax);
+int les_aveflow_download(les_handle* h, int which, float* host);
+
 /* ---- restart and output files (Fortran unformatted sequential records) ---------------
  * les_write_restart writes data20 (press.f95:192-199: u, v, w, p, usum, vsum, wsum over the
  * interior, one record each) and data21 (adam.f95:24-29: fold, gold, hold, fghold = 0); with
@@ -277,15 +292,20 @@ enum {
 };
 int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms);
 
-/* Autotuner (no counterpart in the reference, whose OpenCL launch shapes are fixed): times
- * every workgroup tile (rows of j x levels of k) of the fused SOR iteration on this handle's
- * grid, 'reps' launches each, and keeps the fastest for later solves.  Changes no simulation
- * state.  *out (may be NULL) receives the choice. */
+/* Autotuner (no counterpart in the reference, whose OpenCL launch shapes are fixed): times one
+ * SOR iteration without residual as two colour sweeps and as the fused launch with every
+ * workgroup tile (rows of j x levels of k), 'reps' iterations each, on this handle's grid, and
+ * keeps the fastest for later solves (untuned handles use the two sweeps).  Changes no
+ * simulation state.  *out (may be NULL) receives the choice. */
 typedef struct les_tuning {
-  int sor_tile_j, sor_tile_k;  /* 0 when the fused iteration does not apply */
-  float sor_iter_ms;           /* its average time per iteration, or -1 */
+  int sor_tile_j, sor_tile_k;  /* the fused tile, or 0, 0 for the two colour sweeps */
+  float sor_iter_ms;           /* average time of one iteration with that choice, or -1 when
+                                  the grid allows neither (odd jp, ip > 320 or non-uniform) */
 } les_tuning;
 int les_autotune(les_handle* h, int reps, les_tuning* out);
+/* Sets the SOR iteration form by hand: tile_j = 0 for two colour sweeps, else the fused launch
+ * with tile_j (4, 8 or 16) rows of j and tile_k levels of k per workgroup. */
+int les_set_sor_tile(les_handle* h, int tile_j, int tile_k);
 
 /* ---- the reference's GPU host API (module_LES_combined_ocl) as single-handle drop-ins ---- */
 /* initialise_LES_kernel (Wrappers/module_LES_combined_ocl.f95:15-441), external wind profile

@@ -90,6 +90,7 @@ PROTOS = {
     "les_reduce_sum_exact": (c_int, [H, c_void_p, c_int64, P_float]),
     "les_time_kernel": (c_int, [H, c_int, c_int, P_float]),
     "les_autotune": (c_int, [H, c_int, ctypes.POINTER(LesTuning)]),
+    "les_set_sor_tile": (c_int, [H, c_int, c_int]),
     "les_ensemble_create": (c_int, [ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
     "les_ensemble_run": (c_int, [c_void_p, c_int, c_int]),
     "les_ensemble_sync": (c_int, [c_void_p]),
@@ -108,6 +109,8 @@ PROTOS = {
                                c_uint64, c_void_p]),
     "les_initial_wind_profile": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "les_gis_read_zbm": (c_int, [ctypes.c_char_p, c_int, c_int, c_void_p]),
+    "les_aveflow": (c_int, [H, c_int, c_int, c_int]),
+    "les_aveflow_download": (c_int, [H, c_int, c_void_p]),
     "les_set_terrain": (c_int, [H, c_void_p]),
 }
 

@@ -45,8 +45,12 @@ struct les_handle {
   double* mx_mem = nullptr;
   // second iterate of the fused SOR iteration (ping-pong with sp.p) and its workgroup tile
   float* palt[2] = {nullptr, nullptr};
-  SorIterShape iter_shape{8, 10};
+  SorIterShape iter_shape{0, 0};  // tj == 0: two colour sweeps per iteration (untuned default)
   bool tuned = false;
+  bool p2xy = false;  // see grid_p2xy
+  // aveflow statistics, allocated by the first les_aveflow call
+  float* ave_mem = nullptr;
+  AveStats ave{};
 };
 
 namespace {
@@ -122,8 +126,34 @@ bool spec_of(int f, BoundSpec& s) {
 bool is_mask(int f) { return f >= LES_AMASK1 && f <= LES_DMASK1; }
 bool is_1d_host(int f) { return f >= LES_CN2L && f <= LES_WIND_PROFILE; }
 
+// Every per-lane denominator of the stencils along i and j is a positive power of two with a
+// normal reciprocal: dx1, dy1, dxs, dys and the neighbour sums dx1(i)+dx1(i+1), dy1(j)+dy1(j+1)
+// (as FP32 sums).  Then the kernels divide by multiplying with the exact reciprocal.
+bool pow2_normal_host(float b) {
+  uint32_t bb;
+  std::memcpy(&bb, &b, 4);
+  return (bb & 0x807fffffu) == 0u && (bb & 0x7f800000u) - 0x00800000u <= 0x7e000000u;
+}
+
+bool grid_p2xy(const les_handle* h) {
+  const auto& dx1 = h->hgrid[LES_DX1];
+  const auto& dy1 = h->hgrid[LES_DY1];
+  const auto& dxs = h->hgrid[LES_DXS];
+  const auto& dys = h->hgrid[LES_DYS];
+  if (dx1.empty() || dy1.empty() || dxs.empty() || dys.empty()) return false;
+  for (const auto* a : {&dx1, &dy1, &dxs, &dys})
+    for (float x : *a)
+      if (!pow2_normal_host(x)) return false;
+  for (size_t q = 0; q + 1 < dx1.size(); ++q)
+    if (!pow2_normal_host(dx1[q] + dx1[q + 1])) return false;
+  for (size_t q = 0; q + 1 < dy1.size(); ++q)
+    if (!pow2_normal_host(dy1[q] + dy1[q + 1])) return false;
+  return true;
+}
+
 Grid1D grid_of(les_handle* h) {
   Grid1D g;
+  g.p2xy = h->p2xy ? 1 : 0;
   g.dx1 = h->fld[LES_DX1];
   g.dxs = h->fld[LES_DXS];
   g.dy1 = h->fld[LES_DY1];
@@ -415,6 +445,7 @@ extern "C" int les_destroy(les_handle* h) {
   if (h->scratch_f) hipFree(h->scratch_f);
   if (h->split_mem) hipFree(h->split_mem);
   if (h->mx_mem) hipFree(h->mx_mem);
+  if (h->ave_mem) hipFree(h->ave_mem);
   if (h->st) hipStreamDestroy(h->st);
   if (h == g_default) g_default = nullptr;
   delete h;
@@ -464,6 +495,10 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
     int r = upload_scalar_area(h);
     if (r) return r;
   }
+  if (field == LES_DX1 || field == LES_DY1 || field == LES_DXS || field == LES_DYS) {
+    h->p2xy = grid_p2xy(h);
+    h->graph_valid = false;
+  }
   if (field == LES_P) h->p_consistent = false;
   if (field == LES_CN1) std::memcpy(h->hcn1.data(), host, n * sizeof(float));
   if (field == LES_CN1 || field == LES_DXS || field == LES_DYS || field == LES_DZS ||
@@ -471,6 +506,35 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
     int r = refresh_split_factors(h);
     if (r) return r;
   }
+  return LES_OK;
+}
+
+// aveflow.f95:42-157 after step n (statistics from n1 to nmax), on the device
+extern "C" int les_aveflow(les_handle* h, int n, int n1, int nmax) {
+  if (!h) return LES_E_INVALID;
+  const Dims d = h->d;
+  const Fields F = fields_of(h);
+  if (n1 > nmax) return fail(h, LES_E_INVALID, "aveflow: n1 > nmax");
+  const size_t n3 = (size_t)d.ip * d.jp * d.kp, n2 = (size_t)d.ip * d.kp;
+  if (!h->ave_mem) {
+    const size_t nf = AVE_N3 * n3 + AVE_N2 * n2;
+    HIPCHK(h, hipMalloc(&h->ave_mem, nf * sizeof(float)));
+    HIPCHK(h, hipMemsetAsync(h->ave_mem, 0, nf * sizeof(float), h->st));
+    for (int q = 0; q < AVE_N3; ++q) h->ave.a3[q] = h->ave_mem + q * n3;
+    for (int q = 0; q < AVE_N2; ++q) h->ave.a2[q] = h->ave_mem + AVE_N3 * n3 + q * n2;
+  }
+  launch_aveflow(h->st, d, F, h->ave, n, n1, nmax);
+  return check_launch(h);
+}
+
+extern "C" int les_aveflow_download(les_handle* h, int which, float* host) {
+  if (!h || !host || which < 0 || which >= AVE_N3 + AVE_N2) return LES_E_INVALID;
+  if (!h->ave_mem) return fail(h, LES_E_STATE, "aveflow_download: no statistics yet");
+  const size_t n3 = (size_t)h->d.ip * h->d.jp * h->d.kp, n2 = (size_t)h->d.ip * h->d.kp;
+  const float* src = which < AVE_N3 ? h->ave.a3[which] : h->ave.a2[which - AVE_N3];
+  const size_t cnt = which < AVE_N3 ? n3 : n2;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(host, src, cnt * sizeof(float), hipMemcpyDeviceToHost));
   return LES_OK;
 }
 
@@ -725,7 +789,7 @@ int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true) {
   const bool resid = allow_exit && h->ph.pjuge > 0.0f;
   const int nb = sor_split_blocks(d);
   int l0 = 1;
-  if (!resid && sor_iter_supported(d, h->sp)) {
+  if (!resid && h->iter_shape.tj > 0 && sor_iter_supported(d, h->sp)) {
     const SorPair fwd{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
     const SorPair back{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}};
     for (; l0 + 1 <= h->cfg.nmaxp; l0 += 2) {
@@ -1155,9 +1219,25 @@ extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms
   return check_launch(h);
 }
 
-// Autotuner: times every workgroup tile of the fused SOR iteration on this handle's grid (the
-// split copy is scratch between solves, so the trial launches change no simulation state) and
-// keeps the fastest; the captured step graph is rebuilt with it.
+extern "C" int les_set_sor_tile(les_handle* h, int tile_j, int tile_k) {
+  if (!h) return LES_E_INVALID;
+  if (tile_j == 0) {
+    h->iter_shape = SorIterShape{0, 0};
+  } else {
+    if ((tile_j != 4 && tile_j != 8 && tile_j != 16) || tile_k < 1)
+      return fail(h, LES_E_INVALID, "set_sor_tile: tile_j must be 0, 4, 8 or 16 and tile_k >= 1");
+    if (!sor_iter_supported(h->d, h->sp))
+      return fail(h, LES_E_STATE, "set_sor_tile: the fused SOR iteration does not apply to this grid");
+    h->iter_shape = SorIterShape{tile_j, tile_k};
+  }
+  h->graph_valid = false;
+  return LES_OK;
+}
+
+// Autotuner: times one SOR iteration as two colour sweeps and as the fused launch with every
+// workgroup tile on this handle's grid (the split copy is scratch between solves, so the trial
+// launches change no simulation state) and keeps the fastest; the captured step graph is
+// rebuilt with it.
 extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
   if (!h || reps < 1) return LES_E_INVALID;
   STAGE_BEGIN(h);
@@ -1169,6 +1249,19 @@ extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
     hipEvent_t e0, e1;
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
+    {  // the two colour sweeps of one iteration
+      for (int q = 0; q < 4; ++q) launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, 0, h->sc);
+      HIPCHK(h, hipEventRecord(e0, h->st));
+      for (int q = 0; q < 2 * reps; ++q)
+        launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, 0, h->sc);
+      HIPCHK(h, hipEventRecord(e1, h->st));
+      HIPCHK(h, hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+      best.sor_iter_ms = ms / (float)reps;
+      best.sor_tile_j = 0;
+      best.sor_tile_k = 0;
+    }
     for (int tj : tjs)
       for (int kc : kcs) {
         if (kc > h->d.kp) continue;
@@ -1189,7 +1282,7 @@ extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
         float ms = 0.0f;
         HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
         ms /= (float)reps;
-        if (best.sor_iter_ms < 0.0f || ms < best.sor_iter_ms) {
+        if (ms < best.sor_iter_ms) {
           best.sor_iter_ms = ms;
           best.sor_tile_j = tj;
           best.sor_tile_k = kc;

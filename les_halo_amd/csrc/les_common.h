@@ -54,6 +54,8 @@ struct Dims {
 struct Grid1D {
   const float *dx1, *dxs, *dy1, *dys, *dzn, *dzs;
   const float *cn2l, *cn2s, *cn3l, *cn3s, *cn4l, *cn4s, *delx1;
+  int p2xy;  // every dx1, dy1, dxs, dys and neighbour sum dx1(i)+dx1(i+1), dy1(j)+dy1(j+1) is a
+             // power of two (set on the host; selects xdiv<true>)
   __device__ float DX1(int i) const { return dx1[i + 1]; }
   __device__ float DXS(int i) const { return dxs[i]; }
   __device__ float DY1(int j) const { return dy1[j]; }
@@ -62,17 +64,22 @@ struct Grid1D {
   __device__ float DZS(int k) const { return dzs[k + 1]; }
 };
 
-// a / b with IEEE rounding.  When b is a positive normal power of two whose reciprocal is
-// normal (biased exponent 1..253), 1/b is exact and a * (1/b) is the same real number as a / b,
-// so both round to the same float (subnormal results included): one multiply instead of the
-// division sequence (scale, reciprocal, four FMAs, fixup).  Grid spacings here are mostly
-// powers of two (4 m horizontal, 2 m vertical below the stretching), so the fast path is the
-// common one and, for a wavefront-uniform b, a uniform branch.
-__device__ __forceinline__ float qdiv(float a, float b) {
+// a / b with IEEE rounding, faster when b is a power of two.  For a positive normal b = 2^e
+// whose reciprocal is normal (biased exponent 1..253), 1/b is exact and a * (1/b) is the same
+// real number as a / b, so both round to the same float (subnormal results included): one
+// multiply instead of the division sequence (scale, reciprocal, four FMAs, fixup).
+__device__ __forceinline__ bool pow2_normal(float b) {
   const uint32_t bb = __float_as_uint(b);
-  const uint32_t e = bb & 0x7f800000u;
-  if ((bb & 0x807fffffu) == 0u && e - 0x00800000u <= 0x7e000000u)
-    return a * __uint_as_float(0x7f000000u - e);
+  return (bb & 0x807fffffu) == 0u && (bb & 0x7f800000u) - 0x00800000u <= 0x7e000000u;
+}
+__device__ __forceinline__ float pow2_recip(float b) {  // requires pow2_normal(b)
+  return __uint_as_float(0x7f000000u - __float_as_uint(b));
+}
+// per-lane denominators (grid spacings along i and j): the launch knows on the host whether
+// every one of them is a power of two (Grid1D::p2xy) and picks P2 accordingly
+template <bool P2>
+__device__ __forceinline__ float xdiv(float a, float b) {
+  if constexpr (P2) return a * pow2_recip(b);
   return a / b;
 }
 

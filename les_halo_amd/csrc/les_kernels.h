@@ -102,6 +102,17 @@ int reduction_blocks(Dims d);
 // exact sum of n device floats into *out (device), for tests of the reduction itself
 void launch_exact_sum(hipStream_t st, const float* x, long n, RedBuf rb, float* out);
 
+// aveflow.f95 statistics: (ip,jp,kp) arrays a3[AVE_*] and (ip,kp) arrays a2 (avesu, avesv,
+// avesw, avesuu, avesvv, avesww, uwfxs), flat with i fastest
+enum { AVE_U, AVE_V, AVE_W, AVE_P, AVE_L, AVE_UU, AVE_VV, AVE_WW, AVE_SM, AVE_SMSM, AVE_UWFX,
+       AVE_N3 };
+constexpr int AVE_N2 = 7;
+struct AveStats {
+  float* a3[AVE_N3];
+  float* a2[AVE_N2];
+};
+void launch_aveflow(hipStream_t st, Dims d, Fields F, AveStats s, int n, int n1, int nmax);
+
 // feedbfm.f95:42-63 masks from building heights
 void launch_feedbfm(hipStream_t st, Dims d, Grid1D g, const float* z2, const float* zbm,
                     float* am, float* bm, float* cm, float* dm, uint8_t* bits);

@@ -27,6 +27,7 @@ __device__ __forceinline__ float mbit(uint8_t m, int b) { return (m >> b) & 1u ?
 // velnw.f95:27-59 fused with the inflow of bondv1_ext_wind_profile.f95:27-50
 // (state 1 of run_LES_kernel).  Threads cover i = 0..imax, j = 1..jm, k = 1..km.
 // ==========================================================================================
+template <bool P2>
 __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F, Physics ph,
                                                       int do_velnw, int inflow_n, int imax) {
   const int i = blockIdx.x * BX + threadIdx.x;
@@ -46,12 +47,12 @@ __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F
   const float ro = ph.ro, dt = ph.dt;
   const float p0 = F.p[d.P(i, j, k)];
   const long ifg = d.F(i, j, k);
-  float pz = qdiv((-p0 + F.p[d.P(i + 1, j, k)]) / ro, g.DXS(i));
+  float pz = xdiv<P2>((-p0 + F.p[d.P(i + 1, j, k)]) / ro, g.DXS(i));
   F.u[iu] = F.u[iu] + dt * (F.f[ifg] - pz);
-  pz = qdiv((-p0 + F.p[d.P(i, j + 1, k)]) / ro, g.DYS(j));
+  pz = xdiv<P2>((-p0 + F.p[d.P(i, j + 1, k)]) / ro, g.DYS(j));
   F.v[iu] = F.v[iu] + dt * (F.g[ifg] - pz);
   if (k <= d.kp - 1) {
-    pz = qdiv((-p0 + F.p[d.P(i, j, k + 1)]) / ro, g.DZS(k));
+    pz = ((-p0 + F.p[d.P(i, j, k + 1)]) / ro) / g.DZS(k);
     F.w[iw] = F.w[iw] + dt * (F.h[ifg] - pz);
   }
 }
@@ -60,8 +61,12 @@ void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph,
                          int inflow_n) {
   int imax = (do_velnw || inflow_n == 1) ? d.ip : 1;
   dim3 grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
-  hipLaunchKernelGGL(k_velnw_inflow, grid, dim3(BX, BY), 0, st, d, g, F, ph, do_velnw, inflow_n,
-                     imax);
+  if (g.p2xy)
+    hipLaunchKernelGGL(k_velnw_inflow<true>, grid, dim3(BX, BY), 0, st, d, g, F, ph, do_velnw,
+                       inflow_n, imax);
+  else
+    hipLaunchKernelGGL(k_velnw_inflow<false>, grid, dim3(BX, BY), 0, st, d, g, F, ph, do_velnw,
+                       inflow_n, imax);
 }
 
 // ==========================================================================================
@@ -210,6 +215,7 @@ void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, D
 // copy (value at im), at j+1 = jm+1 the periodic copy (value at j = 1); k+1 values are
 // computed (vel2 computes them up to km+1).  diu1..9 are stored with every halo copy of
 // vel2.f95:125-188, each written by the thread owning its source.
+template <bool P2>
 __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F, Physics ph,
                                                       int do_velfg, int do_feedbf) {
   const int i = 1 + blockIdx.x * BX + threadIdx.x;
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
   const int k = 1 + blockIdx.z;
   const int im = d.ip, jm = d.jp, km = d.kp;
   if (i > im || j > jm) return;
-  VelCtx c{d, g, F.u, F.v, F.w};
+  VelCtx<P2> c{d, g, F.u, F.v, F.w};
   const int ip1 = (i == im) ? im : i + 1;  // i+1 with the im+1 -> im copy
   const int jp1 = (j == jm) ? 1 : j + 1;   // j+1 with the jm+1 -> 1 copy
   const long ifg = d.F(i, j, k);
@@ -236,24 +242,24 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
     const float dznk = c.dzn(k), dznkp = c.dzn(k + 1);
     // u momentum (velFG.f95:59-72)
     {
-      float covx1 = qdiv(dx1ip * c.cov1(i, j, k) + dx1i * c.cov1(ip1, j, k), dx1i + dx1ip);
+      float covx1 = xdiv<P2>(dx1ip * c.cov1(i, j, k) + dx1i * c.cov1(ip1, j, k), dx1i + dx1ip);
       float covy1 = (c.cov2(i, j, k) + c.cov2(i, jp1, k)) / two;
       float covz1 = (c.cov3(i, j, k) + c.cov3(i, j, k + 1)) / two;
       float covc = covx1 + covy1 + covz1;
-      float dfu1 = qdiv(two * (-d1 + c.diu1(ip1, j, k)), dx1i + dx1ip) +
-                   qdiv(-d2 + c.diu2(i, jp1, k), dy1j) + qdiv(-d3 + c.diu3(i, j, k + 1), dznk);
+      float dfu1 = xdiv<P2>(two * (-d1 + c.diu1(ip1, j, k)), dx1i + dx1ip) +
+                   xdiv<P2>(-d2 + c.diu2(i, jp1, k), dy1j) + (-d3 + c.diu3(i, j, k + 1)) / dznk;
       float df = ph.vn * dfu1;
       fv = (-covc + df);
     }
     // v momentum (velFG.f95:75-88)
     {
       float covx1 = (c.cov4(i, j, k) + c.cov4(ip1, j, k)) / two;
-      float covy1 = qdiv(dy1jp * c.cov5(i, j, k) + dy1j * c.cov5(i, jp1, k), dy1j + dy1jp);
+      float covy1 = xdiv<P2>(dy1jp * c.cov5(i, j, k) + dy1j * c.cov5(i, jp1, k), dy1j + dy1jp);
       float covz1 = (c.cov6(i, j, k) + c.cov6(i, j, k + 1)) / two;
       float covc = covx1 + covy1 + covz1;
-      float dfv1 = qdiv(-d4 + c.diu4(ip1, j, k), dx1i) +
-                   qdiv(two * (-d5 + c.diu5(i, jp1, k)), dy1j + dy1jp) +
-                   qdiv(-d6 + c.diu6(i, j, k + 1), dznk);
+      float dfv1 = xdiv<P2>(-d4 + c.diu4(ip1, j, k), dx1i) +
+                   xdiv<P2>(two * (-d5 + c.diu5(i, jp1, k)), dy1j + dy1jp) +
+                   (-d6 + c.diu6(i, j, k + 1)) / dznk;
       float df = ph.vn * dfv1;
       gv = (-covc + df);
     }
@@ -261,10 +267,10 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
     if (hk) {
       float covx1 = (c.cov7(i, j, k) + c.cov7(ip1, j, k)) / two;
       float covy1 = (c.cov8(i, j, k) + c.cov8(i, jp1, k)) / two;
-      float covz1 = qdiv(dznkp * c.cov9(i, j, k) + dznk * c.cov9(i, j, k + 1), dznk + dznkp);
+      float covz1 = (dznkp * c.cov9(i, j, k) + dznk * c.cov9(i, j, k + 1)) / (dznk + dznkp);
       float covc = covx1 + covy1 + covz1;
-      float dfw1 = qdiv(-d7 + c.diu7(ip1, j, k), dx1i) + qdiv(-d8 + c.diu8(i, jp1, k), dy1j) +
-                   qdiv(-d9 + c.diu9(i, j, k + 1), c.dzs(k));
+      float dfw1 = xdiv<P2>(-d7 + c.diu7(ip1, j, k), dx1i) + xdiv<P2>(-d8 + c.diu8(i, jp1, k), dy1j) +
+                   (-d9 + c.diu9(i, j, k + 1)) / c.dzs(k);
       float df = ph.vn * dfw1;
       hv = (-covc + df);
     } else {
@@ -368,8 +374,12 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
 
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf) {
-  hipLaunchKernelGGL(k_velfg_feedbf, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F, ph,
-                     do_velfg, do_feedbf);
+  if (g.p2xy)
+    hipLaunchKernelGGL(k_velfg_feedbf<true>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g,
+                       F, ph, do_velfg, do_feedbf);
+  else
+    hipLaunchKernelGGL(k_velfg_feedbf<false>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g,
+                       F, ph, do_velfg, do_feedbf);
 }
 
 // ==========================================================================================
@@ -464,6 +474,7 @@ void launch_boundsm(hipStream_t st, Dims d, Fields F) {
 // ==========================================================================================
 // les.f95:78-182 (viscous terms) fused with adam.f95:32-46 (Adams-Bashforth)
 // ==========================================================================================
+template <bool P2>
 __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
                                                    int do_adam) {
   const int i = 1 + blockIdx.x * BX + threadIdx.x;
@@ -492,81 +503,73 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
     {
       float evsx2 = s100;
       float evsx1 = s000;
-      float evsy2 = qdiv(dy1jp * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dy1j * (qdiv(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
+      float evsy2 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                         dy1j * (xdiv<P2>(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
                          dy1j + dy1jp);
-      float evsy1 = qdiv(dy1jp * (qdiv(dx1ip * sm(i, j - 1, k) + dx1i * sm(i + 1, j - 1, k), dx1i + dx1ip)) +
-                         dy1j * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
+      float evsy1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * sm(i, j - 1, k) + dx1i * sm(i + 1, j - 1, k), dx1i + dx1ip)) +
+                         dy1j * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
                          dy1j + dy1jp);
-      float evsz2 = qdiv(dznkp * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dznk * (qdiv(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip)),
-                         dznk + dznkp);
-      float evsz1 = qdiv(dznk * (qdiv(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
-                         dznkm * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
-                         dznkm + dznk);
+      float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                         dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
+      float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
+                         dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
       float visux2 = (evsx2) * two * D1(i + 1, j, k);
       float visux1 = (evsx1) * two * D1(i, j, k);
       float visuy2 = (evsy2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
       float visuy1 = (evsy1) * (D2(i, j, k) + D4(i + 1, j - 1, k));
       float visuz2 = (evsz2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
       float visuz1 = (evsz1) * (D3(i, j, k) + D7(i + 1, j, k - 1));
-      float vfu = qdiv(visux2 - visux1, dx1i) +
-                  qdiv(visuy2 - visuy1, dy1j) +
-                  qdiv(visuz2 - visuz1, dznk);
+      float vfu = xdiv<P2>(visux2 - visux1, dx1i) +
+                  xdiv<P2>(visuy2 - visuy1, dy1j) +
+                  (visuz2 - visuz1) / dznk;
       fv = (fv + vfu);
     }
     // y-momentum (les.f95:112-143)
     {
       float evsy2 = sm(i, j + 1, k);
       float evsy1 = s000;
-      float evsx2 = qdiv(dy1jp * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dy1j * (qdiv(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
+      float evsx2 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                         dy1j * (xdiv<P2>(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
                          dy1j + dy1jp);
-      float evsx1 = qdiv(dy1jp * (qdiv(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
-                         dy1j * (qdiv(dx1i * sm(i - 1, j + 1, k) + dx1im * sm(i, j + 1, k), dx1im + dx1i)),
+      float evsx1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
+                         dy1j * (xdiv<P2>(dx1i * sm(i - 1, j + 1, k) + dx1im * sm(i, j + 1, k), dx1im + dx1i)),
                          dy1j + dy1jp);
-      float evsz2 = qdiv(dznkp * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dznk * (qdiv(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip)),
-                         dznk + dznkp);
-      float evsz1 = qdiv(dznk * (qdiv(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
-                         dznkm * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
-                         dznkm + dznk);
+      float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                         dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
+      float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
+                         dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
       float visvx2 = (evsx2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
       float visvx1 = (evsx1) * (D2(i - 1, j + 1, k) + D4(i, j, k));
       float visvy2 = (evsy2) * two * D5(i, j + 1, k);
       float visvy1 = (evsy1) * two * D5(i, j, k);
       float visvz2 = (evsz2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
       float visvz1 = (evsz1) * (D6(i, j, k) + D8(i, j + 1, k - 1));
-      float vfv = qdiv(visvx2 - visvx1, dx1i) +
-                  qdiv(visvy2 - visvy1, dy1j) +
-                  qdiv(visvz2 - visvz1, dznk);
+      float vfv = xdiv<P2>(visvx2 - visvx1, dx1i) +
+                  xdiv<P2>(visvy2 - visvy1, dy1j) +
+                  (visvz2 - visvz1) / dznk;
       gv = (gv + vfv);
     }
     // z-momentum (les.f95:146-180)
     {
       float evsz2 = sm(i, j, k + 1);
       float evsz1 = s000;
-      float evsx2 = qdiv(dznkp * (qdiv(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dznk * (qdiv(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip)),
-                         dznk + dznkp);
-      float evsx1 = qdiv(dznkp * (qdiv(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
-                         dznk * (qdiv(dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1), dx1im + dx1i)),
-                         dznk + dznkp);
-      float evsy2 = qdiv(dznkp * (qdiv(dy1jp * s000 + dy1j * sm(i, j + 1, k), dy1j + dy1jp)) +
-                         dznk * (qdiv(dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1), dy1j + dy1jp)),
-                         dznk + dznkp);
-      float evsy1 = qdiv(dznkp * (qdiv(dy1j * sm(i, j - 1, k) + dy1jm * s000, dy1jm + dy1j)) +
-                         dznk * (qdiv(dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1), dy1jm + dy1j)),
-                         dznk + dznkp);
+      float evsx2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                         dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
+      float evsx1 = (dznkp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
+                         dznk * (xdiv<P2>(dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1), dx1im + dx1i))) / (dznk + dznkp);
+      float evsy2 = (dznkp * (xdiv<P2>(dy1jp * s000 + dy1j * sm(i, j + 1, k), dy1j + dy1jp)) +
+                         dznk * (xdiv<P2>(dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1), dy1j + dy1jp))) / (dznk + dznkp);
+      float evsy1 = (dznkp * (xdiv<P2>(dy1j * sm(i, j - 1, k) + dy1jm * s000, dy1jm + dy1j)) +
+                         dznk * (xdiv<P2>(dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1), dy1jm + dy1j))) / (dznk + dznkp);
       float viswx2 = (evsx2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
       float viswx1 = (evsx1) * (D3(i - 1, j, k + 1) + D7(i, j, k));
       float viswy2 = (evsy2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
       float viswy1 = (evsy1) * (D6(i, j - 1, k + 1) + D8(i, j, k));
       float viswz2 = (evsz2) * two * D9(i, j, k + 1);
       float viswz1 = (evsz1) * two * D9(i, j, k);
-      float vfw = qdiv(viswx2 - viswx1, dx1i) +
-                  qdiv(viswy2 - viswy1, dy1j) +
-                  qdiv(viswz2 - viswz1, dznk);
+      float vfw = xdiv<P2>(viswx2 - viswx1, dx1i) +
+                  xdiv<P2>(viswy2 - viswy1, dy1j) +
+                  (viswz2 - viswz1) / dznk;
       hv = (hv + vfw);
     }
   }
@@ -586,8 +589,12 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
 }
 
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam) {
-  hipLaunchKernelGGL(k_visc_adam, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F, do_visc,
-                     do_adam);
+  if (g.p2xy)
+    hipLaunchKernelGGL(k_visc_adam<true>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F,
+                       do_visc, do_adam);
+  else
+    hipLaunchKernelGGL(k_visc_adam<false>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F,
+                       do_visc, do_adam);
 }
 
 // ==========================================================================================
@@ -629,6 +636,7 @@ void launch_bondfg(hipStream_t st, Dims d, Fields F) {
 }
 
 // ---- press.f95:49-75: bondfg + rhs + exact partial sums of dx1*dy1*dzn*rhs ----
+template <bool P2>
 __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics ph,
                                              int64_t* partials) {
   __shared__ unsigned long long lds[NLIMB];
@@ -655,11 +663,11 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
       const float gjm = F.g[d.F(i, jm1, k)];
       const float hk = (k == km) ? 0.0f : F.h[ifg];
       const float hkm = (k == 1) ? 0.0f : F.h[d.F(i, j, k - 1)];
-      float r = qdiv(-F.u[d.U(i - 1, j, k)] + F.u[d.U(i, j, k)], dx1i) +
-                qdiv(-F.v[d.U(i, j - 1, k)] + F.v[d.U(i, j, k)], dy1j) +
-                qdiv(-F.w[d.W(i, j, k - 1)] + F.w[d.W(i, j, k)], dznk);
-      r = qdiv(F.f[ifg] - fim, dx1i) + qdiv(F.g[ifg] - gjm, dy1j) + qdiv(hk - hkm, dznk) +
-          qdiv(r, dt);
+      float r = xdiv<P2>(-F.u[d.U(i - 1, j, k)] + F.u[d.U(i, j, k)], dx1i) +
+                xdiv<P2>(-F.v[d.U(i, j - 1, k)] + F.v[d.U(i, j, k)], dy1j) +
+                (-F.w[d.W(i, j, k - 1)] + F.w[d.W(i, j, k)]) / dznk;
+      r = xdiv<P2>(F.f[ifg] - fim, dx1i) + xdiv<P2>(F.g[ifg] - gjm, dy1j) + (hk - hkm) / dznk +
+          (r) / dt;
       F.rhs[d.R(i, j, k)] = r;
       acc.add(dx1i * dy1j * dznk * r, lds);
     }
@@ -689,7 +697,10 @@ void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf r
                 DevScalars* sc) {
   dim3 grid = interior_grid(d, RED_KC);
   int nb = (int)(grid.x * grid.y * grid.z);
-  hipLaunchKernelGGL(k_rhs, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
+  if (g.p2xy)
+    hipLaunchKernelGGL(k_rhs<true>, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
+  else
+    hipLaunchKernelGGL(k_rhs<false>, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
   hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
 }
 

@@ -190,21 +190,23 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
   if (k0 > d.kp) return;  // padding workgroups of the XCD remap (uniform per workgroup)
   const int k1 = min(d.kp, k0 + kc - 1);
   const int tid = threadIdx.x;
-  const int ng = sp.nr >> 2;
+  const int ng = sp.nr >> 2;  // float4 groups per row (<= 64 since nr <= SOR_ITER_NRMAX)
   const int jp = d.jp, kp = d.kp, ip = d.ip;
   const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
   const float om = ph.omega;
   auto wrapj = [&](int j) { return ((j - 1) % jp + jp) % jp + 1; };  // periodic j
   auto rowbase = [&](int j, int k) { return sp.nr * ((j - 1) + jp * (k - 1)); };
 
+  // Work split inside the workgroup: wavefront wv takes rows wv, wv+4, ...; its lanes take
+  // consecutive cells of the row, so LDS reads and global accesses are unit-stride.
+  const int wv = tid >> 6, lane = tid & 63;
   // old colour 0 of level k (1 <= k <= kp) -> sB ring
   auto load_b = [&](int k) {
     float(*dst)[SOR_ITER_NRMAX] = sB[k % 3];
-    for (int w = tid; w < (TJ + 4) * ng; w += 256) {
-      const int r = w / ng, q = w - r * ng;
-      const float4 v =
-          *reinterpret_cast<const float4*>(io.pin[0] + rowbase(wrapj(j0 - 2 + r), k) + 4 * q);
-      *reinterpret_cast<float4*>(&dst[r][4 * q]) = v;
+    for (int r = wv; r < TJ + 4; r += 4) {
+      const float* src = io.pin[0] + rowbase(wrapj(j0 - 2 + r), k);
+      if (lane < ng)
+        *reinterpret_cast<float4*>(&dst[r][4 * lane]) = *reinterpret_cast<const float4*>(src + 4 * lane);
     }
   };
   // new colour 1 of level k (1 <= k <= kp) on rows j0-1 .. j0+TJ -> sA ring
@@ -214,20 +216,13 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
     const float(*bu)[SOR_ITER_NRMAX] = sB[(k + 1) % 3];
     const float(*bd)[SOR_ITER_NRMAX] = sB[(k + 2) % 3];  // == level k-1
     float(*dst)[SOR_ITER_NRMAX] = sA[k % 3];
-    for (int w = tid; w < (TJ + 2) * ng; w += 256) {
-      const int r = w / ng, q = w - r * ng;
+    for (int r = wv; r < TJ + 2; r += 4) {
       const int j = wrapj(j0 - 1 + r);
       const int f = ((1 + j + k) & 1) == 1 ? 1 : 2;  // first i of colour 1 in this row
-      const int base = rowbase(j, k) + 4 * q;
-      const float4 own = *reinterpret_cast<const float4*>(io.pin[1] + base);
-      const float4 rhs = *reinterpret_cast<const float4*>(sp.r[1] + base);
-      const float pv[4] = {own.x, own.y, own.z, own.w};
-      const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int ii = 4 * q + e;
+      const int rb = rowbase(j, k);
+      for (int ii = lane; ii < sp.nr; ii += 64) {
         const int i = f + 2 * ii;
-        float pc = pv[e];
+        float pc = io.pin[1][rb + ii];
         if (i <= ip) {
           const float pe = (i == ip) ? pc : bm[r + 1][ii + f - 1];
           const float pw = (i == 1) ? pc : bm[r + 1][ii + f - 2];
@@ -235,7 +230,8 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
           const float pu = (k < kp) ? bu[r + 1][ii] : pc;
           const float pd = (k > 1) ? bd[r + 1][ii] : pc;
           const float reltmp =
-              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd - rv[e]) -
+              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd -
+                           sp.r[1][rb + ii]) -
                     pc);
           pc = pc + reltmp;
         }
@@ -250,18 +246,12 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
     const float(*au)[SOR_ITER_NRMAX] = sA[(k + 1) % 3];
     const float(*ad)[SOR_ITER_NRMAX] = sA[(k + 2) % 3];  // == level k-1
     const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
-    for (int w = tid; w < TJ * ng; w += 256) {
-      const int r = w / ng, q = w - r * ng;
+    for (int r = wv; r < TJ; r += 4) {
       const int j = j0 + r;
-      if (j > jp) continue;
+      if (j > jp) break;
       const int f = ((1 + j + k) & 1) == 0 ? 1 : 2;  // first i of colour 0 in this row
-      const int base = rowbase(j, k) + 4 * q;
-      const float4 rhs = *reinterpret_cast<const float4*>(sp.r[0] + base);
-      const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
-      float out[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int ii = 4 * q + e;
+      const int rb = rowbase(j, k);
+      for (int ii = lane; ii < sp.nr; ii += 64) {
         const int i = f + 2 * ii;
         float pc = bm[r + 2][ii];
         if (i <= ip) {
@@ -271,15 +261,14 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
           const float pu = (k < kp) ? au[r + 1][ii] : pc;
           const float pd = (k > 1) ? ad[r + 1][ii] : pc;
           const float reltmp =
-              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd - rv[e]) -
+              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd -
+                           sp.r[0][rb + ii]) -
                     pc);
           pc = pc + reltmp;
         }
-        out[e] = pc;
+        io.pout[0][rb + ii] = pc;
+        io.pout[1][rb + ii] = am[r + 1][ii];
       }
-      *reinterpret_cast<float4*>(io.pout[0] + base) = make_float4(out[0], out[1], out[2], out[3]);
-      *reinterpret_cast<float4*>(io.pout[1] + base) =
-          make_float4(am[r + 1][4 * q], am[r + 1][4 * q + 1], am[r + 1][4 * q + 2], am[r + 1][4 * q + 3]);
     }
   };
 

@@ -5,6 +5,7 @@
 
 namespace les {
 
+template <bool P2>
 struct VelCtx {
   Dims d;
   Grid1D g;
@@ -18,64 +19,64 @@ struct VelCtx {
   __device__ float dzs(int k) const { return g.DZS(k); }
 
   // vel2.f95:51-60
-  __device__ float diu1(int i, int j, int k) const { return qdiv(-U(i - 1, j, k) + U(i, j, k), dx1(i)); }
+  __device__ float diu1(int i, int j, int k) const { return xdiv<P2>(-U(i - 1, j, k) + U(i, j, k), dx1(i)); }
   __device__ float cov1(int i, int j, int k) const {
     return ((U(i - 1, j, k) + U(i, j, k)) / 2.0f) * diu1(i, j, k);
   }
-  __device__ float diu5(int i, int j, int k) const { return qdiv(-V(i, j - 1, k) + V(i, j, k), dy1(j)); }
+  __device__ float diu5(int i, int j, int k) const { return xdiv<P2>(-V(i, j - 1, k) + V(i, j, k), dy1(j)); }
   __device__ float cov5(int i, int j, int k) const {
     return ((V(i, j - 1, k) + V(i, j, k)) / 2.0f) * diu5(i, j, k);
   }
-  __device__ float diu9(int i, int j, int k) const { return qdiv(-W(i, j, k - 1) + W(i, j, k), dzn(k)); }
+  __device__ float diu9(int i, int j, int k) const { return (-W(i, j, k - 1) + W(i, j, k)) / dzn(k); }
   __device__ float cov9(int i, int j, int k) const {
     return ((W(i, j, k - 1) + W(i, j, k)) / 2.0f) * diu9(i, j, k);
   }
   // vel2.f95:68-70
   __device__ float diu2(int i, int j, int k) const {
-    return qdiv(2.0f * (-U(i, j - 1, k) + U(i, j, k)), dy1(j - 1) + dy1(j));
+    return xdiv<P2>(2.0f * (-U(i, j - 1, k) + U(i, j, k)), dy1(j - 1) + dy1(j));
   }
   __device__ float cov2(int i, int j, int k) const {
-    float nou = qdiv(dx1(i + 1) * V(i, j - 1, k) + dx1(i) * V(i + 1, j - 1, k), dx1(i) + dx1(i + 1));
+    float nou = xdiv<P2>(dx1(i + 1) * V(i, j - 1, k) + dx1(i) * V(i + 1, j - 1, k), dx1(i) + dx1(i + 1));
     return nou * diu2(i, j, k);
   }
   // vel2.f95:78-80
   __device__ float diu3(int i, int j, int k) const {
-    return qdiv(-U(i, j, k - 1) + U(i, j, k), dzs(k - 1));
+    return (-U(i, j, k - 1) + U(i, j, k)) / dzs(k - 1);
   }
   __device__ float cov3(int i, int j, int k) const {
-    float nou = qdiv(dx1(i + 1) * W(i, j, k - 1) + dx1(i) * W(i + 1, j, k - 1), dx1(i) + dx1(i + 1));
+    float nou = xdiv<P2>(dx1(i + 1) * W(i, j, k - 1) + dx1(i) * W(i + 1, j, k - 1), dx1(i) + dx1(i + 1));
     return nou * diu3(i, j, k);
   }
   // vel2.f95:88-90 (the "- u0" with u0 = 0 is exact)
   __device__ float diu4(int i, int j, int k) const {
-    return qdiv(2.0f * (-V(i - 1, j, k) + V(i, j, k)), dx1(i - 1) + dx1(i));
+    return xdiv<P2>(2.0f * (-V(i - 1, j, k) + V(i, j, k)), dx1(i - 1) + dx1(i));
   }
   __device__ float cov4(int i, int j, int k) const {
-    float nou = qdiv(dy1(j + 1) * U(i - 1, j, k) + dy1(j) * U(i - 1, j + 1, k), dy1(j) + dy1(j + 1));
+    float nou = xdiv<P2>(dy1(j + 1) * U(i - 1, j, k) + dy1(j) * U(i - 1, j + 1, k), dy1(j) + dy1(j + 1));
     return nou * diu4(i, j, k);
   }
   // vel2.f95:98-100
   __device__ float diu6(int i, int j, int k) const {
-    return qdiv(-V(i, j, k - 1) + V(i, j, k), dzs(k - 1));
+    return (-V(i, j, k - 1) + V(i, j, k)) / dzs(k - 1);
   }
   __device__ float cov6(int i, int j, int k) const {
-    float nou = qdiv(dy1(j + 1) * W(i, j, k - 1) + dy1(j) * W(i, j + 1, k - 1), dy1(j) + dy1(j + 1));
+    float nou = xdiv<P2>(dy1(j + 1) * W(i, j, k - 1) + dy1(j) * W(i, j + 1, k - 1), dy1(j) + dy1(j + 1));
     return nou * diu6(i, j, k);
   }
   // vel2.f95:108-110
   __device__ float diu7(int i, int j, int k) const {
-    return qdiv(2.0f * (-W(i - 1, j, k) + W(i, j, k)), dx1(i - 1) + dx1(i));
+    return xdiv<P2>(2.0f * (-W(i - 1, j, k) + W(i, j, k)), dx1(i - 1) + dx1(i));
   }
   __device__ float cov7(int i, int j, int k) const {
-    float nou = qdiv(dzn(k + 1) * U(i - 1, j, k) + dzn(k) * U(i - 1, j, k + 1), dzn(k) + dzn(k + 1));
+    float nou = (dzn(k + 1) * U(i - 1, j, k) + dzn(k) * U(i - 1, j, k + 1)) / (dzn(k) + dzn(k + 1));
     return nou * diu7(i, j, k);
   }
   // vel2.f95:118-120
   __device__ float diu8(int i, int j, int k) const {
-    return qdiv(2.0f * (-W(i, j - 1, k) + W(i, j, k)), dy1(j - 1) + dy1(j));
+    return xdiv<P2>(2.0f * (-W(i, j - 1, k) + W(i, j, k)), dy1(j - 1) + dy1(j));
   }
   __device__ float cov8(int i, int j, int k) const {
-    float nou = qdiv(dzn(k + 1) * V(i, j - 1, k) + dzn(k) * V(i, j - 1, k + 1), dzn(k) + dzn(k + 1));
+    float nou = (dzn(k + 1) * V(i, j - 1, k) + dzn(k) * V(i, j - 1, k + 1)) / (dzn(k) + dzn(k + 1));
     return nou * diu8(i, j, k);
   }
 };

@@ -97,6 +97,23 @@ class LES:
         if coefficients:
             self.init_coefficients()
 
+    AVE_NAMES = ["aveu", "avev", "avew", "avep", "avel", "aveuu", "avevv", "aveww", "avesm",
+                 "avesmsm", "uwfx", "avesu", "avesv", "avesw", "avesuu", "avesvv", "avesww", "uwfxs"]
+
+    def aveflow(self, n, n1, nmax):
+        """aveflow.f95:42-157 after step n (les_aveflow)."""
+        self._chk(lib().les_aveflow(self.h, n, n1, nmax), "aveflow")
+
+    def aveflow_stats(self):
+        """The eighteen statistics arrays (flat, i fastest)."""
+        ip, jp, kp = self.dims
+        out = {}
+        for q, n in enumerate(self.AVE_NAMES):
+            a = np.empty(ip * jp * kp if q < 11 else ip * kp, np.float32)
+            self._chk(lib().les_aveflow_download(self.h, q, a.ctypes.data), "aveflow_download")
+            out[n] = a
+        return out
+
     def set_terrain(self, zbm):
         """feedbfm.f95:42-63 on the device from zbm(-1:ip+1,-1:jp+1) (z2, dzn uploaded)."""
         z = np.ascontiguousarray(zbm, dtype=np.float32)
@@ -261,6 +278,10 @@ class LES:
         self._chk(lib().les_autotune(self.h, reps, ctypes.byref(t)), "autotune")
         return {"sor_tile_j": t.sor_tile_j, "sor_tile_k": t.sor_tile_k,
                 "sor_iter_ms": float(t.sor_iter_ms)}
+
+    def set_sor_tile(self, tile_j, tile_k=0):
+        """SOR iteration form: tile_j = 0 two colour sweeps, else the fused launch's tile."""
+        self._chk(lib().les_set_sor_tile(self.h, tile_j, tile_k), "set_sor_tile")
 
     def time_kernel(self, which, reps=20):
         """Average milliseconds per launch of one kernel (HIP events on the handle's stream)."""

@@ -14,6 +14,7 @@
 //                   (the semantics the HIP path implements); FP64 instantiations treat it as SEQ.
 //
 // Parity pins: see tests/test_oracle_*.py (reference-build fixtures, closed forms, invariants).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1074,4 +1075,110 @@ extern "C" float orc_exact_sum_f32(const float* x, long n) {
   BigAcc b;
   for (long i = 0; i < n; ++i) b.add(x[i]);
   return b.to_float();
+}
+
+// ---------------------------------------------------------------------------------------------
+// aveflow.f95:42-157: time and spanwise statistics of u, v, w, p, sm after each step.
+// The eleven (ip,jp,kp) arrays and seven (ip,kp) arrays are flat, i fastest.  The final uwfx
+// loop reads aveu and avew at level k-1 = 0 for k = 1, before the arrays' first element; here the
+// ip*jp values there read as 0 (flat indexing, so avew(ip+1,jp,0) is avew(1,1,1)), which is
+// what the reference computes when the arrays are preceded by zeros.
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+typedef struct {
+  void *aveu, *avev, *avew, *avep, *avel, *aveuu, *avevv, *aveww, *avesm, *avesmsm, *uwfx;
+  void *avesu, *avesv, *avesw, *avesuu, *avesvv, *avesww, *uwfxs;
+} orc_ave;
+}
+
+namespace {
+template <typename T>
+void aveflow(F<T>& a, const orc_ave& s, int n, int n1, int nmax) {
+  const int im = a.im, jm = a.jm, km = a.km;
+  const long n3 = (long)im * jm * km, n2 = (long)im * km;
+  T* aveu = (T*)s.aveu; T* avev = (T*)s.avev; T* avew = (T*)s.avew; T* avep = (T*)s.avep;
+  T* avel = (T*)s.avel; T* aveuu = (T*)s.aveuu; T* avevv = (T*)s.avevv; T* aveww = (T*)s.aveww;
+  T* avesm = (T*)s.avesm; T* avesmsm = (T*)s.avesmsm; T* uwfx = (T*)s.uwfx;
+  T* avesu = (T*)s.avesu; T* avesv = (T*)s.avesv; T* avesw = (T*)s.avesw;
+  T* avesuu = (T*)s.avesuu; T* avesvv = (T*)s.avesvv; T* avesww = (T*)s.avesww;
+  T* uwfxs = (T*)s.uwfxs;
+  auto o3 = [&](int i, int j, int k) { return (long)(i - 1) + (long)im * ((j - 1) + (long)jm * (k - 1)); };
+  auto o2 = [&](int i, int k) { return (long)(i - 1) + (long)im * (k - 1); };
+  auto rd = [&](const T* x, long off) { return off < 0 ? T(0) : x[off]; };
+  if (n == n1) {
+    T* z3[] = {aveu, avev, avew, avep, avel, aveuu, avevv, aveww, avesm, avesmsm, uwfx};
+    for (T* x : z3) std::fill(x, x + n3, T(0));
+    T* z2[] = {avesu, avesv, avesw, avesuu, avesvv, avesww};
+    for (T* x : z2) std::fill(x, x + n2, T(0));
+  }
+  if (n >= n1) {  // aveflow.f95:70-86
+    for (int k = 1; k <= km; ++k)
+      for (int j = 1; j <= jm; ++j)
+        for (int i = 1; i <= im; ++i) {
+          const long q = o3(i, j, k);
+          const T uu = a.u(i, j, k), vv = a.v(i, j, k), ww = a.w(i, j, k);
+          aveu[q] = aveu[q] + uu;
+          avev[q] = avev[q] + vv;
+          avew[q] = avew[q] + ww;
+          avep[q] = avep[q] + a.p(i, j, k);
+          aveuu[q] = aveuu[q] + uu * uu;
+          avevv[q] = avevv[q] + vv * vv;
+          aveww[q] = aveww[q] + ww * ww;
+          const T smv = a.sm(i, j, k);
+          avesm[q] = avesm[q] + smv;
+          avesmsm[q] = avesmsm[q] + smv * smv;
+          uwfx[q] = uwfx[q] + T(0.5) * (a.u(i, j, k - 1) + a.u(i, j, k)) * T(0.5) *
+                                  (a.w(i, j, k - 1) + a.w(i + 1, j, k - 1));
+        }
+  }
+  if (n != nmax) return;
+  const T cnt = (T)(float)(nmax - n1 + 1);  // float(nmax-n1+1)
+  T* d3[] = {aveu, avev, avew, avep, avel, aveuu, avevv, aveww, avesm, avesmsm};
+  for (T* x : d3)
+    for (long q = 0; q < n3; ++q) x[q] = x[q] / cnt;
+  for (int k = 1; k <= km; ++k)  // aveflow.f95:104-110
+    for (int j = 1; j <= jm; ++j)
+      for (int i = 1; i <= im; ++i) {
+        const long q = o3(i, j, k);
+        uwfx[q] = uwfx[q] / cnt - T(0.5) * (rd(aveu, o3(i, j, k - 1)) + aveu[q]) * T(0.5) *
+                                      (rd(avew, o3(i, j, k - 1)) + rd(avew, o3(i + 1, j, k - 1)));
+      }
+  for (int k = 1; k <= km; ++k)  // aveflow.f95:111-123
+    for (int j = 1; j <= jm; ++j)
+      for (int i = 1; i <= im; ++i) {
+        const long q = o3(i, j, k), r = o2(i, k);
+        avesu[r] = avesu[r] + aveu[q];
+        avesv[r] = avesv[r] + avev[q];
+        avesw[r] = avesw[r] + avew[q];
+        avesuu[r] = avesuu[r] + aveuu[q];
+        avesvv[r] = avesvv[r] + avevv[q];
+        avesww[r] = avesww[r] + aveww[q];
+        uwfxs[r] = uwfxs[r] + uwfx[q];
+      }
+  const T fjm = (T)(float)jm;
+  T* s2[] = {avesu, avesv, avesw, avesuu, avesvv, avesww, uwfxs};
+  for (T* x : s2)
+    for (long r = 0; r < n2; ++r) x[r] = x[r] / fjm;
+  for (long q = 0; q < n3; ++q) {  // aveflow.f95:136-149
+    aveuu[q] = std::sqrt(std::fabs(aveuu[q] - aveu[q] * aveu[q]));
+    avevv[q] = std::sqrt(std::fabs(avevv[q] - avev[q] * avev[q]));
+    aveww[q] = std::sqrt(std::fabs(aveww[q] - avew[q] * avew[q]));
+  }
+  for (long r = 0; r < n2; ++r) {
+    avesuu[r] = std::sqrt(std::fabs(avesuu[r] - avesu[r] * avesu[r]));
+    avesvv[r] = std::sqrt(std::fabs(avesvv[r] - avesv[r] * avesv[r]));
+    avesww[r] = std::sqrt(std::fabs(avesww[r] - avesw[r] * avesw[r]));
+  }
+}
+}  // namespace
+
+extern "C" int orc_aveflow_f32(const orc_fields* s, const orc_ave* st, int n, int n1, int nmax) {
+  F<float> a(*s);
+  aveflow<float>(a, *st, n, n1, nmax);
+  return 0;
+}
+extern "C" int orc_aveflow_f64(const orc_fields* s, const orc_ave* st, int n, int n1, int nmax) {
+  F<double> a(*s);
+  aveflow<double>(a, *st, n, n1, nmax);
+  return 0;
 }

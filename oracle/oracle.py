@@ -43,6 +43,15 @@ class OrcScalars(ctypes.Structure):
                 ("refine_passes", ctypes.c_int)]
 
 
+AVE3 = ["aveu", "avev", "avew", "avep", "avel", "aveuu", "avevv", "aveww", "avesm", "avesmsm",
+        "uwfx"]
+AVE2 = ["avesu", "avesv", "avesw", "avesuu", "avesvv", "avesww", "uwfxs"]
+
+
+class OrcAve(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in AVE3 + AVE2]
+
+
 SUM_SEQ, SUM_EXACT = 0, 1
 _STAGES = ["velnw", "bondv1", "vel2", "velfg", "feedbf", "les_calc_sm", "boundsm",
            "les_calc_visc", "les", "adam", "bondfg", "boundp1", "boundp2", "press_rhsav",
@@ -195,3 +204,15 @@ class OracleState:
 
     def step(self, n, **over):
         return self.call("step", n, **over)
+
+    def new_stats(self):
+        """Zeroed aveflow statistics arrays (flat, i fastest)."""
+        ip, jp, kp = self.ip, self.jp, self.kp
+        return {n: np.zeros(ip * jp * kp if n in AVE3 else ip * kp, self.dtype) for n in AVE3 + AVE2}
+
+    def aveflow(self, stats, n, n1, nmax):
+        """aveflow.f95:42-157 on 'stats' (new_stats()) after step n."""
+        st = OrcAve(*[stats[k].ctypes.data for k in AVE3 + AVE2])
+        r = getattr(lib(), "orc_aveflow" + self.sfx)(ctypes.byref(self._struct()), ctypes.byref(st),
+                                                   n, n1, nmax)
+        assert r == 0

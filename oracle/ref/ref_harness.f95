@@ -12,6 +12,10 @@
 !      state is dumped.
 !   2  terrain masks: feedbfm.f95 with its own GIS read (run with cwd = src/Fortran), zbm and
 !      the four masks are dumped.
+!   3  statistics: as mode 0 (no state dumps), with aveflow.f95 called after every step
+!      (n1 from the input, nmax = nsteps); the eighteen statistics arrays are dumped at the end.
+!      aveu and avew are passed as sections of buffers that start with ip*jp zeros, so the
+!      reads of aveflow.f95:105-106 at level k-1 = 0 (before the arrays) see zeros.
 
 subroutine noop
   ! press.f95:116 calls noop, which the reference leaves to an absent noop.c
@@ -33,6 +37,7 @@ program ref_harness
   use module_boundp
   use module_init
   use module_feedbfm
+  use module_aveflow
   implicit none
 
   real(kind=4), dimension(0:ip+2,0:jp+2,0:kp+1) :: p
@@ -63,6 +68,14 @@ program ref_harness
   real(kind=4), dimension(kp) :: cn4l, cn4s, delx1
   real(kind=4), dimension(jp,kp) :: wind_profile
   real(kind=4), dimension(-1:ipmax+1,-1:jpmax+1) :: zbm
+  ! mode 3 only; on the heap so that the large grids keep their static arrays below 2 GiB
+  real(kind=4), allocatable :: aveubuf(:), avewbuf(:)
+  real(kind=4), allocatable, dimension(:,:,:) :: avev, avep, avel, aveuu, avevv, aveww, avesm, &
+                                                 avesmsm, uwfx
+  real(kind=4), allocatable, dimension(:,:) :: avesu, avesv, avesw, avesuu, avesvv, avesww, uwfxs
+  integer :: n1
+  real(kind=4) :: time
+  character(len=70) :: data10, data11
   real(kind=4) :: dt, ro, vn, alpha, beta
   integer :: mode, nip, njp, nkp, nsteps, ndump, stage, nstage, n, idump, im, jm, km
   integer, allocatable :: dump_steps(:)
@@ -98,10 +111,20 @@ program ref_harness
   read(10) dx1, dy1, dzn, dzs, dxs, dys, z2
   read(10) amask1, bmask1, cmask1, dmask1, wind_profile
 
-  if (mode == 0) then
+  if (mode == 0 .or. mode == 3) then
     read(10) nsteps, ndump
     allocate(dump_steps(ndump))
     read(10) dump_steps
+    if (mode == 3) then
+      read(10) n1
+      allocate(aveubuf(ip*jp + ip*jp*kp), avewbuf(ip*jp + ip*jp*kp))
+      allocate(avev(ip,jp,kp), avep(ip,jp,kp), avel(ip,jp,kp), aveuu(ip,jp,kp), avevv(ip,jp,kp))
+      allocate(aveww(ip,jp,kp), avesm(ip,jp,kp), avesmsm(ip,jp,kp), uwfx(ip,jp,kp))
+      allocate(avesu(ip,kp), avesv(ip,kp), avesw(ip,kp), avesuu(ip,kp), avesvv(ip,kp))
+      allocate(avesww(ip,kp), uwfxs(ip,kp))
+      aveubuf = 0.0; avewbuf = 0.0; uwfxs = 0.0; time = 0.0
+      data10 = ''; data11 = ''
+    end if
     call init(km, jm, im, u, v, w, p, cn2s, dxs, cn2l, cn3s, dys, cn3l, dzs, cn4s, cn4l, cn1, &
               amask1, bmask1, cmask1, dmask1, zbm, z2, dzn)
     ! init.f95 is built with IFBF=0 (no GIS read): the masks come from the input file
@@ -129,12 +152,21 @@ program ref_harness
     call system_clock(clk0, clkrate)
     do n = 1, nsteps
       call one_step(n)
+      if (mode == 3) then
+        call aveflow(n, n1, km, jm, im, aveubuf(ip*jp+1), avev, avewbuf(ip*jp+1), avep, avel, &
+                     aveuu, avevv, aveww, avesm, avesmsm, uwfx, avesu, avesv, avesw, avesuu, &
+                     avesvv, avesww, u, v, w, p, sm, nsteps, uwfxs, data10, time, data11)
+      end if
       do idump = 1, ndump
         if (dump_steps(idump) == n) call dump_state()
       end do
     end do
     call system_clock(clk1)
     print '(A,F12.6)', 'steps_seconds=', real(clk1 - clk0, 8) / real(clkrate, 8)
+    if (mode == 3) then
+      write(11) aveubuf(ip*jp+1:), avev, avewbuf(ip*jp+1:), avep, avel, aveuu, avevv, aveww, &
+                avesm, avesmsm, uwfx, avesu, avesv, avesw, avesuu, avesvv, avesww, uwfxs
+    end if
   else if (mode == 1) then
     read(10) stage, nstage
     call read_state()

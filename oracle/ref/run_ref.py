@@ -101,6 +101,26 @@ def trajectory(case, phys, nsteps, dump_steps):
     return _parse(_run(binary(ip, jp, kp), payload), ip, jp, kp, len(dump_steps))
 
 
+AVE3 = ["aveu", "avev", "avew", "avep", "avel", "aveuu", "avevv", "aveww", "avesm", "avesmsm",
+        "uwfx"]
+AVE2 = ["avesu", "avesv", "avesw", "avesuu", "avesvv", "avesww", "uwfxs"]
+
+
+def aveflow_stats(case, phys, nsteps, n1):
+    """Mode 3: init sequence, nsteps steps with aveflow.f95 after each (nmax = nsteps);
+    returns the statistics arrays (flat, i fastest)."""
+    ip, jp, kp = case["dims"]
+    payload = _header(3, ip, jp, kp, phys, case) + _i32(nsteps, 0) + _i32(n1)
+    raw = np.frombuffer(_run(binary(ip, jp, kp), payload), np.float32)
+    out, pos = {}, 0
+    for n in AVE3 + AVE2:
+        cnt = ip * jp * kp if n in AVE3 else ip * kp
+        out[n] = raw[pos:pos + cnt].copy()
+        pos += cnt
+    assert pos == raw.size, (pos, raw.size)
+    return out
+
+
 def stage(case, phys, state, stage_id, n):
     """Mode 1: one reference stage on a full state (dict of flat float32 arrays)."""
     ip, jp, kp = case["dims"]

@@ -83,7 +83,15 @@ def main():
                              "dumps": {str(n): digest(s) for n, s in zip(dumps_odd, traj)}}
     print("odd trajectory done")
 
-    # 4. the reference's own grid size and GIS terrain
+    # 4. aveflow statistics over a short run (aveflow.f95 after every step)
+    st = run_ref.aveflow_stats(case, phys, 6, 3)
+    out["aveflow"] = {"dims": list(case["dims"]), "seed": 7, "hmax": 12.0, "coverage": 0.3,
+                      "nsteps": 6, "n1": 3,
+                      "stats": {n: [common.sha(v), float(np.linalg.norm(v.astype(np.float64)))]
+                                for n, v in st.items()}}
+    print("aveflow done")
+
+    # 5. the reference's own grid size and GIS terrain
     if args.big:
         ip, jp, kp = 150, 150, 90
         case = inputs.make_case(ip, jp, kp)

@@ -279,22 +279,51 @@ def test_run_after_inconsistent_upload(case):
     _compare(L, o, "run after inconsistent upload")
 
 
-@pytest.mark.parametrize("dims", [(40, 32, 20), (37, 18, 23)])
-def test_fused_sor_iteration_after_autotune(dims):
-    """The fused iteration (both colours per launch, ping-pong iterates) with the tile the
-    autotuner picks reproduces the oracle's trajectory bit for bit (50 iterations at omega 1.5,
-    no early exit)."""
+@pytest.mark.parametrize("dims,tile", [((40, 32, 20), (8, 10)), ((37, 18, 23), (4, 5)),
+                                       ((37, 18, 23), (16, 23)), ((40, 32, 20), (4, 45))])
+def test_fused_sor_iteration(dims, tile):
+    """The fused iteration (both colours per launch, ping-pong iterates) with a given tile
+    reproduces the oracle's trajectory bit for bit (50 iterations at omega 1.5, no early exit),
+    eagerly and from the captured graph."""
     case = common.tiny_case(*dims, seed=21)
     L, o = _pair(case, **common.CFG_SOR)
-    t = L.autotune(2)
-    assert t["sor_tile_j"] in (4, 8, 16) and t["sor_tile_k"] > 0, t
+    L.set_sor_tile(*tile)
     L.ifdata()
     o.ifdata()
     for n in range(1, 4):
         L.step(n)
         o.step(n)
-    _compare(L, o, "fused SOR after autotune %s" % (t,), ["u", "v", "w", "p", "rhs", "sm"])
+    _compare(L, o, "fused SOR tile %s" % (tile,), ["u", "v", "w", "p", "rhs", "sm"])
     L.run(4, 2)
     for n in (4, 5):
         o.step(n)
     _compare(L, o, "fused SOR, graph replay", ["u", "v", "w", "p"])
+
+
+def test_autotune_keeps_results(case):
+    L, o = _pair(case, **common.CFG_SOR)
+    t = L.autotune(2)
+    assert t["sor_tile_j"] in (0, 4, 8, 16) and t["sor_iter_ms"] > 0, t
+    L.ifdata()
+    o.ifdata()
+    L.run(1, 3)
+    for n in range(1, 4):
+        o.step(n)
+    _compare(L, o, "after autotune %s" % (t,), ["u", "v", "w", "p"])
+
+
+def test_aveflow_statistics_bitwise(case):
+    """aveflow.f95:42-157 on the device after every step equals the oracle's statistics."""
+    L, o = _pair(case, **common.CFG_SOR)
+    L.ifdata()
+    o.ifdata()
+    st = o.new_stats()
+    nsteps, n1 = 6, 3
+    for n in range(1, nsteps + 1):
+        L.step(n)
+        L.aveflow(n, n1, nsteps)
+        o.step(n)
+        o.aveflow(st, n, n1, nsteps)
+    got = L.aveflow_stats()
+    bad = [common.mismatch_report(got[n], st[n], n) for n in st if not common.bits_equal(got[n], st[n])]
+    assert not bad, "\n".join(bad)

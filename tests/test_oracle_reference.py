@@ -101,3 +101,20 @@ def test_tokyo_mask_rule():
     o.feedbfm(d["zbm"])
     for n in ref:
         assert common.bits_equal(o.a[n], ref[n]), n
+
+
+def test_aveflow_statistics_bitwise():
+    """aveflow.f95:42-157 after every step of a short run: every statistics array of the oracle
+    equals the reference's (run with aveu/avew preceded by zeros, see ref_harness.f95 mode 3)."""
+    ent = VEC["aveflow"]
+    ip, jp, kp = ent["dims"]
+    case = common.tiny_case(ip, jp, kp, seed=ent["seed"])
+    o = common.oracle_from_case(case, **common.REF_PHYS, **common.REF_SOR, sum_mode=SUM_SEQ)
+    o.ifdata()
+    st = o.new_stats()
+    for n in range(1, ent["nsteps"] + 1):
+        o.step(n)
+        o.aveflow(st, n, ent["n1"], ent["nsteps"])
+    bad = [n for n, (h, _) in ent["stats"].items() if common.sha(st[n]) != h]
+    assert not bad, ["%s: |%g| vs |%g|" % (n, np.linalg.norm(st[n].astype(np.float64)),
+                                          ent["stats"][n][1]) for n in bad]
