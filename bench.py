@@ -139,13 +139,44 @@ def ensemble_bench(nmembers, steps, warmup):
             "ms_per_step_all_members": round(t_ens / steps * 1e3, 3)}
 
 
+def mixed_bench(steps, tol=1e-7, refine_max=200):
+    """The mixed-precision configuration: 600x600x90, stencils and SOR sweeps in FP32, rhs mean,
+    residual and refined pressure in FP64 (LES_PREC_MIXED), refinement to a relative L2
+    residual of tol.  Steps run eagerly (the host reads the residual between passes)."""
+    import torch
+    from les_halo_amd import LES, inputs
+    from les_halo_amd.les import LES_PREC_MIXED
+    ip, jp, kp = 600, 600, 90
+    L = LES(ip, jp, kp, **PHYS, precision=LES_PREC_MIXED, refine_tol=tol, refine_max=refine_max)
+    L.load_case(inputs.make_case(ip, jp, kp))
+    L.ifdata()
+    L.run(1, 2)
+    L.sync()
+    passes, resid = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(3, 3 + steps):
+        L.step(n)
+        s = L.scalars()
+        passes.append(s["refine_passes"])
+        resid.append(s["resid_l2"])
+    L.sync()
+    secs = time.perf_counter() - t0
+    return {"grid": [ip, jp, kp], "steps": steps, "ms_per_step": round(secs / steps * 1e3, 3),
+            "value": round(ip * jp * kp * steps / secs / 1e6, 2), "unit": "M cell-steps/s",
+            "refine_tol": tol, "refine_max": refine_max, "refine_passes": passes,
+            "resid_l2_last": resid[-1], "resid_l2_max": max(resid)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed", type=int, default=0, metavar="STEPS",
+                    help="also run the mixed-precision 600x600x90 configuration for STEPS steps")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="also run the batched-ensemble configuration with N members")
     ap.add_argument("--profile-sor", type=int, default=0, metavar="REPS",
@@ -208,6 +239,8 @@ def main():
                        "parallelism": "single domain, 1 GPU"},
             "step_hbm_frac": round(step_hbm_frac, 4), "step_bytes_per_cell": B_STEP,
             "roofline": roofline}
+    if args.mixed:
+        line["mixed"] = mixed_bench(args.mixed)
     if args.ensemble:
         del L
         line["ensemble"] = ensemble_bench(args.ensemble, args.steps, args.warmup)
