@@ -48,6 +48,7 @@ struct les_handle {
   SorIterShape iter_shape{0, 0};  // tj == 0: two colour sweeps per iteration (untuned default)
   bool tuned = false;
   bool p2xy = false;  // see grid_p2xy
+  bool split_ok = false;  // the colour-split sweep's index arithmetic covers this grid
   // aveflow statistics, allocated by the first les_aveflow call
   float* ave_mem = nullptr;
   AveStats ave{};
@@ -411,6 +412,7 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     sp.cz = fac + (cfg->ip + 2) + (cfg->jp + 2);
     sp.cn1k = fac + (cfg->ip + 2) + (cfg->jp + 2) + (cfg->kp + 2);
     sp.uniform = 0;
+    h->split_ok = sor_split_divisors(h->d, sp);
     h->hcn1.assign((size_t)cfg->ip * cfg->jp * cfg->kp, 0.0f);
     if (cfg->precision == LES_PREC_MIXED) {
       Mixed& mx = h->mx;
@@ -759,7 +761,7 @@ namespace {
 // The colour-split solve (les_sor.hip) applies when its equivalence to the reference's
 // in-place sweep holds: even jp, mirror-consistent p halos, cn1 of the separable form.
 bool use_split(const les_handle* h) {
-  return (h->d.jp % 2 == 0) && h->p_consistent && h->cn1_separable;
+  return (h->d.jp % 2 == 0) && h->p_consistent && h->cn1_separable && h->split_ok;
 }
 
 // press.f95:87-115 on the Fortran arrays: colour sweep, boundp1 per colour, boundp2 per
@@ -823,7 +825,7 @@ int enqueue_sor_solve(les_handle* h) {
 // eagerly: the host reads the residual after each pass.
 int enqueue_press_mixed(les_handle* h) {
   STAGE_BEGIN(h);
-  if (!(h->d.jp % 2 == 0 && h->cn1_separable))
+  if (!(h->d.jp % 2 == 0 && h->cn1_separable && h->split_ok))
     return fail(h, LES_E_STATE, "mixed precision needs an even jp and cn1 of the init.f95 form");
   int r;
   if (!h->p_consistent) {  // the refinement works on mirror-consistent boundary images
@@ -1243,7 +1245,7 @@ extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
   STAGE_BEGIN(h);
   les_tuning best{};
   best.sor_iter_ms = -1.0f;
-  if (sor_iter_supported(d, h->sp)) {
+  if (h->split_ok && sor_iter_supported(d, h->sp)) {
     const int tjs[] = {4, 8, 16};
     const int kcs[] = {5, 10, 15, 23, 30, 45};
     hipEvent_t e0, e1;

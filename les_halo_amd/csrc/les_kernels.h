@@ -40,6 +40,7 @@ struct SplitSOR {
   const float *ax, *by, *cz, *cn1k;
   int nr, nj;  // padded colour row length, jp
   int uniform;
+  uint32_t m_ng, m_jp;  // multiply-high reciprocals of nr/4 and jp (sor_split_divisors)
 };
 
 // Fused SOR iteration (k_sor_iter): previous and next iterate, colour 0 and colour 1 each
@@ -91,6 +92,8 @@ void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScal
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc);
 int sor_split_blocks(Dims d);
+// sets SplitSOR::m_ng, m_jp; false when the grid is too large for the multiply-high quotients
+bool sor_split_divisors(Dims d, SplitSOR& sp);
 int sor_iter_supported(Dims d, const SplitSOR& sp);
 void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
                      DevScalars* sc, SorIterShape shape);

@@ -61,101 +61,101 @@ void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScal
 }
 
 // ---- one colour sweep: 4 consecutive cells of one colour row per thread ----
-template <bool UNIFORM>
+template <bool UNIFORM, bool RESID>
 __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp, Physics ph,
-                                                   int nrd, int want_resid, int64_t* partials,
+                                                   int nrd, int64_t* partials,
                                                    const DevScalars* sc, int ng, int nthreads) {
   __shared__ unsigned long long lds[NLIMB];
   if (sc->sor_done) return;
-  if (want_resid) block_limbs_init(lds);
+  if (RESID) block_limbs_init(lds);
   ThreadAcc acc;
   acc.init();
   const int L = xcd_tile(blockIdx.x, gridDim.x);
   const int t = L * 256 + threadIdx.x;
-  if (t < nthreads) {
-    const int grp = t % ng;
-    const int row = t / ng;
-    const int j = 1 + row % d.jp;
-    const int k = 1 + row / d.jp;
-    const int c = (1 + nrd) & 1;  // nrd = 0 updates i = 1 + mod(k+j,2): odd i+j+k
-    const int i0 = first_i(c, j, k) + 8 * grp;  // i of the group's first cell
-    if (i0 <= d.ip) {
-      const int rs = sp.nr * ((j - 1) + sp.nj * (k - 1));
-      const int base = rs + 4 * grp;
-      const float* po = sp.p[c ^ 1];
-      // neighbour rows: periodic in j, the cell itself across k faces
-      const int rn = (j == d.jp) ? -sp.nr * (d.jp - 1) : sp.nr;  // j+1
-      const int rsj = (j == 1) ? sp.nr * (d.jp - 1) : -sp.nr;    // j-1
-      const int pl = sp.nr * sp.nj;
-      const float4 own = *reinterpret_cast<const float4*>(sp.p[c] + base);
-      const float4 rhs = *reinterpret_cast<const float4*>(sp.r[c] + base);
-      const float4 on = *reinterpret_cast<const float4*>(po + base + rn);
-      const float4 os = *reinterpret_cast<const float4*>(po + base + rsj);
-      float4 ou, od;
-      if (k < d.kp) ou = *reinterpret_cast<const float4*>(po + base + pl);
-      else ou = own;
-      if (k > 1) od = *reinterpret_cast<const float4*>(po + base - pl);
-      else od = own;
-      // i neighbours: odd i -> east o[ii], west o[ii-1]; even i -> east o[ii+1], west o[ii]
-      const float4 oc = *reinterpret_cast<const float4*>(po + base);
-      float xe[4], xw[4];
-      const float ov[4] = {oc.x, oc.y, oc.z, oc.w};
-      if (i0 & 1) {
-        const float prev = (grp > 0) ? po[base - 1] : 0.0f;
-        xe[0] = ov[0]; xe[1] = ov[1]; xe[2] = ov[2]; xe[3] = ov[3];
-        xw[0] = prev; xw[1] = ov[0]; xw[2] = ov[1]; xw[3] = ov[2];
-      } else {
-        const float next = (i0 + 7 <= d.ip) ? po[base + 4] : 0.0f;
-        xw[0] = ov[0]; xw[1] = ov[1]; xw[2] = ov[2]; xw[3] = ov[3];
-        xe[0] = ov[1]; xe[1] = ov[2]; xe[2] = ov[3]; xe[3] = next;
-      }
-      const float pv[4] = {own.x, own.y, own.z, own.w};
-      const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
-      const float nv[4] = {on.x, on.y, on.z, on.w};
-      const float sv[4] = {os.x, os.y, os.z, os.w};
-      const float uv[4] = {ou.x, ou.y, ou.z, ou.w};
-      const float dv[4] = {od.x, od.y, od.z, od.w};
-      float out[4];
-      float c2l = 0.0f, c2s = 0.0f, c3l, c3s, cn1k = 0.0f;
-      const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-      if (UNIFORM) {
-        c2l = g.cn2l[0];
-        c2s = g.cn2s[0];
-        c3l = g.cn3l[0];
-        c3s = g.cn3s[0];
-        cn1k = sp.cn1k[k];
-      } else {
-        c3l = g.cn3l[j - 1];
-        c3s = g.cn3s[j - 1];
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = i0 + 2 * q;
-        float pc = pv[q];
-        if (i <= d.ip) {
-          const float pe = (i == d.ip) ? pc : xe[q];
-          const float pw = (i == 1) ? pc : xw[q];
-          float cn1, a2l = c2l, a2s = c2s;
-          if (UNIFORM) {
-            cn1 = cn1k;
-          } else {
-            cn1 = 1.0f / (sp.ax[i] + sp.by[j] + sp.cz[k]);
-            a2l = g.cn2l[i - 1];
-            a2s = g.cn2s[i - 1];
-          }
-          const float reltmp =
-              ph.omega * (cn1 * (a2l * pe + a2s * pw + c3l * nv[q] + c3s * sv[q] + c4l * uv[q] +
-                                 c4s * dv[q] - rv[q]) -
-                          pc);
-          pc = pc + reltmp;
-          if (want_resid) acc.add(reltmp * reltmp, lds);
-        }
-        out[q] = pc;
-      }
-      *reinterpret_cast<float4*>(sp.p[c] + base) = make_float4(out[0], out[1], out[2], out[3]);
+  const int tc = min(t, nthreads - 1);  // tail lanes index a valid group
+  // tc = grp + ng * row, row = (j-1) + jp * (k-1); quotients by multiply-high (exact for the
+  // sizes sor_split_divisors admits)
+  const int row = (int)__umulhi((unsigned)tc, sp.m_ng);
+  const int grp = tc - row * ng;
+  const int kk = (int)__umulhi((unsigned)row, sp.m_jp);
+  const int j = 1 + row - kk * d.jp;
+  const int k = 1 + kk;
+  const int c = (1 + nrd) & 1;  // nrd = 0 updates i = 1 + mod(k+j,2): odd i+j+k
+  const int i0 = first_i(c, j, k) + 8 * grp;  // i of the group's first cell
+  const int base = sp.nr * ((j - 1) + sp.nj * (k - 1)) + 4 * grp;
+  const float* po = sp.p[c ^ 1];
+  if (t < nthreads && i0 <= d.ip) {
+    const float4 oc = *reinterpret_cast<const float4*>(po + base);
+    // neighbour rows: periodic in j, the cell itself across k faces
+    const int rn = (j == d.jp) ? -sp.nr * (d.jp - 1) : sp.nr;  // j+1
+    const int rsj = (j == 1) ? sp.nr * (d.jp - 1) : -sp.nr;    // j-1
+    const int pl = sp.nr * sp.nj;
+    const float4 own = *reinterpret_cast<const float4*>(sp.p[c] + base);
+    const float4 rhs = *reinterpret_cast<const float4*>(sp.r[c] + base);
+    const float4 on = *reinterpret_cast<const float4*>(po + base + rn);
+    const float4 os = *reinterpret_cast<const float4*>(po + base + rsj);
+    float4 ou, od;
+    if (k < d.kp) ou = *reinterpret_cast<const float4*>(po + base + pl);
+    else ou = own;
+    if (k > 1) od = *reinterpret_cast<const float4*>(po + base - pl);
+    else od = own;
+    // i neighbours: odd i -> east o[ii], west o[ii-1]; even i -> east o[ii+1], west o[ii]
+    float xe[4], xw[4];
+    const float ov[4] = {oc.x, oc.y, oc.z, oc.w};
+    if (i0 & 1) {
+      const float prev = (grp > 0) ? po[base - 1] : 0.0f;
+      xe[0] = ov[0]; xe[1] = ov[1]; xe[2] = ov[2]; xe[3] = ov[3];
+      xw[0] = prev; xw[1] = ov[0]; xw[2] = ov[1]; xw[3] = ov[2];
+    } else {
+      const float next = (i0 + 7 <= d.ip) ? po[base + 4] : 0.0f;
+      xw[0] = ov[0]; xw[1] = ov[1]; xw[2] = ov[2]; xw[3] = ov[3];
+      xe[0] = ov[1]; xe[1] = ov[2]; xe[2] = ov[3]; xe[3] = next;
     }
+    const float pv[4] = {own.x, own.y, own.z, own.w};
+    const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
+    const float nv[4] = {on.x, on.y, on.z, on.w};
+    const float sv[4] = {os.x, os.y, os.z, os.w};
+    const float uv[4] = {ou.x, ou.y, ou.z, ou.w};
+    const float dv[4] = {od.x, od.y, od.z, od.w};
+    float out[4];
+    float c2l = 0.0f, c2s = 0.0f, c3l, c3s, cn1k = 0.0f;
+    const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+    if (UNIFORM) {
+      c2l = g.cn2l[0];
+      c2s = g.cn2s[0];
+      c3l = g.cn3l[0];
+      c3s = g.cn3s[0];
+      cn1k = sp.cn1k[k];
+    } else {
+      c3l = g.cn3l[j - 1];
+      c3s = g.cn3s[j - 1];
+    }
+    auto relax = [&](int q, int i, float pe, float pw) {
+      const float pc = pv[q];
+      float cn1, a2l = c2l, a2s = c2s;
+      if (UNIFORM) {
+        cn1 = cn1k;
+      } else {
+        cn1 = 1.0f / (sp.ax[i] + sp.by[j] + sp.cz[k]);
+        a2l = g.cn2l[i - 1];
+        a2s = g.cn2s[i - 1];
+      }
+      const float reltmp =
+          ph.omega * (cn1 * (a2l * pe + a2s * pw + c3l * nv[q] + c3s * sv[q] + c4l * uv[q] +
+                             c4s * dv[q] - rv[q]) -
+                      pc);
+      if (RESID) acc.add(reltmp * reltmp, lds);
+      return pc + reltmp;
+    };
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 2 * q;
+      const float pc = pv[q];
+      out[q] = (i <= d.ip) ? relax(q, i, (i == d.ip) ? pc : xe[q], (i == 1) ? pc : xw[q]) : pc;
+    }
+    *reinterpret_cast<float4*>(sp.p[c] + base) = make_float4(out[0], out[1], out[2], out[3]);
   }
-  if (want_resid) {
+  if (RESID) {
     acc.flush(lds);
     block_limbs_store(lds, partials + (long)blockIdx.x * NLIMB);
   }
@@ -330,16 +330,34 @@ int sor_split_blocks(Dims d) {
   return nwg;
 }
 
+// floor(2^32 / n) + 1: umulhi(t, m) == t / n for every t with t * n < 2^32
+uint32_t magic_div(uint32_t n) { return (uint32_t)(0xffffffffu / n) + 1u; }
+
+bool sor_split_divisors(Dims d, SplitSOR& sp) {
+  const uint64_t ng = (uint64_t)sp.nr / 4, rows = (uint64_t)d.jp * d.kp;
+  if (ng * rows * ng >= (1ull << 32) || rows * (uint64_t)d.jp >= (1ull << 32)) return false;
+  sp.m_ng = magic_div((uint32_t)ng);
+  sp.m_jp = magic_div((uint32_t)d.jp);
+  return true;
+}
+
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc) {
   int ng, nt, nwg;
   split_threads(d, sp, ng, nt, nwg);
-  if (sp.uniform)
-    hipLaunchKernelGGL(k_sor_split<true>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, nrd,
-                       want_resid, rb.partials + (long)blk_offset * NLIMB, sc, ng, nt);
-  else
-    hipLaunchKernelGGL(k_sor_split<false>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, nrd,
-                       want_resid, rb.partials + (long)blk_offset * NLIMB, sc, ng, nt);
+  int64_t* part = rb.partials + (long)blk_offset * NLIMB;
+  const dim3 grid(nwg), blk(256);
+  if (sp.uniform) {
+    if (want_resid)
+      hipLaunchKernelGGL((k_sor_split<true, true>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+    else
+      hipLaunchKernelGGL((k_sor_split<true, false>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+  } else {
+    if (want_resid)
+      hipLaunchKernelGGL((k_sor_split<false, true>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+    else
+      hipLaunchKernelGGL((k_sor_split<false, false>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+  }
 }
 
 // ---- pav partial sums (press.f95:125-132) from the split copy ----
