@@ -139,7 +139,7 @@ def ensemble_bench(nmembers, steps, warmup):
             "ms_per_step_all_members": round(t_ens / steps * 1e3, 3)}
 
 
-def mixed_bench(steps, tol=1e-7, refine_max=200):
+def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95):
     """The mixed-precision configuration: 600x600x90, stencils and SOR sweeps in FP32, rhs mean,
     residual and refined pressure in FP64 (LES_PREC_MIXED), refinement to a relative L2
     residual of tol.  Steps run eagerly (the host reads the residual between passes)."""
@@ -147,7 +147,8 @@ def mixed_bench(steps, tol=1e-7, refine_max=200):
     from les_halo_amd import LES, inputs
     from les_halo_amd.les import LES_PREC_MIXED
     ip, jp, kp = 600, 600, 90
-    L = LES(ip, jp, kp, **PHYS, precision=LES_PREC_MIXED, refine_tol=tol, refine_max=refine_max)
+    L = LES(ip, jp, kp, **PHYS, precision=LES_PREC_MIXED, refine_tol=tol, refine_max=refine_max,
+            refine_omega=refine_omega)
     L.load_case(inputs.make_case(ip, jp, kp))
     L.ifdata()
     L.run(1, 2)
@@ -164,7 +165,8 @@ def mixed_bench(steps, tol=1e-7, refine_max=200):
     secs = time.perf_counter() - t0
     return {"grid": [ip, jp, kp], "steps": steps, "ms_per_step": round(secs / steps * 1e3, 3),
             "value": round(ip * jp * kp * steps / secs / 1e6, 2), "unit": "M cell-steps/s",
-            "refine_tol": tol, "refine_max": refine_max, "refine_passes": passes,
+            "refine_tol": tol, "refine_max": refine_max, "refine_omega": refine_omega,
+            "refine_passes": passes,
             "resid_l2_last": resid[-1], "resid_l2_max": max(resid)}
 
 
@@ -177,6 +179,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", type=int, default=0, metavar="STEPS",
                     help="also run the mixed-precision 600x600x90 configuration for STEPS steps")
+    ap.add_argument("--refine-omega", type=float, default=1.95,
+                    help="relaxation of the mixed-precision correction solves")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="also run the batched-ensemble configuration with N members")
     ap.add_argument("--profile-sor", type=int, default=0, metavar="REPS",
@@ -240,7 +244,7 @@ def main():
             "step_hbm_frac": round(step_hbm_frac, 4), "step_bytes_per_cell": B_STEP,
             "roofline": roofline}
     if args.mixed:
-        line["mixed"] = mixed_bench(args.mixed)
+        line["mixed"] = mixed_bench(args.mixed, refine_omega=args.refine_omega)
     if args.ensemble:
         del L
         line["ensemble"] = ensemble_bench(args.ensemble, args.steps, args.warmup)

@@ -110,6 +110,7 @@ typedef struct les_config {
   int refine_max;     /* LES_PREC_MIXED: maximum outer refinement passes */
   int device;         /* HIP device ordinal */
   int sub_i, sub_j;   /* subdomains along i and j on the one device (1,1 = single domain) */
+  float refine_omega; /* LES_PREC_MIXED: SOR relaxation of the correction solves; 0 = omega */
 } les_config;
 
 /* Fills the reference's defaults (set.f95, les.f95, press.f95) for an ip x jp x kp grid. */

@@ -22,7 +22,7 @@ class LesConfig(ctypes.Structure):
                 ("vn", c_float), ("alpha", c_float), ("beta", c_float), ("cs0", c_float),
                 ("omega", c_float), ("pjuge", c_float), ("nmaxp", c_int), ("precision", c_int),
                 ("refine_tol", c_double), ("refine_max", c_int), ("device", c_int),
-                ("sub_i", c_int), ("sub_j", c_int)]
+                ("sub_i", c_int), ("sub_j", c_int), ("refine_omega", c_float)]
 
 
 class LesScalars(ctypes.Structure):

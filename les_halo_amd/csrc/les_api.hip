@@ -355,7 +355,8 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     return LES_E_INVALID;
   if (cfg->sub_i != 1 || cfg->sub_j != 1) return LES_E_INVALID;  // one domain per handle
   if (cfg->precision != LES_PREC_FP32 && cfg->precision != LES_PREC_MIXED) return LES_E_INVALID;
-  if (cfg->precision == LES_PREC_MIXED && (!(cfg->refine_tol > 0.0) || cfg->refine_max < 0))
+  if (cfg->precision == LES_PREC_MIXED && (!(cfg->refine_tol > 0.0) || cfg->refine_max < 0 ||
+                                            cfg->refine_omega < 0.0f || cfg->refine_omega >= 2.0f))
     return LES_E_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LES_E_NODEV;
@@ -785,8 +786,9 @@ int enqueue_sor_inplace(les_handle* h) {
 // the same iterations on the colour-split copy (already scattered); early exit only when
 // pjuge > 0 and allowed.  Without a residual, pairs of iterations run as fused launches
 // (k_sor_iter) from sp.p to the second iterate and back, so the result ends in sp.p.
-int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true) {
+int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true, const Physics* phys = nullptr) {
   STAGE_BEGIN(h);
+  const Physics ph = phys ? *phys : h->ph;
   launch_sor_reset(h->st, h->sc);
   const bool resid = allow_exit && h->ph.pjuge > 0.0f;
   const int nb = sor_split_blocks(d);
@@ -795,14 +797,14 @@ int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true) {
     const SorPair fwd{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
     const SorPair back{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}};
     for (; l0 + 1 <= h->cfg.nmaxp; l0 += 2) {
-      launch_sor_iter(h->st, d, g, h->sp, h->ph, fwd, h->sc, h->iter_shape);
-      launch_sor_iter(h->st, d, g, h->sp, h->ph, back, h->sc, h->iter_shape);
+      launch_sor_iter(h->st, d, g, h->sp, ph, fwd, h->sc, h->iter_shape);
+      launch_sor_iter(h->st, d, g, h->sp, ph, back, h->sc, h->iter_shape);
     }
   }
   for (int l = l0; l <= h->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd)
-      launch_sor_split(h->st, d, g, h->sp, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
-    if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, h->ph, h->sc);
+      launch_sor_split(h->st, d, g, h->sp, ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
+    if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, ph, h->sc);
   }
   return check_launch(h);
 }
@@ -837,6 +839,8 @@ int enqueue_press_mixed(les_handle* h) {
   launch_mx_scatter(h->st, d, F, h->sp, h->mx, h->sc);
   if ((r = enqueue_sor_split_loop(h, false))) return r;
   launch_mx_promote(h->st, d, h->sp, h->mx);
+  Physics ph_corr = h->ph;  // the correction solves may relax with their own omega
+  if (h->cfg.refine_omega > 0.0f) ph_corr.omega = h->cfg.refine_omega;
   int passes = 0;
   for (;;) {
     launch_mx_resid(h->st, d, g, h->sp, h->mx, passes == 0 ? 1 : 0, h->cfg.refine_tol, h->sc);
@@ -844,7 +848,7 @@ int enqueue_press_mixed(les_handle* h) {
     HIPCHK(h, hipMemcpyAsync(&done, &h->sc->refine_done, sizeof(int), hipMemcpyDeviceToHost, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
     if (done || passes >= h->cfg.refine_max) break;
-    if ((r = enqueue_sor_split_loop(h, false))) return r;
+    if ((r = enqueue_sor_split_loop(h, false, &ph_corr))) return r;
     launch_mx_accumulate(h->st, d, h->sp, h->mx);
     ++passes;
   }

@@ -73,6 +73,7 @@ typedef struct {
   int sum_mode;  // 0 = sequential, 1 = exact
   double refine_tol;  // > 0: after the SOR, more blocks of nmaxp iterations until the relative
   int refine_max;     //      residual |rhs - L p|_2/|rhs|_2 <= refine_tol (at most refine_max)
+  double refine_omega;  // > 0: relaxation of those further blocks (else omega)
 } orc_params;
 
 typedef struct {
@@ -885,7 +886,8 @@ void press(F<T>& a, const orc_params& q, orc_scalars* out) {
     res = rel_residual(a);
     while (res > q.refine_tol && passes < q.refine_max) {
       T s2;
-      press_sor(a, (T)q.omega, T(0), q.nmaxp, q.sum_mode, &s2);
+      press_sor(a, (T)(q.refine_omega > 0.0 ? q.refine_omega : q.omega), T(0), q.nmaxp,
+                q.sum_mode, &s2);
       ++passes;
       res = rel_residual(a);
     }

@@ -34,7 +34,8 @@ class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ["dt", "ro", "vn", "alpha", "beta", "cs0", "omega", "pjuge"]] + \
                [("nmaxp", ctypes.c_int), ("sum_mode", ctypes.c_int),
-                ("refine_tol", ctypes.c_double), ("refine_max", ctypes.c_int)]
+                ("refine_tol", ctypes.c_double), ("refine_max", ctypes.c_int),
+                ("refine_omega", ctypes.c_double)]
 
 
 class OrcScalars(ctypes.Structure):
@@ -131,7 +132,7 @@ class OracleState:
             self.a[n] = np.zeros(_scratch_size(n, ip, jp, kp), self.dtype)
         self.params = dict(dt=0.2, ro=1.1763, vn=1.583e-5, alpha=-10.0, beta=-1.0, cs0=0.1,
                            omega=1.0, pjuge=1e-4, nmaxp=50, sum_mode=SUM_SEQ,
-                           refine_tol=0.0, refine_max=0)
+                           refine_tol=0.0, refine_max=0, refine_omega=0.0)
         self.params.update(physics)
         self.scalars = OrcScalars()
 
@@ -177,6 +178,9 @@ class OracleState:
         q.sum_mode = int(p["sum_mode"])
         q.refine_tol = float(p["refine_tol"])
         q.refine_max = int(p["refine_max"])
+        # the correction blocks' omega as the FP32 path sees it
+        ro = float(p["refine_omega"])
+        q.refine_omega = float(np.float32(ro)) if self.dtype == np.float32 else ro
         return q
 
     def call(self, stage, n=0, **over):

@@ -33,13 +33,15 @@ def case():
     return common.tiny_case(40, 32, 20, seed=11)
 
 
-def test_mixed_reaches_tolerance_and_tracks_fp64(case):
+@pytest.mark.parametrize("romega", [0.0, 1.7])
+def test_mixed_reaches_tolerance_and_tracks_fp64(case, romega):
+    """romega: relaxation of the correction solves (0 = the step's omega)."""
     from les_halo_amd.les import LES_PREC_MIXED
     nsteps = 4
-    L = _gpu(case, precision=LES_PREC_MIXED, refine_tol=TOL, refine_max=RMAX)
+    L = _gpu(case, precision=LES_PREC_MIXED, refine_tol=TOL, refine_max=RMAX, refine_omega=romega)
     P = _gpu(case)
     o = common.oracle_from_case(case, np.float64, **common.REF_PHYS, **SOR, sum_mode=SUM_SEQ,
-                                refine_tol=TOL, refine_max=RMAX)
+                                refine_tol=TOL, refine_max=RMAX, refine_omega=romega)
     o.ifdata()
     for n in range(1, nsteps + 1):
         L.step(n)
