@@ -303,77 +303,149 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
   F.h[ifg] = hv;
 
   if (!do_velfg) return;
-  // ---- diu stores with the halo copies of vel2.f95:125-188 ----
-  F.diu1[d.D15(i, j, k)] = d1;
-  if (i == im) F.diu1[d.D15(im + 1, j, k)] = d1;
-  F.diu5[d.D15(i, j, k)] = d5;
-  if (j == jm) F.diu5[d.D15(i, 0, k)] = d5;
-  if (j == 1) F.diu5[d.D15(i, jm + 1, k)] = d5;
-  F.diu9[d.D(i, j, k)] = d9;
-  // diu2: periodic j rows (k <= km) and the i = im+1 column for j = 1..jm+1, k = 1..km+1
-  F.diu2[d.D(i, j, k)] = d2;
-  if (j == jm) F.diu2[d.D(i, 0, k)] = d2;
-  if (j == 1) F.diu2[d.D(i, jm + 1, k)] = d2;
-  if (i == im) {
-    F.diu2[d.D(im + 1, j, k)] = d2;
-    if (j == 1) F.diu2[d.D(im + 1, jm + 1, k)] = d2;
-    if (k == km) {  // diu2(im, :, km+1) is never computed: plain copy of memory
-      F.diu2[d.D(im + 1, j, km + 1)] = F.diu2[d.D(im, j, km + 1)];
-      if (j == jm) F.diu2[d.D(im + 1, jm + 1, km + 1)] = F.diu2[d.D(im, jm + 1, km + 1)];
+  const float d3top = (k == km) ? c.diu3(i, j, km + 1) : 0.0f;
+  const float d6top = (k == km) ? c.diu6(i, j, km + 1) : 0.0f;
+  store_diu(d, F, i, j, k, d1, d2, d3, d4, d5, d6, d7, d8, d9, d3top, d6top);
+}
+
+// velfg + feedbf with every i-face and j-face quantity computed once per point.  A 64 x 8
+// workgroup covers 63 x 7 cells: each lane computes the i-face quantities (diu1/cov1, diu4/cov4,
+// diu7/cov7) at its own i and hands them to the lane before it (its i+1), each row the j-face
+// quantities (diu2/cov2, diu5/cov5, diu8/cov8) at its own j through LDS to the row before it;
+// lane 63 and row 7 only produce.  The k-face quantities at k and k+1 are computed in the
+// thread.  Each value comes from the same formula at the same point as in k_velfg_feedbf, so
+// the results are bit-identical; the i-face and j-face work is halved.
+constexpr int VX = 64, VY = 8;
+template <bool P2>
+__global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Physics ph,
+                                                  int do_feedbf) {
+  __shared__ float sy[6][VY][VX];  // cov2 diu2 cov5 diu5 cov8 diu8 of each row
+  const int x = threadIdx.x, y = threadIdx.y;
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const int i = 1 + blockIdx.x * (VX - 1) + x;
+  const int j = 1 + blockIdx.y * (VY - 1) + y;
+  const int k = 1 + blockIdx.z;
+  const bool hk = k <= km - 1;
+  VelCtx<P2> c{d, g, F.u, F.v, F.w};
+  // i-face quantities at (i, j, k)
+  float x1 = 0.0f, xc1 = 0.0f, x4 = 0.0f, xc4 = 0.0f, x7 = 0.0f, xc7 = 0.0f;
+  if (y < VY - 1 && i <= im && j <= jm) {
+    x1 = c.diu1(i, j, k);
+    xc1 = c.cov1(i, j, k);
+    x4 = c.diu4(i, j, k);
+    xc4 = c.cov4(i, j, k);
+    if (hk) {
+      x7 = c.diu7(i, j, k);
+      xc7 = c.cov7(i, j, k);
     }
   }
-  // diu3: k = 1..km+1; i = im+1 column for j = 1..jm+1
-  F.diu3[d.D(i, j, k)] = d3;
-  float d3top = 0.0f;
-  if (k == km) {
-    d3top = c.diu3(i, j, km + 1);
-    F.diu3[d.D(i, j, km + 1)] = d3top;
-  }
-  if (i == im) {
-    F.diu3[d.D(im + 1, j, k)] = d3;
-    if (k == km) F.diu3[d.D(im + 1, j, km + 1)] = d3top;
-    if (j == jm) {  // diu3(im, jm+1, :) is never computed
-      F.diu3[d.D(im + 1, jm + 1, k)] = F.diu3[d.D(im, jm + 1, k)];
-      if (k == km) F.diu3[d.D(im + 1, jm + 1, km + 1)] = F.diu3[d.D(im, jm + 1, km + 1)];
+  // j-face quantities at (i, jq, k); row jm+1 is the periodic copy of row 1
+  if (x < VX - 1 && i <= im && j <= jm + 1) {
+    const int jq = (j == jm + 1) ? 1 : j;
+    sy[0][y][x] = c.cov2(i, jq, k);
+    sy[1][y][x] = c.diu2(i, jq, k);
+    sy[2][y][x] = c.cov5(i, jq, k);
+    sy[3][y][x] = c.diu5(i, jq, k);
+    if (hk) {
+      sy[4][y][x] = c.cov8(i, jq, k);
+      sy[5][y][x] = c.diu8(i, jq, k);
     }
   }
-  // diu4: i = im+1 column (k <= km), then row j = 0 for i = 1..im+1, k = 1..km+1
-  F.diu4[d.D(i, j, k)] = d4;
-  if (i == im) F.diu4[d.D(im + 1, j, k)] = d4;
-  if (j == jm) {
-    F.diu4[d.D(i, 0, k)] = d4;
-    if (i == im) F.diu4[d.D(im + 1, 0, k)] = d4;
-    if (k == km) {  // diu4(:, jm, km+1) is never computed
-      F.diu4[d.D(i, 0, km + 1)] = F.diu4[d.D(i, jm, km + 1)];
-      if (i == im) F.diu4[d.D(im + 1, 0, km + 1)] = F.diu4[d.D(im + 1, jm, km + 1)];
-    }
+  float x1n = __shfl_down(x1, 1), xc1n = __shfl_down(xc1, 1);
+  float x4n = __shfl_down(x4, 1), xc4n = __shfl_down(xc4, 1);
+  float x7n = __shfl_down(x7, 1), xc7n = __shfl_down(xc7, 1);
+  __syncthreads();
+  if (x >= VX - 1 || y >= VY - 1 || i > im || j > jm) return;
+  if (i == im) {  // vel2.f95:125-165: the i-face quantities at im+1 are copies of im
+    x1n = x1;
+    xc1n = xc1;
+    x4n = x4;
+    xc4n = xc4;
+    x7n = x7;
+    xc7n = xc7;
   }
-  // diu6: k = 1..km+1; row j = 0 for i = 1..im+1
-  F.diu6[d.D(i, j, k)] = d6;
-  float d6top = 0.0f;
-  if (k == km) {
-    d6top = c.diu6(i, j, km + 1);
-    F.diu6[d.D(i, j, km + 1)] = d6top;
+  const float y2 = sy[1][y][x], y5 = sy[3][y][x];
+  const float yc2 = sy[0][y][x], yc2n = sy[0][y + 1][x], y2n = sy[1][y + 1][x];
+  const float yc5 = sy[2][y][x], yc5n = sy[2][y + 1][x], y5n = sy[3][y + 1][x];
+  // k-face quantities at k and k+1
+  const float z3 = c.diu3(i, j, k), zc3 = c.cov3(i, j, k);
+  const float z3n = c.diu3(i, j, k + 1), zc3n = c.cov3(i, j, k + 1);
+  const float z6 = c.diu6(i, j, k), zc6 = c.cov6(i, j, k);
+  const float z6n = c.diu6(i, j, k + 1), zc6n = c.cov6(i, j, k + 1);
+  const float z9 = c.diu9(i, j, k);
+  const long ifg = d.F(i, j, k);
+  const float two = 2.0f;
+  const float dx1i = c.dx1(i), dx1ip = c.dx1(i + 1), dy1j = c.dy1(j), dy1jp = c.dy1(j + 1);
+  const float dznk = c.dzn(k), dznkp = c.dzn(k + 1);
+  float fv, gv, hv;
+  {  // velFG.f95:59-72
+    float covx1 = xdiv<P2>(dx1ip * xc1 + dx1i * xc1n, dx1i + dx1ip);
+    float covy1 = (yc2 + yc2n) / two;
+    float covz1 = (zc3 + zc3n) / two;
+    float covc = covx1 + covy1 + covz1;
+    float dfu1 = xdiv<P2>(two * (-x1 + x1n), dx1i + dx1ip) + xdiv<P2>(-y2 + y2n, dy1j) +
+                 (-z3 + z3n) / dznk;
+    float df = ph.vn * dfu1;
+    fv = (-covc + df);
   }
-  if (j == jm) {
-    F.diu6[d.D(i, 0, k)] = d6;
-    if (k == km) F.diu6[d.D(i, 0, km + 1)] = d6top;
-    if (i == im) {  // diu6(im+1, jm, :) is never computed
-      F.diu6[d.D(im + 1, 0, k)] = F.diu6[d.D(im + 1, jm, k)];
-      if (k == km) F.diu6[d.D(im + 1, 0, km + 1)] = F.diu6[d.D(im + 1, jm, km + 1)];
-    }
+  {  // velFG.f95:75-88
+    float covx1 = (xc4 + xc4n) / two;
+    float covy1 = xdiv<P2>(dy1jp * yc5 + dy1j * yc5n, dy1j + dy1jp);
+    float covz1 = (zc6 + zc6n) / two;
+    float covc = covx1 + covy1 + covz1;
+    float dfv1 = xdiv<P2>(-x4 + x4n, dx1i) + xdiv<P2>(two * (-y5 + y5n), dy1j + dy1jp) +
+                 (-z6 + z6n) / dznk;
+    float df = ph.vn * dfv1;
+    gv = (-covc + df);
   }
-  if (hk) {
-    F.diu7[d.D(i, j, k)] = d7;
-    if (i == im) F.diu7[d.D(im + 1, j, k)] = d7;
-    F.diu8[d.D(i, j, k)] = d8;
-    if (j == jm) F.diu8[d.D(i, 0, k)] = d8;
-    if (j == 1) F.diu8[d.D(i, jm + 1, k)] = d8;
+  float y8 = 0.0f;
+  if (hk) {  // velFG.f95:92-105
+    const float yc8 = sy[4][y][x], yc8n = sy[4][y + 1][x];
+    y8 = sy[5][y][x];
+    const float y8n = sy[5][y + 1][x];
+    const float z9n = c.diu9(i, j, k + 1), zc9 = c.cov9(i, j, k), zc9n = c.cov9(i, j, k + 1);
+    float covx1 = (xc7 + xc7n) / two;
+    float covy1 = (yc8 + yc8n) / two;
+    float covz1 = (dznkp * zc9 + dznk * zc9n) / (dznk + dznkp);
+    float covc = covx1 + covy1 + covz1;
+    float dfw1 = xdiv<P2>(-x7 + x7n, dx1i) + xdiv<P2>(-y8 + y8n, dy1j) + (-z9 + z9n) / c.dzs(k);
+    float df = ph.vn * dfw1;
+    hv = (-covc + df);
+  } else {
+    hv = F.h[ifg];  // velfg leaves h(:,:,km) untouched
   }
+  if (do_feedbf) {  // feedbf.f95:41-65
+    const uint8_t m = F.mask[d.M(i, j, k)];
+    const float bm = mbit(m, 0), cm = mbit(m, 1), dm = mbit(m, 2);
+    const float uu = c.U(i, j, k), vv = c.V(i, j, k), ww = c.W(i, j, k);
+    const float us = (F.usum[ifg] + uu) * bm;
+    const float vs = (F.vsum[ifg] + vv) * cm;
+    const float ws = (F.wsum[ifg] + ww) * dm;
+    F.usum[ifg] = us;
+    F.vsum[ifg] = vs;
+    F.wsum[ifg] = ws;
+    const float f1x = ph.alpha * us * ph.dt, f1y = ph.alpha * vs * ph.dt, f1z = ph.alpha * ws * ph.dt;
+    const float f2x = ph.beta * uu * bm, f2y = ph.beta * vv * cm, f2z = ph.beta * ww * dm;
+    fv = fv + (f1x + f2x);
+    gv = gv + (f1y + f2y);
+    hv = hv + (f1z + f2z);
+  }
+  F.f[ifg] = fv;
+  F.g[ifg] = gv;
+  F.h[ifg] = hv;
+  store_diu(d, F, i, j, k, x1, y2, z3, x4, y5, z6, x7, y8, z9, z3n, z6n);
 }
 
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf) {
+  if (do_velfg) {
+    const dim3 grid((d.ip + VX - 2) / (VX - 1), (d.jp + VY - 2) / (VY - 1), d.kp);
+    if (g.p2xy)
+      hipLaunchKernelGGL(k_velfg_xy<true>, grid, dim3(VX, VY), 0, st, d, g, F, ph, do_feedbf);
+    else
+      hipLaunchKernelGGL(k_velfg_xy<false>, grid, dim3(VX, VY), 0, st, d, g, F, ph, do_feedbf);
+    return;
+  }
   if (g.p2xy)
     hipLaunchKernelGGL(k_velfg_feedbf<true>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g,
                        F, ph, do_velfg, do_feedbf);
