@@ -230,7 +230,8 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(kname), "kernel": kname,
                 "bytes_per_launch": bytes_launch, "avg_us": round(kern_ms * 1e3, 3),
-                "tile": [tuning["sor_tile_j"], tuning["sor_tile_k"]]}
+                "tile": [tuning["sor_tile_j"], tuning["sor_tile_k"]],
+                "tuned_iter_us": round(tuning["sor_iter_ms"] * 1e3, 3)}
     step_hbm_frac = B_STEP * cells / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "M cell-steps/s", "n_gpus": 1,

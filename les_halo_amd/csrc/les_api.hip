@@ -1230,8 +1230,9 @@ extern "C" int les_set_sor_tile(les_handle* h, int tile_j, int tile_k) {
   if (tile_j == 0) {
     h->iter_shape = SorIterShape{0, 0};
   } else {
-    if ((tile_j != 4 && tile_j != 8 && tile_j != 16) || tile_k < 1)
-      return fail(h, LES_E_INVALID, "set_sor_tile: tile_j must be 0, 4, 8 or 16 and tile_k >= 1");
+    if ((tile_j != 4 && tile_j != 8 && tile_j != 16) || tile_k < 1 || tile_k > SOR_ITER_KCMAX)
+      return fail(h, LES_E_INVALID,
+                  "set_sor_tile: tile_j must be 0, 4, 8 or 16 and tile_k in 1..128");
     if (!sor_iter_supported(h->d, h->sp))
       return fail(h, LES_E_STATE, "set_sor_tile: the fused SOR iteration does not apply to this grid");
     h->iter_shape = SorIterShape{tile_j, tile_k};

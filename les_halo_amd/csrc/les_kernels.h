@@ -53,6 +53,7 @@ struct SorIterShape {
   int tj, kc;
 };
 constexpr int SOR_ITER_NRMAX = 160;  // longest colour row (ip <= 320) the fused kernel stages in LDS
+constexpr int SOR_ITER_KCMAX = 128;  // most levels per workgroup of the fused kernel
 
 // Reduction scratch: per-block carry-save limbs.
 struct RedBuf {

@@ -171,17 +171,32 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
 //   sA: the new colour-1 values, rows j0-1 .. j0+TJ   (TJ+2 rows), recomputed on the two
 //       rows next to the tile so that colour 0 at the tile's edge rows sees final colour-1
 //       neighbours.
-// At level k it loads old colour 0 at k+2, updates colour 1 at k+1 from it, then updates
-// colour 0 at k from the colour-1 ring and writes both colours of level k.  Every cell gets the
-// same FP32 expression as in k_sor_split with the same operand values, so the result is
-// bit-identical to the two colour sweeps.  Traffic per iteration: p read once, rhs read once,
-// p written once (12 B per cell) instead of 16 B for the two sweeps.
+// At level k it stores old colour 0 of level k+2 into the ring, updates colour 1 at k+1, then
+// colour 0 at k from the colour-1 ring, and writes both colours of level k.  The global loads
+// of the next level (colour 0 at k+3, colour 1 and its rhs at k+2, the colour-0 rhs at k+1) are
+// issued into registers while the current level computes, so their latency overlaps the
+// barriers.  Wavefront wv takes rows wv, wv+4, ...; its lanes take consecutive cells, so LDS
+// and global accesses are unit-stride.  Every cell gets the same FP32 expression as in
+// k_sor_split with the same operand values: the result is bit-identical to the two colour
+// sweeps, with 12 B per cell of traffic per iteration instead of 16 B.
+// Workgroup barrier for LDS hand-offs only: waits for this wavefront's LDS operations, not
+// for its outstanding global loads and stores (which __syncthreads() would drain), so the
+// prefetched loads of the next level stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 template <int TJ>
 __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
                                                   SorPair io, const DevScalars* sc, int kc,
                                                   int njt) {
+  constexpr int NCH = (SOR_ITER_NRMAX + 63) / 64;  // 64-cell chunks of a row
+  constexpr int RB = (TJ + 4 + 3) / 4;             // colour-0 rows per wavefront
+  constexpr int RA = (TJ + 2 + 3) / 4;             // colour-1 rows per wavefront
+  constexpr int RU = (TJ + 3) / 4;                 // tile rows per wavefront
   __shared__ float sB[3][TJ + 4][SOR_ITER_NRMAX];
   __shared__ float sA[3][TJ + 2][SOR_ITER_NRMAX];
+  __shared__ float sC[3][SOR_ITER_KCMAX + 3];  // cn1, cn4l, cn4s of levels k0-1 .. k1+1
   if (sc->sor_done) return;
   const int L = xcd_tile(blockIdx.x, gridDim.x);
   const int jt = L % njt, kt = L / njt;
@@ -189,40 +204,91 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
   const int k0 = 1 + kt * kc;
   if (k0 > d.kp) return;  // padding workgroups of the XCD remap (uniform per workgroup)
   const int k1 = min(d.kp, k0 + kc - 1);
-  const int tid = threadIdx.x;
-  const int ng = sp.nr >> 2;  // float4 groups per row (<= 64 since nr <= SOR_ITER_NRMAX)
-  const int jp = d.jp, kp = d.kp, ip = d.ip;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nr = sp.nr, jp = d.jp, kp = d.kp, ip = d.ip;
   const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
   const float om = ph.omega;
   auto wrapj = [&](int j) { return ((j - 1) % jp + jp) % jp + 1; };  // periodic j
-  auto rowbase = [&](int j, int k) { return sp.nr * ((j - 1) + jp * (k - 1)); };
+  auto rowbase = [&](int j, int k) { return nr * ((j - 1) + jp * (k - 1)); };
+  // The level coefficients come from LDS: a global load of them between the prefetches would
+  // make the wait for it (the vector memory counter drains in order) a wait for the prefetches.
+  for (int t = threadIdx.x; t < k1 - k0 + 3; t += 256) {
+    const int k = k0 - 1 + t;
+    if (k >= 1 && k <= kp) {
+      sC[0][t] = sp.cn1k[k];
+      sC[1][t] = g.cn4l[k - 1];
+      sC[2][t] = g.cn4s[k - 1];
+    }
+  }
 
-  // Work split inside the workgroup: wavefront wv takes rows wv, wv+4, ...; its lanes take
-  // consecutive cells of the row, so LDS reads and global accesses are unit-stride.
-  const int wv = tid >> 6, lane = tid & 63;
-  // old colour 0 of level k (1 <= k <= kp) -> sB ring
-  auto load_b = [&](int k) {
-    float(*dst)[SOR_ITER_NRMAX] = sB[k % 3];
-    for (int r = wv; r < TJ + 4; r += 4) {
+  float bq[RB][NCH];                // old colour 0 of one level, rows j0-2+wv+4q
+  float ao[RA][NCH], ar[RA][NCH];   // old colour 1 and its rhs, rows j0-1+wv+4q
+  float br[RU][NCH];                // colour-0 rhs, tile rows j0+wv+4q
+  // Every lane issues every load of a fetch (rows and cells clamped into the tile, the surplus
+  // values are never stored), so the number of loads in flight is the same on every path and
+  // the compiler's waits for a register count exactly the loads issued after it.
+  auto fetch_b = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int r = min(wv + 4 * q, TJ + 3);
       const float* src = io.pin[0] + rowbase(wrapj(j0 - 2 + r), k);
-      if (lane < ng)
-        *reinterpret_cast<float4*>(&dst[r][4 * lane]) = *reinterpret_cast<const float4*>(src + 4 * lane);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) bq[q][ch] = src[min(lane + 64 * ch, nr - 1)];
     }
   };
-  // new colour 1 of level k (1 <= k <= kp) on rows j0-1 .. j0+TJ -> sA ring
+  auto put_b = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int r = wv + 4 * q;
+      if (r >= TJ + 4) break;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int ii = lane + 64 * ch;
+        if (ii < nr) sB[k % 3][r][ii] = bq[q][ch];
+      }
+    }
+  };
+  auto fetch_a = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < RA; ++q) {
+      const int r = min(wv + 4 * q, TJ + 1);
+      const int rb = rowbase(wrapj(j0 - 1 + r), k);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int ii = min(lane + 64 * ch, nr - 1);
+        ao[q][ch] = io.pin[1][rb + ii];
+        ar[q][ch] = sp.r[1][rb + ii];
+      }
+    }
+  };
+  auto fetch_br = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < RU; ++q) {
+      const int rb = rowbase(min(j0 + min(wv + 4 * q, TJ - 1), jp), k);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) br[q][ch] = sp.r[0][rb + min(lane + 64 * ch, nr - 1)];
+    }
+  };
+  // new colour 1 of level k on rows j0-1 .. j0+TJ -> sA ring (own values and rhs from ao, ar)
   auto update_a = [&](int k) {
-    const float cn1 = sp.cn1k[k], c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+    const int t = k - k0 + 1;
+    const float cn1 = sC[0][t], c4l = sC[1][t], c4s = sC[2][t];
     const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
     const float(*bu)[SOR_ITER_NRMAX] = sB[(k + 1) % 3];
     const float(*bd)[SOR_ITER_NRMAX] = sB[(k + 2) % 3];  // == level k-1
     float(*dst)[SOR_ITER_NRMAX] = sA[k % 3];
-    for (int r = wv; r < TJ + 2; r += 4) {
+#pragma unroll
+    for (int q = 0; q < RA; ++q) {
+      const int r = wv + 4 * q;
+      if (r >= TJ + 2) break;
       const int j = wrapj(j0 - 1 + r);
       const int f = ((1 + j + k) & 1) == 1 ? 1 : 2;  // first i of colour 1 in this row
-      const int rb = rowbase(j, k);
-      for (int ii = lane; ii < sp.nr; ii += 64) {
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int ii = lane + 64 * ch;
+        if (ii >= nr) break;
         const int i = f + 2 * ii;
-        float pc = io.pin[1][rb + ii];
+        float pc = ao[q][ch];
         if (i <= ip) {
           const float pe = (i == ip) ? pc : bm[r + 1][ii + f - 1];
           const float pw = (i == 1) ? pc : bm[r + 1][ii + f - 2];
@@ -231,7 +297,7 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
           const float pd = (k > 1) ? bd[r + 1][ii] : pc;
           const float reltmp =
               om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd -
-                           sp.r[1][rb + ii]) -
+                           ar[q][ch]) -
                     pc);
           pc = pc + reltmp;
         }
@@ -241,17 +307,23 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
   };
   // new colour 0 of level k on the tile rows; writes both colours of level k
   auto update_b = [&](int k) {
-    const float cn1 = sp.cn1k[k], c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+    const int t = k - k0 + 1;
+    const float cn1 = sC[0][t], c4l = sC[1][t], c4s = sC[2][t];
     const float(*am)[SOR_ITER_NRMAX] = sA[k % 3];
     const float(*au)[SOR_ITER_NRMAX] = sA[(k + 1) % 3];
     const float(*ad)[SOR_ITER_NRMAX] = sA[(k + 2) % 3];  // == level k-1
     const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
-    for (int r = wv; r < TJ; r += 4) {
+#pragma unroll
+    for (int q = 0; q < RU; ++q) {
+      const int r = wv + 4 * q;
       const int j = j0 + r;
-      if (j > jp) break;
+      if (r >= TJ || j > jp) break;
       const int f = ((1 + j + k) & 1) == 0 ? 1 : 2;  // first i of colour 0 in this row
       const int rb = rowbase(j, k);
-      for (int ii = lane; ii < sp.nr; ii += 64) {
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int ii = lane + 64 * ch;
+        if (ii >= nr) break;
         const int i = f + 2 * ii;
         float pc = bm[r + 2][ii];
         if (i <= ip) {
@@ -262,7 +334,7 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
           const float pd = (k > 1) ? ad[r + 1][ii] : pc;
           const float reltmp =
               om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd -
-                           sp.r[0][rb + ii]) -
+                           br[q][ch]) -
                     pc);
           pc = pc + reltmp;
         }
@@ -272,23 +344,37 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
     }
   };
 
-  // prologue: colour 1 at k0-1 and k0 (colour 0 at k0 needs both)
-  if (k0 - 2 >= 1) load_b(k0 - 2);
-  if (k0 - 1 >= 1) load_b(k0 - 1);
-  load_b(k0);
-  __syncthreads();
+  // prologue: colour 0 at k0-2 .. k0+1 in the ring, colour 1 at k0-1 and k0.  Fetches of a
+  // level outside 1..kp (or past the tile) load a clamped level and are never used.
+  fetch_b(max(k0 - 2, 1));
+  if (k0 - 2 >= 1) put_b(k0 - 2);
+  fetch_b(max(k0 - 1, 1));
+  if (k0 - 1 >= 1) put_b(k0 - 1);
+  fetch_b(k0);
+  put_b(k0);
+  fetch_b(min(k0 + 1, kp));
+  fetch_a(max(k0 - 1, 1));
+  lds_barrier();
   if (k0 - 1 >= 1) update_a(k0 - 1);
-  __syncthreads();
-  if (k0 + 1 <= kp) load_b(k0 + 1);
-  __syncthreads();
+  lds_barrier();  // level k0-2's slot is read by update_a(k0-1) and refilled next
+  if (k0 + 1 <= kp) put_b(k0 + 1);
+  fetch_a(k0);
+  lds_barrier();
   update_a(k0);
+  // registers for the first step
+  fetch_b(min(k0 + 2, kp));
+  fetch_a(min(k0 + 1, kp));
+  fetch_br(k0);
   for (int k = k0; k <= k1; ++k) {
-    __syncthreads();  // sA(k) complete; sB slot of level k-1 free
-    if (k + 2 <= kp) load_b(k + 2);
-    __syncthreads();
+    lds_barrier();  // sA(k) complete; the ring slots of level k-1 are free
+    if (k + 2 <= kp) put_b(k + 2);
+    fetch_b(min(k + 3, kp));  // for the next step, in flight
+    lds_barrier();
     if (k + 1 <= kp) update_a(k + 1);
-    __syncthreads();
+    fetch_a(min(k + 2, kp));
+    lds_barrier();
     update_b(k);
+    fetch_br(min(k + 1, k1));
   }
 }
 
@@ -296,7 +382,7 @@ int sor_iter_supported(Dims d, const SplitSOR& sp) { return sp.uniform && sp.nr 
 
 void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
                      DevScalars* sc, SorIterShape shape) {
-  const int kc = shape.kc > 0 ? shape.kc : 10;
+  const int kc = shape.kc > 0 ? min(shape.kc, SOR_ITER_KCMAX) : 10;
   const int nkt = (d.kp + kc - 1) / kc;
   if (shape.tj == 4) {
     const int njt = (d.jp + 3) / 4;
