@@ -175,10 +175,12 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
 // colour 0 at k from the colour-1 ring, and writes both colours of level k.  The global loads
 // of the next level (colour 0 at k+3, colour 1 and its rhs at k+2, the colour-0 rhs at k+1) are
 // issued into registers while the current level computes, so their latency overlaps the
-// barriers.  Wavefront wv takes rows wv, wv+4, ...; its lanes take consecutive cells, so LDS
-// and global accesses are unit-stride.  Every cell gets the same FP32 expression as in
-// k_sor_split with the same operand values: the result is bit-identical to the two colour
-// sweeps, with 12 B per cell of traffic per iteration instead of 16 B.
+// barriers.  A thread handles groups of four consecutive cells of a colour row (float4 loads,
+// stores and LDS accesses); the groups of all rows of a ring level are numbered together and
+// dealt to the workgroup's threads, so few lanes idle when a row is shorter than a wavefront.
+// Every cell gets the same FP32 expression as in k_sor_split with the same operand values: the
+// result is bit-identical to the two colour sweeps, with 12 B per cell of traffic per
+// iteration instead of 16 B.
 // Workgroup barrier for LDS hand-offs only: waits for this wavefront's LDS operations, not
 // for its outstanding global loads and stores (which __syncthreads() would drain), so the
 // prefetched loads of the next level stay in flight across it.
@@ -186,16 +188,17 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int TJ>
-__global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
+template <int TJ, int NT>
+__global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
                                                   SorPair io, const DevScalars* sc, int kc,
                                                   int njt) {
-  constexpr int NCH = (SOR_ITER_NRMAX + 63) / 64;  // 64-cell chunks of a row
-  constexpr int RB = (TJ + 4 + 3) / 4;             // colour-0 rows per wavefront
-  constexpr int RA = (TJ + 2 + 3) / 4;             // colour-1 rows per wavefront
-  constexpr int RU = (TJ + 3) / 4;                 // tile rows per wavefront
-  __shared__ float sB[3][TJ + 4][SOR_ITER_NRMAX];
-  __shared__ float sA[3][TJ + 2][SOR_ITER_NRMAX];
+  constexpr int NW = SOR_ITER_NRMAX;         // LDS row stride (floats)
+  constexpr int NGMAX = SOR_ITER_NRMAX / 4;  // groups of four cells in the longest row
+  constexpr int PB = ((TJ + 4) * NGMAX + NT - 1) / NT;  // colour-0 ring groups per thread
+  constexpr int PA = ((TJ + 2) * NGMAX + NT - 1) / NT;  // colour-1 ring groups per thread
+  constexpr int PU = (TJ * NGMAX + NT - 1) / NT;        // tile groups per thread
+  __shared__ float4 sB4[3][TJ + 4][NW / 4];
+  __shared__ float4 sA4[3][TJ + 2][NW / 4];
   __shared__ float sC[3][SOR_ITER_KCMAX + 3];  // cn1, cn4l, cn4s of levels k0-1 .. k1+1
   if (sc->sor_done) return;
   const int L = xcd_tile(blockIdx.x, gridDim.x);
@@ -204,15 +207,15 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
   const int k0 = 1 + kt * kc;
   if (k0 > d.kp) return;  // padding workgroups of the XCD remap (uniform per workgroup)
   const int k1 = min(d.kp, k0 + kc - 1);
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nr = sp.nr, jp = d.jp, kp = d.kp, ip = d.ip;
+  const int nr = sp.nr, ng = nr >> 2, jp = d.jp, kp = d.kp, ip = d.ip;
   const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
   const float om = ph.omega;
+  const int lstride = nr * jp;
   auto wrapj = [&](int j) { return ((j - 1) % jp + jp) % jp + 1; };  // periodic j
-  auto rowbase = [&](int j, int k) { return nr * ((j - 1) + jp * (k - 1)); };
+  auto lev = [&](int k) { return lstride * (k - 1); };
   // The level coefficients come from LDS: a global load of them between the prefetches would
   // make the wait for it (the vector memory counter drains in order) a wait for the prefetches.
-  for (int t = threadIdx.x; t < k1 - k0 + 3; t += 256) {
+  for (int t = threadIdx.x; t < k1 - k0 + 3; t += NT) {
     const int k = k0 - 1 + t;
     if (k >= 1 && k <= kp) {
       sC[0][t] = sp.cn1k[k];
@@ -221,160 +224,193 @@ __global__ __launch_bounds__(256) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp,
     }
   }
 
-  float bq[RB][NCH];                // old colour 0 of one level, rows j0-2+wv+4q
-  float ao[RA][NCH], ar[RA][NCH];   // old colour 1 and its rhs, rows j0-1+wv+4q
-  float br[RU][NCH];                // colour-0 rhs, tile rows j0+wv+4q
-  // Every lane issues every load of a fetch (rows and cells clamped into the tile, the surplus
-  // values are never stored), so the number of loads in flight is the same on every path and
-  // the compiler's waits for a register count exactly the loads issued after it.
+  // Work items: item t of a ring level is group t % ng of ring row t / ng.  Items past the end
+  // are clamped onto the last one for their loads (every lane issues every load, so the number
+  // of loads in flight is the same on every path and the compiler's waits count exactly) and
+  // skip LDS writes, updates and stores.
+  int bRow[PB], bG[PB], bOff[PB];
+  bool bOk[PB];
+#pragma unroll
+  for (int q = 0; q < PB; ++q) {
+    const int t0 = threadIdx.x + NT * q;
+    bOk[q] = t0 < (TJ + 4) * ng;
+    const int t = min(t0, (TJ + 4) * ng - 1);
+    bRow[q] = t / ng;
+    bG[q] = t - bRow[q] * ng;
+    bOff[q] = nr * (wrapj(j0 - 2 + bRow[q]) - 1) + 4 * bG[q];
+  }
+  int aRow[PA], aG[PA], aJ[PA], aOff[PA];
+  bool aOk[PA];
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int t0 = threadIdx.x + NT * q;
+    aOk[q] = t0 < (TJ + 2) * ng;
+    const int t = min(t0, (TJ + 2) * ng - 1);
+    aRow[q] = t / ng;
+    aG[q] = t - aRow[q] * ng;
+    aJ[q] = wrapj(j0 - 1 + aRow[q]);
+    aOff[q] = nr * (aJ[q] - 1) + 4 * aG[q];
+  }
+  const int nUrow = min(TJ, jp - j0 + 1);  // tile rows inside the grid
+  int uRow[PU], uG[PU], uOff[PU];
+  bool uOk[PU];
+#pragma unroll
+  for (int q = 0; q < PU; ++q) {
+    const int t0 = threadIdx.x + NT * q;
+    uOk[q] = t0 < nUrow * ng;
+    const int t = min(t0, nUrow * ng - 1);
+    uRow[q] = t / ng;
+    uG[q] = t - uRow[q] * ng;
+    uOff[q] = nr * (j0 + uRow[q] - 1) + 4 * uG[q];
+  }
+
+  float4 bq[PB];          // old colour 0 of one level (ring rows)
+  float4 ao[PA], ar[PA];  // old colour 1 and its rhs (ring rows)
+  float4 br[PU];          // colour-0 rhs (tile rows)
+  auto ld4 = [](const float* p) { return *reinterpret_cast<const float4*>(p); };
   auto fetch_b = [&](int k) {
 #pragma unroll
-    for (int q = 0; q < RB; ++q) {
-      const int r = min(wv + 4 * q, TJ + 3);
-      const float* src = io.pin[0] + rowbase(wrapj(j0 - 2 + r), k);
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) bq[q][ch] = src[min(lane + 64 * ch, nr - 1)];
-    }
+    for (int q = 0; q < PB; ++q) bq[q] = ld4(io.pin[0] + bOff[q] + lev(k));
   };
-  auto put_b = [&](int k) {
+  auto put_b = [&](int k, auto sl) {  // into ring slot sl of level k
 #pragma unroll
-    for (int q = 0; q < RB; ++q) {
-      const int r = wv + 4 * q;
-      if (r >= TJ + 4) break;
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        const int ii = lane + 64 * ch;
-        if (ii < nr) sB[k % 3][r][ii] = bq[q][ch];
-      }
-    }
+    for (int q = 0; q < PB; ++q)
+      if (bOk[q]) sB4[sl][bRow[q]][bG[q]] = bq[q];
   };
   auto fetch_a = [&](int k) {
 #pragma unroll
-    for (int q = 0; q < RA; ++q) {
-      const int r = min(wv + 4 * q, TJ + 1);
-      const int rb = rowbase(wrapj(j0 - 1 + r), k);
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        const int ii = min(lane + 64 * ch, nr - 1);
-        ao[q][ch] = io.pin[1][rb + ii];
-        ar[q][ch] = sp.r[1][rb + ii];
-      }
+    for (int q = 0; q < PA; ++q) {
+      ao[q] = ld4(io.pin[1] + aOff[q] + lev(k));
+      ar[q] = ld4(sp.r[1] + aOff[q] + lev(k));
     }
   };
   auto fetch_br = [&](int k) {
 #pragma unroll
-    for (int q = 0; q < RU; ++q) {
-      const int rb = rowbase(min(j0 + min(wv + 4 * q, TJ - 1), jp), k);
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) br[q][ch] = sp.r[0][rb + min(lane + 64 * ch, nr - 1)];
-    }
+    for (int q = 0; q < PU; ++q) br[q] = ld4(sp.r[0] + uOff[q] + lev(k));
   };
-  // new colour 1 of level k on rows j0-1 .. j0+TJ -> sA ring (own values and rhs from ao, ar)
-  auto update_a = [&](int k) {
+  // Four cells of one colour row: own values pc, rhs rr; the other colour of the same level
+  // in LDS (om_: row j, on_/os_: rows j+1/j-1, ou_/od_: levels k+1/k-1).  The first own cell
+  // is i0 = f + 8 g; its east/west neighbours are other-colour entries 4g+q+f-1 and 4g+q+f-2.
+  // INT (a compile-time bool) says 1 < k < kp, where the k-boundary selections drop out.
+  auto relax4 = [&](float4 pc4, float4 rr4, const float4* om_, const float4* on_,
+                    const float4* os_, const float4* ou_, const float4* od_, int g4, int f, int k,
+                    float cn1, float c4l, float c4s, auto INT) {
+    const float4 xm = om_[g4];
+    // entry 4g-1 (f == 1) or 4g+4 (f == 2), clamped into the row: only read for cells that
+    // take the value (never for i == 1 or i == ip)
+    const float xo =
+        reinterpret_cast<const float*>(om_)[f == 1 ? max(4 * g4 - 1, 0) : min(4 * g4 + 4, NW - 1)];
+    const float4 xn = on_[g4], xs = os_[g4], xu = ou_[g4], xd = od_[g4];
+    // xmv[1 + m] is entry 4g+m; xmv[0] stands for 4g-1 (f == 1) and xmv[5] for 4g+4 (f == 2)
+    const float xmv[6] = {xo, xm.x, xm.y, xm.z, xm.w, xo};
+    const float pcv[4] = {pc4.x, pc4.y, pc4.z, pc4.w};
+    const float rrv[4] = {rr4.x, rr4.y, rr4.z, rr4.w};
+    const float xnv[4] = {xn.x, xn.y, xn.z, xn.w}, xsv[4] = {xs.x, xs.y, xs.z, xs.w};
+    const float xuv[4] = {xu.x, xu.y, xu.z, xu.w}, xdv[4] = {xd.x, xd.y, xd.z, xd.w};
+    float out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // computed for every cell and kept where i <= ip (padding cells keep their value), so
+      // the LDS reads stay whole float4s outside any branch
+      const int i = f + 2 * (4 * g4 + q);
+      const float pc = pcv[q];
+      const float pe = (i == ip) ? pc : (f == 1 ? xmv[1 + q] : xmv[2 + q]);
+      const float pw = (i == 1) ? pc : (f == 1 ? xmv[q] : xmv[1 + q]);
+      const float pu = (INT || k < kp) ? xuv[q] : pc;
+      const float pd = (INT || k > 1) ? xdv[q] : pc;
+      const float reltmp =
+          om * (cn1 * (c2l * pe + c2s * pw + c3l * xnv[q] + c3s * xsv[q] + c4l * pu + c4s * pd -
+                       rrv[q]) -
+                pc);
+      out[q] = (i <= ip) ? pc + reltmp : pc;
+    }
+    return make_float4(out[0], out[1], out[2], out[3]);
+  };
+  // Ring slot of level k: (k - k0 + 3) % 3.  The main loop passes slots as compile-time
+  // constants (std::integral_constant), so LDS addresses are fixed offsets.
+  auto slot = [&](int k) { return (k - k0 + 3) % 3; };
+  // new colour 1 of level k (slots m, u = k+1, dn = k-1) on ring rows 0 .. TJ+1 -> sA ring
+  auto update_a = [&](int k, auto m, auto u, auto dn, auto INT) {
     const int t = k - k0 + 1;
     const float cn1 = sC[0][t], c4l = sC[1][t], c4s = sC[2][t];
-    const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
-    const float(*bu)[SOR_ITER_NRMAX] = sB[(k + 1) % 3];
-    const float(*bd)[SOR_ITER_NRMAX] = sB[(k + 2) % 3];  // == level k-1
-    float(*dst)[SOR_ITER_NRMAX] = sA[k % 3];
 #pragma unroll
-    for (int q = 0; q < RA; ++q) {
-      const int r = wv + 4 * q;
-      if (r >= TJ + 2) break;
-      const int j = wrapj(j0 - 1 + r);
-      const int f = ((1 + j + k) & 1) == 1 ? 1 : 2;  // first i of colour 1 in this row
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        const int ii = lane + 64 * ch;
-        if (ii >= nr) break;
-        const int i = f + 2 * ii;
-        float pc = ao[q][ch];
-        if (i <= ip) {
-          const float pe = (i == ip) ? pc : bm[r + 1][ii + f - 1];
-          const float pw = (i == 1) ? pc : bm[r + 1][ii + f - 2];
-          const float pn = bm[r + 2][ii], ps = bm[r][ii];
-          const float pu = (k < kp) ? bu[r + 1][ii] : pc;
-          const float pd = (k > 1) ? bd[r + 1][ii] : pc;
-          const float reltmp =
-              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd -
-                           ar[q][ch]) -
-                    pc);
-          pc = pc + reltmp;
-        }
-        dst[r][ii] = pc;
-      }
+    for (int q = 0; q < PA; ++q) {
+      if (!aOk[q]) continue;
+      const int r = aRow[q];
+      const int f = ((1 + aJ[q] + k) & 1) == 1 ? 1 : 2;  // first i of colour 1 in this row
+      const float4 v = relax4(ao[q], ar[q], sB4[m][r + 1], sB4[m][r + 2], sB4[m][r],
+                              sB4[u][r + 1], sB4[dn][r + 1], aG[q], f, k, cn1, c4l, c4s, INT);
+      sA4[m][r][aG[q]] = v;
     }
   };
   // new colour 0 of level k on the tile rows; writes both colours of level k
-  auto update_b = [&](int k) {
+  auto update_b = [&](int k, auto m, auto u, auto dn, auto INT) {
     const int t = k - k0 + 1;
     const float cn1 = sC[0][t], c4l = sC[1][t], c4s = sC[2][t];
-    const float(*am)[SOR_ITER_NRMAX] = sA[k % 3];
-    const float(*au)[SOR_ITER_NRMAX] = sA[(k + 1) % 3];
-    const float(*ad)[SOR_ITER_NRMAX] = sA[(k + 2) % 3];  // == level k-1
-    const float(*bm)[SOR_ITER_NRMAX] = sB[k % 3];
 #pragma unroll
-    for (int q = 0; q < RU; ++q) {
-      const int r = wv + 4 * q;
-      const int j = j0 + r;
-      if (r >= TJ || j > jp) break;
-      const int f = ((1 + j + k) & 1) == 0 ? 1 : 2;  // first i of colour 0 in this row
-      const int rb = rowbase(j, k);
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        const int ii = lane + 64 * ch;
-        if (ii >= nr) break;
-        const int i = f + 2 * ii;
-        float pc = bm[r + 2][ii];
-        if (i <= ip) {
-          const float pe = (i == ip) ? pc : am[r + 1][ii + f - 1];
-          const float pw = (i == 1) ? pc : am[r + 1][ii + f - 2];
-          const float pn = am[r + 2][ii], ps = am[r][ii];
-          const float pu = (k < kp) ? au[r + 1][ii] : pc;
-          const float pd = (k > 1) ? ad[r + 1][ii] : pc;
-          const float reltmp =
-              om * (cn1 * (c2l * pe + c2s * pw + c3l * pn + c3s * ps + c4l * pu + c4s * pd -
-                           br[q][ch]) -
-                    pc);
-          pc = pc + reltmp;
-        }
-        io.pout[0][rb + ii] = pc;
-        io.pout[1][rb + ii] = am[r + 1][ii];
-      }
+    for (int q = 0; q < PU; ++q) {
+      if (!uOk[q]) continue;
+      const int r = uRow[q];
+      const int f = ((1 + j0 + r + k) & 1) == 0 ? 1 : 2;  // first i of colour 0 in this row
+      const float4 pc4 = sB4[m][r + 2][uG[q]];
+      const float4 v = relax4(pc4, br[q], sA4[m][r + 1], sA4[m][r + 2], sA4[m][r],
+                              sA4[u][r + 1], sA4[dn][r + 1], uG[q], f, k, cn1, c4l, c4s, INT);
+      const int o = uOff[q] + lev(k);
+      *reinterpret_cast<float4*>(io.pout[0] + o) = v;
+      *reinterpret_cast<float4*>(io.pout[1] + o) = sA4[m][r + 1][uG[q]];
     }
   };
+  const std::false_type EDGE{};
+  const std::true_type INNER{};
 
   // prologue: colour 0 at k0-2 .. k0+1 in the ring, colour 1 at k0-1 and k0.  Fetches of a
   // level outside 1..kp (or past the tile) load a clamped level and are never used.
   fetch_b(max(k0 - 2, 1));
-  if (k0 - 2 >= 1) put_b(k0 - 2);
+  if (k0 - 2 >= 1) put_b(k0 - 2, slot(k0 - 2));
   fetch_b(max(k0 - 1, 1));
-  if (k0 - 1 >= 1) put_b(k0 - 1);
+  if (k0 - 1 >= 1) put_b(k0 - 1, slot(k0 - 1));
   fetch_b(k0);
-  put_b(k0);
+  put_b(k0, slot(k0));
   fetch_b(min(k0 + 1, kp));
   fetch_a(max(k0 - 1, 1));
   lds_barrier();
-  if (k0 - 1 >= 1) update_a(k0 - 1);
+  if (k0 - 1 >= 1) update_a(k0 - 1, slot(k0 - 1), slot(k0), slot(k0 - 2), EDGE);
   lds_barrier();  // level k0-2's slot is read by update_a(k0-1) and refilled next
-  if (k0 + 1 <= kp) put_b(k0 + 1);
+  if (k0 + 1 <= kp) put_b(k0 + 1, slot(k0 + 1));
   fetch_a(k0);
   lds_barrier();
-  update_a(k0);
+  update_a(k0, slot(k0), slot(k0 + 1), slot(k0 - 1), EDGE);
   // registers for the first step
   fetch_b(min(k0 + 2, kp));
   fetch_a(min(k0 + 1, kp));
   fetch_br(k0);
-  for (int k = k0; k <= k1; ++k) {
+  // One level k with slots s0 = slot(k), s1 = slot(k+1), s2 = slot(k+2) = slot(k-1).  Every
+  // step ends with the same three fetches, so the loads in flight at the top of a step are
+  // the same whichever step ran before it.
+  auto step = [&](int k, auto s0, auto s1, auto s2) {
     lds_barrier();  // sA(k) complete; the ring slots of level k-1 are free
-    if (k + 2 <= kp) put_b(k + 2);
+    if (k + 2 <= kp) put_b(k + 2, s2);
     fetch_b(min(k + 3, kp));  // for the next step, in flight
     lds_barrier();
-    if (k + 1 <= kp) update_a(k + 1);
+    if (k + 1 < kp)
+      update_a(k + 1, s1, s2, s0, INNER);  // 1 < k+1 always
+    else if (k + 1 == kp)
+      update_a(k + 1, s1, s2, s0, EDGE);
     fetch_a(min(k + 2, kp));
     lds_barrier();
-    update_b(k);
+    if (k > 1 && k < kp)
+      update_b(k, s0, s1, s2, INNER);
+    else
+      update_b(k, s0, s1, s2, EDGE);
     fetch_br(min(k + 1, k1));
+  };
+  const std::integral_constant<int, 0> C0{};
+  const std::integral_constant<int, 1> C1{};
+  const std::integral_constant<int, 2> C2{};
+  for (int k = k0; k <= k1; k += 3) {
+    step(k, C0, C1, C2);
+    if (k + 1 <= k1) step(k + 1, C1, C2, C0);
+    if (k + 2 <= k1) step(k + 2, C2, C0, C1);
   }
 }
 
@@ -387,15 +423,15 @@ void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, 
   if (shape.tj == 4) {
     const int njt = (d.jp + 3) / 4;
     const int nwg = (njt * nkt + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_sor_iter<4>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
+    hipLaunchKernelGGL((k_sor_iter<4, 256>), dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
   } else if (shape.tj == 16) {
     const int njt = (d.jp + 15) / 16;
     const int nwg = (njt * nkt + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_sor_iter<16>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
+    hipLaunchKernelGGL((k_sor_iter<16, 512>), dim3(nwg), dim3(512), 0, st, d, g, sp, ph, io, sc, kc, njt);
   } else {
     const int njt = (d.jp + 7) / 8;
     const int nwg = (njt * nkt + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_sor_iter<8>, dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
+    hipLaunchKernelGGL((k_sor_iter<8, 512>), dim3(nwg), dim3(512), 0, st, d, g, sp, ph, io, sc, kc, njt);
   }
 }
 
