@@ -786,10 +786,12 @@ int enqueue_sor_inplace(les_handle* h) {
 // the same iterations on the colour-split copy (already scattered); early exit only when
 // pjuge > 0 and allowed.  Without a residual, pairs of iterations run as fused launches
 // (k_sor_iter) from sp.p to the second iterate and back, so the result ends in sp.p.
-int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true, const Physics* phys = nullptr) {
+// reset = false when the SOR scalars were just re-armed (k_rhsav_finalize does it).
+int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true, const Physics* phys = nullptr,
+                           bool reset = true) {
   STAGE_BEGIN(h);
   const Physics ph = phys ? *phys : h->ph;
-  launch_sor_reset(h->st, h->sc);
+  if (reset) launch_sor_reset(h->st, h->sc);
   const bool resid = allow_exit && h->ph.pjuge > 0.0f;
   const int nb = sor_split_blocks(d);
   int l0 = 1;
@@ -867,7 +869,7 @@ int enqueue_press(les_handle* h) {
   launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
   if (use_split(h)) {
     launch_split_scatter(h->st, d, F, h->sp, h->sc, 1);  // rhs -= rhsav on the way
-    if ((r = enqueue_sor_split_loop(h))) return r;
+    if ((r = enqueue_sor_split_loop(h, true, nullptr, false))) return r;
     launch_pav_split(h->st, d, g, h->sp, h->rb, h->sc);
     launch_split_gather_adj(h->st, d, F, h->sp, h->sc, 1);  // p - pav, halos included
   } else {

@@ -145,12 +145,33 @@ struct ThreadAcc {
       atomicAdd(&lds[t.limb + 1], (unsigned long long)t.hi);
     }
   }
+  // Adds the window into the block's LDS limbs.  Called by every lane of the wavefront
+  // together (the lanes exchange their windows): the lanes whose windows start at the same
+  // limb are summed with cross-lane adds and one lane adds the result, so the block's few
+  // LDS words see one atomic per wavefront and window position instead of one per lane.
   __device__ void flush(unsigned long long* lds) {
-    if (base < 0) return;
-    if (r0) atomicAdd(&lds[base + 0], (unsigned long long)r0);
-    if (r1) atomicAdd(&lds[base + 1], (unsigned long long)r1);
-    if (r2) atomicAdd(&lds[base + 2], (unsigned long long)r2);
-    if (r3) atomicAdd(&lds[base + 3], (unsigned long long)r3);
+    const int lane = (int)__lane_id();
+    unsigned long long pend = __ballot(base >= 0);
+    while (pend) {
+      const int leader = __ffsll((long long)pend) - 1;
+      const int b = __shfl(base, leader);
+      const bool mine = (base == b);
+      int64_t s0 = mine ? r0 : 0, s1 = mine ? r1 : 0, s2 = mine ? r2 : 0, s3 = mine ? r3 : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+        s3 += __shfl_xor(s3, o);
+      }
+      if (lane == leader) {
+        if (s0) atomicAdd(&lds[b + 0], (unsigned long long)s0);
+        if (s1) atomicAdd(&lds[b + 1], (unsigned long long)s1);
+        if (s2) atomicAdd(&lds[b + 2], (unsigned long long)s2);
+        if (s3) atomicAdd(&lds[b + 3], (unsigned long long)s3);
+      }
+      pend &= ~__ballot(mine);
+    }
   }
 };
 
@@ -284,21 +305,31 @@ __device__ inline void block_limbs_store(unsigned long long* lds, int64_t* out) 
   if (tid < NLIMB) out[tid] = (int64_t)lds[tid];
 }
 
-// Sum partials[0..nb) into limbs_out (one workgroup).
+// Sum partials[0..nb) into limbs_out (one workgroup, any size >= NLIMB).  Thread t adds limb
+// t % NLIMB of blocks t / NLIMB, + G, + 2G, ... (G groups of NLIMB threads: consecutive
+// threads read consecutive words), eight loads in flight at a time.
 __device__ inline void sum_partials(const int64_t* partials, int nb, int64_t* limbs_out) {
   __shared__ unsigned long long acc[NLIMB];
   if (threadIdx.x < NLIMB) acc[threadIdx.x] = 0ull;
   __syncthreads();
-  int64_t r[NLIMB];
-  for (int q = 0; q < NLIMB; ++q) r[q] = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    for (int q = 0; q < NLIMB; ++q) r[q] += partials[(long)b * NLIMB + q];
-  for (int q = 0; q < NLIMB; ++q)
-    if (r[q]) atomicAdd(&acc[q], (unsigned long long)r[q]);
+  const int G = (int)blockDim.x / NLIMB;
+  const int q = (int)threadIdx.x % NLIMB, b0 = (int)threadIdx.x / NLIMB;
+  if (b0 < G) {
+    int64_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int b = b0;
+    for (; b + 7 * G < nb; b += 8 * G) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) r[u] += partials[(long)(b + u * G) * NLIMB + q];
+    }
+    for (; b < nb; b += G) r[0] += partials[(long)b * NLIMB + q];
+    const int64_t s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    if (s) atomicAdd(&acc[q], (unsigned long long)s);
+  }
   __syncthreads();
   if (threadIdx.x < NLIMB) limbs_out[threadIdx.x] = (int64_t)acc[threadIdx.x];
   __syncthreads();
 }
+constexpr int FIN_THREADS = 1024;  // workgroup size of the one-workgroup finalisers
 
 // Workgroups are handed to the 8 XCDs (each with its own L2) round-robin.  With a grid padded
 // to a multiple of 8, this maps dispatch id w to a logical tile so that every XCD works on one

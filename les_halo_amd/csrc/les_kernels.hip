@@ -749,11 +749,17 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
   block_limbs_store(lds, partials + (long)blk * NLIMB);
 }
 
-__global__ __launch_bounds__(256) void k_rhsav_finalize(const int64_t* partials, int nb,
-                                                        DevScalars* sc) {
+// Also re-arms the SOR scalars for the solve that follows (the work of k_sor_reset).
+__global__ __launch_bounds__(FIN_THREADS) void k_rhsav_finalize(const int64_t* partials, int nb,
+                                                                DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
   sum_partials(partials, nb, limbs);
-  if (threadIdx.x == 0) sc->rhsav = limbs_to_float(limbs) / sc->area;
+  if (threadIdx.x == 0) {
+    sc->rhsav = limbs_to_float(limbs) / sc->area;
+    sc->sor_done = 0;
+    sc->sor_iters = 0;
+    sc->sor = 0.0f;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_rhs_adjust(Dims d, Fields F, const DevScalars* sc) {
@@ -773,7 +779,7 @@ void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf r
     hipLaunchKernelGGL(k_rhs<true>, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
   else
     hipLaunchKernelGGL(k_rhs<false>, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
-  hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
+  hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
 }
 
 void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc) {
@@ -832,7 +838,7 @@ void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, i
 }
 
 // residual of one SOR iteration (both colours) and the early exit of press.f95:114
-__global__ __launch_bounds__(256) void k_sor_resid_finalize(const int64_t* partials, int nb,
+__global__ __launch_bounds__(FIN_THREADS) void k_sor_resid_finalize(const int64_t* partials, int nb,
                                                             Physics ph, DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
   if (sc->sor_done) return;
@@ -847,7 +853,7 @@ __global__ __launch_bounds__(256) void k_sor_resid_finalize(const int64_t* parti
 
 void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, int nblocks, Physics ph,
                                DevScalars* sc) {
-  hipLaunchKernelGGL(k_sor_resid_finalize, dim3(1), dim3(256), 0, st, rb.partials, nblocks, ph,
+  hipLaunchKernelGGL(k_sor_resid_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nblocks, ph,
                      sc);
 }
 
@@ -940,7 +946,7 @@ __global__ __launch_bounds__(256) void k_pav(Dims d, Grid1D g, Fields F, int64_t
   block_limbs_store(lds, partials + (long)blk * NLIMB);
 }
 
-__global__ __launch_bounds__(256) void k_pav_finalize(const int64_t* partials, int nb,
+__global__ __launch_bounds__(FIN_THREADS) void k_pav_finalize(const int64_t* partials, int nb,
                                                       DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
   sum_partials(partials, nb, limbs);
@@ -951,7 +957,7 @@ void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalar
   dim3 grid = interior_grid(d, RED_KC);
   int nb = (int)(grid.x * grid.y * grid.z);
   hipLaunchKernelGGL(k_pav, grid, dim3(BX, BY), 0, st, d, g, F, rb.partials);
-  hipLaunchKernelGGL(k_pav_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
+  hipLaunchKernelGGL(k_pav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
 }
 
 // p -= pav on the interior; with_boundp also writes every mirror image of the cell
@@ -1003,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_exact_sum_part(const float* x, long n, 
   block_limbs_store(lds, partials + (long)blockIdx.x * NLIMB);
 }
 
-__global__ __launch_bounds__(256) void k_exact_sum_fin(const int64_t* partials, int nb, float* out) {
+__global__ __launch_bounds__(FIN_THREADS) void k_exact_sum_fin(const int64_t* partials, int nb, float* out) {
   __shared__ int64_t limbs[NLIMB];
   sum_partials(partials, nb, limbs);
   if (threadIdx.x == 0) *out = limbs_to_float(limbs);
@@ -1012,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_exact_sum_fin(const int64_t* partials, 
 void launch_exact_sum(hipStream_t st, const float* x, long n, RedBuf rb, float* out) {
   int nb = (int)min((long)rb.max_blocks, max(1L, (n + 255) / 256));
   hipLaunchKernelGGL(k_exact_sum_part, dim3(nb), dim3(256), 0, st, x, n, rb.partials);
-  hipLaunchKernelGGL(k_exact_sum_fin, dim3(1), dim3(256), 0, st, rb.partials, nb, out);
+  hipLaunchKernelGGL(k_exact_sum_fin, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, out);
 }
 
 // ==========================================================================================

@@ -483,38 +483,60 @@ void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph,
 }
 
 // ---- pav partial sums (press.f95:125-132) from the split copy ----
+// Exact partial sums of dx1*dy1*dzn*p over the colour-split copy (press.f95:123-134).  A
+// thread takes one group of four split entries of row j in both colours over a chunk of k
+// levels (float4 loads along the split rows); padding entries (i > ip) are skipped.
 __global__ __launch_bounds__(256) void k_pav_split(Dims d, Grid1D g, SplitSOR sp,
-                                                   int64_t* partials) {
+                                                   int64_t* partials, int ng, int kchunk) {
   __shared__ unsigned long long lds[NLIMB];
   block_limbs_init(lds);
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int kchunk = (d.kp + RED_KC - 1) / RED_KC;
-  const int k0 = 1 + blockIdx.z * kchunk, k1 = min(d.kp, k0 + kchunk - 1);
+  const int t = blockIdx.x * 256 + threadIdx.x;
   ThreadAcc acc;
   acc.init();
-  if (i <= d.ip && j <= d.jp) {
-    const float dx1i = g.DX1(i), dy1j = g.DY1(j);
-    for (int k = k0; k <= k1; ++k)
-      acc.add(sp.p[(i + j + k) & 1][split_index(sp, i, j, k)] * dx1i * dy1j * g.DZN(k), lds);
+  if (t < ng * d.jp) {
+    const int row = t / ng;
+    const int grp = t - row * ng;
+    const int j = 1 + row;
+    const float dy1j = g.DY1(j);
+    const int k0 = 1 + blockIdx.y * kchunk, k1 = min(d.kp, k0 + kchunk - 1);
+    for (int k = k0; k <= k1; ++k) {
+      const float dznk = g.DZN(k);
+      const int base = sp.nr * (row + sp.nj * (k - 1)) + 4 * grp;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float4 v = *reinterpret_cast<const float4*>(sp.p[c] + base);
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+        const int i0 = first_i(c, j, k) + 8 * grp;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = i0 + 2 * q;
+          if (i <= d.ip) acc.add(vv[q] * g.DX1(i) * dy1j * dznk, lds);
+        }
+      }
+    }
   }
   acc.flush(lds);
-  const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  block_limbs_store(lds, partials + (long)blk * NLIMB);
+  block_limbs_store(lds, partials + (long)(blockIdx.x + gridDim.x * blockIdx.y) * NLIMB);
 }
 
-__global__ __launch_bounds__(256) void k_pav_split_finalize(const int64_t* partials, int nb,
-                                                            DevScalars* sc) {
+__global__ __launch_bounds__(FIN_THREADS) void k_pav_split_finalize(const int64_t* partials,
+                                                                    int nb, DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
   sum_partials(partials, nb, limbs);
   if (threadIdx.x == 0) sc->pav = limbs_to_float(limbs) / sc->pco;
 }
 
 void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc) {
-  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, RED_KC);
-  int nb = (int)(grid.x * grid.y * grid.z);
-  hipLaunchKernelGGL(k_pav_split, grid, dim3(64, 4), 0, st, d, g, sp, rb.partials);
-  hipLaunchKernelGGL(k_pav_split_finalize, dim3(1), dim3(256), 0, st, rb.partials, nb, sc);
+  const int ng = sp.nr / 4;
+  const int nbx = (ng * d.jp + 255) / 256;
+  // about 1300 workgroups: enough waves to stream the copy, few partials to add up
+  int nkc = std::min(std::max(1300 / nbx, 1), d.kp);
+  const int kchunk = (d.kp + nkc - 1) / nkc;
+  nkc = (d.kp + kchunk - 1) / kchunk;
+  hipLaunchKernelGGL(k_pav_split, dim3(nbx, nkc), dim3(256), 0, st, d, g, sp, rb.partials, ng,
+                     kchunk);
+  hipLaunchKernelGGL(k_pav_split_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials,
+                     nbx * nkc, sc);
 }
 
 // ---- split copy -> p over the box [0,ip+1]x[0,jp+1]x[0,kp+1], optionally minus pav
