@@ -339,4 +339,36 @@ __device__ __forceinline__ int xcd_tile(int w, int nwg_padded) {
   return (w & 7) * per + (w >> 3);
 }
 
+// A 3-D grid of nbx x nby x nbz blocks (i, j, k) launched as a 1-D grid padded to a multiple
+// of 8, visited in a cache-friendly logical order: the j blocks are cut into strips of sj,
+// the order runs strip by strip, within a strip level by level in k, within a level j then i,
+// and every XCD takes one contiguous share of the order (xcd_tile).  A stencil's i, j and k
+// neighbour cells are then read by blocks resident on the same XCD (same L2) at nearly the
+// same time, and adjacent i blocks share the cache lines at their seams.
+struct StripGrid {
+  int nbx, nby, nbz, sj;
+};
+constexpr int STRIP_SJ = 8;
+inline StripGrid strip_grid(int nbx, int nby, int nbz) { return StripGrid{nbx, nby, nbz, STRIP_SJ}; }
+inline dim3 strip_dim(const StripGrid& s) {
+  const long n = (long)s.nbx * s.nby * s.nbz;
+  return dim3((unsigned)((n + 7) / 8 * 8));
+}
+// Logical block (bx, by, bz) of this workgroup and its logical index L; false for padding.
+__device__ __forceinline__ bool strip_block(const StripGrid& s, int& bx, int& by, int& bz,
+                                            int& L) {
+  L = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (L >= s.nbx * s.nby * s.nbz) return false;
+  bx = L % s.nbx;
+  const int R = L / s.nbx;  // row tile, ordered (strip, bz, j within the strip)
+  const int per = s.sj * s.nbz;
+  const int nstrip = (s.nby + s.sj - 1) / s.sj;
+  const int st = min(R / per, nstrip - 1);
+  const int w = min(s.sj, s.nby - st * s.sj);
+  const int r = R - st * per;
+  bz = r / w;
+  by = st * s.sj + (r - bz * w);
+  return true;
+}
+
 }  // namespace les

@@ -29,10 +29,13 @@ __device__ __forceinline__ float mbit(uint8_t m, int b) { return (m >> b) & 1u ?
 // ==========================================================================================
 template <bool P2>
 __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F, Physics ph,
-                                                      int do_velnw, int inflow_n, int imax) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+                                                      int do_velnw, int inflow_n, int imax,
+                                                      StripGrid sg) {
+  int bx, by, bz, L;
+  if (!strip_block(sg, bx, by, bz, L)) return;
+  const int i = bx * BX + threadIdx.x;
+  const int j = 1 + by * BY + threadIdx.y;
+  const int k = 1 + bz;
   if (i > imax || j > d.jp) return;
   const long iu = d.U(i, j, k);
   const long iw = d.W(i, j, k);
@@ -60,13 +63,13 @@ __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F
 void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
                          int inflow_n) {
   int imax = (do_velnw || inflow_n == 1) ? d.ip : 1;
-  dim3 grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+  const StripGrid sg = strip_grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
   if (g.p2xy)
-    hipLaunchKernelGGL(k_velnw_inflow<true>, grid, dim3(BX, BY), 0, st, d, g, F, ph, do_velnw,
-                       inflow_n, imax);
+    hipLaunchKernelGGL(k_velnw_inflow<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
+                       do_velnw, inflow_n, imax, sg);
   else
-    hipLaunchKernelGGL(k_velnw_inflow<false>, grid, dim3(BX, BY), 0, st, d, g, F, ph, do_velnw,
-                       inflow_n, imax);
+    hipLaunchKernelGGL(k_velnw_inflow<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
+                       do_velnw, inflow_n, imax, sg);
 }
 
 // ==========================================================================================
@@ -318,13 +321,15 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
 constexpr int VX = 64, VY = 8;
 template <bool P2>
 __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Physics ph,
-                                                  int do_feedbf) {
+                                                  int do_feedbf, StripGrid sg) {
   __shared__ float sy[6][VY][VX];  // cov2 diu2 cov5 diu5 cov8 diu8 of each row
+  int bx, by, bz, L;
+  if (!strip_block(sg, bx, by, bz, L)) return;
   const int x = threadIdx.x, y = threadIdx.y;
   const int im = d.ip, jm = d.jp, km = d.kp;
-  const int i = 1 + blockIdx.x * (VX - 1) + x;
-  const int j = 1 + blockIdx.y * (VY - 1) + y;
-  const int k = 1 + blockIdx.z;
+  const int i = 1 + bx * (VX - 1) + x;
+  const int j = 1 + by * (VY - 1) + y;
+  const int k = 1 + bz;
   const bool hk = k <= km - 1;
   VelCtx<P2> c{d, g, F.u, F.v, F.w};
   // i-face quantities at (i, j, k)
@@ -439,11 +444,13 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf) {
   if (do_velfg) {
-    const dim3 grid((d.ip + VX - 2) / (VX - 1), (d.jp + VY - 2) / (VY - 1), d.kp);
+    const StripGrid sg = strip_grid((d.ip + VX - 2) / (VX - 1), (d.jp + VY - 2) / (VY - 1), d.kp);
     if (g.p2xy)
-      hipLaunchKernelGGL(k_velfg_xy<true>, grid, dim3(VX, VY), 0, st, d, g, F, ph, do_feedbf);
+      hipLaunchKernelGGL(k_velfg_xy<true>, strip_dim(sg), dim3(VX, VY), 0, st, d, g, F, ph,
+                         do_feedbf, sg);
     else
-      hipLaunchKernelGGL(k_velfg_xy<false>, grid, dim3(VX, VY), 0, st, d, g, F, ph, do_feedbf);
+      hipLaunchKernelGGL(k_velfg_xy<false>, strip_dim(sg), dim3(VX, VY), 0, st, d, g, F, ph,
+                         do_feedbf, sg);
     return;
   }
   if (g.p2xy)
@@ -457,10 +464,13 @@ void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do
 // ==========================================================================================
 // les.f95:40-60: Smagorinsky eddy viscosity sm from the stored diu fields
 // ==========================================================================================
-__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph) {
-  const int i = 1 + blockIdx.x * BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
+                                                 StripGrid sg) {
+  int bx, by, bz, L;
+  if (!strip_block(sg, bx, by, bz, L)) return;
+  const int i = 1 + bx * BX + threadIdx.x;
+  const int j = 1 + by * BY + threadIdx.y;
+  const int k = 1 + bz;
   if (i > d.ip || j > d.jp) return;
   const float q = 0.25f, two = 2.0f;
   const float* diu2 = F.diu2;
@@ -486,7 +496,8 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
 }
 
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
-  hipLaunchKernelGGL(k_calc_sm, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F, ph);
+  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+  hipLaunchKernelGGL(k_calc_sm, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, sg);
 }
 
 // ==========================================================================================
@@ -548,10 +559,12 @@ void launch_boundsm(hipStream_t st, Dims d, Fields F) {
 // ==========================================================================================
 template <bool P2>
 __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
-                                                   int do_adam) {
-  const int i = 1 + blockIdx.x * BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+                                                   int do_adam, StripGrid sg) {
+  int bx, by, bz, L;
+  if (!strip_block(sg, bx, by, bz, L)) return;
+  const int i = 1 + bx * BX + threadIdx.x;
+  const int j = 1 + by * BY + threadIdx.y;
+  const int k = 1 + bz;
   if (i > d.ip || j > d.jp) return;
   const long ifg = d.F(i, j, k);
   float fv = F.f[ifg], gv = F.g[ifg], hv = F.h[ifg];
@@ -661,12 +674,13 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
 }
 
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam) {
+  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
   if (g.p2xy)
-    hipLaunchKernelGGL(k_visc_adam<true>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F,
-                       do_visc, do_adam);
+    hipLaunchKernelGGL(k_visc_adam<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, do_visc,
+                       do_adam, sg);
   else
-    hipLaunchKernelGGL(k_visc_adam<false>, interior_grid(d, d.kp), dim3(BX, BY), 0, st, d, g, F,
-                       do_visc, do_adam);
+    hipLaunchKernelGGL(k_visc_adam<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, do_visc,
+                       do_adam, sg);
 }
 
 // ==========================================================================================
@@ -710,14 +724,16 @@ void launch_bondfg(hipStream_t st, Dims d, Fields F) {
 // ---- press.f95:49-75: bondfg + rhs + exact partial sums of dx1*dy1*dzn*rhs ----
 template <bool P2>
 __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics ph,
-                                             int64_t* partials) {
+                                             int64_t* partials, StripGrid sg) {
   __shared__ unsigned long long lds[NLIMB];
+  int bx, by, bz, L;
+  if (!strip_block(sg, bx, by, bz, L)) return;  // padding: writes no partial
   block_limbs_init(lds);
-  const int i = 1 + blockIdx.x * BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * BY + threadIdx.y;
+  const int i = 1 + bx * BX + threadIdx.x;
+  const int j = 1 + by * BY + threadIdx.y;
   const int im = d.ip, jm = d.jp, km = d.kp;
   const int kchunk = (km + RED_KC - 1) / RED_KC;
-  const int k0 = 1 + blockIdx.z * kchunk, k1 = min(km, k0 + kchunk - 1);
+  const int k0 = 1 + bz * kchunk, k1 = min(km, k0 + kchunk - 1);
   ThreadAcc acc;
   acc.init();
   if (i <= im && j <= jm) {
@@ -745,8 +761,7 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
     }
   }
   acc.flush(lds);
-  const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  block_limbs_store(lds, partials + (long)blk * NLIMB);
+  block_limbs_store(lds, partials + (long)L * NLIMB);
 }
 
 // Also re-arms the SOR scalars for the solve that follows (the work of k_sor_reset).
@@ -773,12 +788,14 @@ __global__ __launch_bounds__(256) void k_rhs_adjust(Dims d, Fields F, const DevS
 
 void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
                 DevScalars* sc) {
-  dim3 grid = interior_grid(d, RED_KC);
-  int nb = (int)(grid.x * grid.y * grid.z);
+  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, RED_KC);
+  const int nb = sg.nbx * sg.nby * sg.nbz;
   if (g.p2xy)
-    hipLaunchKernelGGL(k_rhs<true>, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
+    hipLaunchKernelGGL(k_rhs<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, rb.partials,
+                       sg);
   else
-    hipLaunchKernelGGL(k_rhs<false>, grid, dim3(BX, BY), 0, st, d, g, F, ph, rb.partials);
+    hipLaunchKernelGGL(k_rhs<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
+                       rb.partials, sg);
   hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
 }
 
