@@ -353,6 +353,10 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   if (cfg->ip < 2 || cfg->jp < 2 || cfg->kp < 2 || cfg->ip > 16384 || cfg->jp > 16384 ||
       cfg->kp > 4096 || cfg->nmaxp < 0)
     return LES_E_INVALID;
+  // 32-bit device indexing and buffer byte offsets: the largest array,
+  // diu1(-1:ip+2,0:jp+2,0:kp+2), below 2^30 elements
+  if ((int64_t)(cfg->ip + 4) * (cfg->jp + 3) * (cfg->kp + 3) >= (int64_t(1) << 30))
+    return LES_E_INVALID;
   if (cfg->sub_i != 1 || cfg->sub_j != 1) return LES_E_INVALID;  // one domain per handle
   if (cfg->precision != LES_PREC_FP32 && cfg->precision != LES_PREC_MIXED) return LES_E_INVALID;
   if (cfg->precision == LES_PREC_MIXED && (!(cfg->refine_tol > 0.0) || cfg->refine_max < 0 ||

@@ -7,46 +7,68 @@ namespace les {
 
 // Interior sizes; every array keeps the reference's Fortran bounds (main_ocl.f95:40-167),
 // column-major with i fastest.  Index helpers take Fortran indices.
+// Linear indices of the Fortran arrays (column-major, each with its own bounds).  The index
+// arithmetic is 32-bit (les_create admits only grids whose largest array, diu1, has fewer than
+// 2^30 elements), so a neighbour's index is the centre's plus a constant or a stride and the
+// compiler folds it into the load's offset instead of redoing 64-bit products.
 struct Dims {
   int ip, jp, kp;
   // p(0:ip+2,0:jp+2,0:kp+1)
-  __host__ __device__ long P(int i, int j, int k) const {
-    return (long)i + (long)(ip + 3) * ((long)j + (long)(jp + 3) * (long)k);
-  }
+  __host__ __device__ long P(int i, int j, int k) const { return i + (ip + 3) * (j + (jp + 3) * k); }
   // u,v(0:ip+1,-1:jp+1,0:kp+1)
   __host__ __device__ long U(int i, int j, int k) const {
-    return (long)i + (long)(ip + 2) * ((long)(j + 1) + (long)(jp + 3) * (long)k);
+    return i + (ip + 2) * ((j + 1) + (jp + 3) * k);
   }
   // w(0:ip+1,-1:jp+1,-1:kp+1)
   __host__ __device__ long W(int i, int j, int k) const {
-    return (long)i + (long)(ip + 2) * ((long)(j + 1) + (long)(jp + 3) * (long)(k + 1));
+    return i + (ip + 2) * ((j + 1) + (jp + 3) * (k + 1));
   }
   // f,g,h,usum,vsum,wsum(0:ip,0:jp,0:kp)
-  __host__ __device__ long F(int i, int j, int k) const {
-    return (long)i + (long)(ip + 1) * ((long)j + (long)(jp + 1) * (long)k);
-  }
+  __host__ __device__ long F(int i, int j, int k) const { return i + (ip + 1) * (j + (jp + 1) * k); }
   // fold,gold,hold,cn1(1:ip,1:jp,1:kp)
   __host__ __device__ long O(int i, int j, int k) const {
-    return (long)(i - 1) + (long)ip * ((long)(j - 1) + (long)jp * (long)(k - 1));
+    return (i - 1) + ip * ((j - 1) + jp * (k - 1));
   }
   // rhs(0:ip+1,0:jp+1,0:kp+1)
-  __host__ __device__ long R(int i, int j, int k) const {
-    return (long)i + (long)(ip + 2) * ((long)j + (long)(jp + 2) * (long)k);
-  }
+  __host__ __device__ long R(int i, int j, int k) const { return i + (ip + 2) * (j + (jp + 2) * k); }
   // sm(-1:ip+1,-1:jp+1,0:kp+1)
   __host__ __device__ long S(int i, int j, int k) const {
-    return (long)(i + 1) + (long)(ip + 3) * ((long)(j + 1) + (long)(jp + 3) * (long)k);
+    return (i + 1) + (ip + 3) * ((j + 1) + (jp + 3) * k);
   }
   // diu1,diu5(-1:ip+2,0:jp+2,0:kp+2)
   __host__ __device__ long D15(int i, int j, int k) const {
-    return (long)(i + 1) + (long)(ip + 4) * ((long)j + (long)(jp + 3) * (long)k);
+    return (i + 1) + (ip + 4) * (j + (jp + 3) * k);
   }
   // diu2..4,6..9(0:ip+2,0:jp+2,0:kp+2)
-  __host__ __device__ long D(int i, int j, int k) const {
-    return (long)i + (long)(ip + 3) * ((long)j + (long)(jp + 3) * (long)k);
-  }
+  __host__ __device__ long D(int i, int j, int k) const { return i + (ip + 3) * (j + (jp + 3) * k); }
   // amask1,dmask1 / interior mask byte array (1:ip,1:jp,1:kp)
   __host__ __device__ long M(int i, int j, int k) const { return O(i, j, k); }
+  // element counts of the arrays above
+  __host__ __device__ int nP() const { return (ip + 3) * (jp + 3) * (kp + 2); }
+  __host__ __device__ int nU() const { return (ip + 2) * (jp + 3) * (kp + 2); }
+  __host__ __device__ int nW() const { return (ip + 2) * (jp + 3) * (kp + 3); }
+  __host__ __device__ int nF() const { return (ip + 1) * (jp + 1) * (kp + 1); }
+  __host__ __device__ int nO() const { return ip * jp * kp; }
+  __host__ __device__ int nR() const { return (ip + 2) * (jp + 2) * (kp + 2); }
+  __host__ __device__ int nS() const { return (ip + 3) * (jp + 3) * (kp + 2); }
+  __host__ __device__ int nD15() const { return (ip + 4) * (jp + 3) * (kp + 3); }
+  __host__ __device__ int nD() const { return (ip + 3) * (jp + 3) * (kp + 3); }
+};
+
+// Raw-buffer view of an FP32 device array of n elements: loads and stores address it with a
+// 32-bit byte offset from a descriptor held in scalar registers, so an access costs one VALU
+// add for its row instead of a 64-bit address computation.  The hardware range check makes
+// an access outside [0, n) read 0 or be dropped (arrays are below 2^30 elements).
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const float* p, int n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * 4, 0x00020000)) {}
+  __device__ __forceinline__ float operator[](long i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)i * 4, 0, 0));
+  }
+  __device__ __forceinline__ void st(long i, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)i * 4, 0, 0);
+  }
 };
 
 // 1-D grid arrays, offsets of their lower bounds: dx1(-1:), dxs(0:), dy1(0:), dys(0:),

@@ -417,27 +417,28 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
     float df = ph.vn * dfw1;
     hv = (-covc + df);
   } else {
-    hv = F.h[ifg];  // velfg leaves h(:,:,km) untouched
+    hv = Buf(F.h, d.nF())[ifg];  // velfg leaves h(:,:,km) untouched
   }
   if (do_feedbf) {  // feedbf.f95:41-65
     const uint8_t m = F.mask[d.M(i, j, k)];
     const float bm = mbit(m, 0), cm = mbit(m, 1), dm = mbit(m, 2);
     const float uu = c.U(i, j, k), vv = c.V(i, j, k), ww = c.W(i, j, k);
-    const float us = (F.usum[ifg] + uu) * bm;
-    const float vs = (F.vsum[ifg] + vv) * cm;
-    const float ws = (F.wsum[ifg] + ww) * dm;
-    F.usum[ifg] = us;
-    F.vsum[ifg] = vs;
-    F.wsum[ifg] = ws;
+    const Buf bus(F.usum, d.nF()), bvs(F.vsum, d.nF()), bws(F.wsum, d.nF());
+    const float us = (bus[ifg] + uu) * bm;
+    const float vs = (bvs[ifg] + vv) * cm;
+    const float ws = (bws[ifg] + ww) * dm;
+    bus.st(ifg, us);
+    bvs.st(ifg, vs);
+    bws.st(ifg, ws);
     const float f1x = ph.alpha * us * ph.dt, f1y = ph.alpha * vs * ph.dt, f1z = ph.alpha * ws * ph.dt;
     const float f2x = ph.beta * uu * bm, f2y = ph.beta * vv * cm, f2z = ph.beta * ww * dm;
     fv = fv + (f1x + f2x);
     gv = gv + (f1y + f2y);
     hv = hv + (f1z + f2z);
   }
-  F.f[ifg] = fv;
-  F.g[ifg] = gv;
-  F.h[ifg] = hv;
+  Buf(F.f, d.nF()).st(ifg, fv);
+  Buf(F.g, d.nF()).st(ifg, gv);
+  Buf(F.h, d.nF()).st(ifg, hv);
   store_diu(d, F, i, j, k, x1, y2, z3, x4, y5, z6, x7, y8, z9, z3n, z6n);
 }
 
@@ -473,26 +474,22 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
   const int k = 1 + bz;
   if (i > d.ip || j > d.jp) return;
   const float q = 0.25f, two = 2.0f;
-  const float* diu2 = F.diu2;
-  const float* diu3 = F.diu3;
-  const float* diu4 = F.diu4;
-  const float* diu6 = F.diu6;
-  const float* diu7 = F.diu7;
-  const float* diu8 = F.diu8;
-  float dudxx1 = F.diu1[d.D15(i, j, k)];
+  const Buf diu2(F.diu2, d.nD()), diu3(F.diu3, d.nD()), diu4(F.diu4, d.nD());
+  const Buf diu6(F.diu6, d.nD()), diu7(F.diu7, d.nD()), diu8(F.diu8, d.nD());
+  float dudxx1 = Buf(F.diu1, d.nD15())[d.D15(i, j, k)];
   float dudyx1 = (diu2[d.D(i - 1, j, k)] + diu2[d.D(i - 1, j + 1, k)] + diu2[d.D(i, j, k)] + diu2[d.D(i, j + 1, k)]) * q;
   float dudzx1 = (diu3[d.D(i - 1, j, k)] + diu3[d.D(i - 1, j, k + 1)] + diu3[d.D(i, j, k)] + diu3[d.D(i, j, k + 1)]) * q;
   float dvdxx1 = (diu4[d.D(i, j, k)] + diu4[d.D(i, j - 1, k)] + diu4[d.D(i + 1, j, k)] + diu4[d.D(i + 1, j - 1, k)]) * q;
-  float dvdyx1 = F.diu5[d.D15(i, j, k)];
+  float dvdyx1 = Buf(F.diu5, d.nD15())[d.D15(i, j, k)];
   float dvdzx1 = (diu6[d.D(i, j - 1, k)] + diu6[d.D(i, j - 1, k + 1)] + diu6[d.D(i, j, k)] + diu6[d.D(i, j, k + 1)]) * q;
   float dwdxx1 = (diu7[d.D(i, j, k)] + diu7[d.D(i, j, k - 1)] + diu7[d.D(i + 1, j, k)] + diu7[d.D(i + 1, j, k - 1)]) * q;
   float dwdyx1 = (diu8[d.D(i, j, k)] + diu8[d.D(i, j, k - 1)] + diu8[d.D(i, j + 1, k)] + diu8[d.D(i, j + 1, k - 1)]) * q;
-  float dwdzx1 = F.diu9[d.D(i, j, k)];
+  float dwdzx1 = Buf(F.diu9, d.nD())[d.D(i, j, k)];
   float cd = ph.cs0 * g.delx1[k - 1];
   float s = two * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
             (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
             (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
-  F.sm[d.S(i, j, k)] = (cd * cd) * sqrtf(s);
+  Buf(F.sm, d.nS()).st(d.S(i, j, k), (cd * cd) * sqrtf(s));
 }
 
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
@@ -567,19 +564,23 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
   const int k = 1 + bz;
   if (i > d.ip || j > d.jp) return;
   const long ifg = d.F(i, j, k);
-  float fv = F.f[ifg], gv = F.g[ifg], hv = F.h[ifg];
+  const Buf bf(F.f, d.nF()), bg(F.g, d.nF()), bh(F.h, d.nF());
+  float fv = bf[ifg], gv = bg[ifg], hv = bh[ifg];
   if (do_visc) {
     const float two = 2.0f;
-    auto sm = [&](int x, int y, int z) -> float { return F.sm[d.S(x, y, z)]; };
-    auto D1 = [&](int x, int y, int z) -> float { return F.diu1[d.D15(x, y, z)]; };
-    auto D5 = [&](int x, int y, int z) -> float { return F.diu5[d.D15(x, y, z)]; };
-    auto D2 = [&](int x, int y, int z) -> float { return F.diu2[d.D(x, y, z)]; };
-    auto D3 = [&](int x, int y, int z) -> float { return F.diu3[d.D(x, y, z)]; };
-    auto D4 = [&](int x, int y, int z) -> float { return F.diu4[d.D(x, y, z)]; };
-    auto D6 = [&](int x, int y, int z) -> float { return F.diu6[d.D(x, y, z)]; };
-    auto D7 = [&](int x, int y, int z) -> float { return F.diu7[d.D(x, y, z)]; };
-    auto D8 = [&](int x, int y, int z) -> float { return F.diu8[d.D(x, y, z)]; };
-    auto D9 = [&](int x, int y, int z) -> float { return F.diu9[d.D(x, y, z)]; };
+    const Buf bsm(F.sm, d.nS()), b1(F.diu1, d.nD15()), b5(F.diu5, d.nD15());
+    const Buf b2(F.diu2, d.nD()), b3(F.diu3, d.nD()), b4(F.diu4, d.nD()), b6(F.diu6, d.nD());
+    const Buf b7(F.diu7, d.nD()), b8(F.diu8, d.nD()), b9(F.diu9, d.nD());
+    auto sm = [&](int x, int y, int z) -> float { return bsm[d.S(x, y, z)]; };
+    auto D1 = [&](int x, int y, int z) -> float { return b1[d.D15(x, y, z)]; };
+    auto D5 = [&](int x, int y, int z) -> float { return b5[d.D15(x, y, z)]; };
+    auto D2 = [&](int x, int y, int z) -> float { return b2[d.D(x, y, z)]; };
+    auto D3 = [&](int x, int y, int z) -> float { return b3[d.D(x, y, z)]; };
+    auto D4 = [&](int x, int y, int z) -> float { return b4[d.D(x, y, z)]; };
+    auto D6 = [&](int x, int y, int z) -> float { return b6[d.D(x, y, z)]; };
+    auto D7 = [&](int x, int y, int z) -> float { return b7[d.D(x, y, z)]; };
+    auto D8 = [&](int x, int y, int z) -> float { return b8[d.D(x, y, z)]; };
+    auto D9 = [&](int x, int y, int z) -> float { return b9[d.D(x, y, z)]; };
     const float dx1i = g.DX1(i), dx1ip = g.DX1(i + 1), dx1im = g.DX1(i - 1);
     const float dy1j = g.DY1(j), dy1jp = g.DY1(j + 1), dy1jm = g.DY1(j - 1);
     const float dznk = g.DZN(k), dznkp = g.DZN(k + 1), dznkm = g.DZN(k - 1);
@@ -660,17 +661,18 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
   }
   if (do_adam) {  // adam.f95:35-43
     const long io = d.O(i, j, k);
-    const float fo = F.fold[io], go = F.gold[io], ho = F.hold[io];
-    F.fold[io] = fv;
-    F.gold[io] = gv;
-    F.hold[io] = hv;
+    const Buf bfo(F.fold, d.nO()), bgo(F.gold, d.nO()), bho(F.hold, d.nO());
+    const float fo = bfo[io], go = bgo[io], ho = bho[io];
+    bfo.st(io, fv);
+    bgo.st(io, gv);
+    bho.st(io, hv);
     fv = 1.5f * fv - 0.5f * fo;
     gv = 1.5f * gv - 0.5f * go;
     hv = 1.5f * hv - 0.5f * ho;
   }
-  F.f[ifg] = fv;
-  F.g[ifg] = gv;
-  F.h[ifg] = hv;
+  bf.st(ifg, fv);
+  bg.st(ifg, gv);
+  bh.st(ifg, hv);
 }
 
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam) {

@@ -9,7 +9,9 @@ template <bool P2>
 struct VelCtx {
   Dims d;
   Grid1D g;
-  const float *u, *v, *w;
+  Buf u, v, w;  // raw-buffer views of u, v (u's bounds) and w
+  __device__ VelCtx(Dims d_, Grid1D g_, const float* u_, const float* v_, const float* w_)
+      : d(d_), g(g_), u(u_, d_.nU()), v(v_, d_.nU()), w(w_, d_.nW()) {}
   __device__ float U(int i, int j, int k) const { return u[d.U(i, j, k)]; }
   __device__ float V(int i, int j, int k) const { return v[d.U(i, j, k)]; }
   __device__ float W(int i, int j, int k) const { return w[d.W(i, j, k)]; }
@@ -91,13 +93,14 @@ __device__ inline void store_diu(const Dims& d, const Fields& F, int i, int j, i
                                  float d8, float d9, float d3top, float d6top) {
   const int im = d.ip, jm = d.jp, km = d.kp;
   const bool hk = k <= km - 1;
-  F.diu1[d.D15(i, j, k)] = d1;
+  // the interior stores through buffer views; the halo copies (few threads) through pointers
+  Buf(F.diu1, d.nD15()).st(d.D15(i, j, k), d1);
   if (i == im) F.diu1[d.D15(im + 1, j, k)] = d1;
-  F.diu5[d.D15(i, j, k)] = d5;
+  Buf(F.diu5, d.nD15()).st(d.D15(i, j, k), d5);
   if (j == jm) F.diu5[d.D15(i, 0, k)] = d5;
   if (j == 1) F.diu5[d.D15(i, jm + 1, k)] = d5;
-  F.diu9[d.D(i, j, k)] = d9;
-  F.diu2[d.D(i, j, k)] = d2;
+  Buf(F.diu9, d.nD()).st(d.D(i, j, k), d9);
+  Buf(F.diu2, d.nD()).st(d.D(i, j, k), d2);
   if (j == jm) F.diu2[d.D(i, 0, k)] = d2;
   if (j == 1) F.diu2[d.D(i, jm + 1, k)] = d2;
   if (i == im) {
@@ -108,7 +111,7 @@ __device__ inline void store_diu(const Dims& d, const Fields& F, int i, int j, i
       if (j == jm) F.diu2[d.D(im + 1, jm + 1, km + 1)] = F.diu2[d.D(im, jm + 1, km + 1)];
     }
   }
-  F.diu3[d.D(i, j, k)] = d3;
+  Buf(F.diu3, d.nD()).st(d.D(i, j, k), d3);
   if (k == km) F.diu3[d.D(i, j, km + 1)] = d3top;
   if (i == im) {
     F.diu3[d.D(im + 1, j, k)] = d3;
@@ -118,7 +121,7 @@ __device__ inline void store_diu(const Dims& d, const Fields& F, int i, int j, i
       if (k == km) F.diu3[d.D(im + 1, jm + 1, km + 1)] = F.diu3[d.D(im, jm + 1, km + 1)];
     }
   }
-  F.diu4[d.D(i, j, k)] = d4;
+  Buf(F.diu4, d.nD()).st(d.D(i, j, k), d4);
   if (i == im) F.diu4[d.D(im + 1, j, k)] = d4;
   if (j == jm) {
     F.diu4[d.D(i, 0, k)] = d4;
@@ -128,7 +131,7 @@ __device__ inline void store_diu(const Dims& d, const Fields& F, int i, int j, i
       if (i == im) F.diu4[d.D(im + 1, 0, km + 1)] = F.diu4[d.D(im + 1, jm, km + 1)];
     }
   }
-  F.diu6[d.D(i, j, k)] = d6;
+  Buf(F.diu6, d.nD()).st(d.D(i, j, k), d6);
   if (k == km) F.diu6[d.D(i, j, km + 1)] = d6top;
   if (j == jm) {
     F.diu6[d.D(i, 0, k)] = d6;
@@ -139,9 +142,9 @@ __device__ inline void store_diu(const Dims& d, const Fields& F, int i, int j, i
     }
   }
   if (hk) {
-    F.diu7[d.D(i, j, k)] = d7;
+    Buf(F.diu7, d.nD()).st(d.D(i, j, k), d7);
     if (i == im) F.diu7[d.D(im + 1, j, k)] = d7;
-    F.diu8[d.D(i, j, k)] = d8;
+    Buf(F.diu8, d.nD()).st(d.D(i, j, k), d8);
     if (j == jm) F.diu8[d.D(i, 0, k)] = d8;
     if (j == 1) F.diu8[d.D(i, jm + 1, k)] = d8;
   }
