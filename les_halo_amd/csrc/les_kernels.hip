@@ -741,25 +741,51 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
   if (i <= im && j <= jm) {
     const float dx1i = g.DX1(i), dy1j = g.DY1(j), dt = ph.dt;
     const int jm1 = (j == 1) ? jm : j - 1;  // g(i,0,k) = g(i,jm,k) (bondFG)
-    for (int k = k0; k <= k1; ++k) {
-      const float dznk = g.DZN(k);
+    const Buf bu(F.u, d.nU()), bv(F.v, d.nU()), bw(F.w, d.nW());
+    const Buf bf(F.f, d.nF()), bg(F.g, d.nF()), bh(F.h, d.nF()), brh(F.rhs, d.nR());
+    // Operands of level k with bondFG's values (f(0) = f(1), g(0) = g(jm), h(0) = h(km) = 0)
+    // taken in place of the halo cells, so nothing read here is written by this kernel.
+    struct In {
+      float um, uc, vm, vc, wm, wc, fc, fm, gc, gm, hc, hm;
+    };
+    auto load = [&](int k) {
+      In t;
+      t.um = bu[d.U(i - 1, j, k)];
+      t.uc = bu[d.U(i, j, k)];
+      t.vm = bv[d.U(i, j - 1, k)];
+      t.vc = bv[d.U(i, j, k)];
+      t.wm = bw[d.W(i, j, k - 1)];
+      t.wc = bw[d.W(i, j, k)];
       const long ifg = d.F(i, j, k);
-      // bondFG: f(0,j,k) = f(1,j,k); g(i,0,k) = g(i,jm,k); h(i,j,0) = h(i,j,km) = 0
-      if (i == 1) F.f[d.F(0, j, k)] = F.f[ifg];
-      if (j == jm) F.g[d.F(i, 0, k)] = F.g[ifg];
-      if (k == 1) F.h[d.F(i, j, 0)] = 0.0f;
-      if (k == km) F.h[ifg] = 0.0f;
-      const float fim = (i == 1) ? F.f[ifg] : F.f[d.F(i - 1, j, k)];
-      const float gjm = F.g[d.F(i, jm1, k)];
-      const float hk = (k == km) ? 0.0f : F.h[ifg];
-      const float hkm = (k == 1) ? 0.0f : F.h[d.F(i, j, k - 1)];
-      float r = xdiv<P2>(-F.u[d.U(i - 1, j, k)] + F.u[d.U(i, j, k)], dx1i) +
-                xdiv<P2>(-F.v[d.U(i, j - 1, k)] + F.v[d.U(i, j, k)], dy1j) +
-                (-F.w[d.W(i, j, k - 1)] + F.w[d.W(i, j, k)]) / dznk;
-      r = xdiv<P2>(F.f[ifg] - fim, dx1i) + xdiv<P2>(F.g[ifg] - gjm, dy1j) + (hk - hkm) / dznk +
+      t.fc = bf[ifg];
+      const float f0 = bf[d.F(i - 1, j, k)];
+      t.fm = (i == 1) ? t.fc : f0;
+      t.gc = bg[ifg];
+      t.gm = bg[d.F(i, jm1, k)];
+      const float h0 = bh[ifg], h1 = bh[d.F(i, j, k - 1)];
+      t.hc = (k == km) ? 0.0f : h0;
+      t.hm = (k == 1) ? 0.0f : h1;
+      return t;
+    };
+    // the loads of level k+1 are issued before the stores of level k
+    In nxt = load(k0);
+    for (int k = k0; k <= k1; ++k) {
+      const In c = nxt;
+      if (k < k1) nxt = load(k + 1);
+      const float dznk = g.DZN(k);
+      float r = xdiv<P2>(-c.um + c.uc, dx1i) + xdiv<P2>(-c.vm + c.vc, dy1j) +
+                (-c.wm + c.wc) / dznk;
+      r = xdiv<P2>(c.fc - c.fm, dx1i) + xdiv<P2>(c.gc - c.gm, dy1j) + (c.hc - c.hm) / dznk +
           (r) / dt;
-      F.rhs[d.R(i, j, k)] = r;
+      brh.st(d.R(i, j, k), r);
       acc.add(dx1i * dy1j * dznk * r, lds);
+    }
+    // bondFG: f(0,j,k) = f(1,j,k); g(i,0,k) = g(i,jm,k); h(i,j,0) = h(i,j,km) = 0
+    for (int k = k0; k <= k1; ++k) {
+      if (i == 1) F.f[d.F(0, j, k)] = F.f[d.F(1, j, k)];
+      if (j == jm) F.g[d.F(i, 0, k)] = F.g[d.F(i, jm, k)];
+      if (k == 1) F.h[d.F(i, j, 0)] = 0.0f;
+      if (k == km) F.h[d.F(i, j, km)] = 0.0f;
     }
   }
   acc.flush(lds);
