@@ -41,7 +41,7 @@ def traffic_from_profiles(kernel):
     if not os.path.exists(path):
         return None
     try:
-        ent = json.load(open(path)).get(kernel.split("<")[0])
+        ent = json.load(open(path)).get(kernel)
         return float(ent["hbm_bytes_per_launch"]) if ent else None
     except (ValueError, KeyError, TypeError):
         return None
@@ -78,8 +78,8 @@ CALIB_COPY_BYTES = 256 << 20  # calibration copy: 256 MiB, far larger than the 3
 
 
 def profile_sor(L, reps):
-    """For rocprofv3 --pmc runs: REPS colour sweeps (k_sor_split), REPS fused iterations
-    (k_sor_iter) and one device copy of CALIB_COPY_BYTES (read and written once each, its source
+    """For rocprofv3 --pmc runs: REPS colour sweeps (k_sor_split), REPS fused iterations of
+    each tile shape (k_sor_iter, 10 levels per workgroup) and one device copy of CALIB_COPY_BYTES (read and written once each, its source
     evicted from L2 by the copy itself), whose FETCH_SIZE/WRITE_SIZE calibrate the counters."""
     import torch
     ip, jp, kp = GRID
@@ -88,14 +88,20 @@ def profile_sor(L, reps):
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     ms = L.time_kernel("sor_sweep", reps)
-    ms_it = L.time_kernel("sor_iter", reps)
+    # every fused variant with a fixed tile depth (10 levels), so that each counter run
+    # profiles the same dispatches whatever tile the autotuner chose
+    iter_us = {}
+    for tj in (4, 8, 16):
+        L.set_sor_tile(tj, 10)
+        iter_us["k_sor_iter<%d, %d>" % (tj, 256 if tj == 4 else 512)] = round(
+            L.time_kernel("sor_iter", reps) * 1e3, 3)
     dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"launches": reps,
                       "sweep_bytes_per_launch": SOR_SWEEP_B_PER_CELL * ip * jp * kp,
                       "iter_bytes_per_launch": SOR_ITER_B_PER_CELL * ip * jp * kp,
                       "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
-                      "sweep_avg_us": round(ms * 1e3, 3), "iter_avg_us": round(ms_it * 1e3, 3)}),
+                      "sweep_avg_us": round(ms * 1e3, 3), "iter_avg_us": iter_us}),
           flush=True)
 
 
@@ -220,11 +226,13 @@ def main():
     if tuning["sor_tile_j"] > 0:
         kern_ms = L.time_kernel("sor_iter", 100)
         bytes_launch = SOR_ITER_B_PER_CELL * cells
-        kname = "k_sor_iter<%d>" % tuning["sor_tile_j"]
+        # the kernel's name as rocprofv3 reports it (tile rows, workgroup size)
+        tj = tuning["sor_tile_j"]
+        kname = "k_sor_iter<%d, %d>" % (tj, 256 if tj == 4 else 512)
     else:
         kern_ms = L.time_kernel("sor_sweep", 200)
         bytes_launch = SOR_SWEEP_B_PER_CELL * cells
-        kname = "k_sor_split<true>"
+        kname = "k_sor_split<true, false>"
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
