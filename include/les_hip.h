@@ -237,6 +237,33 @@ int les_write_restart(les_handle* h, const char* data20, const char* data21, int
                       int with_header);
 int les_read_restart(les_handle* h, const char* data30, const char* data31, int* n, float* time);
 
+/* NetCDF snapshots (module_LES_write_netcdf.f95, called from press.f95:189): les_netcdf_create
+ * is init_netcdf_file (:111-300): it creates dir/LES_output.nc, LES_output_p.nc,
+ * LES_output_u/v/w.nc and LES_output_uvwsum.nc in the netCDF classic format with ntimes time
+ * levels (NTIMESTEPS = 20 in the reference), every data value set to the float fill value.
+ * les_netcdf_write is write_to_netcdf_file(p,u,v,w,usum,vsum,wsum,t) (:302-346): time level t
+ * (1-based; the reference passes n/1000) of every variable from the handle's fields;
+ * les_netcdf_write_arrays takes host arrays in the Fortran bounds instead.  les_netcdf_close is
+ * close_netcdf_file (:348-386). */
+typedef struct les_netcdf les_netcdf;
+int les_netcdf_create(const char* dir, int ip, int jp, int kp, int ntimes, les_netcdf** out);
+int les_netcdf_write(les_netcdf* nc, les_handle* h, int t);
+int les_netcdf_write_arrays(les_netcdf* nc, int t, const float* p, const float* u, const float* v,
+                            const float* w, const float* usum, const float* vsum,
+                            const float* wsum);
+int les_netcdf_close(les_netcdf* nc);
+
+/* anime(n,n0,nmax,...,data22,data23,u,w,v,amask1) (anime.f95:5-49, main.f95:224): when n == n0,
+ * n == nmax or n % 1000 == 0, writes the grid file data22 and the cell-centred velocity and
+ * mask file data23 (Fortran unformatted records); otherwise does nothing.  les_anime_arrays
+ * takes host arrays in the Fortran bounds (dx1(-1:ip+1), dy1(0:jp+1), z2(kp+2), u, v, w,
+ * amask1) instead of a handle. */
+int les_anime(les_handle* h, int n, int n0, int nmax, const char* data22, const char* data23);
+int les_anime_arrays(int n, int n0, int nmax, int ip, int jp, int kp, const float* dx1,
+                     const float* dy1, const float* z2, const float* u, const float* v,
+                     const float* w, const float* amask1, const char* data22,
+                     const char* data23);
+
 /* Scalars of the last stages (synchronises). */
 typedef struct les_scalars {
   float uout;      /* bondv1.f95:75 */

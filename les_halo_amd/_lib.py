@@ -48,6 +48,13 @@ PROTOS = {
     "les_write_restart": (c_int, [H, ctypes.c_char_p, ctypes.c_char_p, c_int, c_float, c_int]),
     "les_read_restart": (c_int, [H, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(c_int),
                                  ctypes.POINTER(c_float)]),
+    "les_netcdf_create": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int,
+                                  ctypes.POINTER(c_void_p)]),
+    "les_netcdf_write": (c_int, [c_void_p, H, c_int]),
+    "les_netcdf_write_arrays": (c_int, [c_void_p, c_int] + [c_void_p] * 7),
+    "les_netcdf_close": (c_int, [c_void_p]),
+    "les_anime": (c_int, [H, c_int, c_int, c_int, ctypes.c_char_p, ctypes.c_char_p]),
+    "les_anime_arrays": (c_int, [c_int] * 6 + [c_void_p] * 7 + [ctypes.c_char_p] * 2),
     "les_upload": (c_int, [H, c_int, c_void_p]),
     "les_download": (c_int, [H, c_int, c_void_p]),
     "les_device_ptr": (c_int, [H, c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_int64)]),

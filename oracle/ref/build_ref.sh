@@ -43,7 +43,7 @@ comp "$PARAMS"
 # bondv1.f95 and ifdata.f95 are left out: bondv1.f95's fixed inflow levels (k=1..78, u(:,:,77))
 # do not compile for kp < 77, and the harness restates ifdata.f95's call sequence itself.
 for f in common_sn boundsm vel2 velFG feedbf feedbfm les bondv1_ext_wind_profile boundp \
-         bondFG velnw adam press aveflow; do
+         bondFG velnw adam press aveflow anime; do
   comp "$REF/$f.f95"
 done
 comp "$REF/init.f95" -UIFBF -DIFBF=0

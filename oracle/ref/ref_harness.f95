@@ -12,6 +12,9 @@
 !      state is dumped.
 !   2  terrain masks: feedbfm.f95 with its own GIS read (run with cwd = src/Fortran), zbm and
 !      the four masks are dumped.
+!   1  (stage 13) anime.f95 on the state read, with n = n0 = nmax (so it writes), into the
+!      files anime22.bin and anime23.bin of the working directory; dxl and dyl are the running
+!      sums of dx1 and dy1 (grid.f95:26-38).
 !   3  statistics: as mode 0 (no state dumps), with aveflow.f95 called after every step
 !      (n1 from the input, nmax = nsteps); the eighteen statistics arrays are dumped at the end.
 !      aveu and avew are passed as sections of buffers that start with ip*jp zeros, so the
@@ -38,6 +41,7 @@ program ref_harness
   use module_init
   use module_feedbfm
   use module_aveflow
+  use module_anime
   implicit none
 
   real(kind=4), dimension(0:ip+2,0:jp+2,0:kp+1) :: p
@@ -81,7 +85,10 @@ program ref_harness
   integer, allocatable :: dump_steps(:)
   integer(kind=8) :: clk0, clk1, clkrate
   character(len=512) :: fin, fout
-  character(len=70) :: data20, data21
+  character(len=70) :: data20, data21, data22, data23
+  real(kind=4), dimension(0:ip) :: dxl
+  real(kind=4), dimension(0:jp) :: dyl
+  integer :: ii
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -201,6 +208,18 @@ program ref_harness
       call boundsm(km, jm, sm, im)
     case (12)
       call one_step(n)
+    case (13)
+      dxl(0) = 0.
+      do ii = 1, ip
+        dxl(ii) = dxl(ii-1) + dx1(ii)
+      end do
+      dyl(0) = 0.
+      do ii = 1, jp
+        dyl(ii) = dyl(ii-1) + dy1(ii)
+      end do
+      data22 = 'anime22.bin'
+      data23 = 'anime23.bin'
+      call anime(n, n, n, km, jm, im, dxl, dx1, dyl, dy1, z2, data22, data23, u, w, v, amask1)
     case default
       print *, 'ref_harness: unknown stage', stage
       stop 3

@@ -33,7 +33,8 @@ def available():
 def binary(ip, jp, kp):
     tag = tag_of(ip, jp, kp)
     exe = os.path.join(REF_OUT, tag, "ref_harness")
-    if not os.path.exists(exe):
+    recipe = [os.path.join(HERE, "ref_harness.f95"), os.path.join(HERE, "build_ref.sh")]
+    if not os.path.exists(exe) or max(map(os.path.getmtime, recipe)) > os.path.getmtime(exe):
         subprocess.run([os.path.join(HERE, "build_ref.sh"), tag], check=True,
                        stdout=subprocess.DEVNULL)
     return exe
@@ -130,7 +131,21 @@ def stage(case, phys, state, stage_id, n):
 
 
 STAGE_IDS = {"velnw": 1, "bondv1": 2, "velfg": 3, "feedbf": 4, "les": 5, "adam": 6, "press": 7,
-             "boundp1": 8, "boundp2": 9, "bondfg": 10, "boundsm": 11, "step": 12}
+             "boundp1": 8, "boundp2": 9, "bondfg": 10, "boundsm": 11, "step": 12, "anime": 13}
+
+
+def anime_files(case, phys, state, n):
+    """Mode 1, stage 13: anime.f95 on a full state; returns the bytes of data22 and data23."""
+    ip, jp, kp = case["dims"]
+    payload = _header(1, ip, jp, kp, phys, case) + _i32(STAGE_IDS["anime"], n)
+    payload += b"".join(_f32(state[nm]) for nm in DUMP_ORDER)
+    with tempfile.TemporaryDirectory() as td:
+        _run(binary(ip, jp, kp), payload, cwd=td)
+        with open(os.path.join(td, "anime22.bin"), "rb") as f:
+            d22 = f.read()
+        with open(os.path.join(td, "anime23.bin"), "rb") as f:
+            d23 = f.read()
+    return d22, d23
 
 
 def tokyo_masks(ip, jp, kp, dzn, z2, zbm):
