@@ -145,6 +145,40 @@ def ensemble_bench(nmembers, steps, warmup):
             "ms_per_step_all_members": round(t_ens / steps * 1e3, 3)}
 
 
+def tiled_bench(steps, case):
+    """The plain step as 2 x 2 tiles of 150 x 150 x 90 on the one GPU (les_tiled: face
+    exchanges between the tiles after every stage and SOR colour, in-place colour sweeps),
+    timed eagerly, then checked bitwise against one handle on the whole domain after 10 steps."""
+    import numpy as np
+    import torch
+    from les_halo_amd import LES, TiledLES
+    ip, jp, kp = GRID
+    sor = {k: PHYS[k] for k in ("omega", "nmaxp")}
+    phys = {k: PHYS[k] for k in ("dt", "ro", "vn", "alpha", "beta", "cs0")}
+    T = TiledLES(ip, jp, kp, sub_i=2, sub_j=2, **phys, **sor)
+    T.load_case(case)
+    T.ifdata()
+    T.run(1, 2)
+    T.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    T.run(3, steps)
+    T.sync()
+    secs = time.perf_counter() - t0
+    L = LES(ip, jp, kp, **phys, **sor, pjuge=0.0)
+    L.load_case(case)
+    L.ifdata()
+    L.run(1, 2 + steps)
+    same = all(np.array_equal(T.download(n).view(np.uint32), L.download(n).view(np.uint32))
+               for n in ("u", "v", "w", "p"))
+    T.close()
+    L.close()
+    return {"tiles": [2, 2], "tile_grid": [ip // 2, jp // 2, kp], "steps": steps,
+            "ms_per_step": round(secs / steps * 1e3, 3),
+            "value": round(ip * jp * kp * steps / secs / 1e6, 2), "unit": "M cell-steps/s",
+            "bitwise_equal_single_domain_uvwp": bool(same), "steps_compared": 2 + steps}
+
+
 def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95):
     """The mixed-precision configuration: 600x600x90, stencils and SOR sweeps in FP32, rhs mean,
     residual and refined pressure in FP64 (LES_PREC_MIXED), refinement to a relative L2
@@ -189,6 +223,8 @@ def main():
                     help="relaxation of the mixed-precision correction solves")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="also run the batched-ensemble configuration with N members")
+    ap.add_argument("--tiled", type=int, default=0, metavar="STEPS",
+                    help="also run the 2x2-tiles configuration for STEPS steps")
     ap.add_argument("--profile-sor", type=int, default=0, metavar="REPS",
                     help="counter runs: set up the workload, launch REPS SOR colour sweeps and "
                          "one device copy of the same algorithmic bytes (calibration), no timing")
@@ -252,6 +288,8 @@ def main():
                        "parallelism": "single domain, 1 GPU"},
             "step_hbm_frac": round(step_hbm_frac, 4), "step_bytes_per_cell": B_STEP,
             "roofline": roofline}
+    if args.tiled:
+        line["tiled"] = tiled_bench(args.tiled, case)
     if args.mixed:
         line["mixed"] = mixed_bench(args.mixed, refine_omega=args.refine_omega)
     if args.ensemble:

@@ -237,6 +237,30 @@ int les_write_restart(les_handle* h, const char* data20, const char* data21, int
                       int with_header);
 int les_read_restart(les_handle* h, const char* data30, const char* data31, int* n, float* time);
 
+/* ---- one domain as sub_i x sub_j tiles on one device ---------------------------------
+ * les_tiled_create takes the whole domain's configuration with cfg->sub_i, cfg->sub_j >= 1
+ * (at most 8 tiles; ip, jp divisible; the tile sizes even along a cut direction, so that the
+ * red-black colours agree; FP32, no SOR early exit).  Each tile is a handle of
+ * (ip/sub_i, jp/sub_j, kp) cells.  les_tiled_upload / les_tiled_download take whole-domain
+ * arrays in the reference's Fortran bounds (main_ocl.f95:40-167) and slice them into / gather
+ * them from the tiles.  les_tiled_ifdata and les_tiled_step are les_ifdata and les_step of
+ * the whole domain: the stages run tile by tile with the faces the tiles share, and the
+ * periodic j faces, exchanged between stages (the halo exchange of halo_exchange.cl's stages
+ * ST_HALO_*, module_LES_combined_ocl_TEMPL_V2.f95:356-363, as device copies since every tile is
+ * on this GPU), the domain's own faces under the reference's boundary rules, and the sums
+ * combined exactly: the results are bitwise those of one handle on the whole domain. */
+typedef struct les_tiled les_tiled;
+int les_tiled_create(const les_config* cfg, les_tiled** out);
+int les_tiled_destroy(les_tiled* t);
+const char* les_tiled_last_error(const les_tiled* t);
+int les_tiled_sync(les_tiled* t);
+int les_tiled_upload(les_tiled* t, int field, const float* host);
+int les_tiled_download(les_tiled* t, int field, float* host);
+/* init.f95:82-107 coefficients and delx1 on every tile, from the uploaded grid */
+int les_tiled_init_coefficients(les_tiled* t);
+int les_tiled_ifdata(les_tiled* t);
+int les_tiled_step(les_tiled* t, int n);
+
 /* NetCDF snapshots (module_LES_write_netcdf.f95, called from press.f95:189): les_netcdf_create
  * is init_netcdf_file (:111-300): it creates dir/LES_output.nc, LES_output_p.nc,
  * LES_output_u/v/w.nc and LES_output_uvwsum.nc in the netCDF classic format with ntimes time

@@ -54,6 +54,15 @@ PROTOS = {
     "les_netcdf_write_arrays": (c_int, [c_void_p, c_int] + [c_void_p] * 7),
     "les_netcdf_close": (c_int, [c_void_p]),
     "les_anime": (c_int, [H, c_int, c_int, c_int, ctypes.c_char_p, ctypes.c_char_p]),
+    "les_tiled_create": (c_int, [ctypes.POINTER(LesConfig), ctypes.POINTER(c_void_p)]),
+    "les_tiled_destroy": (c_int, [c_void_p]),
+    "les_tiled_last_error": (ctypes.c_char_p, [c_void_p]),
+    "les_tiled_sync": (c_int, [c_void_p]),
+    "les_tiled_upload": (c_int, [c_void_p, c_int, c_void_p]),
+    "les_tiled_download": (c_int, [c_void_p, c_int, c_void_p]),
+    "les_tiled_init_coefficients": (c_int, [c_void_p]),
+    "les_tiled_ifdata": (c_int, [c_void_p]),
+    "les_tiled_step": (c_int, [c_void_p, c_int]),
     "les_anime_arrays": (c_int, [c_int] * 6 + [c_void_p] * 7 + [ctypes.c_char_p] * 2),
     "les_upload": (c_int, [H, c_int, c_void_p]),
     "les_download": (c_int, [H, c_int, c_void_p]),

@@ -1376,3 +1376,416 @@ extern "C" int les_finalise_LES_kernel(void) {
   g_default = nullptr;
   return LES_OK;
 }
+
+// ------------------------------------------------------------------------------------------
+// One domain cut into sub_i x sub_j tiles on one device.  Each tile is an ordinary handle of
+// (ip/sub_i, jp/sub_j, kp) cells with the whole domain's arrays sliced into it (halos
+// included).  A step runs the stages tile by tile on one stream; after a stage, the faces a
+// tile shares with a neighbour (and the periodic j faces) are copied between the tiles'
+// arrays (the halo exchange, here a device copy since every tile is on the same GPU), the
+// domain's own faces keep the reference's boundary rules (Edges), and the sums (uout, rhsav,
+// pav) combine the tiles' exact partials, so the tiled step is bitwise the single-domain step.
+// Pressure uses the in-place colour sweep (k_sor_colour) with exchanges after every colour.
+// ------------------------------------------------------------------------------------------
+struct les_tiled {
+  les_config cfg;  // the whole domain
+  int si = 1, sj = 1, ipl = 0, jpl = 0;
+  std::vector<les_handle*> t;  // tile (a, b) at t[a + si * b]
+  std::vector<Edges> e;
+  int64_t* partials = nullptr;
+  std::vector<int> red_off;  // first partial of each tile
+  int red_total = 0;
+  std::vector<float> gdx1, gdy1, gdzn;  // the whole domain's grid, for area = pco
+  std::string err;
+};
+
+namespace {
+
+int tfail(les_tiled* T, int code, const std::string& msg) {
+  if (T) T->err = msg;
+  return code;
+}
+
+// axis of a 1-D or 2-D field along which tiles cut it (0 i, 1 j, 2 k)
+int tiled_axis(int f) {
+  switch (f) {
+    case LES_DX1: case LES_DXS: case LES_CN2L: case LES_CN2S: return 0;
+    case LES_DY1: case LES_DYS: case LES_CN3L: case LES_CN3S: case LES_WIND_PROFILE: return 1;
+    default: return 2;
+  }
+}
+
+bool is_3d(int f) { return f <= LES_DMASK1 || f == LES_CN1; }
+
+ArrView view_of(les_handle* h, int f) {
+  int lo[3], hi[3];
+  les_field_bounds(f, h->d.ip, h->d.jp, h->d.kp, lo, hi);
+  return ArrView{h->fld[f], lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1};
+}
+
+// exchange of the faces of `fields`: dir 0 copies the i faces (rows 1..jpl) between
+// neighbours along i; dir 1 the j faces (columns 0..ipl+1, so corners follow the i faces),
+// wrapping around in j when `periodic`.
+void tiled_exchange(les_tiled* T, hipStream_t st, std::initializer_list<int> fields, int dir,
+                    bool periodic = true) {
+  ExchangePlan plan{};
+  auto flush = [&]() {
+    launch_exchange(st, plan);
+    plan.nbox = 0;
+  };
+  auto add = [&](const BoxCopy& b) {
+    plan.box[plan.nbox++] = b;
+    if (plan.nbox == EXCH_MAX) flush();
+  };
+  for (int f : fields) {
+    for (int b = 0; b < T->sj; ++b)
+      for (int a = 0; a < T->si; ++a) {
+        les_handle* s = T->t[a + T->si * b];
+        const ArrView vs = view_of(s, f);
+        const int klo = vs.lo2;
+        int lo[3], hi[3];
+        les_field_bounds(f, T->ipl, T->jpl, T->cfg.kp, lo, hi);
+        const int nk = hi[2] - lo[2] + 1;
+        // only the halo columns/rows the array has (f, g, h end at ip, jp)
+        const bool has_e = hi[0] >= T->ipl + 1, has_w = lo[0] <= 0;
+        const bool has_n = hi[1] >= T->jpl + 1, has_s = lo[1] <= 0;
+        const int i0 = std::max(0, lo[0]), i1 = std::min(T->ipl + 1, hi[0]);
+        if (dir == 0) {
+          if (a + 1 >= T->si) continue;
+          les_handle* r = T->t[a + 1 + T->si * b];
+          const ArrView vr = view_of(r, f);
+          // right's column 1 -> my column ipl+1; my column ipl -> right's column 0
+          if (has_e) add(BoxCopy{vr, vs, 1, 1, T->ipl + 1, 1, klo, 1, T->jpl, nk});
+          if (has_w) add(BoxCopy{vs, vr, T->ipl, 1, 0, 1, klo, 1, T->jpl, nk});
+        } else {
+          const int bn = (b + 1) % T->sj;
+          if (b + 1 == T->sj && !periodic) continue;
+          les_handle* r = T->t[a + T->si * bn];
+          const ArrView vr = view_of(r, f);
+          // north's row 1 -> my row jpl+1; my row jpl -> north's row 0 (columns i0..i1)
+          if (has_n) add(BoxCopy{vr, vs, i0, 1, i0, T->jpl + 1, klo, i1 - i0 + 1, 1, nk});
+          if (has_s) add(BoxCopy{vs, vr, i0, T->jpl, i0, 0, klo, i1 - i0 + 1, 1, nk});
+        }
+      }
+  }
+  flush();
+}
+
+int tiled_write_area(les_tiled* T) {
+  if (T->gdx1.empty() || T->gdy1.empty() || T->gdzn.empty()) return LES_OK;
+  // press.f95:67-74: the whole domain's sequential FP32 sum (the single-domain handle's value)
+  float area = 0.0f;
+  double area64 = 0.0;
+  for (int k = 1; k <= T->cfg.kp; ++k)
+    for (int j = 1; j <= T->cfg.jp; ++j)
+      for (int i = 1; i <= T->cfg.ip; ++i) {
+        const float v = T->gdx1[i + 1] * T->gdy1[j] * T->gdzn[k + 1];
+        area = area + v;
+        area64 += (double)v;
+      }
+  les_handle* h0 = T->t[0];
+  HIPCHK(h0, hipStreamSynchronize(h0->st));
+  HIPCHK(h0, hipMemcpy(&h0->sc->area, &area, sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h0, hipMemcpy(&h0->sc->pco, &area, sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h0, hipMemcpy(&h0->sc->area64, &area64, sizeof(double), hipMemcpyHostToDevice));
+  return LES_OK;
+}
+
+std::vector<DevScalars*> tile_scalars(les_tiled* T, bool east_only) {
+  std::vector<DevScalars*> v;
+  for (int b = 0; b < T->sj; ++b)
+    for (int a = 0; a < T->si; ++a)
+      if (!east_only || a == T->si - 1) v.push_back(T->t[a + T->si * b]->sc);
+  return v;
+}
+
+void tiled_bcast(les_tiled* T, hipStream_t st, size_t off) {
+  std::vector<DevScalars*> all = tile_scalars(T, false);
+  launch_scalar_bcast(st, all[0], all.data() + 1, (int)all.size() - 1, (int)off);
+}
+
+// velnw + bondv1 (inflow, uout, outflow, periodic and top/bottom faces) on every tile
+void tiled_bondv1(les_tiled* T, hipStream_t st, int do_velnw, int inflow_n) {
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    // n == 1 fills u with the profile everywhere; later steps set the inflow planes only
+    const int in = (inflow_n == 1 || T->e[q].w) ? inflow_n : 0;
+    launch_velnw_inflow(st, h->d, grid_of(h), fields_of(h), h->ph, do_velnw, in);
+  }
+  std::vector<DevScalars*> east = tile_scalars(T, true), all = tile_scalars(T, false);
+  for (size_t q = 0; q < T->t.size(); ++q)
+    if (T->e[q].e) {
+      les_handle* h = T->t[q];
+      launch_uout(st, h->d, fields_of(h), h->sc, reinterpret_cast<float*>(h->rb.partials));
+    }
+  launch_uout_combine(st, east.data(), (int)east.size(), all.data(), (int)all.size());
+  for (size_t q = 0; q < T->t.size(); ++q)
+    if (T->e[q].e) {
+      les_handle* h = T->t[q];
+      launch_bondv1_uvw(st, h->d, grid_of(h), fields_of(h), h->ph, h->sc, T->e[q], -1);
+    }
+  tiled_exchange(T, st, {LES_U, LES_V, LES_W}, 0);
+  tiled_exchange(T, st, {LES_U, LES_V, LES_W}, 1);
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    launch_bondv1_uvw(st, h->d, grid_of(h), fields_of(h), h->ph, h->sc, T->e[q], 0);
+  }
+}
+
+// boundp1 (the domain's i faces), the exchange, boundp2
+void tiled_boundp(les_tiled* T, hipStream_t st, int check_done) {
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    launch_boundp1(st, h->d, fields_of(h), h->sc, check_done, T->e[q]);
+  }
+  tiled_exchange(T, st, {LES_P}, 0);
+  tiled_exchange(T, st, {LES_P}, 1);
+  for (les_handle* h : T->t) launch_boundp2(st, h->d, fields_of(h), h->sc, check_done);
+}
+
+// velfg + feedbf, les (calc_sm, boundsm), visc (+ adam)
+void tiled_velfg_les(les_tiled* T, hipStream_t st, int do_adam) {
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    launch_velfg(st, h->d, grid_of(h), fields_of(h), h->ph, 1, 1, T->e[q]);
+  }
+  const std::initializer_list<int> diu = {LES_DIU1, LES_DIU2, LES_DIU3, LES_DIU4, LES_DIU5,
+                                          LES_DIU6, LES_DIU7, LES_DIU8, LES_DIU9};
+  tiled_exchange(T, st, diu, 0);
+  tiled_exchange(T, st, diu, 1);
+  for (les_handle* h : T->t) launch_calc_sm(st, h->d, grid_of(h), fields_of(h), h->ph);
+  for (size_t q = 0; q < T->t.size(); ++q)
+    launch_boundsm_tiled(st, T->t[q]->d, fields_of(T->t[q]), T->e[q], 0);
+  tiled_exchange(T, st, {LES_SM}, 0);
+  for (size_t q = 0; q < T->t.size(); ++q)
+    launch_boundsm_tiled(st, T->t[q]->d, fields_of(T->t[q]), T->e[q], 1);
+  tiled_exchange(T, st, {LES_SM}, 1, false);  // boundsm's j faces are copies, not periodic
+  for (size_t q = 0; q < T->t.size(); ++q)
+    launch_boundsm_tiled(st, T->t[q]->d, fields_of(T->t[q]), T->e[q], 2);
+  for (les_handle* h : T->t) launch_calc_visc_adam(st, h->d, grid_of(h), fields_of(h), 1, do_adam);
+}
+
+void tiled_press(les_tiled* T, hipStream_t st) {
+  tiled_exchange(T, st, {LES_F, LES_G}, 0);
+  tiled_exchange(T, st, {LES_F, LES_G}, 1);
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    RedBuf rb{T->partials + (long)T->red_off[q] * NLIMB, 0};
+    launch_rhs(st, h->d, grid_of(h), fields_of(h), h->ph, rb, h->sc, T->e[q], 0);
+  }
+  launch_rhsav_finalize(st, T->partials, T->red_total, T->t[0]->sc);
+  tiled_bcast(T, st, offsetof(DevScalars, rhsav));
+  for (size_t q = 1; q < T->t.size(); ++q) launch_sor_reset(st, T->t[q]->sc);
+  for (les_handle* h : T->t) launch_rhs_adjust(st, h->d, fields_of(h), h->sc);
+  for (int l = 1; l <= T->cfg.nmaxp; ++l) {
+    for (int nrd = 0; nrd <= 1; ++nrd) {
+      for (les_handle* h : T->t)
+        launch_sor_colour(st, h->d, grid_of(h), fields_of(h), h->ph, nrd, 0, h->rb, 0, h->sc);
+      for (size_t q = 0; q < T->t.size(); ++q) {
+        les_handle* h = T->t[q];
+        launch_boundp1(st, h->d, fields_of(h), h->sc, 1, T->e[q]);
+      }
+      tiled_exchange(T, st, {LES_P}, 0);
+      tiled_exchange(T, st, {LES_P}, 1);
+    }
+    for (les_handle* h : T->t) launch_boundp2(st, h->d, fields_of(h), h->sc, 1);
+  }
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    RedBuf rb{T->partials + (long)T->red_off[q] * NLIMB, 0};
+    launch_pav(st, h->d, grid_of(h), fields_of(h), rb, h->sc, 0);
+  }
+  launch_pav_finalize(st, T->partials, T->red_total, T->t[0]->sc);
+  tiled_bcast(T, st, offsetof(DevScalars, pav));
+  for (les_handle* h : T->t) launch_adj(st, h->d, fields_of(h), h->sc, 0);
+  tiled_boundp(T, st, 0);
+}
+
+int tiled_check(les_tiled* T) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return tfail(T, LES_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return LES_OK;
+}
+
+}  // namespace
+
+extern "C" int les_tiled_create(const les_config* cfg, les_tiled** out) {
+  if (!cfg || !out) return LES_E_INVALID;
+  *out = nullptr;
+  const int si = cfg->sub_i, sj = cfg->sub_j;
+  if (si < 1 || sj < 1 || si * sj > 8 || cfg->ip % si || cfg->jp % sj) return LES_E_INVALID;
+  const int ipl = cfg->ip / si, jpl = cfg->jp / sj;
+  // red-black colours: a tile's local (i+j+k) parity must be the domain's
+  if ((si > 1 && ipl % 2) || (sj > 1 && jpl % 2) || ipl < 2 || jpl < 2) return LES_E_INVALID;
+  if (cfg->precision != LES_PREC_FP32 || cfg->pjuge > 0.0f) return LES_E_INVALID;
+  les_tiled* T = new les_tiled();
+  T->cfg = *cfg;
+  T->si = si;
+  T->sj = sj;
+  T->ipl = ipl;
+  T->jpl = jpl;
+  les_config c = *cfg;
+  c.ip = ipl;
+  c.jp = jpl;
+  c.sub_i = 1;
+  c.sub_j = 1;
+  for (int b = 0; b < sj; ++b)
+    for (int a = 0; a < si; ++a) {
+      les_handle* h = nullptr;
+      const int r = les_create(&c, &h);
+      if (r) {
+        les_tiled_destroy(T);
+        return r;
+      }
+      T->t.push_back(h);
+      T->e.push_back(Edges{a == 0, a == si - 1, b == 0, b == sj - 1, 1});
+      T->red_off.push_back(T->red_total);
+      T->red_total += reduction_blocks(h->d);
+    }
+  if (hipMalloc(&T->partials, (size_t)T->red_total * NLIMB * sizeof(int64_t)) != hipSuccess) {
+    les_tiled_destroy(T);
+    return LES_E_NOMEM;
+  }
+  *out = T;
+  return LES_OK;
+}
+
+extern "C" int les_tiled_destroy(les_tiled* T) {
+  if (!T) return LES_E_INVALID;
+  for (les_handle* h : T->t) les_destroy(h);
+  if (T->partials) hipFree(T->partials);
+  delete T;
+  return LES_OK;
+}
+
+extern "C" const char* les_tiled_last_error(const les_tiled* T) {
+  if (!T) return "";
+  if (!T->err.empty()) return T->err.c_str();
+  for (les_handle* h : T->t)
+    if (!h->err.empty()) return h->err.c_str();
+  return "";
+}
+
+extern "C" int les_tiled_sync(les_tiled* T) {
+  if (!T) return LES_E_INVALID;
+  for (les_handle* h : T->t) {
+    const int r = les_sync(h);
+    if (r) return r;
+  }
+  return LES_OK;
+}
+
+// whole-domain array -> every tile's slice (halos included)
+extern "C" int les_tiled_upload(les_tiled* T, int field, const float* host) {
+  if (!T || !host || field < 0 || field >= LES_NFIELDS) return LES_E_INVALID;
+  int glo[3], ghi[3], llo[3], lhi[3];
+  const int64_t gn = les_field_bounds(field, T->cfg.ip, T->cfg.jp, T->cfg.kp, glo, ghi);
+  les_field_bounds(field, T->ipl, T->jpl, T->cfg.kp, llo, lhi);
+  if (gn <= 0) return LES_E_INVALID;
+  const long g0 = ghi[0] - glo[0] + 1, g1 = ghi[1] - glo[1] + 1;
+  for (int b = 0; b < T->sj; ++b)
+    for (int a = 0; a < T->si; ++a) {
+      les_handle* h = T->t[a + T->si * b];
+      const int io = a * T->ipl, jo = b * T->jpl;
+      std::vector<float> loc((size_t)h->cnt[field]);
+      if (is_3d(field)) {
+        const long l0 = lhi[0] - llo[0] + 1, l1 = lhi[1] - llo[1] + 1;
+        for (int k = llo[2]; k <= lhi[2]; ++k)
+          for (int j = llo[1]; j <= lhi[1]; ++j)
+            for (int i = llo[0]; i <= lhi[0]; ++i)
+              loc[(i - llo[0]) + l0 * ((j - llo[1]) + l1 * (k - llo[2]))] =
+                  host[(i + io - glo[0]) + g0 * ((j + jo - glo[1]) + g1 * (k - glo[2]))];
+      } else if (field == LES_WIND_PROFILE) {  // (1:jp, 1:kp)
+        for (int k = 0; k < T->cfg.kp; ++k)
+          for (int j = 0; j < T->jpl; ++j)
+            loc[j + (size_t)T->jpl * k] = host[(j + jo) + (size_t)T->cfg.jp * k];
+      } else {
+        const int ax = tiled_axis(field), off = ax == 0 ? io : (ax == 1 ? jo : 0);
+        for (int i = llo[0]; i <= lhi[0]; ++i) loc[i - llo[0]] = host[i + off - glo[0]];
+      }
+      const int r = les_upload(h, field, loc.data());
+      if (r) return r;
+    }
+  if (field == LES_DX1) T->gdx1.assign(host, host + gn);
+  if (field == LES_DY1) T->gdy1.assign(host, host + gn);
+  if (field == LES_DZN) T->gdzn.assign(host, host + gn);
+  if (field == LES_DX1 || field == LES_DY1 || field == LES_DZN) return tiled_write_area(T);
+  return LES_OK;
+}
+
+// every tile's interior, and the halos on the domain's faces, -> the whole-domain array
+extern "C" int les_tiled_download(les_tiled* T, int field, float* host) {
+  if (!T || !host || field < 0 || field >= LES_NFIELDS) return LES_E_INVALID;
+  int glo[3], ghi[3], llo[3], lhi[3];
+  les_field_bounds(field, T->cfg.ip, T->cfg.jp, T->cfg.kp, glo, ghi);
+  les_field_bounds(field, T->ipl, T->jpl, T->cfg.kp, llo, lhi);
+  const long g0 = ghi[0] - glo[0] + 1, g1 = ghi[1] - glo[1] + 1;
+  int r = les_tiled_sync(T);
+  if (r) return r;
+  for (int b = 0; b < T->sj; ++b)
+    for (int a = 0; a < T->si; ++a) {
+      les_handle* h = T->t[a + T->si * b];
+      const int io = a * T->ipl, jo = b * T->jpl;
+      std::vector<float> loc((size_t)h->cnt[field]);
+      if ((r = les_download(h, field, loc.data()))) return r;
+      if (is_3d(field)) {
+        const long l0 = lhi[0] - llo[0] + 1, l1 = lhi[1] - llo[1] + 1;
+        for (int k = llo[2]; k <= lhi[2]; ++k)
+          for (int j = llo[1]; j <= lhi[1]; ++j) {
+            if ((j < 1 && b > 0) || (j > T->jpl && b < T->sj - 1)) continue;
+            for (int i = llo[0]; i <= lhi[0]; ++i) {
+              if ((i < 1 && a > 0) || (i > T->ipl && a < T->si - 1)) continue;
+              host[(i + io - glo[0]) + g0 * ((j + jo - glo[1]) + g1 * (k - glo[2]))] =
+                  loc[(i - llo[0]) + l0 * ((j - llo[1]) + l1 * (k - llo[2]))];
+            }
+          }
+      } else if (field == LES_WIND_PROFILE) {
+        for (int k = 0; k < T->cfg.kp; ++k)
+          for (int j = 0; j < T->jpl; ++j)
+            host[(j + jo) + (size_t)T->cfg.jp * k] = loc[j + (size_t)T->jpl * k];
+      } else {
+        const int ax = tiled_axis(field), off = ax == 0 ? io : (ax == 1 ? jo : 0);
+        for (int i = llo[0]; i <= lhi[0]; ++i) host[i + off - glo[0]] = loc[i - llo[0]];
+      }
+    }
+  return LES_OK;
+}
+
+extern "C" int les_tiled_ifdata(les_tiled* T) {
+  if (!T) return LES_E_INVALID;
+  hipStream_t st = T->t[0]->st;
+  int r = les_tiled_sync(T);
+  if (r) return r;
+  // ifdata.f95:258-273 as les_ifdata runs it
+  tiled_bondv1(T, st, 0, 1);
+  tiled_boundp(T, st, 0);
+  tiled_velfg_les(T, st, 0);
+  return tiled_check(T);
+}
+
+extern "C" int les_tiled_step(les_tiled* T, int n) {
+  if (!T) return LES_E_INVALID;
+  hipStream_t st = T->t[0]->st;
+  tiled_bondv1(T, st, 1, n < 1 ? 2 : n);
+  tiled_velfg_les(T, st, 1);
+  tiled_press(T, st);
+  return tiled_check(T);
+}
+
+// init.f95:82-107 and les.f95:36 on every tile.  The coefficients are pointwise functions of
+// the grid, which the tiles hold sliced with their halos, except delx1, which uses dx1(0) and
+// dy1(0) of the whole domain.
+extern "C" int les_tiled_init_coefficients(les_tiled* T) {
+  if (!T) return LES_E_INVALID;
+  if (T->gdx1.empty() || T->gdy1.empty() || T->gdzn.empty())
+    return tfail(T, LES_E_STATE, "tiled_init_coefficients: upload dx1, dy1 and dzn first");
+  int r;
+  std::vector<float> delx1(T->cfg.kp);
+  for (int k = 1; k <= T->cfg.kp; ++k)
+    delx1[k - 1] = powf(T->gdx1[1] * T->gdy1[0] * T->gdzn[k + 1], 1.0f / 3.0f);
+  for (les_handle* h : T->t) {
+    if ((r = les_init_coefficients(h))) return r;
+    if ((r = les_upload(h, LES_DELX1, delx1.data()))) return r;
+  }
+  return tiled_write_area(T);
+}

@@ -23,6 +23,7 @@ struct DevScalars {
   // mixed precision (FP64 reductions and iterate)
   double rhsav64, area64, pav64, rhs_l2;
   int refine_done;
+  float uout_max, uout_min;  // bondv1.f95:71-72 before the average (max seeded and min seeded 0)
 };
 
 struct Physics {
@@ -61,29 +62,71 @@ struct RedBuf {
   int max_blocks;
 };
 
+// Which faces of a (sub)domain are faces of the whole domain.  A tile of a domain cut into
+// subdomains on one device (les_tiled) takes its internal faces, and the periodic j faces, from
+// its neighbours by copies after the stage instead of applying the single-domain rules there.
+struct Edges {
+  int w, e;   // the west (i = 0) / east (i = ip+1) face is the domain's inflow / outflow face
+  int s, n;   // the south (j = 0) / north (j = jp+1) face is the domain's j = 0 / jm+1 face
+  int tiled;  // 0: one domain (periodic j copies made in place); 1: faces come by exchange
+};
+constexpr Edges EDGES_ALL{1, 1, 1, 1, 0};
+
+// box copy between two arrays of the same field (the halo exchange of les_tiled): element
+// (i,j,k) of an array is at p[(i-lo0) + n0*((j-lo1) + n1*(k-lo2))]
+struct ArrView {
+  float* p;
+  int lo0, lo1, lo2, n0, n1;
+};
+struct BoxCopy {
+  ArrView src, dst;
+  int si, sj, di, dj, k0;  // first source / destination (i, j), first level
+  int ni, nj, nk;          // box extents
+};
+constexpr int EXCH_MAX = 32;  // boxes per exchange launch
+struct ExchangePlan {
+  int nbox;
+  BoxCopy box[EXCH_MAX];
+};
+void launch_exchange(hipStream_t st, const ExchangePlan& plan);
+// copies the float at byte offset off of src into the same place of up to 8 DevScalars
+void launch_scalar_bcast(hipStream_t st, const DevScalars* src, DevScalars* const* dst, int ndst,
+                         int off);
+// uout of les_tiled: (max(0, max of the uout_max) + min(0, min of the uout_min)) / 2 over the
+// east tiles, written to every tile
+void launch_uout_combine(hipStream_t st, DevScalars* const* east, int neast,
+                         DevScalars* const* all, int nall);
+void launch_rhsav_finalize(hipStream_t st, const int64_t* partials, int nb, DevScalars* sc);
+void launch_pav_finalize(hipStream_t st, const int64_t* partials, int nb, DevScalars* sc);
+// boundsm of a tile in three phases: 0 physical i faces, 1 physical j faces, 2 k planes
+void launch_boundsm_tiled(hipStream_t st, Dims d, Fields F, Edges e, int phase);
+
 // ---- launchers (les_kernels.hip); all asynchronous on 'st' ----
 void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
                          int inflow_n);
 void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc, float* part);
-void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc);
+void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc,
+                       Edges e = EDGES_ALL, int do_outflow = 1);
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
-                  int do_feedbf);
+                  int do_feedbf, Edges e = EDGES_ALL);
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph);
 void launch_boundsm(hipStream_t st, Dims d, Fields F);
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc,
                            int do_adam);
 void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
-                DevScalars* sc);
+                DevScalars* sc, Edges e = EDGES_ALL, int finalize = 1);
 void launch_bondfg(hipStream_t st, Dims d, Fields F);
 void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc);
 void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
                        int want_resid, RedBuf rb, int blk_offset, DevScalars* sc);
 void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, int nblocks, Physics ph,
                                DevScalars* sc);
-void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done);
+void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done,
+                    Edges e = EDGES_ALL);
 void launch_boundp2(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done);
 void launch_sor_reset(hipStream_t st, DevScalars* sc);
-void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalars* sc);
+void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalars* sc,
+                int finalize = 1);
 void launch_adj(hipStream_t st, Dims d, Fields F, DevScalars* sc, int with_boundp);
 int sor_resid_blocks(Dims d);
 

@@ -129,6 +129,8 @@ __global__ __launch_bounds__(256) void k_uout(Dims d, Fields F, DevScalars* sc, 
   }
   block_maxmin(mx, mn, smax, smin);
   if (threadIdx.x == 0) {
+    sc->uout_max = mx;
+    sc->uout_min = mn;
     sc->uout = (mx + mn) / 2.0f;
     sc->uout_count = 0u;
   }
@@ -163,10 +165,14 @@ __global__ __launch_bounds__(256) void k_outflow(Dims d, Grid1D g, Fields F, Phy
 // although w starts at i = 0; in the column-major layout w(-1,j,k) is w(ip+1,j-1,k), so the
 // only cells it reaches beyond the i = im+1 column are w(ip+1,jp+1,-1) and w(ip+1,jp+1,km-1),
 // both zeroed here as well.
-__global__ __launch_bounds__(256) void k_bondv1_halo(Dims d, Fields F) {
+// A tile (e.tiled) has its j faces and internal i faces from the exchange before this kernel:
+// it makes no periodic copies, takes the k planes from its own (exchanged) rows, and writes the
+// w(-1,-1,...) cells only where they lie in the whole domain (the tile at its east and north
+// faces).
+__global__ __launch_bounds__(256) void k_bondv1_halo(Dims d, Fields F, Edges e) {
   const int im = d.ip, jm = d.jp, km = d.kp;
   const int ni = im + 2;
-  const long n1 = (long)ni * km * 2;               // j faces, k = 1..km, two faces
+  const long n1 = e.tiled ? 0 : (long)ni * km * 2;  // j faces, k = 1..km, two faces
   const long n2 = (long)ni * (jm + 3) * 2;         // k faces, j = -1..jm+1, two planes
   long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t < n1) {
@@ -190,10 +196,12 @@ __global__ __launch_bounds__(256) void k_bondv1_halo(Dims d, Fields F) {
   const int top = (int)(t / ((long)ni * (jm + 3)));
   // w: zero planes k = 0 and k = km (j = -1..jm+1)
   F.w[d.W(i, j, top ? km : 0)] = 0.0f;
-  if (!top && i == im + 1 && j == jm + 1) F.w[d.W(im + 1, jm + 1, -1)] = 0.0f;  // w(-1,-1,0)
+  const bool corner = i == im + 1 && j == jm + 1 && (!e.tiled || (e.e && e.n));
+  if (!top && corner) F.w[d.W(im + 1, jm + 1, -1)] = 0.0f;  // w(-1,-1,0)
+  if (top && corner && e.tiled) F.w[d.W(im + 1, jm + 1, km - 1)] = 0.0f;  // w(-1,-1,km)
   if (j < 0) return;
   // u, v: k = 0 antisymmetric, k = km+1 copy, from the periodic-completed rows
-  const int js = (j == 0) ? jm : ((j == jm + 1) ? 1 : j);
+  const int js = e.tiled ? j : ((j == 0) ? jm : ((j == jm + 1) ? 1 : j));
   if (top) {
     F.u[d.U(i, j, km + 1)] = F.u[d.U(i, js, km)];
     F.v[d.U(i, j, km + 1)] = F.v[d.U(i, js, km)];
@@ -203,11 +211,16 @@ __global__ __launch_bounds__(256) void k_bondv1_halo(Dims d, Fields F) {
   }
 }
 
-void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc) {
+// do_outflow = 1: outflow then the halo kernel; 0: the halo kernel only; -1: outflow only
+void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc,
+                       Edges e, int do_outflow) {
   int n = d.jp * d.kp;
-  hipLaunchKernelGGL(k_outflow, dim3((n + 255) / 256), dim3(256), 0, st, d, g, F, ph, sc);
-  long nh = (long)(d.ip + 2) * d.kp * 2 + (long)(d.ip + 2) * (d.jp + 3) * 2;
-  hipLaunchKernelGGL(k_bondv1_halo, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, d, F);
+  if (do_outflow != 0)
+    hipLaunchKernelGGL(k_outflow, dim3((n + 255) / 256), dim3(256), 0, st, d, g, F, ph, sc);
+  if (do_outflow < 0) return;
+  long nh = (e.tiled ? 0L : (long)(d.ip + 2) * d.kp * 2) + (long)(d.ip + 2) * (d.jp + 3) * 2;
+  hipLaunchKernelGGL(k_bondv1_halo, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, d, F,
+                     e);
 }
 
 // ==========================================================================================
@@ -321,7 +334,7 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
 constexpr int VX = 64, VY = 8;
 template <bool P2>
 __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Physics ph,
-                                                  int do_feedbf, StripGrid sg) {
+                                                  int do_feedbf, StripGrid sg, Edges e) {
   __shared__ float sy[6][VY][VX];  // cov2 diu2 cov5 diu5 cov8 diu8 of each row
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;
@@ -332,9 +345,11 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
   const int k = 1 + bz;
   const bool hk = k <= km - 1;
   VelCtx<P2> c{d, g, F.u, F.v, F.w};
-  // i-face quantities at (i, j, k)
+  // i-face quantities at (i, j, k); a tile whose east face is internal computes them at im+1
+  // too, from its exchanged halo (the whole domain's values there)
+  const int imx = e.e ? im : im + 1;
   float x1 = 0.0f, xc1 = 0.0f, x4 = 0.0f, xc4 = 0.0f, x7 = 0.0f, xc7 = 0.0f;
-  if (y < VY - 1 && i <= im && j <= jm) {
+  if (y < VY - 1 && i <= imx && j <= jm) {
     x1 = c.diu1(i, j, k);
     xc1 = c.cov1(i, j, k);
     x4 = c.diu4(i, j, k);
@@ -344,9 +359,10 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
       xc7 = c.cov7(i, j, k);
     }
   }
-  // j-face quantities at (i, jq, k); row jm+1 is the periodic copy of row 1
+  // j-face quantities at (i, jq, k); row jm+1 is the periodic copy of row 1 (a tile computes
+  // it from its exchanged halo rows)
   if (x < VX - 1 && i <= im && j <= jm + 1) {
-    const int jq = (j == jm + 1) ? 1 : j;
+    const int jq = (j == jm + 1 && !e.tiled) ? 1 : j;
     sy[0][y][x] = c.cov2(i, jq, k);
     sy[1][y][x] = c.diu2(i, jq, k);
     sy[2][y][x] = c.cov5(i, jq, k);
@@ -361,7 +377,7 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
   float x7n = __shfl_down(x7, 1), xc7n = __shfl_down(xc7, 1);
   __syncthreads();
   if (x >= VX - 1 || y >= VY - 1 || i > im || j > jm) return;
-  if (i == im) {  // vel2.f95:125-165: the i-face quantities at im+1 are copies of im
+  if (i == im && e.e) {  // vel2.f95:125-165: the i-face quantities at im+1 are copies of im
     x1n = x1;
     xc1n = xc1;
     x4n = x4;
@@ -439,19 +455,21 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
   Buf(F.f, d.nF()).st(ifg, fv);
   Buf(F.g, d.nF()).st(ifg, gv);
   Buf(F.h, d.nF()).st(ifg, hv);
-  store_diu(d, F, i, j, k, x1, y2, z3, x4, y5, z6, x7, y8, z9, z3n, z6n);
+  store_diu(d, F, i, j, k, x1, y2, z3, x4, y5, z6, x7, y8, z9, z3n, z6n, e);
 }
 
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
-                  int do_feedbf) {
+                  int do_feedbf, Edges e) {
   if (do_velfg) {
+    // (the last column of blocks always has a lane at i = im+1: lane 63 of a block sits one
+    // cell past its last output)
     const StripGrid sg = strip_grid((d.ip + VX - 2) / (VX - 1), (d.jp + VY - 2) / (VY - 1), d.kp);
     if (g.p2xy)
       hipLaunchKernelGGL(k_velfg_xy<true>, strip_dim(sg), dim3(VX, VY), 0, st, d, g, F, ph,
-                         do_feedbf, sg);
+                         do_feedbf, sg, e);
     else
       hipLaunchKernelGGL(k_velfg_xy<false>, strip_dim(sg), dim3(VX, VY), 0, st, d, g, F, ph,
-                         do_feedbf, sg);
+                         do_feedbf, sg, e);
     return;
   }
   if (g.p2xy)
@@ -725,8 +743,10 @@ void launch_bondfg(hipStream_t st, Dims d, Fields F) {
 
 // ---- press.f95:49-75: bondfg + rhs + exact partial sums of dx1*dy1*dzn*rhs ----
 template <bool P2>
+// A tile reads f(0) and g(i,0) from its exchanged halo unless its west face is the domain's,
+// and makes bondFG's f and g copies only at the domain's faces (the exchange made the rest).
 __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics ph,
-                                             int64_t* partials, StripGrid sg) {
+                                             int64_t* partials, StripGrid sg, Edges e) {
   __shared__ unsigned long long lds[NLIMB];
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;  // padding: writes no partial
@@ -740,7 +760,7 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
   acc.init();
   if (i <= im && j <= jm) {
     const float dx1i = g.DX1(i), dy1j = g.DY1(j), dt = ph.dt;
-    const int jm1 = (j == 1) ? jm : j - 1;  // g(i,0,k) = g(i,jm,k) (bondFG)
+    const int jm1 = (j == 1 && !e.tiled) ? jm : j - 1;  // g(i,0,k) = g(i,jm,k) (bondFG)
     const Buf bu(F.u, d.nU()), bv(F.v, d.nU()), bw(F.w, d.nW());
     const Buf bf(F.f, d.nF()), bg(F.g, d.nF()), bh(F.h, d.nF()), brh(F.rhs, d.nR());
     // Operands of level k with bondFG's values (f(0) = f(1), g(0) = g(jm), h(0) = h(km) = 0)
@@ -759,7 +779,7 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
       const long ifg = d.F(i, j, k);
       t.fc = bf[ifg];
       const float f0 = bf[d.F(i - 1, j, k)];
-      t.fm = (i == 1) ? t.fc : f0;
+      t.fm = (i == 1 && e.w) ? t.fc : f0;
       t.gc = bg[ifg];
       t.gm = bg[d.F(i, jm1, k)];
       const float h0 = bh[ifg], h1 = bh[d.F(i, j, k - 1)];
@@ -782,8 +802,8 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
     }
     // bondFG: f(0,j,k) = f(1,j,k); g(i,0,k) = g(i,jm,k); h(i,j,0) = h(i,j,km) = 0
     for (int k = k0; k <= k1; ++k) {
-      if (i == 1) F.f[d.F(0, j, k)] = F.f[d.F(1, j, k)];
-      if (j == jm) F.g[d.F(i, 0, k)] = F.g[d.F(i, jm, k)];
+      if (i == 1 && e.w) F.f[d.F(0, j, k)] = F.f[d.F(1, j, k)];
+      if (j == jm && !e.tiled) F.g[d.F(i, 0, k)] = F.g[d.F(i, jm, k)];
       if (k == 1) F.h[d.F(i, j, 0)] = 0.0f;
       if (k == km) F.h[d.F(i, j, km)] = 0.0f;
     }
@@ -815,16 +835,21 @@ __global__ __launch_bounds__(256) void k_rhs_adjust(Dims d, Fields F, const DevS
 }
 
 void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
-                DevScalars* sc) {
+                DevScalars* sc, Edges e, int finalize) {
   const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, RED_KC);
   const int nb = sg.nbx * sg.nby * sg.nbz;
   if (g.p2xy)
     hipLaunchKernelGGL(k_rhs<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, rb.partials,
-                       sg);
+                       sg, e);
   else
     hipLaunchKernelGGL(k_rhs<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
-                       rb.partials, sg);
+                       rb.partials, sg, e);
+  if (!finalize) return;
   hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
+}
+
+void launch_rhsav_finalize(hipStream_t st, const int64_t* partials, int nb, DevScalars* sc) {
+  hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, partials, nb, sc);
 }
 
 void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc) {
@@ -914,8 +939,10 @@ void launch_sor_reset(hipStream_t st, DevScalars* sc) {
 
 // boundp.f95:23-44 (boundp1): i faces for j = 0..jm+1 then j faces for i = 0..im+1, at every
 // k = 0..km+1; composed so that every source is an interior (i,j) column.
+// A tile copies only at the domain's i faces and makes no periodic j copies (the exchange
+// after this kernel fills its internal faces and j faces).
 __global__ __launch_bounds__(256) void k_boundp1(Dims d, Fields F, const DevScalars* sc,
-                                                 int check_done) {
+                                                 int check_done, Edges e) {
   if (check_done && sc->sor_done) return;
   const int im = d.ip, jm = d.jp, km = d.kp;
   const long nface_i = (long)jm * (km + 2) * 2;       // a in {0,im+1}, b = 1..jm
@@ -926,11 +953,11 @@ __global__ __launch_bounds__(256) void k_boundp1(Dims d, Fields F, const DevScal
     const int c = (int)((t / jm) % (km + 2));
     const int side = (int)(t / ((long)jm * (km + 2)));
     const int a = side ? im + 1 : 0, as = side ? im : 1;
-    F.p[d.P(a, b, c)] = F.p[d.P(as, b, c)];
+    if (side ? e.e : e.w) F.p[d.P(a, b, c)] = F.p[d.P(as, b, c)];
     return;
   }
   t -= nface_i;
-  if (t >= nface_j) return;
+  if (t >= nface_j || e.tiled) return;
   const int a = (int)(t % (im + 2));
   const int c = (int)((t / (im + 2)) % (km + 2));
   const int side = (int)(t / ((long)(im + 2) * (km + 2)));
@@ -956,10 +983,11 @@ __global__ __launch_bounds__(256) void k_boundp2(Dims d, Fields F, const DevScal
     F.p[d.P(a, b, 0)] = F.p[d.P(a, b, 1)];
 }
 
-void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done) {
+void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done,
+                    Edges e) {
   long n = (long)d.jp * (d.kp + 2) * 2 + (long)(d.ip + 2) * (d.kp + 2) * 2;
   hipLaunchKernelGGL(k_boundp1, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F, sc,
-                     check_done);
+                     check_done, e);
 }
 
 void launch_boundp2(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done) {
@@ -998,11 +1026,17 @@ __global__ __launch_bounds__(FIN_THREADS) void k_pav_finalize(const int64_t* par
   if (threadIdx.x == 0) sc->pav = limbs_to_float(limbs) / sc->pco;
 }
 
-void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalars* sc) {
+void launch_pav(hipStream_t st, Dims d, Grid1D g, Fields F, RedBuf rb, DevScalars* sc,
+                int finalize) {
   dim3 grid = interior_grid(d, RED_KC);
   int nb = (int)(grid.x * grid.y * grid.z);
   hipLaunchKernelGGL(k_pav, grid, dim3(BX, BY), 0, st, d, g, F, rb.partials);
-  hipLaunchKernelGGL(k_pav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
+  if (finalize)
+    hipLaunchKernelGGL(k_pav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
+}
+
+void launch_pav_finalize(hipStream_t st, const int64_t* partials, int nb, DevScalars* sc) {
+  hipLaunchKernelGGL(k_pav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, partials, nb, sc);
 }
 
 // p -= pav on the interior; with_boundp also writes every mirror image of the cell
@@ -1183,6 +1217,110 @@ void launch_halo_write(hipStream_t st, float* const* arrays, int v_dim, int im, 
   long n = (long)max(im, jm) * km;
   hipLaunchKernelGGL(k_halo, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, hp, v_dim, im,
                      jm, km, h_w, 0, 1, const_cast<float*>(buffer));
+}
+
+// ==========================================================================================
+// Tiles of one domain on one device (les_tiled): face exchange and scalar combination
+// ==========================================================================================
+
+// Every box of the plan in one launch: blockIdx.y selects the box, threads run over its cells
+// (i fastest).  Source and destination boxes never overlap a box written in the same launch.
+__global__ __launch_bounds__(256) void k_exchange(ExchangePlan plan) {
+  const BoxCopy& b = plan.box[blockIdx.y];
+  const long n = (long)b.ni * b.nj * b.nk;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int a = (int)(t % b.ni);
+    const int c = (int)((t / b.ni) % b.nj);
+    const int z = b.k0 + (int)(t / ((long)b.ni * b.nj));
+    const ArrView& S = b.src;
+    const ArrView& D = b.dst;
+    const float v = S.p[(b.si + a - S.lo0) + (long)S.n0 * ((b.sj + c - S.lo1) + (long)S.n1 * (z - S.lo2))];
+    D.p[(b.di + a - D.lo0) + (long)D.n0 * ((b.dj + c - D.lo1) + (long)D.n1 * (z - D.lo2))] = v;
+  }
+}
+
+void launch_exchange(hipStream_t st, const ExchangePlan& plan) {
+  if (plan.nbox <= 0) return;
+  long most = 0;
+  for (int q = 0; q < plan.nbox; ++q)
+    most = std::max(most, (long)plan.box[q].ni * plan.box[q].nj * plan.box[q].nk);
+  const unsigned gx = (unsigned)std::min<long>((most + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_exchange, dim3(gx, plan.nbox), dim3(256), 0, st, plan);
+}
+
+struct ScalarDst {
+  DevScalars* p[8];
+};
+
+__global__ void k_scalar_bcast(const DevScalars* src, ScalarDst dst, int ndst, int off) {
+  const int q = threadIdx.x;
+  if (q >= ndst) return;
+  const float v = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(src) + off);
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(dst.p[q]) + off) = v;
+}
+
+void launch_scalar_bcast(hipStream_t st, const DevScalars* src, DevScalars* const* dst, int ndst,
+                         int off) {
+  ScalarDst d{};
+  for (int q = 0; q < ndst && q < 8; ++q) d.p[q] = dst[q];
+  hipLaunchKernelGGL(k_scalar_bcast, dim3(1), dim3(64), 0, st, src, d, std::min(ndst, 8), off);
+}
+
+// bondv1.f95:67-75 over the union of the east tiles' planes: max and min are exact and order
+// free, and every tile's max/min already includes the 0 seed of amax1/amin1
+__global__ void k_uout_combine(ScalarDst east, int neast, ScalarDst all, int nall) {
+  if (threadIdx.x != 0) return;
+  float mx = east.p[0]->uout_max, mn = east.p[0]->uout_min;
+  for (int q = 1; q < neast; ++q) {
+    mx = fmaxf(mx, east.p[q]->uout_max);
+    mn = fminf(mn, east.p[q]->uout_min);
+  }
+  const float u = (mx + mn) / 2.0f;
+  for (int q = 0; q < nall; ++q) all.p[q]->uout = u;
+}
+
+void launch_uout_combine(hipStream_t st, DevScalars* const* east, int neast,
+                         DevScalars* const* all, int nall) {
+  ScalarDst e{}, a{};
+  for (int q = 0; q < neast && q < 8; ++q) e.p[q] = east[q];
+  for (int q = 0; q < nall && q < 8; ++q) a.p[q] = all[q];
+  hipLaunchKernelGGL(k_uout_combine, dim3(1), dim3(64), 0, st, e, std::min(neast, 8), a,
+                     std::min(nall, 8));
+}
+
+// boundsm.f95:13-32 on a tile, in the order the reference composes it, with the exchanges
+// between the phases: phase 0 copies the i faces that are the domain's (rows j = 1..jm),
+// phase 1 the j faces that are the domain's (columns i = 0..im+1, after the i faces have been
+// exchanged), phase 2 the k planes of every column (a, b) of 0..im+1 x 0..jm+1 from the then
+// complete levels 1 and km.
+__global__ __launch_bounds__(256) void k_boundsm_tiled(Dims d, Fields F, Edges e, int phase) {
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (phase == 0) {
+    if (t >= 2L * jm * km) return;
+    const int b = 1 + (int)(t % jm), c = 1 + (int)((t / jm) % km), side = (int)(t / ((long)jm * km));
+    if (side ? e.e : e.w) F.sm[d.S(side ? im + 1 : 0, b, c)] = F.sm[d.S(side ? im : 1, b, c)];
+  } else if (phase == 1) {
+    if (t >= 2L * (im + 2) * km) return;
+    const int a = (int)(t % (im + 2)), c = 1 + (int)((t / (im + 2)) % km);
+    const int side = (int)(t / ((long)(im + 2) * km));
+    if (side ? e.n : e.s) F.sm[d.S(a, side ? jm + 1 : 0, c)] = F.sm[d.S(a, side ? jm : 1, c)];
+  } else {
+    if (t >= 2L * (im + 2) * (jm + 2)) return;
+    const int a = (int)(t % (im + 2)), b = (int)((t / (im + 2)) % (jm + 2));
+    const int top = (int)(t / ((long)(im + 2) * (jm + 2)));
+    if (top)
+      F.sm[d.S(a, b, km + 1)] = F.sm[d.S(a, b, km)];
+    else
+      F.sm[d.S(a, b, 0)] = -F.sm[d.S(a, b, 1)];
+  }
+}
+
+void launch_boundsm_tiled(hipStream_t st, Dims d, Fields F, Edges e, int phase) {
+  const long n = phase == 0 ? 2L * d.jp * d.kp
+                            : (phase == 1 ? 2L * (d.ip + 2) * d.kp : 2L * (d.ip + 2) * (d.jp + 2));
+  hipLaunchKernelGGL(k_boundsm_tiled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F,
+                     e, phase);
 }
 
 }  // namespace les

@@ -88,65 +88,70 @@ struct VelCtx {
 // (i = im+1 column copies, periodic j rows, the "--les" copies for k up to km+1).  d3top/d6top
 // are diu3/diu6 at km+1 (used when k == km).  Halo cells whose source vel2 never computes are
 // copied from memory; the kernel never writes those sources, so the copies are race-free.
+// A tile (e.tiled) makes the k copies and, at the domain's east face, the i = im+1 copies;
+// every copy involving the periodic j rows comes from the exchange after the stage instead.
 __device__ inline void store_diu(const Dims& d, const Fields& F, int i, int j, int k, float d1,
                                  float d2, float d3, float d4, float d5, float d6, float d7,
-                                 float d8, float d9, float d3top, float d6top) {
+                                 float d8, float d9, float d3top, float d6top,
+                                 Edges e = EDGES_ALL) {
   const int im = d.ip, jm = d.jp, km = d.kp;
   const bool hk = k <= km - 1;
+  const bool ie = (i == im) && e.e;  // i = im+1 copies
+  const bool pj = !e.tiled;          // periodic j copies
   // the interior stores through buffer views; the halo copies (few threads) through pointers
   Buf(F.diu1, d.nD15()).st(d.D15(i, j, k), d1);
-  if (i == im) F.diu1[d.D15(im + 1, j, k)] = d1;
+  if (ie) F.diu1[d.D15(im + 1, j, k)] = d1;
   Buf(F.diu5, d.nD15()).st(d.D15(i, j, k), d5);
-  if (j == jm) F.diu5[d.D15(i, 0, k)] = d5;
-  if (j == 1) F.diu5[d.D15(i, jm + 1, k)] = d5;
+  if (pj && j == jm) F.diu5[d.D15(i, 0, k)] = d5;
+  if (pj && j == 1) F.diu5[d.D15(i, jm + 1, k)] = d5;
   Buf(F.diu9, d.nD()).st(d.D(i, j, k), d9);
   Buf(F.diu2, d.nD()).st(d.D(i, j, k), d2);
-  if (j == jm) F.diu2[d.D(i, 0, k)] = d2;
-  if (j == 1) F.diu2[d.D(i, jm + 1, k)] = d2;
-  if (i == im) {
+  if (pj && j == jm) F.diu2[d.D(i, 0, k)] = d2;
+  if (pj && j == 1) F.diu2[d.D(i, jm + 1, k)] = d2;
+  if (ie) {
     F.diu2[d.D(im + 1, j, k)] = d2;
-    if (j == 1) F.diu2[d.D(im + 1, jm + 1, k)] = d2;
+    if (pj && j == 1) F.diu2[d.D(im + 1, jm + 1, k)] = d2;
     if (k == km) {
       F.diu2[d.D(im + 1, j, km + 1)] = F.diu2[d.D(im, j, km + 1)];
-      if (j == jm) F.diu2[d.D(im + 1, jm + 1, km + 1)] = F.diu2[d.D(im, jm + 1, km + 1)];
+      if (pj && j == jm) F.diu2[d.D(im + 1, jm + 1, km + 1)] = F.diu2[d.D(im, jm + 1, km + 1)];
     }
   }
   Buf(F.diu3, d.nD()).st(d.D(i, j, k), d3);
   if (k == km) F.diu3[d.D(i, j, km + 1)] = d3top;
-  if (i == im) {
+  if (ie) {
     F.diu3[d.D(im + 1, j, k)] = d3;
     if (k == km) F.diu3[d.D(im + 1, j, km + 1)] = d3top;
-    if (j == jm) {
+    if (pj && j == jm) {
       F.diu3[d.D(im + 1, jm + 1, k)] = F.diu3[d.D(im, jm + 1, k)];
       if (k == km) F.diu3[d.D(im + 1, jm + 1, km + 1)] = F.diu3[d.D(im, jm + 1, km + 1)];
     }
   }
   Buf(F.diu4, d.nD()).st(d.D(i, j, k), d4);
-  if (i == im) F.diu4[d.D(im + 1, j, k)] = d4;
-  if (j == jm) {
+  if (ie) F.diu4[d.D(im + 1, j, k)] = d4;
+  if (pj && j == jm) {
     F.diu4[d.D(i, 0, k)] = d4;
-    if (i == im) F.diu4[d.D(im + 1, 0, k)] = d4;
+    if (ie) F.diu4[d.D(im + 1, 0, k)] = d4;
     if (k == km) {
       F.diu4[d.D(i, 0, km + 1)] = F.diu4[d.D(i, jm, km + 1)];
-      if (i == im) F.diu4[d.D(im + 1, 0, km + 1)] = F.diu4[d.D(im + 1, jm, km + 1)];
+      if (ie) F.diu4[d.D(im + 1, 0, km + 1)] = F.diu4[d.D(im + 1, jm, km + 1)];
     }
   }
   Buf(F.diu6, d.nD()).st(d.D(i, j, k), d6);
   if (k == km) F.diu6[d.D(i, j, km + 1)] = d6top;
-  if (j == jm) {
+  if (pj && j == jm) {
     F.diu6[d.D(i, 0, k)] = d6;
     if (k == km) F.diu6[d.D(i, 0, km + 1)] = d6top;
-    if (i == im) {
+    if (ie) {
       F.diu6[d.D(im + 1, 0, k)] = F.diu6[d.D(im + 1, jm, k)];
       if (k == km) F.diu6[d.D(im + 1, 0, km + 1)] = F.diu6[d.D(im + 1, jm, km + 1)];
     }
   }
   if (hk) {
     Buf(F.diu7, d.nD()).st(d.D(i, j, k), d7);
-    if (i == im) F.diu7[d.D(im + 1, j, k)] = d7;
+    if (ie) F.diu7[d.D(im + 1, j, k)] = d7;
     Buf(F.diu8, d.nD()).st(d.D(i, j, k), d8);
-    if (j == jm) F.diu8[d.D(i, 0, k)] = d8;
-    if (j == 1) F.diu8[d.D(i, jm + 1, k)] = d8;
+    if (pj && j == jm) F.diu8[d.D(i, 0, k)] = d8;
+    if (pj && j == 1) F.diu8[d.D(i, jm + 1, k)] = d8;
   }
 }
 

@@ -336,3 +336,81 @@ class Ensemble:
 
 
 __all__ = ["LES", "Ensemble", "LESError", "LES_PREC_FP32", "LES_PREC_MIXED"]
+
+
+class TiledLES:
+    """One ip x jp x kp domain as sub_i x sub_j tiles on the GPU (les_tiled_*): the same
+    stages as LES, with face exchanges between the tiles; upload/download take whole-domain
+    arrays."""
+
+    def __init__(self, ip, jp, kp, sub_i=2, sub_j=2, dt=0.2, ro=1.1763, vn=None, alpha=-10.0,
+                 beta=-1.0, cs0=0.1, omega=1.0, nmaxp=50, device=0):
+        L = lib()
+        cfg = LesConfig()
+        L.les_default_config(ctypes.byref(cfg), ip, jp, kp)
+        cfg.dt, cfg.ro, cfg.alpha, cfg.beta, cfg.cs0 = dt, ro, alpha, beta, cs0
+        if vn is not None:
+            cfg.vn = vn
+        cfg.omega, cfg.pjuge, cfg.nmaxp, cfg.device = omega, 0.0, nmaxp, device
+        cfg.sub_i, cfg.sub_j = sub_i, sub_j
+        self.cfg = cfg
+        self.dims = (ip, jp, kp)
+        h = ctypes.c_void_p()
+        r = L.les_tiled_create(ctypes.byref(cfg), ctypes.byref(h))
+        if r != 0:
+            raise LESError(r, "les_tiled_create failed (%dx%dx%d as %dx%d)" % (ip, jp, kp, sub_i, sub_j))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().les_tiled_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, r, what):
+        if r != 0:
+            msg = lib().les_tiled_last_error(self.h)
+            raise LESError(r, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def upload(self, name, data):
+        ip, jp, kp = self.dims
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float32).reshape(-1))
+        if a.size != fl.size(name, ip, jp, kp):
+            raise ValueError("%s: expected %d values, got %d" % (name, fl.size(name, ip, jp, kp), a.size))
+        self._chk(lib().les_tiled_upload(self.h, fl.field_id(name), ptr(a)), "upload " + name)
+
+    def download(self, name):
+        ip, jp, kp = self.dims
+        a = np.empty(fl.size(name, ip, jp, kp), np.float32)
+        self._chk(lib().les_tiled_download(self.h, fl.field_id(name), ptr(a)), "download " + name)
+        return a
+
+    def load_case(self, case, coefficients=True):
+        for n in fl.GRID + fl.MASKS + ["wind_profile"]:
+            self.upload(n, case[n])
+        if coefficients:
+            self._chk(lib().les_tiled_init_coefficients(self.h), "init_coefficients")
+
+    def ifdata(self):
+        self._chk(lib().les_tiled_ifdata(self.h), "ifdata")
+
+    def step(self, n):
+        self._chk(lib().les_tiled_step(self.h, int(n)), "step")
+
+    def run(self, n0, nsteps):
+        for n in range(n0, n0 + nsteps):
+            self.step(n)
+
+    def sync(self):
+        self._chk(lib().les_tiled_sync(self.h), "sync")

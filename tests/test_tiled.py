@@ -1,0 +1,53 @@
+"""One domain as 2 x 2 (and 2 x 1, 1 x 2) tiles on one GPU (les_tiled) against one handle on
+the whole domain: after the init sequence and three steps every interior value, and every
+halo value on the domain's faces that the stages set, is bitwise the single-domain value.  The
+tiles exchange their shared faces and the periodic j faces between stages, keep the domain's
+inflow/outflow and boundary rules on the domain's faces, and combine the exact sums."""
+import numpy as np
+import pytest
+
+import common
+
+pytestmark = pytest.mark.gpu
+
+SOR = dict(omega=1.5, nmaxp=20)
+FIELDS = ["u", "v", "w", "p", "f", "g", "h", "usum", "vsum", "wsum", "fold", "gold", "hold",
+          "sm", "rhs"] + ["diu%d" % q for q in range(1, 10)]
+
+
+def _interior(name, a, dims):
+    return common.interior(name, a, *dims)
+
+
+@pytest.mark.parametrize("tiles", [(2, 2), (2, 1), (1, 2)])
+def test_tiled_equals_single_domain(tiles):
+    from les_halo_amd import LES, TiledLES
+    case = common.tiny_case(32, 24, 16, seed=21)
+    dims = case["dims"]
+    L = LES(*dims, **common.REF_PHYS, pjuge=0.0, **SOR)
+    T = TiledLES(*dims, sub_i=tiles[0], sub_j=tiles[1], **common.REF_PHYS, **SOR)
+    for X in (L, T):
+        X.load_case(case)
+        X.ifdata()
+    u0 = T.download("u")
+    for n in range(1, 4):
+        L.step(n)
+        T.step(n)
+    for nm in FIELDS:
+        a, b = _interior(nm, L.download(nm), dims), _interior(nm, T.download(nm), dims)
+        assert common.bits_equal(b, a), common.mismatch_report(b, a, nm)
+    # the domain's faces of u, v, w and p (inflow, outflow, periodic j, top/bottom)
+    for nm in ("u", "v", "w", "p"):
+        b, a = T.download(nm), L.download(nm)
+        assert common.bits_equal(b, a), common.mismatch_report(b, a, nm + " (full)")
+    # the steps moved the state (a vacuous comparison would pass too)
+    assert not common.bits_equal(T.download("u"), u0)
+    assert np.abs(_interior("p", T.download("p"), dims)).max() > 0
+    T.close()
+    L.close()
+
+
+def test_tiled_rejects_odd_tile_sizes():
+    from les_halo_amd import LESError, TiledLES
+    with pytest.raises(LESError):
+        TiledLES(30, 24, 8, sub_i=2, sub_j=2)  # 15 columns per tile: colours would flip
