@@ -260,6 +260,8 @@ int les_tiled_download(les_tiled* t, int field, float* host);
 int les_tiled_init_coefficients(les_tiled* t);
 int les_tiled_ifdata(les_tiled* t);
 int les_tiled_step(les_tiled* t, int n);
+/* les_tiled_step for n0 .. n0+nsteps-1, the steps n >= 2 as replays of one captured HIP graph */
+int les_tiled_run(les_tiled* t, int n0, int nsteps);
 
 /* NetCDF snapshots (module_LES_write_netcdf.f95, called from press.f95:189): les_netcdf_create
  * is init_netcdf_file (:111-300): it creates dir/LES_output.nc, LES_output_p.nc,

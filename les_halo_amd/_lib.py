@@ -63,6 +63,7 @@ PROTOS = {
     "les_tiled_init_coefficients": (c_int, [c_void_p]),
     "les_tiled_ifdata": (c_int, [c_void_p]),
     "les_tiled_step": (c_int, [c_void_p, c_int]),
+    "les_tiled_run": (c_int, [c_void_p, c_int, c_int]),
     "les_anime_arrays": (c_int, [c_int] * 6 + [c_void_p] * 7 + [ctypes.c_char_p] * 2),
     "les_upload": (c_int, [H, c_int, c_void_p]),
     "les_download": (c_int, [H, c_int, c_void_p]),

@@ -1397,6 +1397,8 @@ struct les_tiled {
   int red_total = 0;
   std::vector<float> gdx1, gdy1, gdzn;  // the whole domain's grid, for area = pco
   std::string err;
+  hipGraph_t graph = nullptr;  // one captured step (n >= 2) for les_tiled_run
+  hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -1652,6 +1654,8 @@ extern "C" int les_tiled_create(const les_config* cfg, les_tiled** out) {
 
 extern "C" int les_tiled_destroy(les_tiled* T) {
   if (!T) return LES_E_INVALID;
+  if (T->gexec) hipGraphExecDestroy(T->gexec);
+  if (T->graph) hipGraphDestroy(T->graph);
   for (les_handle* h : T->t) les_destroy(h);
   if (T->partials) hipFree(T->partials);
   delete T;
@@ -1788,4 +1792,33 @@ extern "C" int les_tiled_init_coefficients(les_tiled* T) {
     if ((r = les_upload(h, LES_DELX1, delx1.data()))) return r;
   }
   return tiled_write_area(T);
+}
+
+// les_tiled_step for n = n0 .. n0+nsteps-1: steps with n <= 1 eagerly, the others as
+// replays of one captured step (every launch of a step n >= 2 is the same).  A later upload
+// does not change the captured launches (they take the tiles' arrays by address).
+extern "C" int les_tiled_run(les_tiled* T, int n0, int nsteps) {
+  if (!T || nsteps < 0) return LES_E_INVALID;
+  int r;
+  for (; nsteps > 0 && n0 <= 1; ++n0, --nsteps)
+    if ((r = les_tiled_step(T, n0))) return r;
+  if (nsteps == 0) return LES_OK;
+  hipStream_t st = T->t[0]->st;
+  if (!T->gexec) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return tfail(T, LES_E_HIP, "tiled_run: begin capture");
+    r = les_tiled_step(T, 2);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &gr);
+    if (r || e != hipSuccess) {
+      if (gr) hipGraphDestroy(gr);
+      return r ? r : tfail(T, LES_E_HIP, std::string("tiled_run: capture: ") + hipGetErrorString(e));
+    }
+    T->graph = gr;
+    if (hipGraphInstantiate(&T->gexec, T->graph, nullptr, nullptr, 0) != hipSuccess)
+      return tfail(T, LES_E_HIP, "tiled_run: graph instantiate");
+  }
+  for (int q = 0; q < nsteps; ++q)
+    if (hipGraphLaunch(T->gexec, st) != hipSuccess) return tfail(T, LES_E_HIP, "tiled_run: launch");
+  return LES_OK;
 }

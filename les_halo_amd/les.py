@@ -409,8 +409,8 @@ class TiledLES:
         self._chk(lib().les_tiled_step(self.h, int(n)), "step")
 
     def run(self, n0, nsteps):
-        for n in range(n0, n0 + nsteps):
-            self.step(n)
+        """Steps n0 .. n0+nsteps-1 (graph replays for n >= 2)."""
+        self._chk(lib().les_tiled_run(self.h, int(n0), int(nsteps)), "run")
 
     def sync(self):
         self._chk(lib().les_tiled_sync(self.h), "sync")
