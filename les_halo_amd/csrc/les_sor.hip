@@ -36,27 +36,34 @@ __device__ __forceinline__ int split_index(const SplitSOR& sp, int i, int j, int
 }  // namespace
 
 // ---- p, rhs (Fortran layouts) -> split copy; optionally rhs -= rhsav (press.f95:76-82) ----
+// A thread takes split entry ii of row (j,k) in both colours, i.e. the two neighbouring cells
+// i = 1 + 2ii and 2 + 2ii: its split stores are unit-stride across the wavefront, and its two
+// Fortran loads (and rhs stores) fill each other's cache lines.
 __global__ __launch_bounds__(256) void k_split_scatter(Dims d, Fields F, SplitSOR sp,
                                                        const DevScalars* sc, int adjust_rhs) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int ii = blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
-  if (i > d.ip || j > d.jp) return;
-  const int par = (i + j + k) & 1;
-  const int is = split_index(sp, i, j, k);
-  sp.p[par][is] = F.p[d.P(i, j, k)];
-  const long ir = d.R(i, j, k);
-  float r = F.rhs[ir];
-  if (adjust_rhs) {
-    r = r - sc->rhsav;
-    F.rhs[ir] = r;
+  if (ii >= sp.nr || j > d.jp) return;
+  const int base = ii + sp.nr * ((j - 1) + sp.nj * (k - 1));
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int i = first_i(c, j, k) + 2 * ii;
+    if (i > d.ip) continue;  // padding entries of the split rows stay untouched
+    sp.p[c][base] = F.p[d.P(i, j, k)];
+    const long ir = d.R(i, j, k);
+    float r = F.rhs[ir];
+    if (adjust_rhs) {
+      r = r - sc->rhsav;
+      F.rhs[ir] = r;
+    }
+    sp.r[c][base] = r;
   }
-  sp.r[par][is] = r;
 }
 
 void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                           int adjust_rhs) {
-  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
+  dim3 grid((sp.nr + 63) / 64, (d.jp + 3) / 4, d.kp);
   hipLaunchKernelGGL(k_split_scatter, grid, dim3(64, 4), 0, st, d, F, sp, sc, adjust_rhs);
 }
 
