@@ -1385,7 +1385,10 @@ extern "C" int les_finalise_LES_kernel(void) {
 // arrays (the halo exchange, here a device copy since every tile is on the same GPU), the
 // domain's own faces keep the reference's boundary rules (Edges), and the sums (uout, rhsav,
 // pav) combine the tiles' exact partials, so the tiled step is bitwise the single-domain step.
-// Pressure uses the in-place colour sweep (k_sor_colour) with exchanges after every colour.
+// Pressure sweeps each tile's colour-split copy (k_sor_split with halo rows and columns) and
+// exchanges the split faces after every colour (k_split_exchange); tiles whose grid the split
+// sweep does not cover use the in-place colour sweep (k_sor_colour) with Fortran-layout
+// exchanges.
 // ------------------------------------------------------------------------------------------
 struct les_tiled {
   les_config cfg;  // the whole domain
@@ -1399,6 +1402,13 @@ struct les_tiled {
   std::string err;
   hipGraph_t graph = nullptr;  // one captured step (n >= 2) for les_tiled_run
   hipGraphExec_t gexec = nullptr;
+  // colour-split copies of p and rhs with halo rows and i-halo columns, per tile (nullptr:
+  // the tiles sweep in place)
+  float* split_mem = nullptr;
+  long split_ns = 0, split_per_tile = 0;
+  // the boundary images of p are mirror images of the interior (after tiled_boundp; not after
+  // an upload of p), which the split sweep assumes at the domain's faces
+  bool p_consistent = false;
 };
 
 namespace {
@@ -1543,6 +1553,7 @@ void tiled_boundp(les_tiled* T, hipStream_t st, int check_done) {
   tiled_exchange(T, st, {LES_P}, 0);
   tiled_exchange(T, st, {LES_P}, 1);
   for (les_handle* h : T->t) launch_boundp2(st, h->d, fields_of(h), h->sc, check_done);
+  T->p_consistent = true;
 }
 
 // velfg + feedbf, les (calc_sm, boundsm), visc (+ adam)
@@ -1567,6 +1578,69 @@ void tiled_velfg_les(les_tiled* T, hipStream_t st, int do_adam) {
   for (les_handle* h : T->t) launch_calc_visc_adam(st, h->d, grid_of(h), fields_of(h), 1, do_adam);
 }
 
+// tile q's colour-split copy (its handle's coefficients, the tiled arrays)
+SplitSOR tile_split(les_tiled* T, int q) {
+  les_handle* h = T->t[q];
+  SplitSOR sp = h->sp;
+  float* base = T->split_mem + (long)q * T->split_per_tile;
+  const long ns = T->split_ns;
+  sp.p[0] = base;
+  sp.p[1] = base + ns;
+  sp.r[0] = base + 2 * ns;
+  sp.r[1] = base + 3 * ns;
+  sp.nj = T->jpl + 2;
+  sp.jhalo = 1;
+  sp.hw = T->e[q].w ? nullptr : base + 4 * ns;
+  sp.he = T->e[q].e ? nullptr : base + 4 * ns + (long)T->jpl * T->cfg.kp;
+  return sp;
+}
+
+SplitTiles split_tiles(les_tiled* T) {
+  SplitTiles x{};
+  x.ntiles = (int)T->t.size();
+  for (int b = 0; b < T->sj; ++b)
+    for (int a = 0; a < T->si; ++a) {
+      const int q = a + T->si * b;
+      const SplitSOR sp = tile_split(T, q);
+      x.p[q][0] = sp.p[0];
+      x.p[q][1] = sp.p[1];
+      x.hw[q] = const_cast<float*>(sp.hw);
+      x.he[q] = const_cast<float*>(sp.he);
+      x.n[q] = a + T->si * ((b + 1) % T->sj);
+      x.s[q] = a + T->si * ((b + T->sj - 1) % T->sj);
+      x.w[q] = a > 0 ? q - 1 : -1;
+      x.e[q] = a + 1 < T->si ? q + 1 : -1;
+      x.nr = sp.nr;
+      x.nj = sp.nj;
+    }
+  return x;
+}
+
+// press.f95:87-115 on the tiles' split copies: the colour sweeps of every tile, then the face
+// exchange of the split layout, per colour; the k faces need no images (a cell reads itself
+// there), and the gather leaves the interior of p for pav, adj and the boundary images
+void tiled_sor_split(les_tiled* T, hipStream_t st) {
+  const SplitTiles x = split_tiles(T);
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    launch_split_scatter(st, h->d, fields_of(h), tile_split(T, (int)q), h->sc, 0);
+  }
+  launch_split_exchange(st, T->t[0]->d, x);
+  for (int l = 1; l <= T->cfg.nmaxp; ++l)
+    for (int nrd = 0; nrd <= 1; ++nrd) {
+      for (size_t q = 0; q < T->t.size(); ++q) {
+        les_handle* h = T->t[q];
+        launch_sor_split(st, h->d, grid_of(h), tile_split(T, (int)q), h->ph, nrd, 0, h->rb, 0,
+                         h->sc);
+      }
+      launch_split_exchange(st, T->t[0]->d, x);
+    }
+  for (size_t q = 0; q < T->t.size(); ++q) {
+    les_handle* h = T->t[q];
+    launch_split_gather_adj(st, h->d, fields_of(h), tile_split(T, (int)q), h->sc, 0);
+  }
+}
+
 void tiled_press(les_tiled* T, hipStream_t st) {
   tiled_exchange(T, st, {LES_F, LES_G}, 0);
   tiled_exchange(T, st, {LES_F, LES_G}, 1);
@@ -1579,7 +1653,10 @@ void tiled_press(les_tiled* T, hipStream_t st) {
   tiled_bcast(T, st, offsetof(DevScalars, rhsav));
   for (size_t q = 1; q < T->t.size(); ++q) launch_sor_reset(st, T->t[q]->sc);
   for (les_handle* h : T->t) launch_rhs_adjust(st, h->d, fields_of(h), h->sc);
-  for (int l = 1; l <= T->cfg.nmaxp; ++l) {
+  bool split = T->split_mem && T->p_consistent;
+  for (les_handle* h : T->t) split = split && h->cn1_separable;
+  if (split) tiled_sor_split(T, st);
+  for (int l = 1; !split && l <= T->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd) {
       for (les_handle* h : T->t)
         launch_sor_colour(st, h->d, grid_of(h), fields_of(h), h->ph, nrd, 0, h->rb, 0, h->sc);
@@ -1648,6 +1725,24 @@ extern "C" int les_tiled_create(const les_config* cfg, les_tiled** out) {
     les_tiled_destroy(T);
     return LES_E_NOMEM;
   }
+  // the split sweep needs an even jp per tile (the colour across the periodic seam) and its
+  // index arithmetic on the tile grid
+  bool split = jpl % 2 == 0 && (int)T->t.size() <= SPLIT_TILES_MAX;
+  for (les_handle* h : T->t) split = split && h->split_ok;
+  if (split) {
+    T->split_ns = (long)T->t[0]->sp.nr * (jpl + 2) * cfg->kp;
+    T->split_per_tile = 4 * T->split_ns + 2L * jpl * cfg->kp;
+    const size_t bytes = (size_t)T->split_per_tile * T->t.size() * sizeof(float);
+    if (hipMalloc(&T->split_mem, bytes) != hipSuccess) {
+      les_tiled_destroy(T);
+      return LES_E_NOMEM;
+    }
+    // padding entries are read (never used) by whole-float4 loads
+    if (hipMemset(T->split_mem, 0, bytes) != hipSuccess) {
+      les_tiled_destroy(T);
+      return LES_E_HIP;
+    }
+  }
   *out = T;
   return LES_OK;
 }
@@ -1658,6 +1753,7 @@ extern "C" int les_tiled_destroy(les_tiled* T) {
   if (T->graph) hipGraphDestroy(T->graph);
   for (les_handle* h : T->t) les_destroy(h);
   if (T->partials) hipFree(T->partials);
+  if (T->split_mem) hipFree(T->split_mem);
   delete T;
   return LES_OK;
 }
@@ -1710,6 +1806,7 @@ extern "C" int les_tiled_upload(les_tiled* T, int field, const float* host) {
       const int r = les_upload(h, field, loc.data());
       if (r) return r;
     }
+  if (field == LES_P) T->p_consistent = false;
   if (field == LES_DX1) T->gdx1.assign(host, host + gn);
   if (field == LES_DY1) T->gdy1.assign(host, host + gn);
   if (field == LES_DZN) T->gdzn.assign(host, host + gn);

@@ -39,9 +39,25 @@ struct SplitSOR {
   float* p[2];
   float* r[2];
   const float *ax, *by, *cz, *cn1k;
-  int nr, nj;  // padded colour row length, jp
+  int nr, nj;  // padded colour row length, rows per level (jp, or jp + 2 with jhalo)
   int uniform;
   uint32_t m_ng, m_jp;  // multiply-high reciprocals of nr/4 and jp (sor_split_divisors)
+  // A tile of a cut domain (k_sor_split only): jhalo stores rows j = jp+1 and j = 0 of each
+  // colour after the interior rows (row indices jp and jp+1) instead of wrapping around in j;
+  // hw / he, when set, hold p(0,j,k) / p(ip+1,j,k) at (j-1) + jp*(k-1) for a west / east face
+  // shared with a neighbour (else the cell's own value is its image, boundp1).
+  int jhalo;
+  const float *hw, *he;
+};
+// The split copies of the tiles of one cut domain and their neighbours, for the face exchange
+// of the split layout (k_split_exchange) after every colour sweep
+constexpr int SPLIT_TILES_MAX = 8;
+struct SplitTiles {
+  float* p[SPLIT_TILES_MAX][2];
+  float* hw[SPLIT_TILES_MAX];  // nullptr on the domain's west face
+  float* he[SPLIT_TILES_MAX];  // nullptr on the domain's east face
+  int n[SPLIT_TILES_MAX], s[SPLIT_TILES_MAX], w[SPLIT_TILES_MAX], e[SPLIT_TILES_MAX];
+  int ntiles, nr, nj;
 };
 
 // Fused SOR iteration (k_sor_iter): previous and next iterate, colour 0 and colour 1 each
@@ -142,6 +158,8 @@ int sor_iter_supported(Dims d, const SplitSOR& sp);
 void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
                      DevScalars* sc, SorIterShape shape);
 void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc);
+// fills every tile's halo rows (both colours) and i-halo columns from its neighbours' interior
+void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T);
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                              int subtract_pav);
 int reduction_blocks(Dims d);

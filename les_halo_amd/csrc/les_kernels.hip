@@ -483,14 +483,9 @@ void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do
 // ==========================================================================================
 // les.f95:40-60: Smagorinsky eddy viscosity sm from the stored diu fields
 // ==========================================================================================
-__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
-                                                 StripGrid sg) {
-  int bx, by, bz, L;
-  if (!strip_block(sg, bx, by, bz, L)) return;
-  const int i = 1 + bx * BX + threadIdx.x;
-  const int j = 1 + by * BY + threadIdx.y;
-  const int k = 1 + bz;
-  if (i > d.ip || j > d.jp) return;
+// sm(i,j,k) at one interior point (les.f95:40-56)
+__device__ __forceinline__ float calc_sm_at(const Dims& d, const Grid1D& g, const Fields& F,
+                                            const Physics& ph, int i, int j, int k) {
   const float q = 0.25f, two = 2.0f;
   const Buf diu2(F.diu2, d.nD()), diu3(F.diu3, d.nD()), diu4(F.diu4, d.nD());
   const Buf diu6(F.diu6, d.nD()), diu7(F.diu7, d.nD()), diu8(F.diu8, d.nD());
@@ -507,7 +502,18 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
   float s = two * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
             (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
             (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
-  Buf(F.sm, d.nS()).st(d.S(i, j, k), (cd * cd) * sqrtf(s));
+  return (cd * cd) * sqrtf(s);
+}
+
+__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
+                                                 StripGrid sg) {
+  int bx, by, bz, L;
+  if (!strip_block(sg, bx, by, bz, L)) return;
+  const int i = 1 + bx * BX + threadIdx.x;
+  const int j = 1 + by * BY + threadIdx.y;
+  const int k = 1 + bz;
+  if (i > d.ip || j > d.jp) return;
+  Buf(F.sm, d.nS()).st(d.S(i, j, k), calc_sm_at(d, g, F, ph, i, j, k));
 }
 
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
@@ -572,6 +578,117 @@ void launch_boundsm(hipStream_t st, Dims d, Fields F) {
 // ==========================================================================================
 // les.f95:78-182 (viscous terms) fused with adam.f95:32-46 (Adams-Bashforth)
 // ==========================================================================================
+// les.f95:78-180: the viscous terms added to f, g, h at interior point (i,j,k); sm(x,y,z)
+// returns the eddy viscosity (boundsm applied) at the 15 points it is called with
+template <bool P2, class SM>
+__device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const Fields& F, int i,
+                                           int j, int k, SM sm, float& fv, float& gv, float& hv) {
+  const float two = 2.0f;
+  const Buf b1(F.diu1, d.nD15()), b5(F.diu5, d.nD15());
+  const Buf b2(F.diu2, d.nD()), b3(F.diu3, d.nD()), b4(F.diu4, d.nD()), b6(F.diu6, d.nD());
+  const Buf b7(F.diu7, d.nD()), b8(F.diu8, d.nD()), b9(F.diu9, d.nD());
+  auto D1 = [&](int x, int y, int z) -> float { return b1[d.D15(x, y, z)]; };
+  auto D5 = [&](int x, int y, int z) -> float { return b5[d.D15(x, y, z)]; };
+  auto D2 = [&](int x, int y, int z) -> float { return b2[d.D(x, y, z)]; };
+  auto D3 = [&](int x, int y, int z) -> float { return b3[d.D(x, y, z)]; };
+  auto D4 = [&](int x, int y, int z) -> float { return b4[d.D(x, y, z)]; };
+  auto D6 = [&](int x, int y, int z) -> float { return b6[d.D(x, y, z)]; };
+  auto D7 = [&](int x, int y, int z) -> float { return b7[d.D(x, y, z)]; };
+  auto D8 = [&](int x, int y, int z) -> float { return b8[d.D(x, y, z)]; };
+  auto D9 = [&](int x, int y, int z) -> float { return b9[d.D(x, y, z)]; };
+  const float dx1i = g.DX1(i), dx1ip = g.DX1(i + 1), dx1im = g.DX1(i - 1);
+  const float dy1j = g.DY1(j), dy1jp = g.DY1(j + 1), dy1jm = g.DY1(j - 1);
+  const float dznk = g.DZN(k), dznkp = g.DZN(k + 1), dznkm = g.DZN(k - 1);
+  const float s000 = sm(i, j, k), s100 = sm(i + 1, j, k);
+  // x-momentum (les.f95:78-109)
+  {
+    float evsx2 = s100;
+    float evsx1 = s000;
+    float evsy2 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dy1j * (xdiv<P2>(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
+                       dy1j + dy1jp);
+    float evsy1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * sm(i, j - 1, k) + dx1i * sm(i + 1, j - 1, k), dx1i + dx1ip)) +
+                       dy1j * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
+                       dy1j + dy1jp);
+    float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
+    float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
+                       dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
+    float visux2 = (evsx2) * two * D1(i + 1, j, k);
+    float visux1 = (evsx1) * two * D1(i, j, k);
+    float visuy2 = (evsy2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
+    float visuy1 = (evsy1) * (D2(i, j, k) + D4(i + 1, j - 1, k));
+    float visuz2 = (evsz2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
+    float visuz1 = (evsz1) * (D3(i, j, k) + D7(i + 1, j, k - 1));
+    float vfu = xdiv<P2>(visux2 - visux1, dx1i) +
+                xdiv<P2>(visuy2 - visuy1, dy1j) +
+                (visuz2 - visuz1) / dznk;
+    fv = (fv + vfu);
+  }
+  // y-momentum (les.f95:112-143)
+  {
+    float evsy2 = sm(i, j + 1, k);
+    float evsy1 = s000;
+    float evsx2 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dy1j * (xdiv<P2>(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
+                       dy1j + dy1jp);
+    float evsx1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
+                       dy1j * (xdiv<P2>(dx1i * sm(i - 1, j + 1, k) + dx1im * sm(i, j + 1, k), dx1im + dx1i)),
+                       dy1j + dy1jp);
+    float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
+    float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
+                       dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
+    float visvx2 = (evsx2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
+    float visvx1 = (evsx1) * (D2(i - 1, j + 1, k) + D4(i, j, k));
+    float visvy2 = (evsy2) * two * D5(i, j + 1, k);
+    float visvy1 = (evsy1) * two * D5(i, j, k);
+    float visvz2 = (evsz2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
+    float visvz1 = (evsz1) * (D6(i, j, k) + D8(i, j + 1, k - 1));
+    float vfv = xdiv<P2>(visvx2 - visvx1, dx1i) +
+                xdiv<P2>(visvy2 - visvy1, dy1j) +
+                (visvz2 - visvz1) / dznk;
+    gv = (gv + vfv);
+  }
+  // z-momentum (les.f95:146-180)
+  {
+    float evsz2 = sm(i, j, k + 1);
+    float evsz1 = s000;
+    float evsx2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
+    float evsx1 = (dznkp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
+                       dznk * (xdiv<P2>(dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1), dx1im + dx1i))) / (dznk + dznkp);
+    float evsy2 = (dznkp * (xdiv<P2>(dy1jp * s000 + dy1j * sm(i, j + 1, k), dy1j + dy1jp)) +
+                       dznk * (xdiv<P2>(dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1), dy1j + dy1jp))) / (dznk + dznkp);
+    float evsy1 = (dznkp * (xdiv<P2>(dy1j * sm(i, j - 1, k) + dy1jm * s000, dy1jm + dy1j)) +
+                       dznk * (xdiv<P2>(dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1), dy1jm + dy1j))) / (dznk + dznkp);
+    float viswx2 = (evsx2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
+    float viswx1 = (evsx1) * (D3(i - 1, j, k + 1) + D7(i, j, k));
+    float viswy2 = (evsy2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
+    float viswy1 = (evsy1) * (D6(i, j - 1, k + 1) + D8(i, j, k));
+    float viswz2 = (evsz2) * two * D9(i, j, k + 1);
+    float viswz1 = (evsz1) * two * D9(i, j, k);
+    float vfw = xdiv<P2>(viswx2 - viswx1, dx1i) +
+                xdiv<P2>(viswy2 - viswy1, dy1j) +
+                (viswz2 - viswz1) / dznk;
+    hv = (hv + vfw);
+  }
+}
+
+// adam.f95:35-43 at interior point (i,j,k): fold <- f, f <- 1.5 f - 0.5 fold (g, h alike)
+__device__ __forceinline__ void adam_point(const Dims& d, const Fields& F, int i, int j, int k,
+                                           float& fv, float& gv, float& hv) {
+  const long io = d.O(i, j, k);
+  const Buf bfo(F.fold, d.nO()), bgo(F.gold, d.nO()), bho(F.hold, d.nO());
+  const float fo = bfo[io], go = bgo[io], ho = bho[io];
+  bfo.st(io, fv);
+  bgo.st(io, gv);
+  bho.st(io, hv);
+  fv = 1.5f * fv - 0.5f * fo;
+  gv = 1.5f * gv - 0.5f * go;
+  hv = 1.5f * hv - 0.5f * ho;
+}
+
 template <bool P2>
 __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
                                                    int do_adam, StripGrid sg) {
@@ -585,109 +702,11 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
   const Buf bf(F.f, d.nF()), bg(F.g, d.nF()), bh(F.h, d.nF());
   float fv = bf[ifg], gv = bg[ifg], hv = bh[ifg];
   if (do_visc) {
-    const float two = 2.0f;
-    const Buf bsm(F.sm, d.nS()), b1(F.diu1, d.nD15()), b5(F.diu5, d.nD15());
-    const Buf b2(F.diu2, d.nD()), b3(F.diu3, d.nD()), b4(F.diu4, d.nD()), b6(F.diu6, d.nD());
-    const Buf b7(F.diu7, d.nD()), b8(F.diu8, d.nD()), b9(F.diu9, d.nD());
+    const Buf bsm(F.sm, d.nS());
     auto sm = [&](int x, int y, int z) -> float { return bsm[d.S(x, y, z)]; };
-    auto D1 = [&](int x, int y, int z) -> float { return b1[d.D15(x, y, z)]; };
-    auto D5 = [&](int x, int y, int z) -> float { return b5[d.D15(x, y, z)]; };
-    auto D2 = [&](int x, int y, int z) -> float { return b2[d.D(x, y, z)]; };
-    auto D3 = [&](int x, int y, int z) -> float { return b3[d.D(x, y, z)]; };
-    auto D4 = [&](int x, int y, int z) -> float { return b4[d.D(x, y, z)]; };
-    auto D6 = [&](int x, int y, int z) -> float { return b6[d.D(x, y, z)]; };
-    auto D7 = [&](int x, int y, int z) -> float { return b7[d.D(x, y, z)]; };
-    auto D8 = [&](int x, int y, int z) -> float { return b8[d.D(x, y, z)]; };
-    auto D9 = [&](int x, int y, int z) -> float { return b9[d.D(x, y, z)]; };
-    const float dx1i = g.DX1(i), dx1ip = g.DX1(i + 1), dx1im = g.DX1(i - 1);
-    const float dy1j = g.DY1(j), dy1jp = g.DY1(j + 1), dy1jm = g.DY1(j - 1);
-    const float dznk = g.DZN(k), dznkp = g.DZN(k + 1), dznkm = g.DZN(k - 1);
-    const float s000 = sm(i, j, k), s100 = sm(i + 1, j, k);
-    // x-momentum (les.f95:78-109)
-    {
-      float evsx2 = s100;
-      float evsx1 = s000;
-      float evsy2 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dy1j * (xdiv<P2>(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
-                         dy1j + dy1jp);
-      float evsy1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * sm(i, j - 1, k) + dx1i * sm(i + 1, j - 1, k), dx1i + dx1ip)) +
-                         dy1j * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
-                         dy1j + dy1jp);
-      float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
-      float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
-                         dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
-      float visux2 = (evsx2) * two * D1(i + 1, j, k);
-      float visux1 = (evsx1) * two * D1(i, j, k);
-      float visuy2 = (evsy2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
-      float visuy1 = (evsy1) * (D2(i, j, k) + D4(i + 1, j - 1, k));
-      float visuz2 = (evsz2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
-      float visuz1 = (evsz1) * (D3(i, j, k) + D7(i + 1, j, k - 1));
-      float vfu = xdiv<P2>(visux2 - visux1, dx1i) +
-                  xdiv<P2>(visuy2 - visuy1, dy1j) +
-                  (visuz2 - visuz1) / dznk;
-      fv = (fv + vfu);
-    }
-    // y-momentum (les.f95:112-143)
-    {
-      float evsy2 = sm(i, j + 1, k);
-      float evsy1 = s000;
-      float evsx2 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dy1j * (xdiv<P2>(dx1ip * sm(i, j + 1, k) + dx1i * sm(i + 1, j + 1, k), dx1i + dx1ip)),
-                         dy1j + dy1jp);
-      float evsx1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
-                         dy1j * (xdiv<P2>(dx1i * sm(i - 1, j + 1, k) + dx1im * sm(i, j + 1, k), dx1im + dx1i)),
-                         dy1j + dy1jp);
-      float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
-      float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
-                         dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
-      float visvx2 = (evsx2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
-      float visvx1 = (evsx1) * (D2(i - 1, j + 1, k) + D4(i, j, k));
-      float visvy2 = (evsy2) * two * D5(i, j + 1, k);
-      float visvy1 = (evsy1) * two * D5(i, j, k);
-      float visvz2 = (evsz2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
-      float visvz1 = (evsz1) * (D6(i, j, k) + D8(i, j + 1, k - 1));
-      float vfv = xdiv<P2>(visvx2 - visvx1, dx1i) +
-                  xdiv<P2>(visvy2 - visvy1, dy1j) +
-                  (visvz2 - visvz1) / dznk;
-      gv = (gv + vfv);
-    }
-    // z-momentum (les.f95:146-180)
-    {
-      float evsz2 = sm(i, j, k + 1);
-      float evsz1 = s000;
-      float evsx2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                         dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
-      float evsx1 = (dznkp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
-                         dznk * (xdiv<P2>(dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1), dx1im + dx1i))) / (dznk + dznkp);
-      float evsy2 = (dznkp * (xdiv<P2>(dy1jp * s000 + dy1j * sm(i, j + 1, k), dy1j + dy1jp)) +
-                         dznk * (xdiv<P2>(dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1), dy1j + dy1jp))) / (dznk + dznkp);
-      float evsy1 = (dznkp * (xdiv<P2>(dy1j * sm(i, j - 1, k) + dy1jm * s000, dy1jm + dy1j)) +
-                         dznk * (xdiv<P2>(dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1), dy1jm + dy1j))) / (dznk + dznkp);
-      float viswx2 = (evsx2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
-      float viswx1 = (evsx1) * (D3(i - 1, j, k + 1) + D7(i, j, k));
-      float viswy2 = (evsy2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
-      float viswy1 = (evsy1) * (D6(i, j - 1, k + 1) + D8(i, j, k));
-      float viswz2 = (evsz2) * two * D9(i, j, k + 1);
-      float viswz1 = (evsz1) * two * D9(i, j, k);
-      float vfw = xdiv<P2>(viswx2 - viswx1, dx1i) +
-                  xdiv<P2>(viswy2 - viswy1, dy1j) +
-                  (viswz2 - viswz1) / dznk;
-      hv = (hv + vfw);
-    }
+    visc_terms<P2>(d, g, F, i, j, k, sm, fv, gv, hv);
   }
-  if (do_adam) {  // adam.f95:35-43
-    const long io = d.O(i, j, k);
-    const Buf bfo(F.fold, d.nO()), bgo(F.gold, d.nO()), bho(F.hold, d.nO());
-    const float fo = bfo[io], go = bgo[io], ho = bho[io];
-    bfo.st(io, fv);
-    bgo.st(io, gv);
-    bho.st(io, hv);
-    fv = 1.5f * fv - 0.5f * fo;
-    gv = 1.5f * gv - 0.5f * go;
-    hv = 1.5f * hv - 0.5f * ho;
-  }
+  if (do_adam) adam_point(d, F, i, j, k, fv, gv, hv);
   bf.st(ifg, fv);
   bg.st(ifg, gv);
   bh.st(ifg, hv);

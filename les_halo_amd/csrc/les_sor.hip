@@ -93,9 +93,14 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
   const float* po = sp.p[c ^ 1];
   if (t < nthreads && i0 <= d.ip) {
     const float4 oc = *reinterpret_cast<const float4*>(po + base);
-    // neighbour rows: periodic in j, the cell itself across k faces
-    const int rn = (j == d.jp) ? -sp.nr * (d.jp - 1) : sp.nr;  // j+1
-    const int rsj = (j == 1) ? sp.nr * (d.jp - 1) : -sp.nr;    // j-1
+    // neighbour rows: periodic in j (a tile: its halo rows, j = jp+1 at row index jp and
+    // j = 0 at jp+1), the cell itself across k faces
+    const int rn = (j == d.jp && !sp.jhalo) ? -sp.nr * (d.jp - 1) : sp.nr;     // j+1
+    const int rsj = (j == 1) ? sp.nr * (sp.jhalo ? d.jp + 1 : d.jp - 1) : -sp.nr;  // j-1
+    // i faces shared with a neighbour tile: its cells p(0,j,k) and p(ip+1,j,k)
+    const int ih = (j - 1) + d.jp * (k - 1);
+    const float hwv = (sp.hw && i0 == 1) ? sp.hw[ih] : 0.0f;
+    const float hev = (sp.he && i0 + 6 >= d.ip) ? sp.he[ih] : 0.0f;
     const int pl = sp.nr * sp.nj;
     const float4 own = *reinterpret_cast<const float4*>(sp.p[c] + base);
     const float4 rhs = *reinterpret_cast<const float4*>(sp.r[c] + base);
@@ -158,7 +163,9 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
     for (int q = 0; q < 4; ++q) {
       const int i = i0 + 2 * q;
       const float pc = pv[q];
-      out[q] = (i <= d.ip) ? relax(q, i, (i == d.ip) ? pc : xe[q], (i == 1) ? pc : xw[q]) : pc;
+      const float pe = (i == d.ip) ? (sp.he ? hev : pc) : xe[q];
+      const float pw = (i == 1) ? (sp.hw ? hwv : pc) : xw[q];
+      out[q] = (i <= d.ip) ? relax(q, i, pe, pw) : pc;
     }
     *reinterpret_cast<float4*>(sp.p[c] + base) = make_float4(out[0], out[1], out[2], out[3]);
   }
@@ -544,6 +551,46 @@ void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, 
                      kchunk);
   hipLaunchKernelGGL(k_pav_split_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials,
                      nbx * nkc, sc);
+}
+
+// ---- face exchange of the split copies of the tiles of one cut domain (after a colour
+// sweep).  Job 0: my row j = jp+1 (row index jp) <- the north neighbour's row j = 1; job 1: my
+// row j = 0 (row index jp+1) <- the south neighbour's row j = jp, both colours, every entry; job
+// 2: hw(j,k) <- the west neighbour's p(ip,j,k); job 3: he(j,k) <- the east neighbour's
+// p(1,j,k).  Tiles have the same even sizes, so a cell's colour and split position along a
+// shared face are the same in both tiles; p(ip,j,k) has colour (j+k)&1 (first i 2, entry
+// ip/2 - 1) and p(1,j,k) colour (1+j+k)&1 (entry 0).  Sources are interior entries, targets
+// halo entries: no job reads what another writes. ----
+__global__ __launch_bounds__(256) void k_split_exchange(Dims d, SplitTiles T) {
+  const int q = blockIdx.y, job = blockIdx.z;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const int nr = T.nr, nj = T.nj;
+  if (job < 2) {
+    if (t >= 2L * d.kp * nr) return;
+    const int ii = (int)(t % nr);
+    const int rest = (int)(t / nr);
+    const int c = rest & 1, k = 1 + (rest >> 1);
+    const int src = job == 0 ? T.n[q] : T.s[q];
+    const int srow = job == 0 ? 0 : d.jp - 1;
+    const int drow = job == 0 ? d.jp : d.jp + 1;
+    const long lev = (long)nr * nj * (k - 1);
+    T.p[q][c][ii + nr * drow + lev] = T.p[src][c][ii + nr * srow + lev];
+  } else {
+    if (t >= (long)d.jp * d.kp) return;
+    const int j = 1 + (int)(t % d.jp), k = 1 + (int)(t / d.jp);
+    const long row = (long)nr * ((j - 1) + (long)nj * (k - 1));
+    if (job == 2) {
+      if (T.hw[q]) T.hw[q][t] = T.p[T.w[q]][(j + k) & 1][(d.ip - 2) / 2 + row];
+    } else if (T.he[q]) {
+      T.he[q][t] = T.p[T.e[q]][(1 + j + k) & 1][row];
+    }
+  }
+}
+
+void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T) {
+  const long n = std::max(2L * d.kp * T.nr, (long)d.jp * d.kp);
+  hipLaunchKernelGGL(k_split_exchange, dim3((unsigned)((n + 255) / 256), T.ntiles, 4), dim3(256),
+                     0, st, d, T);
 }
 
 // ---- split copy -> p over the box [0,ip+1]x[0,jp+1]x[0,kp+1], optionally minus pav

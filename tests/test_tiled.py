@@ -19,10 +19,14 @@ def _interior(name, a, dims):
     return common.interior(name, a, *dims)
 
 
-@pytest.mark.parametrize("tiles", [(2, 2), (2, 1), (1, 2)])
-def test_tiled_equals_single_domain(tiles):
+# even tile sizes sweep the tiles' colour-split copies with split-layout face exchanges; an
+# odd jp (32 x 21: the colour flips across the periodic seam) sweeps in place, as the single
+# domain does there
+@pytest.mark.parametrize("tiles,grid", [((2, 2), (32, 24)), ((2, 1), (32, 24)),
+                                        ((1, 2), (32, 24)), ((2, 1), (32, 21))])
+def test_tiled_equals_single_domain(tiles, grid):
     from les_halo_amd import LES, TiledLES
-    case = common.tiny_case(32, 24, 16, seed=21)
+    case = common.tiny_case(grid[0], grid[1], 16, seed=21)
     dims = case["dims"]
     L = LES(*dims, **common.REF_PHYS, pjuge=0.0, **SOR)
     T = TiledLES(*dims, sub_i=tiles[0], sub_j=tiles[1], **common.REF_PHYS, **SOR)
