@@ -1626,13 +1626,16 @@ void tiled_sor_split(les_tiled* T, hipStream_t st) {
     launch_split_scatter(st, h->d, fields_of(h), tile_split(T, (int)q), h->sc, 0);
   }
   launch_split_exchange(st, T->t[0]->d, x);
+  SorTiles sw{};
+  sw.ntiles = (int)T->t.size();
+  for (int q = 0; q < sw.ntiles; ++q) {
+    sw.sp[q] = tile_split(T, q);
+    sw.g[q] = grid_of(T->t[q]);
+    sw.sc[q] = T->t[q]->sc;
+  }
   for (int l = 1; l <= T->cfg.nmaxp; ++l)
     for (int nrd = 0; nrd <= 1; ++nrd) {
-      for (size_t q = 0; q < T->t.size(); ++q) {
-        les_handle* h = T->t[q];
-        launch_sor_split(st, h->d, grid_of(h), tile_split(T, (int)q), h->ph, nrd, 0, h->rb, 0,
-                         h->sc);
-      }
+      launch_sor_split_tiles(st, T->t[0]->d, sw, T->t[0]->ph, nrd);
       launch_split_exchange(st, T->t[0]->d, x);
     }
   for (size_t q = 0; q < T->t.size(); ++q) {

@@ -158,6 +158,15 @@ int sor_iter_supported(Dims d, const SplitSOR& sp);
 void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
                      DevScalars* sc, SorIterShape shape);
 void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc);
+// the sweep operands of every tile of a cut domain (k_sor_split_tiles)
+struct SorTiles {
+  SplitSOR sp[SPLIT_TILES_MAX];
+  Grid1D g[SPLIT_TILES_MAX];
+  const DevScalars* sc[SPLIT_TILES_MAX];
+  int ntiles;
+};
+// one colour sweep (no residual) of every tile, all tiles of the same size d
+void launch_sor_split_tiles(hipStream_t st, Dims d, const SorTiles& T, Physics ph, int nrd);
 // fills every tile's halo rows (both colours) and i-halo columns from its neighbours' interior
 void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T);
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,

@@ -68,16 +68,15 @@ void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScal
 }
 
 // ---- one colour sweep: 4 consecutive cells of one colour row per thread ----
+// L: the workgroup's index in the sweep (after the XCD remap); lds: its limb scratch
 template <bool UNIFORM, bool RESID>
-__global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp, Physics ph,
-                                                   int nrd, int64_t* partials,
-                                                   const DevScalars* sc, int ng, int nthreads) {
-  __shared__ unsigned long long lds[NLIMB];
-  if (sc->sor_done) return;
+__device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, const SplitSOR& sp,
+                                               const Physics& ph, int nrd, int64_t* partials,
+                                               int ng, int nthreads, int L,
+                                               unsigned long long* lds) {
   if (RESID) block_limbs_init(lds);
   ThreadAcc acc;
   acc.init();
-  const int L = xcd_tile(blockIdx.x, gridDim.x);
   const int t = L * 256 + threadIdx.x;
   const int tc = min(t, nthreads - 1);  // tail lanes index a valid group
   // tc = grp + ng * row, row = (j-1) + jp * (k-1); quotients by multiply-high (exact for the
@@ -173,6 +172,26 @@ __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp
     acc.flush(lds);
     block_limbs_store(lds, partials + (long)blockIdx.x * NLIMB);
   }
+}
+
+template <bool UNIFORM, bool RESID>
+__global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp, Physics ph,
+                                                   int nrd, int64_t* partials,
+                                                   const DevScalars* sc, int ng, int nthreads) {
+  __shared__ unsigned long long lds[NLIMB];
+  if (sc->sor_done) return;
+  sor_split_body<UNIFORM, RESID>(d, g, sp, ph, nrd, partials, ng, nthreads,
+                                 xcd_tile(blockIdx.x, gridDim.x), lds);
+}
+
+// one colour sweep of every tile of a cut domain (blockIdx.y = tile), no residual
+template <bool UNIFORM>
+__global__ __launch_bounds__(256) void k_sor_split_tiles(Dims d, SorTiles T, Physics ph, int nrd,
+                                                         int ng, int nthreads) {
+  const int q = blockIdx.y;
+  if (T.sc[q]->sor_done) return;
+  sor_split_body<UNIFORM, false>(d, T.g[q], T.sp[q], ph, nrd, nullptr, ng, nthreads,
+                                 xcd_tile(blockIdx.x, gridDim.x), nullptr);
 }
 
 // ---- one whole SOR iteration (colour 1 then colour 0, press.f95:90-100 with nrd = 0, 1) per
@@ -496,6 +515,18 @@ void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph,
   }
 }
 
+void launch_sor_split_tiles(hipStream_t st, Dims d, const SorTiles& T, Physics ph, int nrd) {
+  int ng, nt, nwg;
+  split_threads(d, T.sp[0], ng, nt, nwg);
+  const dim3 grid(nwg, T.ntiles), blk(256);
+  bool uniform = true;
+  for (int q = 0; q < T.ntiles; ++q) uniform = uniform && T.sp[q].uniform;
+  if (uniform)
+    hipLaunchKernelGGL((k_sor_split_tiles<true>), grid, blk, 0, st, d, T, ph, nrd, ng, nt);
+  else
+    hipLaunchKernelGGL((k_sor_split_tiles<false>), grid, blk, 0, st, d, T, ph, nrd, ng, nt);
+}
+
 // ---- pav partial sums (press.f95:125-132) from the split copy ----
 // Exact partial sums of dx1*dy1*dzn*p over the colour-split copy (press.f95:123-134).  A
 // thread takes one group of four split entries of row j in both colours over a chunk of k
@@ -565,16 +596,18 @@ __global__ __launch_bounds__(256) void k_split_exchange(Dims d, SplitTiles T) {
   const int q = blockIdx.y, job = blockIdx.z;
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   const int nr = T.nr, nj = T.nj;
-  if (job < 2) {
-    if (t >= 2L * d.kp * nr) return;
-    const int ii = (int)(t % nr);
-    const int rest = (int)(t / nr);
+  if (job < 2) {  // four entries per thread (rows are float4-aligned: nr % 4 == 0)
+    const int ng = nr / 4;
+    if (t >= 2L * d.kp * ng) return;
+    const int ii = 4 * (int)(t % ng);
+    const int rest = (int)(t / ng);
     const int c = rest & 1, k = 1 + (rest >> 1);
     const int src = job == 0 ? T.n[q] : T.s[q];
     const int srow = job == 0 ? 0 : d.jp - 1;
     const int drow = job == 0 ? d.jp : d.jp + 1;
     const long lev = (long)nr * nj * (k - 1);
-    T.p[q][c][ii + nr * drow + lev] = T.p[src][c][ii + nr * srow + lev];
+    *reinterpret_cast<float4*>(T.p[q][c] + ii + nr * drow + lev) =
+        *reinterpret_cast<const float4*>(T.p[src][c] + ii + nr * srow + lev);
   } else {
     if (t >= (long)d.jp * d.kp) return;
     const int j = 1 + (int)(t % d.jp), k = 1 + (int)(t / d.jp);
@@ -588,7 +621,7 @@ __global__ __launch_bounds__(256) void k_split_exchange(Dims d, SplitTiles T) {
 }
 
 void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T) {
-  const long n = std::max(2L * d.kp * T.nr, (long)d.jp * d.kp);
+  const long n = std::max(2L * d.kp * (T.nr / 4), (long)d.jp * d.kp);
   hipLaunchKernelGGL(k_split_exchange, dim3((unsigned)((n + 255) / 256), T.ntiles, 4), dim3(256),
                      0, st, d, T);
 }
