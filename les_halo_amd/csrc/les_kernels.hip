@@ -48,15 +48,23 @@ __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F
   }
   if (!do_velnw || i < 1) return;
   const float ro = ph.ro, dt = ph.dt;
-  const float p0 = F.p[d.P(i, j, k)];
+  // every operand is loaded before the first store (the stores could alias them for the
+  // compiler, which would otherwise wait for each load group in turn); p(i,j,kp+1), h and w at
+  // k = kp exist, the w update is skipped there
+  const Buf bp(F.p, d.nP()), bf(F.f, d.nF()), bg(F.g, d.nF()), bh(F.h, d.nF());
+  const Buf bu(F.u, d.nU()), bv(F.v, d.nU()), bw(F.w, d.nW());
   const long ifg = d.F(i, j, k);
-  float pz = xdiv<P2>((-p0 + F.p[d.P(i + 1, j, k)]) / ro, g.DXS(i));
-  F.u[iu] = F.u[iu] + dt * (F.f[ifg] - pz);
-  pz = xdiv<P2>((-p0 + F.p[d.P(i, j + 1, k)]) / ro, g.DYS(j));
-  F.v[iu] = F.v[iu] + dt * (F.g[ifg] - pz);
+  const float p0 = bp[d.P(i, j, k)], pe = bp[d.P(i + 1, j, k)];
+  const float pn = bp[d.P(i, j + 1, k)], pt = bp[d.P(i, j, k + 1)];
+  const float fv = bf[ifg], gv = bg[ifg], hv = bh[ifg];
+  const float uv = bu[iu], vv = bv[iu], wv = bw[iw];
+  float pz = xdiv<P2>((-p0 + pe) / ro, g.DXS(i));
+  bu.st(iu, uv + dt * (fv - pz));
+  pz = xdiv<P2>((-p0 + pn) / ro, g.DYS(j));
+  bv.st(iu, vv + dt * (gv - pz));
   if (k <= d.kp - 1) {
-    pz = ((-p0 + F.p[d.P(i, j, k + 1)]) / ro) / g.DZS(k);
-    F.w[iw] = F.w[iw] + dt * (F.h[ifg] - pz);
+    pz = ((-p0 + pt) / ro) / g.DZS(k);
+    bw.st(iw, wv + dt * (hv - pz));
   }
 }
 

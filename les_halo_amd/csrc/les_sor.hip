@@ -46,16 +46,23 @@ __global__ __launch_bounds__(256) void k_split_scatter(Dims d, Fields F, SplitSO
   const int k = 1 + blockIdx.z;
   if (ii >= sp.nr || j > d.jp) return;
   const int base = ii + sp.nr * ((j - 1) + sp.nj * (k - 1));
+  // both colours' loads before any store (a store could alias them for the compiler)
+  float pv[2], rv[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int i = min(first_i(c, j, k) + 2 * ii, d.ip);  // clamped: loaded, not used
+    pv[c] = F.p[d.P(i, j, k)];
+    rv[c] = F.rhs[d.R(i, j, k)];
+  }
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const int i = first_i(c, j, k) + 2 * ii;
     if (i > d.ip) continue;  // padding entries of the split rows stay untouched
-    sp.p[c][base] = F.p[d.P(i, j, k)];
-    const long ir = d.R(i, j, k);
-    float r = F.rhs[ir];
+    sp.p[c][base] = pv[c];
+    float r = rv[c];
     if (adjust_rhs) {
       r = r - sc->rhsav;
-      F.rhs[ir] = r;
+      F.rhs[d.R(i, j, k)] = r;
     }
     sp.r[c][base] = r;
   }
