@@ -27,6 +27,14 @@ __device__ __forceinline__ float mbit(uint8_t m, int b) { return (m >> b) & 1u ?
 // velnw.f95:27-59 fused with the inflow of bondv1_ext_wind_profile.f95:27-50
 // (state 1 of run_LES_kernel).  Threads cover i = 0..imax, j = 1..jm, k = 1..km.
 // ==========================================================================================
+// A thread takes VN_KPT consecutive levels of one (i,j) column: all their operands are loaded
+// before the first store (the stores could alias them for the compiler, which would otherwise
+// wait for each load group in turn), so more bytes are in flight per wavefront.  p(i,j,kp+1), h
+// and w at k = kp exist; the w update is skipped there.
+#ifndef LES_VELNW_KPT
+#define LES_VELNW_KPT 2
+#endif
+constexpr int VN_KPT = LES_VELNW_KPT;
 template <bool P2>
 __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F, Physics ph,
                                                       int do_velnw, int inflow_n, int imax,
@@ -35,43 +43,62 @@ __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F
   if (!strip_block(sg, bx, by, bz, L)) return;
   const int i = bx * BX + threadIdx.x;
   const int j = 1 + by * BY + threadIdx.y;
-  const int k = 1 + bz;
+  const int k0 = 1 + bz * VN_KPT;
   if (i > imax || j > d.jp) return;
-  const long iu = d.U(i, j, k);
-  const long iw = d.W(i, j, k);
   if (inflow_n > 0 && (i <= 1 || inflow_n == 1)) {
     // inflow planes i = 0,1 (and, at n == 1, the fill u(i,:,:) = u(1,:,:) for i = 2..im)
-    F.u[iu] = F.wind[(j - 1) + d.jp * (k - 1)];
-    F.v[iu] = 0.0f;
-    F.w[iw] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < VN_KPT; ++q) {
+      const int k = k0 + q;
+      if (k > d.kp) break;
+      F.u[d.U(i, j, k)] = F.wind[(j - 1) + d.jp * (k - 1)];
+      F.v[d.U(i, j, k)] = 0.0f;
+      F.w[d.W(i, j, k)] = 0.0f;
+    }
     return;
   }
   if (!do_velnw || i < 1) return;
   const float ro = ph.ro, dt = ph.dt;
-  // every operand is loaded before the first store (the stores could alias them for the
-  // compiler, which would otherwise wait for each load group in turn); p(i,j,kp+1), h and w at
-  // k = kp exist, the w update is skipped there
   const Buf bp(F.p, d.nP()), bf(F.f, d.nF()), bg(F.g, d.nF()), bh(F.h, d.nF());
   const Buf bu(F.u, d.nU()), bv(F.v, d.nU()), bw(F.w, d.nW());
-  const long ifg = d.F(i, j, k);
-  const float p0 = bp[d.P(i, j, k)], pe = bp[d.P(i + 1, j, k)];
-  const float pn = bp[d.P(i, j + 1, k)], pt = bp[d.P(i, j, k + 1)];
-  const float fv = bf[ifg], gv = bg[ifg], hv = bh[ifg];
-  const float uv = bu[iu], vv = bv[iu], wv = bw[iw];
-  float pz = xdiv<P2>((-p0 + pe) / ro, g.DXS(i));
-  bu.st(iu, uv + dt * (fv - pz));
-  pz = xdiv<P2>((-p0 + pn) / ro, g.DYS(j));
-  bv.st(iu, vv + dt * (gv - pz));
-  if (k <= d.kp - 1) {
-    pz = ((-p0 + pt) / ro) / g.DZS(k);
-    bw.st(iw, wv + dt * (hv - pz));
+  float p0[VN_KPT], pe[VN_KPT], pn[VN_KPT], pt[VN_KPT];
+  float fv[VN_KPT], gv[VN_KPT], hv[VN_KPT], uv[VN_KPT], vv[VN_KPT], wv[VN_KPT];
+#pragma unroll
+  for (int q = 0; q < VN_KPT; ++q) {
+    const int k = min(k0 + q, d.kp);  // levels past kp: loaded, not stored
+    const long ifg = d.F(i, j, k), iu = d.U(i, j, k);
+    p0[q] = bp[d.P(i, j, k)];
+    pe[q] = bp[d.P(i + 1, j, k)];
+    pn[q] = bp[d.P(i, j + 1, k)];
+    pt[q] = bp[d.P(i, j, k + 1)];
+    fv[q] = bf[ifg];
+    gv[q] = bg[ifg];
+    hv[q] = bh[ifg];
+    uv[q] = bu[iu];
+    vv[q] = bv[iu];
+    wv[q] = bw[d.W(i, j, k)];
+  }
+#pragma unroll
+  for (int q = 0; q < VN_KPT; ++q) {
+    const int k = k0 + q;
+    if (k > d.kp) break;
+    const long iu = d.U(i, j, k);
+    float pz = xdiv<P2>((-p0[q] + pe[q]) / ro, g.DXS(i));
+    bu.st(iu, uv[q] + dt * (fv[q] - pz));
+    pz = xdiv<P2>((-p0[q] + pn[q]) / ro, g.DYS(j));
+    bv.st(iu, vv[q] + dt * (gv[q] - pz));
+    if (k <= d.kp - 1) {
+      pz = ((-p0[q] + pt[q]) / ro) / g.DZS(k);
+      bw.st(d.W(i, j, k), wv[q] + dt * (hv[q] - pz));
+    }
   }
 }
 
 void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
                          int inflow_n) {
   int imax = (do_velnw || inflow_n == 1) ? d.ip : 1;
-  const StripGrid sg = strip_grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+  const StripGrid sg = strip_grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY,
+                                  (d.kp + VN_KPT - 1) / VN_KPT);
   if (g.p2xy)
     hipLaunchKernelGGL(k_velnw_inflow<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
                        do_velnw, inflow_n, imax, sg);
