@@ -313,7 +313,7 @@ __device__ inline double limbs_to_double(const int64_t* in) {
 // writes them to partials[block]; a one-workgroup finaliser adds all partials (integers,
 // exact) and rounds once.
 // ------------------------------------------------------------------------------------------
-constexpr int RED_KC = 6;  // k-chunks of the reduction grids
+constexpr int RED_KC = 8;  // k-chunks of the reduction grids
 
 __device__ inline void block_limbs_init(unsigned long long* lds) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;

@@ -640,18 +640,29 @@ __global__ __launch_bounds__(256) void k_split_gather_adj(Dims d, Fields F, Spli
                                                           const DevScalars* sc, int subtract_pav) {
   const int a = blockIdx.x * 64 + threadIdx.x;
   const int b = blockIdx.y * 4 + threadIdx.y;
-  const int c = blockIdx.z;
+  const int c0 = 2 * blockIdx.z;  // levels c0 and c0 + 1: both loads before the stores
   if (a > d.ip + 1 || b > d.jp + 1) return;
   const int i = (a == 0) ? 1 : ((a == d.ip + 1) ? d.ip : a);
   const int j = (b == 0) ? d.jp : ((b == d.jp + 1) ? 1 : b);
-  const int k = (c == 0) ? 1 : ((c == d.kp + 1) ? d.kp : c);
-  const float v = sp.p[(i + j + k) & 1][split_index(sp, i, j, k)];
-  F.p[d.P(a, b, c)] = subtract_pav ? v - sc->pav : v;
+  float v[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int c = min(c0 + q, d.kp + 1);
+    const int k = (c == 0) ? 1 : ((c == d.kp + 1) ? d.kp : c);
+    v[q] = sp.p[(i + j + k) & 1][split_index(sp, i, j, k)];
+  }
+  const float pav = subtract_pav ? sc->pav : 0.0f;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int c = c0 + q;
+    if (c > d.kp + 1) break;
+    F.p[d.P(a, b, c)] = subtract_pav ? v[q] - pav : v[q];
+  }
 }
 
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                              int subtract_pav) {
-  dim3 grid((d.ip + 2 + 63) / 64, (d.jp + 2 + 3) / 4, d.kp + 2);
+  dim3 grid((d.ip + 2 + 63) / 64, (d.jp + 2 + 3) / 4, (d.kp + 3) / 2);
   hipLaunchKernelGGL(k_split_gather_adj, grid, dim3(64, 4), 0, st, d, F, sp, sc, subtract_pav);
 }
 
