@@ -518,41 +518,91 @@ void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do
 // ==========================================================================================
 // les.f95:40-60: Smagorinsky eddy viscosity sm from the stored diu fields
 // ==========================================================================================
-// sm(i,j,k) at one interior point (les.f95:40-56)
-__device__ __forceinline__ float calc_sm_at(const Dims& d, const Grid1D& g, const Fields& F,
-                                            const Physics& ph, int i, int j, int k) {
-  const float q = 0.25f, two = 2.0f;
+// les.f95:40-56 at one interior point: the diu values sm(i,j,k) reads, loaded by sm_load
+// before any arithmetic, and the formula (sm_value) in the reference's operation order
+struct SmIn {
+  float d1, d5, d9, d2[4], d3[4], d4[4], d6[4], d7[4], d8[4];
+};
+__device__ __forceinline__ SmIn sm_load(const Dims& d, const Fields& F, int i, int j, int k) {
   const Buf diu2(F.diu2, d.nD()), diu3(F.diu3, d.nD()), diu4(F.diu4, d.nD());
   const Buf diu6(F.diu6, d.nD()), diu7(F.diu7, d.nD()), diu8(F.diu8, d.nD());
-  float dudxx1 = Buf(F.diu1, d.nD15())[d.D15(i, j, k)];
-  float dudyx1 = (diu2[d.D(i - 1, j, k)] + diu2[d.D(i - 1, j + 1, k)] + diu2[d.D(i, j, k)] + diu2[d.D(i, j + 1, k)]) * q;
-  float dudzx1 = (diu3[d.D(i - 1, j, k)] + diu3[d.D(i - 1, j, k + 1)] + diu3[d.D(i, j, k)] + diu3[d.D(i, j, k + 1)]) * q;
-  float dvdxx1 = (diu4[d.D(i, j, k)] + diu4[d.D(i, j - 1, k)] + diu4[d.D(i + 1, j, k)] + diu4[d.D(i + 1, j - 1, k)]) * q;
-  float dvdyx1 = Buf(F.diu5, d.nD15())[d.D15(i, j, k)];
-  float dvdzx1 = (diu6[d.D(i, j - 1, k)] + diu6[d.D(i, j - 1, k + 1)] + diu6[d.D(i, j, k)] + diu6[d.D(i, j, k + 1)]) * q;
-  float dwdxx1 = (diu7[d.D(i, j, k)] + diu7[d.D(i, j, k - 1)] + diu7[d.D(i + 1, j, k)] + diu7[d.D(i + 1, j, k - 1)]) * q;
-  float dwdyx1 = (diu8[d.D(i, j, k)] + diu8[d.D(i, j, k - 1)] + diu8[d.D(i, j + 1, k)] + diu8[d.D(i, j + 1, k - 1)]) * q;
-  float dwdzx1 = Buf(F.diu9, d.nD())[d.D(i, j, k)];
-  float cd = ph.cs0 * g.delx1[k - 1];
+  SmIn v;
+  v.d1 = Buf(F.diu1, d.nD15())[d.D15(i, j, k)];
+  v.d5 = Buf(F.diu5, d.nD15())[d.D15(i, j, k)];
+  v.d9 = Buf(F.diu9, d.nD())[d.D(i, j, k)];
+  v.d2[0] = diu2[d.D(i - 1, j, k)];
+  v.d2[1] = diu2[d.D(i - 1, j + 1, k)];
+  v.d2[2] = diu2[d.D(i, j, k)];
+  v.d2[3] = diu2[d.D(i, j + 1, k)];
+  v.d3[0] = diu3[d.D(i - 1, j, k)];
+  v.d3[1] = diu3[d.D(i - 1, j, k + 1)];
+  v.d3[2] = diu3[d.D(i, j, k)];
+  v.d3[3] = diu3[d.D(i, j, k + 1)];
+  v.d4[0] = diu4[d.D(i, j, k)];
+  v.d4[1] = diu4[d.D(i, j - 1, k)];
+  v.d4[2] = diu4[d.D(i + 1, j, k)];
+  v.d4[3] = diu4[d.D(i + 1, j - 1, k)];
+  v.d6[0] = diu6[d.D(i, j - 1, k)];
+  v.d6[1] = diu6[d.D(i, j - 1, k + 1)];
+  v.d6[2] = diu6[d.D(i, j, k)];
+  v.d6[3] = diu6[d.D(i, j, k + 1)];
+  v.d7[0] = diu7[d.D(i, j, k)];
+  v.d7[1] = diu7[d.D(i, j, k - 1)];
+  v.d7[2] = diu7[d.D(i + 1, j, k)];
+  v.d7[3] = diu7[d.D(i + 1, j, k - 1)];
+  v.d8[0] = diu8[d.D(i, j, k)];
+  v.d8[1] = diu8[d.D(i, j, k - 1)];
+  v.d8[2] = diu8[d.D(i, j + 1, k)];
+  v.d8[3] = diu8[d.D(i, j + 1, k - 1)];
+  return v;
+}
+__device__ __forceinline__ float sm_value(const SmIn& v, float delx1k, float cs0) {
+  const float q = 0.25f, two = 2.0f;
+  float dudxx1 = v.d1;
+  float dudyx1 = (v.d2[0] + v.d2[1] + v.d2[2] + v.d2[3]) * q;
+  float dudzx1 = (v.d3[0] + v.d3[1] + v.d3[2] + v.d3[3]) * q;
+  float dvdxx1 = (v.d4[0] + v.d4[1] + v.d4[2] + v.d4[3]) * q;
+  float dvdyx1 = v.d5;
+  float dvdzx1 = (v.d6[0] + v.d6[1] + v.d6[2] + v.d6[3]) * q;
+  float dwdxx1 = (v.d7[0] + v.d7[1] + v.d7[2] + v.d7[3]) * q;
+  float dwdyx1 = (v.d8[0] + v.d8[1] + v.d8[2] + v.d8[3]) * q;
+  float dwdzx1 = v.d9;
+  float cd = cs0 * delx1k;
   float s = two * (dudxx1 * dudxx1 + dvdyx1 * dvdyx1 + dwdzx1 * dwdzx1) +
             (dudyx1 + dvdxx1) * (dudyx1 + dvdxx1) + (dwdyx1 + dvdzx1) * (dwdyx1 + dvdzx1) +
             (dudzx1 + dwdxx1) * (dudzx1 + dwdxx1);
   return (cd * cd) * sqrtf(s);
 }
 
+// A thread takes SM_KPT consecutive levels of one column, every load before the first store
+// (which could alias them for the compiler); one level measured fastest (2 or 4 levels: +2 %)
+#ifndef LES_SM_KPT
+#define LES_SM_KPT 1
+#endif
+constexpr int SM_KPT = LES_SM_KPT;
 __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
                                                  StripGrid sg) {
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;
   const int i = 1 + bx * BX + threadIdx.x;
   const int j = 1 + by * BY + threadIdx.y;
-  const int k = 1 + bz;
+  const int k0 = 1 + bz * SM_KPT;
   if (i > d.ip || j > d.jp) return;
-  Buf(F.sm, d.nS()).st(d.S(i, j, k), calc_sm_at(d, g, F, ph, i, j, k));
+  SmIn v[SM_KPT];
+#pragma unroll
+  for (int q = 0; q < SM_KPT; ++q) v[q] = sm_load(d, F, i, j, min(k0 + q, d.kp));
+  const Buf bsm(F.sm, d.nS());
+#pragma unroll
+  for (int q = 0; q < SM_KPT; ++q) {
+    const int k = k0 + q;
+    if (k > d.kp) break;
+    bsm.st(d.S(i, j, k), sm_value(v[q], g.delx1[k - 1], ph.cs0));
+  }
 }
 
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
-  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY,
+                                  (d.kp + SM_KPT - 1) / SM_KPT);
   hipLaunchKernelGGL(k_calc_sm, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, sg);
 }
 
