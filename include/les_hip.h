@@ -412,6 +412,37 @@ int les_gen_inflow(int jp, int kp, const float* z2, const float* dzn, float usta
 int les_initial_wind_profile(int jp, int kp, const float* z2, const float* dzn,
                              float* wind_profile);
 
+/* The adjoint form.  No entry point of the reference corresponds: the reference has no
+ * derivative.  One handle holds the step's arrays in FP32 (fp64 = 0) or FP64 and runs the step
+ * of main_ocl.f95:264-299 for n >= 2 (EXTERNAL_WIND_PROFILE) as generated statement kernels,
+ * forward (les_adj_step) and transposed in reverse order (les_adj_reverse_step: one step's
+ * vector-Jacobian product, the step recomputed from the current state first).  feedbf's beta
+ * term is beta * cd(i,j,k) * u * mask with a drag coefficient field cd (field LES_ADJ_CD, 1 =
+ * the reference).  Fields go in and out as doubles in the reference's Fortran bounds;
+ * les_adj_{upload,download}_adjoint address the adjoints of the carried state (u, v, w, p, f, g,
+ * h, fold, gold, hold, usum, vsum, wsum) and of the controls (LES_WIND_PROFILE, LES_ADJ_CD).
+ * les_adj_set_sums gives press.f95's volume sums (area, pco).  les_adj_window advances nsteps
+ * steps from the current state with a checkpoint every ckpt steps, returns the objective
+ * J = mean over the steps and the nprobes (i,j,k) probes of u after each step, and leaves the
+ * gradient of J in the adjoints of the controls and of the initial state (recomputing each
+ * segment from its checkpoint and each step from its state); the handle's state is the state
+ * after the window. */
+#define LES_ADJ_CD 100
+typedef struct les_adj les_adj;
+int les_adj_create(const les_config* cfg, int fp64, les_adj** out);
+int les_adj_destroy(les_adj* h);
+const char* les_adj_last_error(const les_adj* h);
+int les_adj_upload(les_adj* h, int field, const double* host);
+int les_adj_download(les_adj* h, int field, double* host);
+int les_adj_upload_adjoint(les_adj* h, int field, const double* host);
+int les_adj_download_adjoint(les_adj* h, int field, double* host);
+int les_adj_set_sums(les_adj* h, double area, double pco);
+int les_adj_zero_adjoints(les_adj* h);
+int les_adj_step(les_adj* h, int nsteps);
+int les_adj_reverse_step(les_adj* h);
+int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probes, int nprobes,
+                   double* objective);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,18 @@ PROTOS = {
     "les_tiled_ifdata": (c_int, [c_void_p]),
     "les_tiled_step": (c_int, [c_void_p, c_int]),
     "les_tiled_run": (c_int, [c_void_p, c_int, c_int]),
+    "les_adj_create": (c_int, [ctypes.POINTER(LesConfig), c_int, ctypes.POINTER(c_void_p)]),
+    "les_adj_destroy": (c_int, [c_void_p]),
+    "les_adj_last_error": (ctypes.c_char_p, [c_void_p]),
+    "les_adj_upload": (c_int, [c_void_p, c_int, c_void_p]),
+    "les_adj_download": (c_int, [c_void_p, c_int, c_void_p]),
+    "les_adj_upload_adjoint": (c_int, [c_void_p, c_int, c_void_p]),
+    "les_adj_download_adjoint": (c_int, [c_void_p, c_int, c_void_p]),
+    "les_adj_set_sums": (c_int, [c_void_p, c_double, c_double]),
+    "les_adj_zero_adjoints": (c_int, [c_void_p]),
+    "les_adj_step": (c_int, [c_void_p, c_int]),
+    "les_adj_reverse_step": (c_int, [c_void_p]),
+    "les_adj_window": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, ctypes.POINTER(c_double)]),
     "les_anime_arrays": (c_int, [c_int] * 6 + [c_void_p] * 7 + [ctypes.c_char_p] * 2),
     "les_upload": (c_int, [H, c_int, c_void_p]),
     "les_download": (c_int, [H, c_int, c_void_p]),

@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libles_hip.so")
 
 SOURCES = ["les_api.hip", "les_kernels.hip", "les_sor.hip", "les_mixed.hip", "les_ave.hip",
-           "les_gen.cpp", "les_restart.cpp", "les_output.cpp"]
+           "les_adj.hip", "les_gen.cpp", "les_restart.cpp", "les_output.cpp"]
 
 # -ffp-contract=off: every FP32 operation rounds on its own, as in the reference.
 # Correctly rounded FP32 division/sqrt (the HIP default, stated explicitly).
@@ -46,6 +46,8 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
+    from les_halo_amd import adjgen  # the adjoint form's statement kernels
+    adjgen.write()
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"

@@ -1,0 +1,630 @@
+// The adjoint form of the LES step: forward and adjoint statement kernels (generated from the
+// statement list in les_halo_amd/adjgen.py into les_adj_gen.inc) in FP32 or FP64, the
+// reductions the step makes (uout, rhsav, pav) and their adjoints, and a window driver with
+// timestep checkpointing: the gradient of an objective (the mean of u over probe points and
+// steps) with respect to the inflow wind profile and a per-cell drag coefficient field.
+//
+// State carried from step to step: u, v, w, p, f, g, h, fold, gold, hold, usum, vsum, wsum.
+// Every other array is written by the step before it is read (or never written: a constant).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "les_hip.h"
+
+namespace les_adj_impl {
+
+template <typename T>
+struct AdjScal {
+  T uout, rhsav, pav;
+};
+template <typename T>
+struct AdjPar {
+  T dt, ro, vn, alpha, beta, cs0, omega, area, pco;
+};
+
+// sum over the wavefront, one atomic add per wavefront (every lane calls it)
+template <typename T>
+__device__ __forceinline__ void wave_add(T* p, T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0 && v != T(0)) atomicAdd(p, v);
+}
+
+#include "les_adj_gen.inc"
+
+// ---- reductions of the step and their adjoints -----------------------------------------------
+constexpr int RED_THREADS = 1024;
+
+// bondv1_ext_wind_profile.f95:67-75 on the pre-outflow u: uout = (max(0, max u(im,:,:)) +
+// min(0, min u(im,:,:))) / 2 (one workgroup)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_uout(AdjArr<T> A, AdjScal<T>* S, int im, int jm,
+                                                       int km) {
+  __shared__ T smax[RED_THREADS], smin[RED_THREADS];
+  T mx = T(0), mn = T(0);
+  for (int t = threadIdx.x; t < jm * km; t += RED_THREADS) {
+    const int j = 1 + t % jm, k = 1 + t / jm;
+    const T x = A.u_pre[(long)im + (long)(im + 2) * ((long)(j + 1) + (long)(jm + 3) * k)];
+    mx = max(mx, x);
+    mn = min(mn, x);
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
+      smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S->uout = (smax[0] + smin[0]) / T(2);
+}
+
+// adjoint: the max (if positive) and the min (if negative) take half of uout's adjoint each; the
+// element is the first (in k, then j) holding the extreme value
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_uout_bw(AdjArr<T> A, AdjArr<T> B, AdjScal<T>* SB,
+                                                          int im, int jm, int km) {
+  __shared__ T smax[RED_THREADS], smin[RED_THREADS];
+  __shared__ int amax[RED_THREADS], amin[RED_THREADS];
+  T mx = T(0), mn = T(0);
+  int ax = -1, an = -1;
+  for (int t = threadIdx.x; t < jm * km; t += RED_THREADS) {
+    const int j = 1 + t % jm, k = 1 + t / jm;
+    const T x = A.u_pre[(long)im + (long)(im + 2) * ((long)(j + 1) + (long)(jm + 3) * k)];
+    if (x > mx) {
+      mx = x;
+      ax = t;
+    }
+    if (x < mn) {
+      mn = x;
+      an = t;
+    }
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  amax[threadIdx.x] = ax;
+  amin[threadIdx.x] = an;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const int q = threadIdx.x + o;
+      if (smax[q] > smax[threadIdx.x] ||
+          (smax[q] == smax[threadIdx.x] && amax[q] >= 0 && (amax[threadIdx.x] < 0 || amax[q] < amax[threadIdx.x]))) {
+        smax[threadIdx.x] = smax[q];
+        amax[threadIdx.x] = amax[q];
+      }
+      if (smin[q] < smin[threadIdx.x] ||
+          (smin[q] == smin[threadIdx.x] && amin[q] >= 0 && (amin[threadIdx.x] < 0 || amin[q] < amin[threadIdx.x]))) {
+        smin[threadIdx.x] = smin[q];
+        amin[threadIdx.x] = amin[q];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const T half = SB->uout / T(2);
+    auto at = [&](int t) {
+      const int j = 1 + t % jm, k = 1 + t / jm;
+      return (long)im + (long)(im + 2) * ((long)(j + 1) + (long)(jm + 3) * k);
+    };
+    if (amax[0] >= 0 && smax[0] > T(0)) B.u[at(amax[0])] += half;
+    if (amin[0] >= 0 && smin[0] < T(0)) B.u[at(amin[0])] += half;
+    SB->uout = T(0);
+  }
+}
+
+// which: 0 rhsav = sum dx1*dy1*dzn*rhs / area (press.f95:67-75); 1 pav = sum p*dx1*dy1*dzn / pco
+// (press.f95:125-134).  Per-block FP64 partials in a fixed order, then one workgroup adds them
+// in a fixed order: deterministic.
+template <typename T>
+__device__ __forceinline__ T wsum_term(const AdjArr<T>& A, int which, int i, int j, int k, int im,
+                                       int jm) {
+  const T dx = A.dx1[i + 1], dy = A.dy1[j], dz = A.dzn[k + 1];
+  if (which == 0) return dx * dy * dz * A.rhs[(long)i + (long)(im + 2) * ((long)j + (long)(jm + 2) * k)];
+  return A.p[(long)i + (long)(im + 3) * ((long)j + (long)(jm + 3) * k)] * dx * dy * dz;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_wsum_part(AdjArr<T> A, int which, double* part, int im,
+                                                   int jm, int km) {
+  __shared__ double s[256];
+  const long n = (long)im * jm * km;
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int i = 1 + (int)(t % im), j = 1 + (int)((t / im) % jm), k = 1 + (int)(t / ((long)im * jm));
+    acc += (double)wsum_term(A, which, i, j, k, im, jm);
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_wsum_final(const double* part, int nb, int which,
+                                                    AdjScal<T>* S, AdjPar<T> P) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < nb; q += 256) acc += part[q];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (which == 0)
+      S->rhsav = (T)s[0] / P.area;
+    else
+      S->pav = (T)s[0] / P.pco;
+  }
+}
+
+// adjoint: d(sum w*x / c)/dx = w / c
+template <typename T>
+__global__ __launch_bounds__(256) void k_wsum_bw(AdjArr<T> A, AdjArr<T> B, AdjScal<T>* SB, int which,
+                                                 AdjPar<T> P, int im, int jm, int km) {
+  const long n = (long)im * jm * km;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const int i = 1 + (int)(t % im), j = 1 + (int)((t / im) % jm), k = 1 + (int)(t / ((long)im * jm));
+  const T dx = A.dx1[i + 1], dy = A.dy1[j], dz = A.dzn[k + 1];
+  if (which == 0)
+    B.rhs[(long)i + (long)(im + 2) * ((long)j + (long)(jm + 2) * k)] += SB->rhsav * (dx * dy * dz) / P.area;
+  else
+    B.p[(long)i + (long)(im + 3) * ((long)j + (long)(jm + 3) * k)] += SB->pav * (dx * dy * dz) / P.pco;
+}
+
+template <typename T>
+__global__ void k_zero_scal(AdjScal<T>* S) {
+  S->uout = S->rhsav = S->pav = T(0);
+}
+
+// objective: acc += sum over probes of u at the probe (i, j, k triples)
+template <typename T>
+__global__ void k_probe_sum(AdjArr<T> A, const int* probes, int np, double* acc, int im, int jm) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int q = 0; q < np; ++q) {
+    const int i = probes[3 * q], j = probes[3 * q + 1], k = probes[3 * q + 2];
+    s += (double)A.u[(long)i + (long)(im + 2) * ((long)(j + 1) + (long)(jm + 3) * k)];
+  }
+  *acc += s;
+}
+
+template <typename T>
+__global__ void k_probe_seed(AdjArr<T> B, const int* probes, int np, T wgt, int im, int jm) {
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q < np; ++q) {
+    const int i = probes[3 * q], j = probes[3 * q + 1], k = probes[3 * q + 2];
+    B.u[(long)i + (long)(im + 2) * ((long)(j + 1) + (long)(jm + 3) * k)] += wgt;
+  }
+}
+
+const int CARRIED[] = {ADJ_A_u, ADJ_A_v, ADJ_A_w, ADJ_A_p, ADJ_A_f, ADJ_A_g, ADJ_A_h, ADJ_A_fold,
+                       ADJ_A_gold, ADJ_A_hold, ADJ_A_usum, ADJ_A_vsum, ADJ_A_wsum};
+constexpr int NCARRIED = sizeof(CARRIED) / sizeof(CARRIED[0]);
+
+}  // namespace les_adj_impl
+
+using namespace les_adj_impl;
+
+struct les_adj {
+  les_config cfg;
+  int fp64 = 0;
+  int im = 0, jm = 0, km = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  void* A[ADJ_NARRAYS] = {};  // forward arrays (T)
+  void* B[ADJ_NARRAYS] = {};  // adjoints (T), active arrays only
+  long n[ADJ_NARRAYS] = {};
+  void* S = nullptr;   // AdjScal<T>
+  void* SB = nullptr;  // adjoint scalars
+  double* part = nullptr;
+  int npart = 0;
+  double* jacc = nullptr;
+  int* probes = nullptr;
+  int nprobes_cap = 0;
+  double area = 0.0, pco = 0.0;
+};
+
+namespace {
+
+int afail(les_adj* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define ACHK(h, x)                                                                  \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return afail(h, LES_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t tsize(const les_adj* h) { return h->fp64 ? sizeof(double) : sizeof(float); }
+
+template <typename T>
+AdjArr<T> arrs(les_adj* h, bool adj) {
+  AdjArr<T> a{};
+  for (int q = 0; q < ADJ_NARRAYS; ++q) adj_slot(a, q) = static_cast<T*>(adj ? h->B[q] : h->A[q]);
+  return a;
+}
+
+template <typename T>
+AdjPar<T> pars(const les_adj* h) {
+  const les_config& c = h->cfg;
+  return AdjPar<T>{(T)c.dt, (T)c.ro, (T)c.vn, (T)c.alpha, (T)c.beta, (T)c.cs0, (T)c.omega,
+                   (T)h->area, (T)h->pco};
+}
+
+// field id of les_hip.h (or LES_ADJ_CD) -> adjoint array index
+int array_of_field(int f) {
+  static const char* names[LES_NFIELDS] = {
+      "p", "u", "v", "w", "usum", "vsum", "wsum", "f", "g", "h", "fold", "gold", "hold", "rhs", "sm",
+      "diu1", "diu2", "diu3", "diu4", "diu5", "diu6", "diu7", "diu8", "diu9", nullptr, "bmask1",
+      "cmask1", "dmask1", "cn1", "cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s", "dx1", "dxs", "dy1",
+      "dys", "dzn", "dzs", nullptr, "delx1", "wind"};
+  const char* nm = nullptr;
+  if (f == LES_ADJ_CD) nm = "cd";
+  else if (f >= 0 && f < LES_NFIELDS) nm = names[f];
+  if (!nm) return -1;
+  for (int q = 0; q < ADJ_NARRAYS; ++q)
+    if (!strcmp(ADJ_ARRAY_NAMES[q], nm)) return q;
+  return -1;
+}
+
+template <typename T>
+void run_sum(les_adj* h, int which) {
+  AdjArr<T> A = arrs<T>(h, false);
+  hipLaunchKernelGGL(k_wsum_part<T>, dim3(h->npart), dim3(256), 0, h->st, A, which, h->part, h->im,
+                     h->jm, h->km);
+  hipLaunchKernelGGL(k_wsum_final<T>, dim3(1), dim3(256), 0, h->st, h->part, h->npart, which,
+                     static_cast<AdjScal<T>*>(h->S), pars<T>(h));
+}
+
+template <typename T>
+void run_ops_fw(les_adj* h, const int* ops, int nops) {
+  const AdjArr<T> A = arrs<T>(h, false);
+  const AdjPar<T> P = pars<T>(h);
+  AdjScal<T>* S = static_cast<AdjScal<T>*>(h->S);
+  for (int q = 0; q < nops; ++q) {
+    const int op = ops[q];
+    if (op >= 0) {
+      adj_launch_fw<T>(op, h->st, A, S, P, h->im, h->jm, h->km);
+    } else if (op == ADJ_OP_SNAP) {
+      for (const char* nm : {"u", "v", "w"}) {
+        int a = -1, b = -1;
+        for (int z = 0; z < ADJ_NARRAYS; ++z) {
+          if (!strcmp(ADJ_ARRAY_NAMES[z], nm)) a = z;
+          if (!strcmp(ADJ_ARRAY_NAMES[z], (std::string(nm) + "_pre").c_str())) b = z;
+        }
+        hipMemcpyAsync(h->A[b], h->A[a], h->n[a] * tsize(h), hipMemcpyDeviceToDevice, h->st);
+      }
+    } else if (op == ADJ_OP_UOUT) {
+      hipLaunchKernelGGL(k_uout<T>, dim3(1), dim3(RED_THREADS), 0, h->st, A, S, h->im, h->jm, h->km);
+    } else if (op == ADJ_OP_RHSAV) {
+      run_sum<T>(h, 0);
+    } else if (op == ADJ_OP_PAV) {
+      run_sum<T>(h, 1);
+    } else if (op == ADJ_OP_SOR) {
+      for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_fw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
+    }
+  }
+}
+
+template <typename T>
+void run_ops_bw(les_adj* h, const int* ops, int nops) {
+  const AdjArr<T> A = arrs<T>(h, false), B = arrs<T>(h, true);
+  const AdjPar<T> P = pars<T>(h);
+  AdjScal<T>* S = static_cast<AdjScal<T>*>(h->S);
+  AdjScal<T>* SB = static_cast<AdjScal<T>*>(h->SB);
+  const long ncell = (long)h->im * h->jm * h->km;
+  for (int q = nops - 1; q >= 0; --q) {
+    const int op = ops[q];
+    if (op >= 0) {
+      adj_launch_bw<T>(op, h->st, A, B, S, SB, P, h->im, h->jm, h->km);
+    } else if (op == ADJ_OP_UOUT) {
+      hipLaunchKernelGGL(k_uout_bw<T>, dim3(1), dim3(RED_THREADS), 0, h->st, A, B, SB, h->im, h->jm,
+                         h->km);
+    } else if (op == ADJ_OP_RHSAV || op == ADJ_OP_PAV) {
+      const int which = op == ADJ_OP_RHSAV ? 0 : 1;
+      hipLaunchKernelGGL(k_wsum_bw<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, h->st,
+                         A, B, SB, which, P, h->im, h->jm, h->km);
+      hipLaunchKernelGGL(k_zero_scal<T>, dim3(1), dim3(1), 0, h->st, SB);
+    } else if (op == ADJ_OP_SOR) {
+      for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_bw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
+    }
+  }
+}
+
+void step_fw(les_adj* h) {
+  if (h->fp64)
+    run_ops_fw<double>(h, ADJ_STEP_OPS, ADJ_STEP_NOPS);
+  else
+    run_ops_fw<float>(h, ADJ_STEP_OPS, ADJ_STEP_NOPS);
+}
+
+void step_bw(les_adj* h) {
+  if (h->fp64)
+    run_ops_bw<double>(h, ADJ_STEP_OPS, ADJ_STEP_NOPS);
+  else
+    run_ops_bw<float>(h, ADJ_STEP_OPS, ADJ_STEP_NOPS);
+}
+
+bool is_carried(int a) {
+  for (int q = 0; q < NCARRIED; ++q)
+    if (CARRIED[q] == a) return true;
+  return false;
+}
+
+// the adjoints of everything but the carried state and the controls (wind, cd) start each
+// reversed step at zero
+int zero_temporaries(les_adj* h) {
+  for (int q = 0; q < ADJ_NARRAYS; ++q)
+    if (h->B[q] && !is_carried(q) && q != ADJ_A_wind && q != ADJ_A_cd)
+      ACHK(h, hipMemsetAsync(h->B[q], 0, h->n[q] * tsize(h), h->st));
+  ACHK(h, hipMemsetAsync(h->SB, 0, 3 * tsize(h), h->st));
+  return LES_OK;
+}
+
+int save_state(les_adj* h, char* dst) {
+  for (int q = 0; q < NCARRIED; ++q) {
+    const int a = CARRIED[q];
+    ACHK(h, hipMemcpyAsync(dst, h->A[a], h->n[a] * tsize(h), hipMemcpyDeviceToDevice, h->st));
+    dst += h->n[a] * tsize(h);
+  }
+  return LES_OK;
+}
+
+int load_state(les_adj* h, const char* src) {
+  for (int q = 0; q < NCARRIED; ++q) {
+    const int a = CARRIED[q];
+    ACHK(h, hipMemcpyAsync(h->A[a], src, h->n[a] * tsize(h), hipMemcpyDeviceToDevice, h->st));
+    src += h->n[a] * tsize(h);
+  }
+  return LES_OK;
+}
+
+size_t state_bytes(const les_adj* h) {
+  size_t b = 0;
+  for (int q = 0; q < NCARRIED; ++q) b += h->n[CARRIED[q]] * tsize(h);
+  return b;
+}
+
+int check(les_adj* h) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return afail(h, LES_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return LES_OK;
+}
+
+int set_probes(les_adj* h, const int* probes, int np) {
+  if (np > h->nprobes_cap) {
+    if (h->probes) hipFree(h->probes);
+    h->probes = nullptr;
+    ACHK(h, hipMalloc(&h->probes, 3 * sizeof(int) * np));
+    h->nprobes_cap = np;
+  }
+  for (int q = 0; q < np; ++q) {
+    const int i = probes[3 * q], j = probes[3 * q + 1], k = probes[3 * q + 2];
+    if (i < 1 || i > h->im || j < 1 || j > h->jm || k < 1 || k > h->km)
+      return afail(h, LES_E_INVALID, "probe outside the interior");
+  }
+  ACHK(h, hipMemcpyAsync(h->probes, probes, 3 * sizeof(int) * np, hipMemcpyHostToDevice, h->st));
+  return LES_OK;
+}
+
+void seed_probes(les_adj* h, int np, double wgt) {
+  if (h->fp64)
+    hipLaunchKernelGGL(k_probe_seed<double>, dim3(1), dim3(64), 0, h->st, arrs<double>(h, true),
+                       h->probes, np, wgt, h->im, h->jm);
+  else
+    hipLaunchKernelGGL(k_probe_seed<float>, dim3(1), dim3(64), 0, h->st, arrs<float>(h, true),
+                       h->probes, np, (float)wgt, h->im, h->jm);
+}
+
+void sum_probes(les_adj* h, int np) {
+  if (h->fp64)
+    hipLaunchKernelGGL(k_probe_sum<double>, dim3(1), dim3(64), 0, h->st, arrs<double>(h, false),
+                       h->probes, np, h->jacc, h->im, h->jm);
+  else
+    hipLaunchKernelGGL(k_probe_sum<float>, dim3(1), dim3(64), 0, h->st, arrs<float>(h, false),
+                       h->probes, np, h->jacc, h->im, h->jm);
+}
+
+}  // namespace
+
+extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
+  if (!cfg || !out || cfg->ip < 2 || cfg->jp < 2 || cfg->kp < 2 || cfg->nmaxp < 0) return LES_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_E_NODEV;
+  les_adj* h = new les_adj();
+  h->cfg = *cfg;
+  h->fp64 = fp64 ? 1 : 0;
+  h->im = cfg->ip;
+  h->jm = cfg->jp;
+  h->km = cfg->kp;
+  auto bail = [&](int code) {
+    les_adj_destroy(h);
+    return code;
+  };
+  if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreate(&h->st) != hipSuccess)
+    return bail(LES_E_HIP);
+  for (int q = 0; q < ADJ_NARRAYS; ++q) {
+    h->n[q] = adj_array_size(q, h->im, h->jm, h->km);
+    const size_t bytes = h->n[q] * tsize(h);
+    if (hipMalloc(&h->A[q], bytes) != hipSuccess) return bail(LES_E_NOMEM);
+    if (hipMemset(h->A[q], 0, bytes) != hipSuccess) return bail(LES_E_HIP);
+    if (!ADJ_PASSIVE[q]) {
+      if (hipMalloc(&h->B[q], bytes) != hipSuccess) return bail(LES_E_NOMEM);
+      if (hipMemset(h->B[q], 0, bytes) != hipSuccess) return bail(LES_E_HIP);
+    }
+  }
+  // the drag coefficient of the reference's feedback (beta * u * mask) is 1
+  {
+    const int a = ADJ_A_cd;
+    std::vector<double> one((size_t)h->n[a], 1.0);
+    les_adj* hh = h;
+    const int r = les_adj_upload(hh, LES_ADJ_CD, one.data());
+    if (r) return bail(r);
+  }
+  h->npart = 512;
+  if (hipMalloc(&h->S, 3 * sizeof(double)) != hipSuccess || hipMalloc(&h->SB, 3 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&h->part, h->npart * sizeof(double)) != hipSuccess ||
+      hipMalloc(&h->jacc, sizeof(double)) != hipSuccess)
+    return bail(LES_E_NOMEM);
+  if (hipMemset(h->S, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(h->SB, 0, 3 * sizeof(double)) != hipSuccess)
+    return bail(LES_E_HIP);
+  *out = h;
+  return LES_OK;
+}
+
+extern "C" int les_adj_destroy(les_adj* h) {
+  if (!h) return LES_E_INVALID;
+  if (h->st) hipStreamSynchronize(h->st);
+  for (int q = 0; q < ADJ_NARRAYS; ++q) {
+    if (h->A[q]) hipFree(h->A[q]);
+    if (h->B[q]) hipFree(h->B[q]);
+  }
+  for (void* p : {h->S, h->SB, (void*)h->part, (void*)h->jacc, (void*)h->probes})
+    if (p) hipFree(p);
+  if (h->st) hipStreamDestroy(h->st);
+  delete h;
+  return LES_OK;
+}
+
+extern "C" const char* les_adj_last_error(const les_adj* h) { return h ? h->err.c_str() : ""; }
+
+static int xfer(les_adj* h, int field, double* host, bool up, bool adj) {
+  if (!h || !host) return LES_E_INVALID;
+  const int a = array_of_field(field);
+  if (a < 0) return afail(h, LES_E_INVALID, "field not held by the adjoint form");
+  void* dev = adj ? h->B[a] : h->A[a];
+  if (!dev) return afail(h, LES_E_INVALID, "no adjoint is kept for this field");
+  const long n = h->n[a];
+  ACHK(h, hipStreamSynchronize(h->st));
+  if (h->fp64) {
+    if (up)
+      ACHK(h, hipMemcpy(dev, host, n * sizeof(double), hipMemcpyHostToDevice));
+    else
+      ACHK(h, hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp((size_t)n);
+    if (up) {
+      for (long q = 0; q < n; ++q) tmp[q] = (float)host[q];
+      ACHK(h, hipMemcpy(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      ACHK(h, hipMemcpy(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (long q = 0; q < n; ++q) host[q] = tmp[q];
+    }
+  }
+  return LES_OK;
+}
+
+extern "C" int les_adj_upload(les_adj* h, int field, const double* host) {
+  return xfer(h, field, const_cast<double*>(host), true, false);
+}
+extern "C" int les_adj_download(les_adj* h, int field, double* host) {
+  return xfer(h, field, host, false, false);
+}
+extern "C" int les_adj_upload_adjoint(les_adj* h, int field, const double* host) {
+  return xfer(h, field, const_cast<double*>(host), true, true);
+}
+extern "C" int les_adj_download_adjoint(les_adj* h, int field, double* host) {
+  return xfer(h, field, host, false, true);
+}
+
+extern "C" int les_adj_set_sums(les_adj* h, double area, double pco) {
+  if (!h) return LES_E_INVALID;
+  h->area = area;
+  h->pco = pco;
+  return LES_OK;
+}
+
+extern "C" int les_adj_step(les_adj* h, int nsteps) {
+  if (!h || nsteps < 0) return LES_E_INVALID;
+  if (h->area <= 0.0 || h->pco <= 0.0) return afail(h, LES_E_STATE, "set the volume sums first");
+  for (int s = 0; s < nsteps; ++s) step_fw(h);
+  return check(h);
+}
+
+extern "C" int les_adj_reverse_step(les_adj* h) {
+  if (!h) return LES_E_INVALID;
+  if (h->area <= 0.0 || h->pco <= 0.0) return afail(h, LES_E_STATE, "set the volume sums first");
+  int r;
+  step_fw(h);
+  if ((r = zero_temporaries(h))) return r;
+  step_bw(h);
+  return check(h);
+}
+
+extern "C" int les_adj_zero_adjoints(les_adj* h) {
+  if (!h) return LES_E_INVALID;
+  for (int q = 0; q < ADJ_NARRAYS; ++q)
+    if (h->B[q]) ACHK(h, hipMemsetAsync(h->B[q], 0, h->n[q] * tsize(h), h->st));
+  ACHK(h, hipMemsetAsync(h->SB, 0, 3 * tsize(h), h->st));
+  return check(h);
+}
+
+extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probes, int nprobes,
+                              double* objective) {
+  if (!h || nsteps < 1 || ckpt < 1 || !probes || nprobes < 1 || !objective) return LES_E_INVALID;
+  if (h->area <= 0.0 || h->pco <= 0.0) return afail(h, LES_E_STATE, "set the volume sums first");
+  int r;
+  if ((r = set_probes(h, probes, nprobes))) return r;
+  const size_t sb = state_bytes(h);
+  const int nck = (nsteps + ckpt - 1) / ckpt;
+  char *ck = nullptr, *seg = nullptr, *fin = nullptr;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(h->st);
+    if (ck) hipFree(ck);
+    if (seg) hipFree(seg);
+    if (fin) hipFree(fin);
+  };
+  if (hipMalloc(&ck, sb * nck) != hipSuccess || hipMalloc(&seg, sb * ckpt) != hipSuccess ||
+      hipMalloc(&fin, sb) != hipSuccess) {
+    cleanup();
+    return afail(h, LES_E_NOMEM, "window: checkpoint memory");
+  }
+  ACHK(h, hipMemsetAsync(h->jacc, 0, sizeof(double), h->st));
+  // forward window: a checkpoint every ckpt steps, the objective after every step
+  for (int s = 0; s < nsteps; ++s) {
+    if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return cleanup(), r;
+    step_fw(h);
+    sum_probes(h, nprobes);
+  }
+  if ((r = save_state(h, fin))) return cleanup(), r;
+  double jsum = 0.0;
+  ACHK(h, hipMemcpyAsync(&jsum, h->jacc, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  // reverse: each segment recomputed from its checkpoint, each step recomputed from its state
+  if ((r = les_adj_zero_adjoints(h))) return cleanup(), r;
+  const double wgt = 1.0 / ((double)nprobes * nsteps);
+  for (int c = nck - 1; c >= 0; --c) {
+    const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+    if ((r = load_state(h, ck + sb * c))) return cleanup(), r;
+    for (int s = s0; s < s1; ++s) {
+      if ((r = save_state(h, seg + sb * (s - s0)))) return cleanup(), r;
+      if (s + 1 < s1) step_fw(h);
+    }
+    for (int s = s1 - 1; s >= s0; --s) {
+      if ((r = load_state(h, seg + sb * (s - s0)))) return cleanup(), r;
+      step_fw(h);
+      if ((r = zero_temporaries(h))) return cleanup(), r;
+      seed_probes(h, nprobes, wgt);
+      step_bw(h);
+    }
+  }
+  if ((r = load_state(h, fin))) return cleanup(), r;
+  ACHK(h, hipStreamSynchronize(h->st));
+  *objective = jsum * wgt;
+  cleanup();
+  return check(h);
+}
