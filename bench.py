@@ -180,6 +180,58 @@ def tiled_bench(steps, case):
             "bitwise_equal_single_domain_uvwp": bool(same), "steps_compared": 2 + steps}
 
 
+def adjoint_bench(steps, ckpt=20, nprobes=16):
+    """The adjoint configuration: a window of STEPS steps at 150 x 150 x 90 FP32 from the plain
+    path's state after the init sequence and two steps, drag field 0.5, objective the mean u at
+    16 seeded probes 5-30 m high; checkpoints every ckpt steps.  Times the gradient (forward
+    window, segment and step recomputation, reverse) against the plain step and against the
+    adjoint form's own forward step."""
+    import numpy as np
+    import torch
+    from les_halo_amd import LES, inputs
+    from les_halo_amd.adjoint import AdjointLES, probe_points
+    ip, jp, kp = 150, 150, 90
+    case = inputs.make_case(ip, jp, kp)
+    L = LES(ip, jp, kp, **PHYS)
+    L.load_case(case)
+    L.ifdata()
+    L.run(1, 2)
+    L.sync()
+    A = AdjointLES.from_les(L, fp64=False)
+    A.upload("cd", np.full(ip * jp * kp, 0.5))
+    probes = probe_points(case, n=nprobes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L.run(3, steps)
+    L.sync()
+    t_plain = time.perf_counter() - t0
+    L.close()
+    state = {n: A.download(n) for n in ("u", "v", "w", "p", "f", "g", "h", "fold", "gold", "hold",
+                                        "usum", "vsum", "wsum")}
+    A.step(1)  # warm-up of every kernel
+    for n, a in state.items():
+        A.upload(n, a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A.step(steps)
+    A.download("u")
+    t_fwd = time.perf_counter() - t0
+    for n, a in state.items():
+        A.upload(n, a)
+    t0 = time.perf_counter()
+    J = A.window(steps, ckpt, probes)
+    t_grad = time.perf_counter() - t0
+    gw, gc = A.gradient("wind_profile"), A.gradient("cd")
+    A.close()
+    return {"grid": [ip, jp, kp], "dtype": "f32", "steps": steps, "checkpoint_every": ckpt,
+            "probes": nprobes, "objective": J, "grad_wind_norm": float(np.linalg.norm(gw)),
+            "grad_cd_norm": float(np.linalg.norm(gc)),
+            "plain_forward_s": round(t_plain, 4), "adjoint_form_forward_s": round(t_fwd, 4),
+            "gradient_s": round(t_grad, 4),
+            "gradient_over_plain_forward": round(t_grad / t_plain, 2),
+            "gradient_over_own_forward": round(t_grad / t_fwd, 2)}
+
+
 def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95):
     """The mixed-precision configuration: 600x600x90, stencils and SOR sweeps in FP32, rhs mean,
     residual and refined pressure in FP64 (LES_PREC_MIXED), refinement to a relative L2
@@ -226,6 +278,8 @@ def main():
                     help="also run the batched-ensemble configuration with N members")
     ap.add_argument("--tiled", type=int, default=0, metavar="STEPS",
                     help="also run the 2x2-tiles configuration for STEPS steps")
+    ap.add_argument("--adjoint", type=int, default=0, metavar="STEPS",
+                    help="also time the adjoint form's gradient over a STEPS-step window")
     ap.add_argument("--profile-sor", type=int, default=0, metavar="REPS",
                     help="counter runs: set up the workload, launch REPS SOR colour sweeps and "
                          "one device copy of the same algorithmic bytes (calibration), no timing")
@@ -291,6 +345,8 @@ def main():
             "roofline": roofline}
     if args.tiled:
         line["tiled"] = tiled_bench(args.tiled, case)
+    if args.adjoint:
+        line["adjoint"] = adjoint_bench(args.adjoint)
     if args.mixed:
         line["mixed"] = mixed_bench(args.mixed, refine_omega=args.refine_omega)
     if args.ensemble:

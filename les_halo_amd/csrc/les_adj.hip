@@ -209,6 +209,54 @@ __global__ void k_probe_seed(AdjArr<T> B, const int* probes, int np, T wgt, int 
   }
 }
 
+// Adjoint of one colour sweep of press.f95:91-98 (statement sor<nrd>) in gather form:
+//   p(y) <- p(y) + omega*(cn1(y)*(cn2l p(y+ei) + cn2s p(y-ei) + cn3l p(y+ej) + cn3s p(y-ej)
+//                                 + cn4l p(y+ek) + cn4s p(y-ek) - rhs(y)) - p(y))
+// for the interior cells y of the colour.  Its transpose adds omega*cn1(y)*c*pb(y) to the
+// adjoint of every neighbour x of such a y (c the coefficient through which y reads x), then
+// scales pb(y) by 1 - omega and takes omega*cn1(y)*pb(y) off rhsb(y).  The first kernel gathers
+// into every point x of p's box that is not a cell of the colour (each x reads its up to six
+// neighbours of the colour), the second updates the colour's cells: no atomics, and the same
+// arithmetic for every neighbour as the scatter form.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sor_bw_gather(AdjArr<T> A, AdjArr<T> B, AdjPar<T> P,
+                                                       int nrd, int im, int jm, int km) {
+  const int ni = im + 3, nj = jm + 3, nk = km + 2;  // p(0:im+2, 0:jm+2, 0:km+1)
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = (int)(t % ni), j = (int)((t / ni) % nj), k = (int)(t / ((long)ni * nj));
+  auto inside = [&](int a, int b, int c) { return a >= 1 && a <= im && b >= 1 && b <= jm && c >= 1 && c <= km; };
+  auto mine = [&](int a, int b, int c) { return inside(a, b, c) && ((a + b + c + nrd) & 1) == 1; };
+  if (mine(i, j, k)) return;
+  auto P_ = [&](int a, int b, int c) { return (long)a + (long)ni * ((long)b + (long)nj * c); };
+  auto C1 = [&](int a, int b, int c) {
+    return A.cn1[(long)(a - 1) + (long)im * ((long)(b - 1) + (long)jm * (c - 1))];
+  };
+  T acc = T(0);
+  // y = x - e_i reads x through cn2l(i_y); y = x + e_i through cn2s(i_y); likewise j, k
+  if (mine(i - 1, j, k)) acc += P.omega * C1(i - 1, j, k) * A.cn2l[i - 2] * B.p[P_(i - 1, j, k)];
+  if (mine(i + 1, j, k)) acc += P.omega * C1(i + 1, j, k) * A.cn2s[i] * B.p[P_(i + 1, j, k)];
+  if (mine(i, j - 1, k)) acc += P.omega * C1(i, j - 1, k) * A.cn3l[j - 2] * B.p[P_(i, j - 1, k)];
+  if (mine(i, j + 1, k)) acc += P.omega * C1(i, j + 1, k) * A.cn3s[j] * B.p[P_(i, j + 1, k)];
+  if (mine(i, j, k - 1)) acc += P.omega * C1(i, j, k - 1) * A.cn4l[k - 2] * B.p[P_(i, j, k - 1)];
+  if (mine(i, j, k + 1)) acc += P.omega * C1(i, j, k + 1) * A.cn4s[k] * B.p[P_(i, j, k + 1)];
+  if (acc != T(0)) B.p[P_(i, j, k)] += acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sor_bw_scale(AdjArr<T> A, AdjArr<T> B, AdjPar<T> P,
+                                                      int nrd, int im, int jm, int km) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)im * jm * km) return;
+  const int i = 1 + (int)(t % im), j = 1 + (int)((t / im) % jm), k = 1 + (int)(t / ((long)im * jm));
+  if (((i + j + k + nrd) & 1) != 1) return;
+  const long ip = (long)i + (long)(im + 3) * ((long)j + (long)(jm + 3) * k);
+  const long ir = (long)i + (long)(im + 2) * ((long)j + (long)(jm + 2) * k);
+  const T pb = B.p[ip];
+  B.rhs[ir] -= P.omega * A.cn1[t] * pb;
+  B.p[ip] = (T(1) - P.omega) * pb;
+}
+
 const int CARRIED[] = {ADJ_A_u, ADJ_A_v, ADJ_A_w, ADJ_A_p, ADJ_A_f, ADJ_A_g, ADJ_A_h, ADJ_A_fold,
                        ADJ_A_gold, ADJ_A_hold, ADJ_A_usum, ADJ_A_vsum, ADJ_A_wsum};
 constexpr int NCARRIED = sizeof(CARRIED) / sizeof(CARRIED[0]);
@@ -329,7 +377,14 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
   const long ncell = (long)h->im * h->jm * h->km;
   for (int q = nops - 1; q >= 0; --q) {
     const int op = ops[q];
-    if (op >= 0) {
+    if (op == ADJ_sor0 || op == ADJ_sor1) {
+      const int nrd = op == ADJ_sor0 ? 0 : 1;
+      const long nbox = (long)(h->im + 3) * (h->jm + 3) * (h->km + 2);
+      hipLaunchKernelGGL(k_sor_bw_gather<T>, dim3((unsigned)((nbox + 255) / 256)), dim3(256), 0,
+                         h->st, A, B, P, nrd, h->im, h->jm, h->km);
+      hipLaunchKernelGGL(k_sor_bw_scale<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
+                         h->st, A, B, P, nrd, h->im, h->jm, h->km);
+    } else if (op >= 0) {
       adj_launch_bw<T>(op, h->st, A, B, S, SB, P, h->im, h->jm, h->km);
     } else if (op == ADJ_OP_UOUT) {
       hipLaunchKernelGGL(k_uout_bw<T>, dim3(1), dim3(RED_THREADS), 0, h->st, A, B, SB, h->im, h->jm,
