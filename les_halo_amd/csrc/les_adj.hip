@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -257,6 +258,76 @@ __global__ __launch_bounds__(256) void k_sor_bw_scale(AdjArr<T> A, AdjArr<T> B, 
   B.p[ip] = (T(1) - P.omega) * pb;
 }
 
+// boundp1 (boundp.f95:23-44) as one copy: its i-face then j-face loops leave every halo cell
+// (a, b, c) of the ring a in {0, im+1} or b in {0, jm+1} (0 <= a <= im+1, 0 <= b <= jm+1, every
+// level c of p) holding p(mi(a), pj(b), c) with mi(0) = 1, mi(im+1) = im, pj(0) = jm,
+// pj(jm+1) = 1 (identity otherwise), sources that are never ring cells.  The transpose adds
+// each ring cell's adjoint into its source's and zeroes it.  boundp2 (boundp.f95:5-21): the
+// levels 0 and km+1 copy levels 1 and km.  These replace the six statement kernels of the two
+// loops in the SOR iterations and after pav (same results).
+__device__ __forceinline__ bool bp1_cell(long t, int im, int jm, int km, long& dst, long& src) {
+  const int ring = 2 * (jm + 2) + 2 * im;
+  if (t >= (long)ring * (km + 2)) return false;
+  const int r = (int)(t % ring), c = (int)(t / ring);
+  int a, b;
+  if (r < 2 * (jm + 2)) {
+    a = (r < jm + 2) ? 0 : im + 1;
+    b = r % (jm + 2);
+  } else {
+    const int q = r - 2 * (jm + 2);
+    b = (q < im) ? 0 : jm + 1;
+    a = 1 + q % im;
+  }
+  const int sa = a == 0 ? 1 : (a == im + 1 ? im : a);
+  const int sb = b == 0 ? jm : (b == jm + 1 ? 1 : b);
+  const long ni = im + 3, nj = jm + 3;
+  dst = a + ni * (b + nj * c);
+  src = sa + ni * (sb + nj * c);
+  return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bp1_fw(AdjArr<T> A, int im, int jm, int km) {
+  long dst, src;
+  if (bp1_cell((long)blockIdx.x * 256 + threadIdx.x, im, jm, km, dst, src)) A.p[dst] = A.p[src];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bp1_bw(AdjArr<T> B, int im, int jm, int km) {
+  long dst, src;
+  if (!bp1_cell((long)blockIdx.x * 256 + threadIdx.x, im, jm, km, dst, src)) return;
+  const T v = B.p[dst];
+  B.p[dst] = T(0);
+  if (v != T(0)) atomicAdd(&B.p[src], v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bp2(AdjArr<T> A, AdjArr<T> B, int adjoint, int im, int jm,
+                                             int km) {
+  const long ni = im + 3, nj = jm + 3, plane = ni * nj;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= 2 * (long)(im + 2) * (jm + 2)) return;
+  const long cell = t % ((long)(im + 2) * (jm + 2));
+  const int a = (int)(cell % (im + 2)), b = (int)(cell / (im + 2));
+  const bool top = t >= (long)(im + 2) * (jm + 2);
+  const long dst = a + ni * b + plane * (top ? km + 1 : 0);
+  const long src = a + ni * b + plane * (top ? km : 1);
+  if (!adjoint) {
+    A.p[dst] = A.p[src];
+  } else {
+    B.p[src] += B.p[dst];  // one ring cell per source: no other thread touches src
+    B.p[dst] = T(0);
+  }
+}
+
+bool is_seq(const int* ops, int n, int q, std::initializer_list<int> want) {
+  if (q + (int)want.size() > n) return false;
+  int z = q;
+  for (int w : want)
+    if (ops[z++] != w) return false;
+  return true;
+}
+
 const int CARRIED[] = {ADJ_A_u, ADJ_A_v, ADJ_A_w, ADJ_A_p, ADJ_A_f, ADJ_A_g, ADJ_A_h, ADJ_A_fold,
                        ADJ_A_gold, ADJ_A_hold, ADJ_A_usum, ADJ_A_vsum, ADJ_A_wsum};
 constexpr int NCARRIED = sizeof(CARRIED) / sizeof(CARRIED[0]);
@@ -343,9 +414,19 @@ void run_ops_fw(les_adj* h, const int* ops, int nops) {
   const AdjArr<T> A = arrs<T>(h, false);
   const AdjPar<T> P = pars<T>(h);
   AdjScal<T>* S = static_cast<AdjScal<T>*>(h->S);
+  const long nbp1 = (long)(2 * (h->jm + 2) + 2 * h->im) * (h->km + 2);
+  const long nbp2 = 2L * (h->im + 2) * (h->jm + 2);
   for (int q = 0; q < nops; ++q) {
     const int op = ops[q];
-    if (op >= 0) {
+    if (is_seq(ops, nops, q, {ADJ_bp1_i0, ADJ_bp1_i1, ADJ_bp1_j0, ADJ_bp1_j1})) {
+      hipLaunchKernelGGL(k_bp1_fw<T>, dim3((unsigned)((nbp1 + 255) / 256)), dim3(256), 0, h->st, A,
+                         h->im, h->jm, h->km);
+      q += 3;
+    } else if (is_seq(ops, nops, q, {ADJ_bp2_k0, ADJ_bp2_k1})) {
+      hipLaunchKernelGGL(k_bp2<T>, dim3((unsigned)((nbp2 + 255) / 256)), dim3(256), 0, h->st, A, A,
+                         0, h->im, h->jm, h->km);
+      q += 1;
+    } else if (op >= 0) {
       adj_launch_fw<T>(op, h->st, A, S, P, h->im, h->jm, h->km);
     } else if (op == ADJ_OP_SNAP) {
       for (const char* nm : {"u", "v", "w"}) {
@@ -375,9 +456,19 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
   AdjScal<T>* S = static_cast<AdjScal<T>*>(h->S);
   AdjScal<T>* SB = static_cast<AdjScal<T>*>(h->SB);
   const long ncell = (long)h->im * h->jm * h->km;
+  const long nbp1 = (long)(2 * (h->jm + 2) + 2 * h->im) * (h->km + 2);
+  const long nbp2 = 2L * (h->im + 2) * (h->jm + 2);
   for (int q = nops - 1; q >= 0; --q) {
     const int op = ops[q];
-    if (op == ADJ_sor0 || op == ADJ_sor1) {
+    if (q >= 3 && is_seq(ops, nops, q - 3, {ADJ_bp1_i0, ADJ_bp1_i1, ADJ_bp1_j0, ADJ_bp1_j1})) {
+      hipLaunchKernelGGL(k_bp1_bw<T>, dim3((unsigned)((nbp1 + 255) / 256)), dim3(256), 0, h->st, B,
+                         h->im, h->jm, h->km);
+      q -= 3;
+    } else if (q >= 1 && is_seq(ops, nops, q - 1, {ADJ_bp2_k0, ADJ_bp2_k1})) {
+      hipLaunchKernelGGL(k_bp2<T>, dim3((unsigned)((nbp2 + 255) / 256)), dim3(256), 0, h->st, A, B,
+                         1, h->im, h->jm, h->km);
+      q -= 1;
+    } else if (op == ADJ_sor0 || op == ADJ_sor1) {
       const int nrd = op == ADJ_sor0 ? 0 : 1;
       const long nbox = (long)(h->im + 3) * (h->jm + 3) * (h->km + 2);
       hipLaunchKernelGGL(k_sor_bw_gather<T>, dim3((unsigned)((nbox + 255) / 256)), dim3(256), 0,
