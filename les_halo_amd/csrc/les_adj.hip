@@ -740,7 +740,13 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
     cleanup();
     return afail(h, LES_E_NOMEM, "window: checkpoint memory");
   }
-  ACHK(h, hipMemsetAsync(h->jacc, 0, sizeof(double), h->st));
+  auto hipfail = [&](hipError_t e, const char* what) {
+    cleanup();
+    return afail(h, LES_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMemsetAsync(h->jacc, 0, sizeof(double), h->st)) != hipSuccess)
+    return hipfail(e, "window: clear the objective");
   // forward window: a checkpoint every ckpt steps, the objective after every step
   for (int s = 0; s < nsteps; ++s) {
     if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return cleanup(), r;
@@ -749,7 +755,9 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   }
   if ((r = save_state(h, fin))) return cleanup(), r;
   double jsum = 0.0;
-  ACHK(h, hipMemcpyAsync(&jsum, h->jacc, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  if ((e = hipMemcpyAsync(&jsum, h->jacc, sizeof(double), hipMemcpyDeviceToHost, h->st)) != hipSuccess ||
+      (e = hipStreamSynchronize(h->st)) != hipSuccess)
+    return hipfail(e, "window: read the objective");
   // reverse: each segment recomputed from its checkpoint, each step recomputed from its state
   if ((r = les_adj_zero_adjoints(h))) return cleanup(), r;
   const double wgt = 1.0 / ((double)nprobes * nsteps);
@@ -769,7 +777,7 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
     }
   }
   if ((r = load_state(h, fin))) return cleanup(), r;
-  ACHK(h, hipStreamSynchronize(h->st));
+  if ((e = hipStreamSynchronize(h->st)) != hipSuccess) return hipfail(e, "window");
   *objective = jsum * wgt;
   cleanup();
   return check(h);

@@ -149,6 +149,17 @@ def test_create_without_gpu_fails_loudly():
     assert e.value.code == 5  # LES_E_NODEV
 
 
+def test_adjoint_create_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from les_halo_amd import LESError
+    from les_halo_amd.adjoint import AdjointLES
+    with pytest.raises(LESError) as e:
+        AdjointLES(8, 8, 8)
+    assert e.value.code == 5  # LES_E_NODEV
+
+
 def test_invalid_sizes_rejected():
     from les_halo_amd._lib import LesConfig
     cfg = LesConfig()
