@@ -232,7 +232,7 @@ def adjoint_bench(steps, ckpt=20, nprobes=16):
             "gradient_over_own_forward": round(t_grad / t_fwd, 2)}
 
 
-def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95):
+def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95, refine_solver=1):
     """The mixed-precision configuration: 600x600x90, stencils and SOR sweeps in FP32, rhs mean,
     residual and refined pressure in FP64 (LES_PREC_MIXED), refinement to a relative L2
     residual of tol.  Steps run eagerly (the host reads the residual between passes)."""
@@ -241,12 +241,12 @@ def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95):
     from les_halo_amd.les import LES_PREC_MIXED
     ip, jp, kp = 600, 600, 90
     L = LES(ip, jp, kp, **PHYS, precision=LES_PREC_MIXED, refine_tol=tol, refine_max=refine_max,
-            refine_omega=refine_omega)
+            refine_omega=refine_omega, refine_solver=refine_solver)
     L.load_case(inputs.make_case(ip, jp, kp))
     L.ifdata()
     L.run(1, 2)
     L.sync()
-    passes, resid = [], []
+    passes, resid, cg_iters = [], [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for n in range(3, 3 + steps):
@@ -254,12 +254,14 @@ def mixed_bench(steps, tol=1e-7, refine_max=200, refine_omega=1.95):
         s = L.scalars()
         passes.append(s["refine_passes"])
         resid.append(s["resid_l2"])
+        cg_iters.append(s["cg_iters"])
     L.sync()
     secs = time.perf_counter() - t0
     return {"grid": [ip, jp, kp], "steps": steps, "ms_per_step": round(secs / steps * 1e3, 3),
             "value": round(ip * jp * kp * steps / secs / 1e6, 2), "unit": "M cell-steps/s",
             "refine_tol": tol, "refine_max": refine_max, "refine_omega": refine_omega,
-            "refine_passes": passes,
+            "refine_solver": ["sor", "pcg"][refine_solver],
+            "refine_passes": passes, "cg_iterations": cg_iters,
             "resid_l2_last": resid[-1], "resid_l2_max": max(resid)}
 
 
@@ -272,7 +274,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", type=int, default=0, metavar="STEPS",
                     help="also run the mixed-precision 600x600x90 configuration for STEPS steps")
-    ap.add_argument("--refine-omega", type=float, default=1.95,
+    ap.add_argument("--refine-solver", type=int, default=1, choices=(0, 1),
+                    help="mixed precision corrections: 0 SOR sweeps, 1 preconditioned CG")
+    ap.add_argument("--refine-omega", type=float, default=None,
                     help="relaxation of the mixed-precision correction solves")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="also run the batched-ensemble configuration with N members")
@@ -348,7 +352,11 @@ def main():
     if args.adjoint:
         line["adjoint"] = adjoint_bench(args.adjoint)
     if args.mixed:
-        line["mixed"] = mixed_bench(args.mixed, refine_omega=args.refine_omega)
+        romega = args.refine_omega
+        if romega is None:  # SOR corrections over-relax hard; the CG preconditioner uses omega
+            romega = 1.95 if args.refine_solver == 0 else 0.0
+        line["mixed"] = mixed_bench(args.mixed, refine_omega=romega,
+                                    refine_solver=args.refine_solver)
     if args.ensemble:
         del L
         line["ensemble"] = ensemble_bench(args.ensemble, args.steps, args.warmup)

@@ -111,6 +111,9 @@ typedef struct les_config {
   int device;         /* HIP device ordinal */
   int sub_i, sub_j;   /* subdomains along i and j on the one device (1,1 = single domain) */
   float refine_omega; /* LES_PREC_MIXED: SOR relaxation of the correction solves; 0 = omega */
+  int refine_solver;  /* LES_PREC_MIXED: 0 = each correction by nmaxp SOR iterations from zero;
+                         1 = by nmaxp conjugate-gradient iterations preconditioned by symmetric
+                         red-black Gauss-Seidel (the FP32 colour sweeps), FP64 inner products */
 } les_config;
 
 /* Fills the reference's defaults (set.f95, les.f95, press.f95) for an ip x jp x kp grid. */
@@ -301,6 +304,8 @@ typedef struct les_scalars {
   int sor_iters;   /* SOR iterations run in the last solve */
   double resid_l2; /* LES_PREC_MIXED: final L2 norm of the pressure residual */
   int refine_passes;
+  int cg_iters;    /* LES_PREC_MIXED, refine_solver 1: conjugate-gradient iterations of the last
+                      solve, all passes */
 } les_scalars;
 int les_get_scalars(les_handle* h, les_scalars* out);
 

@@ -22,13 +22,14 @@ class LesConfig(ctypes.Structure):
                 ("vn", c_float), ("alpha", c_float), ("beta", c_float), ("cs0", c_float),
                 ("omega", c_float), ("pjuge", c_float), ("nmaxp", c_int), ("precision", c_int),
                 ("refine_tol", c_double), ("refine_max", c_int), ("device", c_int),
-                ("sub_i", c_int), ("sub_j", c_int), ("refine_omega", c_float)]
+                ("sub_i", c_int), ("sub_j", c_int), ("refine_omega", c_float),
+                ("refine_solver", c_int)]
 
 
 class LesScalars(ctypes.Structure):
     _fields_ = [("uout", c_float), ("rhsav", c_float), ("area", c_float), ("pav", c_float),
                 ("pco", c_float), ("sor", c_float), ("sor_iters", c_int),
-                ("resid_l2", c_double), ("refine_passes", c_int)]
+                ("resid_l2", c_double), ("refine_passes", c_int), ("cg_iters", c_int)]
 
 
 class LesTuning(ctypes.Structure):

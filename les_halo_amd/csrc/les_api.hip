@@ -43,6 +43,7 @@ struct les_handle {
   // mixed precision: FP64 iterate and rhs in the split layout, block partials
   Mixed mx{};
   double* mx_mem = nullptr;
+  float* cg_mem = nullptr;  // mixed precision: the conjugate-gradient vectors
   // second iterate of the fused SOR iteration (ping-pong with sp.p) and its workgroup tile
   float* palt[2] = {nullptr, nullptr};
   SorIterShape iter_shape{0, 0};  // tj == 0: two colour sweeps per iteration (untuned default)
@@ -344,6 +345,7 @@ extern "C" int les_default_config(les_config* c, int ip, int jp, int kp) {
   c->device = 0;
   c->sub_i = 1;
   c->sub_j = 1;
+  c->refine_solver = 0;
   return LES_OK;
 }
 
@@ -360,7 +362,8 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   if (cfg->sub_i != 1 || cfg->sub_j != 1) return LES_E_INVALID;  // one domain per handle
   if (cfg->precision != LES_PREC_FP32 && cfg->precision != LES_PREC_MIXED) return LES_E_INVALID;
   if (cfg->precision == LES_PREC_MIXED && (!(cfg->refine_tol > 0.0) || cfg->refine_max < 0 ||
-                                            cfg->refine_omega < 0.0f || cfg->refine_omega >= 2.0f))
+                                            cfg->refine_omega < 0.0f || cfg->refine_omega >= 2.0f ||
+                                            cfg->refine_solver < 0 || cfg->refine_solver > 1))
     return LES_E_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LES_E_NODEV;
@@ -422,7 +425,7 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     if (cfg->precision == LES_PREC_MIXED) {
       Mixed& mx = h->mx;
       mx.nblk = mixed_blocks(h->d);
-      const size_t nd = 4 * ns + 2 * (size_t)mx.nblk;
+      const size_t nd = 4 * ns + 2 * (size_t)mx.nblk + 8;
       if (hipMalloc(&h->mx_mem, nd * sizeof(double)) != hipSuccess) return bail(LES_E_NOMEM);
       if (hipMemsetAsync(h->mx_mem, 0, nd * sizeof(double), h->st) != hipSuccess)
         return bail(LES_E_HIP);
@@ -431,7 +434,17 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
       mx.r64[0] = mx.p64[1] + ns;
       mx.r64[1] = mx.r64[0] + ns;
       mx.part = mx.r64[1] + ns;
+      mx.cg = mx.part + 2 * (size_t)mx.nblk;
       mx.cn1 = h->fld[LES_CN1];
+      // FP32 vectors of the conjugate-gradient correction solve
+      if (hipMalloc(&h->cg_mem, 6 * ns * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
+      if (hipMemsetAsync(h->cg_mem, 0, 6 * ns * sizeof(float), h->st) != hipSuccess)
+        return bail(LES_E_HIP);
+      for (int c = 0; c < 2; ++c) {
+        mx.z[c] = h->cg_mem + c * ns;
+        mx.dir[c] = h->cg_mem + (2 + c) * ns;
+        mx.q[c] = h->cg_mem + (4 + c) * ns;
+      }
     }
   }
   if (hipStreamSynchronize(h->st) != hipSuccess) return bail(LES_E_HIP);
@@ -452,6 +465,7 @@ extern "C" int les_destroy(les_handle* h) {
   if (h->scratch_f) hipFree(h->scratch_f);
   if (h->split_mem) hipFree(h->split_mem);
   if (h->mx_mem) hipFree(h->mx_mem);
+  if (h->cg_mem) hipFree(h->cg_mem);
   if (h->ave_mem) hipFree(h->ave_mem);
   if (h->st) hipStreamDestroy(h->st);
   if (h == g_default) g_default = nullptr;
@@ -851,10 +865,20 @@ int enqueue_press_mixed(les_handle* h) {
   for (;;) {
     launch_mx_resid(h->st, d, g, h->sp, h->mx, passes == 0 ? 1 : 0, h->cfg.refine_tol, h->sc);
     int done = 0;
+    double resid = 0.0;
     HIPCHK(h, hipMemcpyAsync(&done, &h->sc->refine_done, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipMemcpyAsync(&resid, &h->sc->resid_l2, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
     if (done || passes >= h->cfg.refine_max) break;
-    if ((r = enqueue_sor_split_loop(h, false, &ph_corr))) return r;
+    if (h->cfg.refine_solver == 1) {
+      // CG on the correction until its preconditioned residual norm has fallen by what this pass
+      // needs to reach a third of the tolerance (at most 1e-6: FP32 vectors), up to 20 x nmaxp
+      // iterations
+      const double need = resid > 0.0 ? 0.3 * h->cfg.refine_tol / resid : 1.0;
+      const double stop = std::max(1e-12, std::min(1e-2, need * need));
+      launch_mx_cg(h->st, d, g, h->sp, h->mx, 20 * h->cfg.nmaxp, ph_corr.omega, stop);
+    }
+    if (h->cfg.refine_solver != 1 && (r = enqueue_sor_split_loop(h, false, &ph_corr))) return r;
     launch_mx_accumulate(h->st, d, h->sp, h->mx);
     ++passes;
   }
@@ -1075,6 +1099,12 @@ extern "C" int les_get_scalars(les_handle* h, les_scalars* out) {
   out->sor_iters = (h->ph.pjuge > 0.0f) ? s.sor_iters : h->cfg.nmaxp;
   out->resid_l2 = s.resid_l2;
   out->refine_passes = s.refine_passes;
+  out->cg_iters = 0;
+  if (h->cfg.precision == LES_PREC_MIXED) {
+    double it = 0.0;
+    HIPCHK(h, hipMemcpy(&it, h->mx.cg + 6, sizeof(double), hipMemcpyDeviceToHost));
+    out->cg_iters = (int)it;
+  }
   return LES_OK;
 }
 

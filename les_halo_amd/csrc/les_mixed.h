@@ -11,6 +11,11 @@ struct Mixed {
   double* part;       // per-block partial sums, 2 * nblk
   const float* cn1;   // cn1(1:ip,1:jp,1:kp)
   int nblk;
+  // conjugate-gradient correction solve (refine_solver 1): FP32 split vectors and FP64 scalars
+  float* z[2];        // preconditioned residual
+  float* dir[2];      // search direction
+  float* q[2];        // A * dir
+  double* cg;         // rz, pq, alpha, beta, stopped flag, rz at the start, iterations
 };
 
 int mixed_blocks(Dims d);
@@ -20,6 +25,12 @@ void launch_mx_promote(hipStream_t st, Dims d, SplitSOR sp, Mixed mx);
 void launch_mx_resid(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int first,
                      double tol, DevScalars* sc);
 void launch_mx_accumulate(hipStream_t st, Dims d, SplitSOR sp, Mixed mx);
+// the correction solve as conjugate gradients on -L e = -r (sp.r: r, sp.p: e from zero),
+// preconditioned by symmetric red-black SOR with relaxation om; at most iters iterations,
+// stopping on the device once the preconditioned residual norm squared falls to stop times its
+// start value
+void launch_mx_cg(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int iters, float om,
+                  double stop);
 void launch_mx_pav_gather(hipStream_t st, Dims d, Grid1D g, Fields F, SplitSOR sp, Mixed mx,
                           DevScalars* sc);
 

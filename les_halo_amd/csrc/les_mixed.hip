@@ -146,7 +146,10 @@ __global__ __launch_bounds__(256) void k_mx_resid_finalize(Mixed mx, int first, 
   __syncthreads();
   const double sb = block_sum(b, lds);
   if (threadIdx.x == 0) {
-    if (first) sc->rhs_l2 = sqrt(sb);
+    if (first) {
+      sc->rhs_l2 = sqrt(sb);
+      mx.cg[6] = 0.0;  // the conjugate-gradient iteration count of this press
+    }
     const double rn = sqrt(sa);
     sc->resid_l2 = (sc->rhs_l2 > 0.0) ? rn / sc->rhs_l2 : rn;
     sc->refine_done = (sc->resid_l2 <= tol) ? 1 : 0;
@@ -198,6 +201,233 @@ __global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp
   const int j = (b == 0) ? d.jp : ((b == d.jp + 1) ? 1 : b);
   const int k = (c == 0) ? 1 : ((c == d.kp + 1) ? d.kp : c);
   F.p[d.P(a, b, c)] = (float)(p64_at(sp, mx, i, j, k) - sc->pav64);
+}
+
+// ---- conjugate gradients for the correction (refine_solver 1) ----
+// A = -L with the boundary images folded into the matrix (a face neighbour across an i or k face
+// is the cell itself, so its coefficient joins the diagonal): (A x)_c = x_c * dg_c - sum over the
+// real neighbours of c_nb * x_nb, dg_c = 1/cn1 - (image coefficients).  With cn2l(i) =
+// 2/(dxs(i)(dxs(i-1)+dxs(i))) and cn2s(i) = 2/(dxs(i-1)(dxs(i-1)+dxs(i))) (init.f95:82-107, and
+// alike along j and k), w*cn2l(i) = w(i+1)*cn2s(i+1) for the weight w = (dxs(i-1)+dxs(i)) *
+// (dys(j-1)+dys(j)) * (dzs(k-1)+dzs(k)): A is symmetric in the inner product weighted by w (the
+// periodic j coupling as well when dys(0) = dys(jp), as on a uniform horizontal grid).  CG runs
+// in that inner product, preconditioned by symmetric Gauss-Seidel (colour 0, colour 1, colour 0,
+// from zero) on the same matrix.  Vectors FP32, inner products FP64.
+namespace {
+
+struct Nb {  // real neighbours' values and coefficients of one cell, and its diagonal
+  float c[6];
+  float x[6];
+  float dg;
+};
+
+__device__ __forceinline__ float xs(const float* const* v, const SplitSOR& sp, int i, int j, int k) {
+  return v[(i + j + k) & 1][split_index(sp, i, j, k)];
+}
+
+// the matrix row of cell (i,j,k) applied to v (its own value excluded)
+__device__ __forceinline__ float offdiag_sum(Dims d, const Grid1D& g, const SplitSOR& sp,
+                                             const float* const* v, int i, int j, int k,
+                                             float& dg, float cn1) {
+  float s = 0.0f;
+  dg = 1.0f / cn1;
+  const float c2l = g.cn2l[i - 1], c2s = g.cn2s[i - 1], c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
+  const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+  if (i < d.ip) s += c2l * xs(v, sp, i + 1, j, k); else dg -= c2l;
+  if (i > 1) s += c2s * xs(v, sp, i - 1, j, k); else dg -= c2s;
+  s += c3l * xs(v, sp, i, j == d.jp ? 1 : j + 1, k);
+  s += c3s * xs(v, sp, i, j == 1 ? d.jp : j - 1, k);
+  if (k < d.kp) s += c4l * xs(v, sp, i, j, k + 1); else dg -= c4l;
+  if (k > 1) s += c4s * xs(v, sp, i, j, k - 1); else dg -= c4s;
+  return s;
+}
+
+__device__ __forceinline__ double vol(const Grid1D& g, int i, int j, int k) {
+  return ((double)g.DXS(i - 1) + (double)g.DXS(i)) * ((double)g.DYS(j - 1) + (double)g.DYS(j)) *
+         ((double)g.DZS(k - 1) + (double)g.DZS(k));
+}
+
+}  // namespace
+
+// q = A dir; block partials of <dir, q>
+__global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp, Mixed mx) {
+  __shared__ double lds[4];
+  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double acc = 0.0;
+  if (i <= d.ip && j <= d.jp) {
+    const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+    float dg;
+    const float s = offdiag_sum(d, g, sp, mx.dir, i, j, k, dg, mx.cn1[d.O(i, j, k)]);
+    const float xv = mx.dir[par][is];
+    const float qv = dg * xv - s;
+    mx.q[par][is] = qv;
+    acc = vol(g, i, j, k) * (double)xv * (double)qv;
+  }
+  const double t = block_sum(acc, lds);
+  if (tid() == 0) mx.part[flat_block()] = t;
+}
+
+// one colour of the symmetric SOR preconditioner z = M^-1 rr (rr in sp.r, negated: the system
+// is -L e = -r), relaxation om: z_c <- (1 - om) z_c + om (rr_c + sum c_nb z_nb) / dg_c.  The
+// application starts from z = 0: first (colour 0 forward) ignores the neighbours and its own
+// value, fresh (colour 1 forward) its own value.
+__global__ __launch_bounds__(256) void k_cg_sgs(Dims d, Grid1D g, SplitSOR sp, Mixed mx, int colour,
+                                                int first, int fresh, float om) {
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (mx.cg[4] != 0.0) return;  // converged
+  if (i > d.ip || j > d.jp || ((i + j + k) & 1) != colour) return;
+  const int is = split_index(sp, i, j, k);
+  float dg;
+  const float s = first ? 0.0f : offdiag_sum(d, g, sp, mx.z, i, j, k, dg, mx.cn1[d.O(i, j, k)]);
+  if (first) {
+    dg = 1.0f / mx.cn1[d.O(i, j, k)];
+    if (i == d.ip) dg -= g.cn2l[i - 1];
+    if (i == 1) dg -= g.cn2s[i - 1];
+    if (k == d.kp) dg -= g.cn4l[k - 1];
+    if (k == 1) dg -= g.cn4s[k - 1];
+  }
+  const float gs = (-sp.r[colour][is] + s) / dg;
+  mx.z[colour][is] = (first || fresh) ? om * gs : (1.0f - om) * mx.z[colour][is] + om * gs;
+}
+
+// block partials of <rr, z> (rr = -sp.r)
+__global__ __launch_bounds__(256) void k_cg_rz(Dims d, Grid1D g, SplitSOR sp, Mixed mx) {
+  __shared__ double lds[4];
+  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double acc = 0.0;
+  if (i <= d.ip && j <= d.jp) {
+    const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+    acc = vol(g, i, j, k) * -(double)sp.r[par][is] * (double)mx.z[par][is];
+  }
+  const double t = block_sum(acc, lds);
+  if (tid() == 0) mx.part[flat_block()] = t;
+}
+
+// which 0: rz = rz0 = sum (start); 1: alpha = rz / sum(pq); 2: beta = sum / rz, rz = sum, and
+// stop (cg[4] = 1) once rz <= stop * rz0.  cg: rz, pq, alpha, beta, stopped, rz0
+__global__ __launch_bounds__(256) void k_cg_scalar(Mixed mx, int which, double stop) {
+  __shared__ double lds[4];
+  if (mx.cg[4] != 0.0) return;
+  double a = 0.0;
+  for (int q = threadIdx.x; q < mx.nblk; q += 256) a += mx.part[q];
+  const double s = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    if (which == 0) {
+      mx.cg[0] = s;
+      mx.cg[5] = s;
+    } else if (which == 1) {
+      mx.cg[1] = s;
+      mx.cg[2] = (s > 0.0) ? mx.cg[0] / s : 0.0;
+      if (!(s > 0.0)) mx.cg[4] = 1.0;  // direction in the null space or lost to rounding: stop
+    } else {
+      mx.cg[3] = (mx.cg[0] != 0.0) ? s / mx.cg[0] : 0.0;
+      mx.cg[0] = s;
+      mx.cg[6] += 1.0;  // iterations of this press
+      if (!(s > stop * mx.cg[5])) mx.cg[4] = 1.0;  // also stops on 0 or NaN
+    }
+  }
+}
+
+// which 0: e += alpha dir, rr -= alpha q (sp.r += alpha q); 1: dir = z + beta dir; 2: dir = z
+__global__ __launch_bounds__(256) void k_cg_update(Dims d, SplitSOR sp, Mixed mx, int which) {
+  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+  if (which == 0) {
+    const float a = (float)mx.cg[2];
+    sp.p[par][is] = sp.p[par][is] + a * mx.dir[par][is];
+    sp.r[par][is] = sp.r[par][is] + a * mx.q[par][is];
+  } else if (which == 1) {
+    mx.dir[par][is] = mx.z[par][is] + (float)mx.cg[3] * mx.dir[par][is];
+  } else {
+    mx.dir[par][is] = mx.z[par][is];
+  }
+}
+
+__global__ void k_cg_reset(Mixed mx) {
+  if (threadIdx.x < 6) mx.cg[threadIdx.x] = 0.0;
+}
+
+// A is singular (constants); its range is the w-orthogonal complement of the constants.  The
+// FP32 right-hand side is projected onto it before the iteration (its w-mean removed), or CG
+// would chase the inconsistent part into the null space.
+__global__ __launch_bounds__(256) void k_cg_mean_part(Dims d, Grid1D g, SplitSOR sp, Mixed mx) {
+  __shared__ double lds[4];
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double a = 0.0, b = 0.0;
+  if (i <= d.ip && j <= d.jp) {
+    const double w = vol(g, i, j, k);
+    a = w * (double)sp.r[(i + j + k) & 1][split_index(sp, i, j, k)];
+    b = w;
+  }
+  const double sa = block_sum(a, lds);
+  if (tid() == 0) mx.part[flat_block()] = sa;
+  __syncthreads();
+  const double sb = block_sum(b, lds);
+  if (tid() == 0) mx.part[mx.nblk + flat_block()] = sb;
+}
+
+__global__ __launch_bounds__(256) void k_cg_mean(Mixed mx) {
+  __shared__ double lds[4];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < mx.nblk; q += 256) {
+    a += mx.part[q];
+    b += mx.part[mx.nblk + q];
+  }
+  const double sa = block_sum(a, lds);
+  __syncthreads();
+  const double sb = block_sum(b, lds);
+  if (threadIdx.x == 0) mx.cg[1] = (sb > 0.0) ? sa / sb : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_cg_project(Dims d, SplitSOR sp, Mixed mx) {
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+  sp.r[par][is] = (float)((double)sp.r[par][is] - mx.cg[1]);
+}
+
+void launch_mx_cg(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int iters, float om,
+                  double stop) {
+  const dim3 grid = mx_grid(d), blk(MX_BX, MX_BY);
+  // symmetric SOR: colour 0, colour 1 forward, colour 1, colour 0 backward
+  auto precondition = [&]() {
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 1, 1, om);
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 1, om);
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 0, om);
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 0, 0, om);
+    hipLaunchKernelGGL(k_cg_rz, grid, blk, 0, st, d, g, sp, mx);
+  };
+  hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, st, mx);
+  hipLaunchKernelGGL(k_cg_mean_part, grid, blk, 0, st, d, g, sp, mx);
+  hipLaunchKernelGGL(k_cg_mean, dim3(1), dim3(256), 0, st, mx);
+  hipLaunchKernelGGL(k_cg_project, grid, blk, 0, st, d, sp, mx);
+  precondition();
+  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 0, stop);
+  hipLaunchKernelGGL(k_cg_update, grid, blk, 0, st, d, sp, mx, 2);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_cg_apply, grid, blk, 0, st, d, g, sp, mx);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 1, stop);
+    hipLaunchKernelGGL(k_cg_update, grid, blk, 0, st, d, sp, mx, 0);
+    precondition();
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 2, stop);
+    hipLaunchKernelGGL(k_cg_update, grid, blk, 0, st, d, sp, mx, 1);
+  }
 }
 
 int mixed_blocks(Dims d) { return mx_blocks(d); }

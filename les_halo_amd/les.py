@@ -25,7 +25,7 @@ class LES:
 
     def __init__(self, ip, jp, kp, dt=0.2, ro=1.1763, vn=None, alpha=-10.0, beta=-1.0, cs0=0.1,
                  omega=1.0, pjuge=1e-4, nmaxp=50, precision=LES_PREC_FP32, refine_tol=1e-7,
-                 refine_max=8, device=0, refine_omega=0.0):
+                 refine_max=8, device=0, refine_omega=0.0, refine_solver=0):
         L = lib()
         cfg = LesConfig()
         L.les_default_config(ctypes.byref(cfg), ip, jp, kp)
@@ -35,6 +35,7 @@ class LES:
         cfg.omega, cfg.pjuge, cfg.nmaxp, cfg.precision = omega, pjuge, nmaxp, precision
         cfg.refine_tol, cfg.refine_max, cfg.device = refine_tol, refine_max, device
         cfg.refine_omega = refine_omega
+        cfg.refine_solver = refine_solver
         self.cfg = cfg
         self.dims = (ip, jp, kp)
         h = ctypes.c_void_p()

@@ -33,12 +33,14 @@ def case():
     return common.tiny_case(40, 32, 20, seed=11)
 
 
-@pytest.mark.parametrize("romega", [0.0, 1.7])
-def test_mixed_reaches_tolerance_and_tracks_fp64(case, romega):
-    """romega: relaxation of the correction solves (0 = the step's omega)."""
+@pytest.mark.parametrize("romega,solver", [(0.0, 0), (1.7, 0), (0.0, 1)])
+def test_mixed_reaches_tolerance_and_tracks_fp64(case, romega, solver):
+    """romega: relaxation of the SOR correction solves (0 = the step's omega); solver 1: the
+    corrections by preconditioned conjugate gradients."""
     from les_halo_amd.les import LES_PREC_MIXED
     nsteps = 4
-    L = _gpu(case, precision=LES_PREC_MIXED, refine_tol=TOL, refine_max=RMAX, refine_omega=romega)
+    L = _gpu(case, precision=LES_PREC_MIXED, refine_tol=TOL, refine_max=RMAX, refine_omega=romega,
+             refine_solver=solver)
     P = _gpu(case)
     o = common.oracle_from_case(case, np.float64, **common.REF_PHYS, **SOR, sum_mode=SUM_SEQ,
                                 refine_tol=TOL, refine_max=RMAX, refine_omega=romega)
@@ -48,6 +50,8 @@ def test_mixed_reaches_tolerance_and_tracks_fp64(case, romega):
         P.step(n)
         o.step(n)
         s = L.scalars()
+        print("step %d: residual %.3e after %d refinement passes (%d CG iterations)"
+              % (n, s["resid_l2"], s["refine_passes"], s["cg_iters"]))
         assert s["resid_l2"] <= TOL, (n, s["resid_l2"], s["refine_passes"])
         assert o.scalars.resid_l2 <= TOL
     err_mixed = {nm: common.rel_l2(L.download(nm), o.a[nm]) for nm in ("u", "v", "w", "p")}
