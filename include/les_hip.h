@@ -110,7 +110,8 @@ typedef struct les_config {
   int refine_max;     /* LES_PREC_MIXED: maximum outer refinement passes */
   int device;         /* HIP device ordinal */
   int sub_i, sub_j;   /* subdomains along i and j on the one device (1,1 = single domain) */
-  float refine_omega; /* LES_PREC_MIXED: SOR relaxation of the correction solves; 0 = omega */
+  float refine_omega; /* LES_PREC_MIXED: relaxation of the correction solves' SOR sweeps; 0 =
+                         omega (SOR corrections) or 1 (the CG preconditioner) */
   int refine_solver;  /* LES_PREC_MIXED: 0 = each correction by nmaxp SOR iterations from zero;
                          1 = by nmaxp conjugate-gradient iterations preconditioned by symmetric
                          red-black Gauss-Seidel (the FP32 colour sweeps), FP64 inner products */

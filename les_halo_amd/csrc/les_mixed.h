@@ -13,7 +13,8 @@ struct Mixed {
   int nblk;
   // conjugate-gradient correction solve (refine_solver 1): FP32 split vectors and FP64 scalars
   float* z[2];        // preconditioned residual
-  float* dir[2];      // search direction
+  float* dir[2];      // search direction (two buffers: the next is formed from the last)
+  float* dir2[2];
   float* q[2];        // A * dir
   double* cg;         // rz, pq, alpha, beta, stopped flag, rz at the start, iterations
 };

@@ -215,10 +215,9 @@ __global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp
 // from zero) on the same matrix.  Vectors FP32, inner products FP64.
 namespace {
 
-struct Nb {  // real neighbours' values and coefficients of one cell, and its diagonal
-  float c[6];
-  float x[6];
-  float dg;
+struct CgDir {  // the search direction of the last iteration and the one being formed
+  float* old_[2];
+  float* new_[2];
 };
 
 __device__ __forceinline__ float xs(const float* const* v, const SplitSOR& sp, int i, int j, int k) {
@@ -249,8 +248,11 @@ __device__ __forceinline__ double vol(const Grid1D& g, int i, int j, int k) {
 
 }  // namespace
 
-// q = A dir; block partials of <dir, q>
-__global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp, Mixed mx) {
+// dnew = z + beta dold (beta = 0 on the first iteration: dnew = z), q = A dnew, block partials of
+// <dnew, q>.  The neighbours' dnew values are formed from their z and dold the same way, so no
+// thread reads what another writes (dnew and dold are different arrays).
+__global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
+                                                  CgDir dd, int first) {
   __shared__ double lds[4];
   if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
   const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
@@ -258,10 +260,24 @@ __global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp,
   const int k = 1 + blockIdx.z;
   double acc = 0.0;
   if (i <= d.ip && j <= d.jp) {
+    const float beta = first ? 0.0f : (float)mx.cg[3];
+    auto dn = [&](int a, int b, int c) {
+      const int par = (a + b + c) & 1, is = split_index(sp, a, b, c);
+      return first ? mx.z[par][is] : mx.z[par][is] + beta * dd.old_[par][is];
+    };
     const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
-    float dg;
-    const float s = offdiag_sum(d, g, sp, mx.dir, i, j, k, dg, mx.cn1[d.O(i, j, k)]);
-    const float xv = mx.dir[par][is];
+    float dg = 1.0f / mx.cn1[d.O(i, j, k)];
+    float s = 0.0f;
+    const float c2l = g.cn2l[i - 1], c2s = g.cn2s[i - 1], c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
+    const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+    if (i < d.ip) s += c2l * dn(i + 1, j, k); else dg -= c2l;
+    if (i > 1) s += c2s * dn(i - 1, j, k); else dg -= c2s;
+    s += c3l * dn(i, j == d.jp ? 1 : j + 1, k);
+    s += c3s * dn(i, j == 1 ? d.jp : j - 1, k);
+    if (k < d.kp) s += c4l * dn(i, j, k + 1); else dg -= c4l;
+    if (k > 1) s += c4s * dn(i, j, k - 1); else dg -= c4s;
+    const float xv = dn(i, j, k);
+    dd.new_[par][is] = xv;
     const float qv = dg * xv - s;
     mx.q[par][is] = qv;
     acc = vol(g, i, j, k) * (double)xv * (double)qv;
@@ -270,18 +286,9 @@ __global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp,
   if (tid() == 0) mx.part[flat_block()] = t;
 }
 
-// one colour of the symmetric SOR preconditioner z = M^-1 rr (rr in sp.r, negated: the system
-// is -L e = -r), relaxation om: z_c <- (1 - om) z_c + om (rr_c + sum c_nb z_nb) / dg_c.  The
-// application starts from z = 0: first (colour 0 forward) ignores the neighbours and its own
-// value, fresh (colour 1 forward) its own value.
-__global__ __launch_bounds__(256) void k_cg_sgs(Dims d, Grid1D g, SplitSOR sp, Mixed mx, int colour,
-                                                int first, int fresh, float om) {
-  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (mx.cg[4] != 0.0) return;  // converged
-  if (i > d.ip || j > d.jp || ((i + j + k) & 1) != colour) return;
-  const int is = split_index(sp, i, j, k);
+__device__ __forceinline__ void cg_relax_body(Dims d, const Grid1D& g, SplitSOR sp, Mixed mx,
+                                              int i, int j, int k, int is, int colour, int first,
+                                              int fresh, float om) {
   float dg;
   const float s = first ? 0.0f : offdiag_sum(d, g, sp, mx.z, i, j, k, dg, mx.cn1[d.O(i, j, k)]);
   if (first) {
@@ -295,20 +302,35 @@ __global__ __launch_bounds__(256) void k_cg_sgs(Dims d, Grid1D g, SplitSOR sp, M
   mx.z[colour][is] = (first || fresh) ? om * gs : (1.0f - om) * mx.z[colour][is] + om * gs;
 }
 
-// block partials of <rr, z> (rr = -sp.r)
-__global__ __launch_bounds__(256) void k_cg_rz(Dims d, Grid1D g, SplitSOR sp, Mixed mx) {
+// one colour of the symmetric SOR preconditioner z = M^-1 rr (rr in sp.r, negated: the system
+// is -L e = -r), relaxation om: z_c <- (1 - om) z_c + om (rr_c + sum c_nb z_nb) / dg_c.  The
+// application starts from z = 0: first (colour 0 forward) ignores the neighbours and its own
+// value, fresh (colour 1 forward) its own value.
+// update (first pass of an iteration): every cell first takes e += alpha dnew and rr -= alpha q.
+// rz (last pass): block partials of <rr, z> over both colours (colour 1 is final by then).
+__global__ __launch_bounds__(256) void k_cg_sgs(Dims d, Grid1D g, SplitSOR sp, Mixed mx, int colour,
+                                                int first, int fresh, float om, CgDir dd,
+                                                int update, int rz) {
   __shared__ double lds[4];
-  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
   const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
   const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
   const int k = 1 + blockIdx.z;
-  double acc = 0.0;
-  if (i <= d.ip && j <= d.jp) {
-    const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
-    acc = vol(g, i, j, k) * -(double)sp.r[par][is] * (double)mx.z[par][is];
+  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
+  const bool in = i <= d.ip && j <= d.jp;
+  const int par = (i + j + k) & 1;
+  const int is = in ? split_index(sp, i, j, k) : 0;
+  if (in && update) {
+    const float a = (float)mx.cg[2];
+    sp.p[par][is] = sp.p[par][is] + a * dd.new_[par][is];
+    sp.r[par][is] = sp.r[par][is] + a * mx.q[par][is];
   }
-  const double t = block_sum(acc, lds);
-  if (tid() == 0) mx.part[flat_block()] = t;
+  if (in && par == colour) cg_relax_body(d, g, sp, mx, i, j, k, is, colour, first, fresh, om);
+  if (rz) {
+    double acc = 0.0;
+    if (in) acc = vol(g, i, j, k) * -(double)sp.r[par][is] * (double)mx.z[par][is];
+    const double t = block_sum(acc, lds);
+    if (tid() == 0) mx.part[flat_block()] = t;
+  }
 }
 
 // which 0: rz = rz0 = sum (start); 1: alpha = rz / sum(pq); 2: beta = sum / rz, rz = sum, and
@@ -333,25 +355,6 @@ __global__ __launch_bounds__(256) void k_cg_scalar(Mixed mx, int which, double s
       mx.cg[6] += 1.0;  // iterations of this press
       if (!(s > stop * mx.cg[5])) mx.cg[4] = 1.0;  // also stops on 0 or NaN
     }
-  }
-}
-
-// which 0: e += alpha dir, rr -= alpha q (sp.r += alpha q); 1: dir = z + beta dir; 2: dir = z
-__global__ __launch_bounds__(256) void k_cg_update(Dims d, SplitSOR sp, Mixed mx, int which) {
-  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
-  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > d.ip || j > d.jp) return;
-  const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
-  if (which == 0) {
-    const float a = (float)mx.cg[2];
-    sp.p[par][is] = sp.p[par][is] + a * mx.dir[par][is];
-    sp.r[par][is] = sp.r[par][is] + a * mx.q[par][is];
-  } else if (which == 1) {
-    mx.dir[par][is] = mx.z[par][is] + (float)mx.cg[3] * mx.dir[par][is];
-  } else {
-    mx.dir[par][is] = mx.z[par][is];
   }
 }
 
@@ -405,28 +408,29 @@ __global__ __launch_bounds__(256) void k_cg_project(Dims d, SplitSOR sp, Mixed m
 void launch_mx_cg(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int iters, float om,
                   double stop) {
   const dim3 grid = mx_grid(d), blk(MX_BX, MX_BY);
-  // symmetric SOR: colour 0, colour 1 forward, colour 1, colour 0 backward
-  auto precondition = [&]() {
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 1, 1, om);
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 1, om);
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 0, om);
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 0, 0, om);
-    hipLaunchKernelGGL(k_cg_rz, grid, blk, 0, st, d, g, sp, mx);
+  CgDir dd{{mx.dir[0], mx.dir[1]}, {mx.dir2[0], mx.dir2[1]}};
+  // symmetric SOR from zero: colour 0, colour 1 forward, colour 1, colour 0 backward; the first
+  // pass may take the iteration's e and rr update, the last adds up <rr, z>
+  auto precondition = [&](int update) {
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 1, 1, om, dd, update, 0);
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 1, om, dd, 0, 0);
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 0, om, dd, 0, 0);
+    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 0, 0, om, dd, 0, 1);
   };
   hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, st, mx);
   hipLaunchKernelGGL(k_cg_mean_part, grid, blk, 0, st, d, g, sp, mx);
   hipLaunchKernelGGL(k_cg_mean, dim3(1), dim3(256), 0, st, mx);
   hipLaunchKernelGGL(k_cg_project, grid, blk, 0, st, d, sp, mx);
-  precondition();
+  precondition(0);
   hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 0, stop);
-  hipLaunchKernelGGL(k_cg_update, grid, blk, 0, st, d, sp, mx, 2);
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_cg_apply, grid, blk, 0, st, d, g, sp, mx);
+    // dnew = z + beta dold into the other buffer; q = A dnew; alpha
+    hipLaunchKernelGGL(k_cg_apply, grid, blk, 0, st, d, g, sp, mx, dd, it == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 1, stop);
-    hipLaunchKernelGGL(k_cg_update, grid, blk, 0, st, d, sp, mx, 0);
-    precondition();
+    precondition(1);  // e += alpha dnew, rr -= alpha q, z = M^-1 rr, <rr, z>
     hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 2, stop);
-    hipLaunchKernelGGL(k_cg_update, grid, blk, 0, st, d, sp, mx, 1);
+    std::swap(dd.old_[0], dd.new_[0]);
+    std::swap(dd.old_[1], dd.new_[1]);
   }
 }
 
