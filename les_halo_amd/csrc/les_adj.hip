@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "les_hip.h"
+#include "les_kernels.h"
 
 namespace les_adj_impl {
 
@@ -353,6 +354,12 @@ struct les_adj {
   int* probes = nullptr;
   int nprobes_cap = 0;
   double area = 0.0, pco = 0.0;
+  // FP32: a plain handle on the same grid whose colour-split solve runs the forward's SOR
+  // iterations (the same values as the statement sweeps when p's halos are mirror images on
+  // entry; the transposed sweeps do not read the iterates, the solve being linear)
+  les_handle* ph = nullptr;
+  bool ph_tuned = false;
+  bool p_mirror = true;  // every halo of p equals its mirror interior cell
 };
 
 namespace {
@@ -409,6 +416,20 @@ void run_sum(les_adj* h, int which) {
                      static_cast<AdjScal<T>*>(h->S), pars<T>(h));
 }
 
+// The forward's SOR iterations by the plain handle's split solve (FP32 with mirror-consistent
+// p halos); false when it does not apply and the statement sweeps run instead.
+template <typename T>
+bool split_sor_fw(les_adj* h) {
+  if (sizeof(T) != sizeof(float) || !h->ph || !h->p_mirror || h->cfg.nmaxp < 1) return false;
+  if (!h->ph_tuned) {
+    les_tuning t{};
+    if (les_autotune(h->ph, 5, &t) != LES_OK) return false;
+    h->ph_tuned = true;
+  }
+  return les::sor_solve_external(h->ph, h->st, static_cast<float*>(h->A[ADJ_A_p]),
+                                 static_cast<float*>(h->A[ADJ_A_rhs])) == LES_OK;
+}
+
 template <typename T>
 void run_ops_fw(les_adj* h, const int* ops, int nops) {
   const AdjArr<T> A = arrs<T>(h, false);
@@ -444,7 +465,8 @@ void run_ops_fw(les_adj* h, const int* ops, int nops) {
     } else if (op == ADJ_OP_PAV) {
       run_sum<T>(h, 1);
     } else if (op == ADJ_OP_SOR) {
-      for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_fw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
+      if (!split_sor_fw<T>(h))
+        for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_fw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
     }
   }
 }
@@ -629,6 +651,10 @@ extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
     return bail(LES_E_NOMEM);
   if (hipMemset(h->S, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(h->SB, 0, 3 * sizeof(double)) != hipSuccess)
     return bail(LES_E_HIP);
+  if (!h->fp64) {  // without it (les_create failed) the statement sweeps run the SOR
+    les_handle* ph = nullptr;
+    if (les_create(cfg, &ph) == LES_OK) h->ph = ph;
+  }
   *out = h;
   return LES_OK;
 }
@@ -636,6 +662,7 @@ extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
 extern "C" int les_adj_destroy(les_adj* h) {
   if (!h) return LES_E_INVALID;
   if (h->st) hipStreamSynchronize(h->st);
+  if (h->ph) les_destroy(h->ph);
   for (int q = 0; q < ADJ_NARRAYS; ++q) {
     if (h->A[q]) hipFree(h->A[q]);
     if (h->B[q]) hipFree(h->B[q]);
@@ -648,6 +675,37 @@ extern "C" int les_adj_destroy(les_adj* h) {
 }
 
 extern "C" const char* les_adj_last_error(const les_adj* h) { return h ? h->err.c_str() : ""; }
+
+// every halo cell of p(0:ip+2, 0:jp+2, 0:kp+1) in the box (0:ip+1, 0:jp+1, 0:kp+1) equals its
+// mirror interior cell (i: 0 -> 1, ip+1 -> ip; j periodic: 0 -> jp, jp+1 -> 1; k: 0 -> 1,
+// kp+1 -> kp), bit for bit
+static bool p_halos_mirror(const les_adj* h, const std::vector<float>& p) {
+  const int ip = h->im, jp = h->jm, kp = h->km;
+  auto at = [&](int i, int j, int k) { return p[i + (size_t)(ip + 3) * (j + (size_t)(jp + 3) * k)]; };
+  for (int k = 0; k <= kp + 1; ++k)
+    for (int j = 0; j <= jp + 1; ++j)
+      for (int i = 0; i <= ip + 1; ++i) {
+        const bool halo = i == 0 || i == ip + 1 || j == 0 || j == jp + 1 || k == 0 || k == kp + 1;
+        if (!halo) continue;
+        const int mi = i == 0 ? 1 : (i == ip + 1 ? ip : i);
+        const int mj = j == 0 ? jp : (j == jp + 1 ? 1 : j);
+        const int mk = k == 0 ? 1 : (k == kp + 1 ? kp : k);
+        const float a = at(i, j, k), b = at(mi, mj, mk);
+        if (std::memcmp(&a, &b, sizeof(float)) != 0) return false;
+      }
+  return true;
+}
+
+// grid and coefficient arrays the plain handle's split solve reads
+static bool sor_input(int field) {
+  switch (field) {
+    case LES_DXS: case LES_DYS: case LES_DZS: case LES_CN1: case LES_CN2L: case LES_CN2S:
+    case LES_CN3L: case LES_CN3S: case LES_CN4L: case LES_CN4S:
+      return true;
+    default:
+      return false;
+  }
+}
 
 static int xfer(les_adj* h, int field, double* host, bool up, bool adj) {
   if (!h || !host) return LES_E_INVALID;
@@ -667,6 +725,14 @@ static int xfer(les_adj* h, int field, double* host, bool up, bool adj) {
     if (up) {
       for (long q = 0; q < n; ++q) tmp[q] = (float)host[q];
       ACHK(h, hipMemcpy(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+      if (!adj && field == LES_P) h->p_mirror = p_halos_mirror(h, tmp);
+      if (!adj && h->ph && sor_input(field)) {
+        h->ph_tuned = false;
+        if (les_upload(h->ph, field, tmp.data()) != LES_OK) {  // fall back to the statements
+          les_destroy(h->ph);
+          h->ph = nullptr;
+        }
+      }
     } else {
       ACHK(h, hipMemcpy(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost));
       for (long q = 0; q < n; ++q) host[q] = tmp[q];

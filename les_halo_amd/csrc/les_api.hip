@@ -844,6 +844,32 @@ int enqueue_sor_solve(les_handle* h) {
   return check_launch(h);
 }
 
+}  // namespace
+
+namespace les {
+// press.f95:87-115 (nmaxp iterations, no early exit) on p and rhs arrays of the handle's
+// Fortran layouts held elsewhere, enqueued on stream st: the colour-split solve of the handle
+// (its split copy, cn1 factors and tuned iteration), which leaves p with the boundary images
+// boundp1/boundp2 leave.  Same conditions as the handle's own split solve (even jp, cn1 of the
+// init.f95 form) and, up to the caller, mirror-consistent halos of p on entry.
+int sor_solve_external(les_handle* h, hipStream_t st, float* p, float* rhs) {
+  if (!h || !p || !rhs) return LES_E_INVALID;
+  if (!(h->d.jp % 2 == 0 && h->cn1_separable && h->split_ok))
+    return fail(h, LES_E_STATE, "split solve does not apply to this grid");
+  const hipStream_t own = h->st;
+  h->st = st;
+  Fields F = fields_of(h);
+  F.p = p;
+  F.rhs = rhs;
+  launch_split_scatter(st, h->d, F, h->sp, h->sc, 0);
+  int r = enqueue_sor_split_loop(h, false);
+  if (!r) launch_split_gather_adj(st, h->d, F, h->sp, h->sc, 0);
+  h->st = own;
+  return r ? r : check_launch(h);
+}
+}  // namespace les
+
+namespace {
 // press.f95:49-146 with FP64 means and outer iterative refinement (les_mixed.hip).  Runs
 // eagerly: the host reads the residual after each pass.
 int enqueue_press_mixed(les_handle* h) {

@@ -2,6 +2,8 @@
 #pragma once
 #include "les_common.h"
 
+struct les_handle;
+
 namespace les {
 
 // Device pointers of every field, in the reference's Fortran layouts.
@@ -196,5 +198,9 @@ void launch_halo_read(hipStream_t st, const float* const* arrays, int v_dim, int
                       int km, int h_w, int correct, float* buffer);
 void launch_halo_write(hipStream_t st, float* const* arrays, int v_dim, int im, int jm, int km,
                        int h_w, const float* buffer);
+
+// press.f95:87-115 by a plain handle's colour-split solve on p and rhs held elsewhere, on
+// stream st (les_api.hip; the adjoint form's FP32 forward)
+int sor_solve_external(::les_handle* h, hipStream_t st, float* p, float* rhs);
 
 }  // namespace les

@@ -280,11 +280,14 @@ def test_run_after_inconsistent_upload(case):
 
 
 @pytest.mark.parametrize("dims,tile", [((40, 32, 20), (8, 10)), ((37, 18, 23), (4, 5)),
-                                       ((37, 18, 23), (16, 23)), ((40, 32, 20), (4, 45))])
+                                       ((37, 18, 23), (16, 23)), ((40, 32, 20), (4, 45)),
+                                       ((33, 18, 13), (4, 6)), ((34, 20, 11), (8, 4))])
 def test_fused_sor_iteration(dims, tile):
     """The fused iteration (both colours per launch, ping-pong iterates) with a given tile
     reproduces the oracle's trajectory bit for bit (50 iterations at omega 1.5, no early exit),
-    eagerly and from the captured graph."""
+    eagerly and from the captured graph.  The widths put the last cell of a colour row (i = ip
+    or ip - 1) at every position of its four-cell group; at ip = 33 the rows starting at i = 2
+    end in a group of padding only."""
     case = common.tiny_case(*dims, seed=21)
     L, o = _pair(case, **common.CFG_SOR)
     L.set_sor_tile(*tile)
