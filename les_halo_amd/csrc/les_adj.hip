@@ -359,6 +359,7 @@ struct les_adj {
   // entry; the transposed sweeps do not read the iterates, the solve being linear)
   les_handle* ph = nullptr;
   bool ph_tuned = false;
+  bool sor_split = false;  // the last forward SOR ran through ph (its transpose does too)
   bool p_mirror = true;  // every halo of p equals its mirror interior cell
 };
 
@@ -420,14 +421,26 @@ void run_sum(les_adj* h, int which) {
 // p halos); false when it does not apply and the statement sweeps run instead.
 template <typename T>
 bool split_sor_fw(les_adj* h) {
+  h->sor_split = false;
   if (sizeof(T) != sizeof(float) || !h->ph || !h->p_mirror || h->cfg.nmaxp < 1) return false;
   if (!h->ph_tuned) {
     les_tuning t{};
     if (les_autotune(h->ph, 5, &t) != LES_OK) return false;
     h->ph_tuned = true;
   }
-  return les::sor_solve_external(h->ph, h->st, static_cast<float*>(h->A[ADJ_A_p]),
-                                 static_cast<float*>(h->A[ADJ_A_rhs])) == LES_OK;
+  h->sor_split = les::sor_solve_external(h->ph, h->st, static_cast<float*>(h->A[ADJ_A_p]),
+                                         static_cast<float*>(h->A[ADJ_A_rhs])) == LES_OK;
+  return h->sor_split;
+}
+
+// Its transpose after a split forward (the statement transposes otherwise: on p with mirror
+// halos both forwards are the same function, so either transpose is its adjoint).
+template <typename T>
+bool split_sor_bw(les_adj* h) {
+  if (sizeof(T) != sizeof(float) || !h->sor_split || !h->B[ADJ_A_p] || !h->B[ADJ_A_rhs])
+    return false;
+  return les::sor_adjoint_external(h->ph, h->st, static_cast<float*>(h->B[ADJ_A_p]),
+                                   static_cast<float*>(h->B[ADJ_A_rhs])) == LES_OK;
 }
 
 template <typename T>
@@ -508,7 +521,8 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
                          A, B, SB, which, P, h->im, h->jm, h->km);
       hipLaunchKernelGGL(k_zero_scal<T>, dim3(1), dim3(1), 0, h->st, SB);
     } else if (op == ADJ_OP_SOR) {
-      for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_bw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
+      if (!split_sor_bw<T>(h))
+        for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_bw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
     }
   }
 }

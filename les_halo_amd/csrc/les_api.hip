@@ -867,6 +867,31 @@ int sor_solve_external(les_handle* h, hipStream_t st, float* p, float* rhs) {
   h->st = own;
   return r ? r : check_launch(h);
 }
+
+// The transpose of sor_solve_external on a uniform horizontal grid: given the adjoints pb of p
+// (after the solve) and rb of rhs, pb becomes the adjoint of p before it and rb gains the
+// rhs part.  The split copy of the handle holds the split adjoints meanwhile.
+int sor_adjoint_external(les_handle* h, hipStream_t st, float* pb, float* rb) {
+  if (!h || !pb || !rb) return LES_E_INVALID;
+  if (!(h->d.jp % 2 == 0 && h->cn1_separable && h->split_ok && h->sp.uniform))
+    return fail(h, LES_E_STATE, "split solve transpose needs a uniform horizontal grid");
+  const Dims d = h->d;
+  const Grid1D g = grid_of(h);
+  const float om = h->ph.omega;
+  const int n = h->cfg.nmaxp;
+  launch_split_gather_t(st, d, pb, h->sp);
+  if (n > 0) {
+    // the iterations swept colour 1 then colour 0: their transposes run in reverse, colour 0
+    // first; the other-colour part of each transposed sweep joins the own part of the next
+    launch_split_sweep_t(st, d, g, h->sp, om, 1, 0);
+    for (int l = n; l >= 1; --l) {
+      launch_split_sweep_t(st, d, g, h->sp, om, 0, 1);
+      launch_split_sweep_t(st, d, g, h->sp, om, 1, l > 1 ? 1 : 2);
+    }
+  }
+  launch_split_scatter_t(st, d, pb, rb, h->sp);
+  return check_launch(h);
+}
 }  // namespace les
 
 namespace {

@@ -173,6 +173,16 @@ void launch_sor_split_tiles(hipStream_t st, Dims d, const SorTiles& T, Physics p
 void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T);
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                              int subtract_pav);
+// Transposes of the colour-split solve (scatter, iterations, gather) for adjoints pb of p and
+// rb of rhs in their Fortran layouts; the split adjoints live in sp.p (of p) and sp.r (of rhs).
+// gather_t: sp.p = the sum of pb over the box cells mirroring each interior cell, those pb
+// zeroed, sp.r zeroed.  sweep_t on the colour-c cells (uniform horizontal grid), mode 0: add
+// the gather from the other colour; 1: scale by the own coefficient, accumulate the rhs
+// adjoint, add the gather; 2: scale and accumulate only.  scatter_t: pb, rb += sp.p, sp.r.
+void launch_split_gather_t(hipStream_t st, Dims d, float* pb, SplitSOR sp);
+void launch_split_sweep_t(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, float omega, int c,
+                          int mode);
+void launch_split_scatter_t(hipStream_t st, Dims d, float* pb, float* rb, SplitSOR sp);
 int reduction_blocks(Dims d);
 
 // exact sum of n device floats into *out (device), for tests of the reduction itself
@@ -202,5 +212,7 @@ void launch_halo_write(hipStream_t st, float* const* arrays, int v_dim, int im, 
 // press.f95:87-115 by a plain handle's colour-split solve on p and rhs held elsewhere, on
 // stream st (les_api.hip; the adjoint form's FP32 forward)
 int sor_solve_external(::les_handle* h, hipStream_t st, float* p, float* rhs);
+// its transpose (uniform horizontal grid): adjoints pb of p and rb of rhs
+int sor_adjoint_external(::les_handle* h, hipStream_t st, float* pb, float* rb);
 
 }  // namespace les

@@ -666,4 +666,118 @@ void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevS
   hipLaunchKernelGGL(k_split_gather_adj, grid, dim3(64, 4), 0, st, d, F, sp, sc, subtract_pav);
 }
 
+// ---- transposes of the colour-split solve (the adjoint form's FP32 reverse) ----
+// One thread per interior cell (i,j,k) of the Fortran box, i fastest.
+
+// gather: every box cell b of [0,ip+1]x[0,jp+1]x[0,kp+1] took the split value of its mirror
+// interior cell, so the split adjoint of cell x is the sum of pb over the cells mirroring x
+// (x itself, and across the faces it touches: i = 0 / ip+1, j = jp+1 / 0, k = 0 / kp+1), and
+// those pb are consumed.
+__global__ __launch_bounds__(256) void k_split_gather_t(Dims d, float* pb, SplitSOR sp) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int is[2] = {i, i == 1 ? 0 : (i == d.ip ? d.ip + 1 : -1)};
+  const int js[2] = {j, j == 1 ? d.jp + 1 : (j == d.jp ? 0 : -1)};
+  const int ks[2] = {k, k == 1 ? 0 : (k == d.kp ? d.kp + 1 : -1)};
+  float v = 0.0f;
+  for (int c = 0; c < 2; ++c) {
+    if (ks[c] < 0) continue;
+    for (int b = 0; b < 2; ++b) {
+      if (js[b] < 0) continue;
+      for (int a = 0; a < 2; ++a) {
+        if (is[a] < 0) continue;
+        const long o = d.P(is[a], js[b], ks[c]);
+        v = v + pb[o];
+        pb[o] = 0.0f;
+      }
+    }
+  }
+  const int cc = (i + j + k) & 1;
+  const int e = split_index(sp, i, j, k);
+  sp.p[cc][e] = v;
+  sp.r[cc][e] = 0.0f;
+}
+
+// One colour sweep p_c <- p_c + omega (cn1 (sum_n a_n q_n - r) - p_c) reads the other colour
+// (q_n) and, across i and k faces, its own value (the mirror image); its transpose, applied in
+// reverse sweep order, gives each colour-c cell y, from its adjoint P(y):
+//   rhs adjoint:       R(y) += -omega cn1(y) P(y)
+//   own:               P(y) <- s(y) P(y), s = 1 - omega + omega cn1(y) (sum of a_n over images)
+//   other colour x:    P(x) += omega cn1(y) a_n(y) P(y) for each neighbour x that y reads.
+// The kernel for the colour-c cells y combines the own part of one sweep's transpose with the
+// other-colour part of the preceding one (in reverse order): it gathers, for y, the terms of
+// its other-colour neighbours x (x reads y with x's coefficient toward y: c2l from the west
+// neighbour, c2s from the east, c3l from the south, c3s from the north, cn4l(k-1) from below,
+// cn4s(k+1) from above, each times omega cn1 of x's level), so every cell is written by one
+// thread and no atomics are needed.
+__global__ __launch_bounds__(256) void k_split_sweep_t(Dims d, Grid1D g, SplitSOR sp, float om,
+                                                       int c, int mode) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp || ((i + j + k) & 1) != c) return;
+  const float* po = sp.p[c ^ 1];
+  const int e = split_index(sp, i, j, k);
+  const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
+  const float cn1 = sp.cn1k[k];
+  const float own = sp.p[c][e];
+  float out;
+  if (mode == 0) {
+    out = own;
+  } else {
+    float img = 0.0f;
+    if (i == 1) img = img + c2s;
+    if (i == d.ip) img = img + c2l;
+    if (k == 1) img = img + g.cn4s[k - 1];
+    if (k == d.kp) img = img + g.cn4l[k - 1];
+    const float s = (1.0f - om) + om * (cn1 * img);
+    sp.r[c][e] = sp.r[c][e] - om * (cn1 * own);
+    out = s * own;
+  }
+  if (mode != 2) {
+    const int row = sp.nr * ((j - 1) + sp.nj * (k - 1));
+    const int ii = (i - 1) >> 1;
+    const int jn = (j == d.jp) ? 1 : j + 1, js = (j == 1) ? d.jp : j - 1;
+    float gsum = 0.0f;
+    if (i > 1) gsum = gsum + c2l * po[((i - 2) >> 1) + row];
+    if (i < d.ip) gsum = gsum + c2s * po[(i >> 1) + row];
+    gsum = gsum + c3l * po[ii + sp.nr * ((js - 1) + sp.nj * (k - 1))];
+    gsum = gsum + c3s * po[ii + sp.nr * ((jn - 1) + sp.nj * (k - 1))];
+    float v = om * (cn1 * gsum);
+    if (k > 1) v = v + om * (sp.cn1k[k - 1] * (g.cn4l[k - 2] * po[e - sp.nr * sp.nj]));
+    if (k < d.kp) v = v + om * (sp.cn1k[k + 1] * (g.cn4s[k] * po[e + sp.nr * sp.nj]));
+    out = out + v;
+  }
+  sp.p[c][e] = out;
+}
+
+__global__ __launch_bounds__(256) void k_split_scatter_t(Dims d, float* pb, float* rb, SplitSOR sp) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  const int cc = (i + j + k) & 1;
+  const int e = split_index(sp, i, j, k);
+  pb[d.P(i, j, k)] = pb[d.P(i, j, k)] + sp.p[cc][e];
+  rb[d.R(i, j, k)] = rb[d.R(i, j, k)] + sp.r[cc][e];
+}
+
+void launch_split_gather_t(hipStream_t st, Dims d, float* pb, SplitSOR sp) {
+  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
+  hipLaunchKernelGGL(k_split_gather_t, grid, dim3(64, 4), 0, st, d, pb, sp);
+}
+
+void launch_split_sweep_t(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, float omega, int c,
+                          int mode) {
+  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
+  hipLaunchKernelGGL(k_split_sweep_t, grid, dim3(64, 4), 0, st, d, g, sp, omega, c, mode);
+}
+
+void launch_split_scatter_t(hipStream_t st, Dims d, float* pb, float* rb, SplitSOR sp) {
+  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
+  hipLaunchKernelGGL(k_split_scatter_t, grid, dim3(64, 4), 0, st, d, pb, rb, sp);
+}
+
 }  // namespace les
