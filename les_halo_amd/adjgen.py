@@ -101,6 +101,10 @@ class Sc(E):  # a scalar the step computes (uout, rhsav, pav): active
         self.name = name
 
 
+# the step's scalars in the order of AdjScal (les_adj.hip); scal_add takes the position
+SCALARS = ("uout", "rhsav", "pav")
+
+
 class Ref(E):  # an array element; idx: tuple of C index strings
     def __init__(self, arr, idx):
         self.arr, self.idx = arr, idx
@@ -629,7 +633,7 @@ def emit_stmt(s, sid):
             out.append("  if (act) atomicAdd(&B.%s[%s], %s);" % (key[0], flat(*key), sumv))
     out.append("  if (act) B.%s[yi] = %s;" % (tgt[0], self_bar if self_bar else "T(0)"))
     for name, sumv in scal:
-        out.append("  wave_add(&SB->%s, act ? (%s) : T(0));" % (name, sumv))
+        out.append("  scal_add(SB, %d, act ? (%s) : T(0));" % (SCALARS.index(name), sumv))
     out.append("}")
     return out
 

@@ -28,11 +28,38 @@ struct AdjPar {
   T dt, ro, vn, alpha, beta, cs0, omega, area, pco;
 };
 
-// sum over the wavefront, one atomic add per wavefront (every lane calls it)
+// Scalar adjoints: the AdjScal<T> (uout, rhsav, pav) followed by ADJ_SLOTS partial sums per
+// scalar.  A statement transpose adds its terms of scalar q as one atomic add per workgroup
+// into slot (workgroup mod ADJ_SLOTS), so the adds of a domain-wide statement spread over
+// many addresses; k_scal_fold adds the slots into the scalar before it is read.
+constexpr int ADJ_SLOTS = 64;
+constexpr int SB_COUNT = 3 + 3 * ADJ_SLOTS;  // scalars and slots (elements of T)
 template <typename T>
-__device__ __forceinline__ void wave_add(T* p, T v) {
+__device__ __forceinline__ T* scal_slots(AdjScal<T>* SB, int q) {
+  return reinterpret_cast<T*>(SB + 1) + q * ADJ_SLOTS;
+}
+// every thread of the workgroup calls it (uniform control flow)
+template <typename T>
+__device__ __forceinline__ void scal_add(AdjScal<T>* SB, int q, T v) {
+  __shared__ T ws[16];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if ((threadIdx.x & 63) == 0 && v != T(0)) atomicAdd(p, v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T s = T(0);
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w];
+    if (s != T(0)) atomicAdd(scal_slots(SB, q) + blockIdx.x % ADJ_SLOTS, s);
+  }
+  __syncthreads();  // ws is reused by the next call
+}
+// one wavefront: SB[q] += the sum of its slots, slots zeroed
+template <typename T>
+__global__ __launch_bounds__(ADJ_SLOTS) void k_scal_fold(AdjScal<T>* SB, int q) {
+  T* slot = scal_slots(SB, q);
+  T v = slot[threadIdx.x];
+  slot[threadIdx.x] = T(0);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (threadIdx.x == 0) reinterpret_cast<T*>(SB)[q] += v;
 }
 
 #include "les_adj_gen.inc"
@@ -513,10 +540,12 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
     } else if (op >= 0) {
       adj_launch_bw<T>(op, h->st, A, B, S, SB, P, h->im, h->jm, h->km);
     } else if (op == ADJ_OP_UOUT) {
+      hipLaunchKernelGGL(k_scal_fold<T>, dim3(1), dim3(ADJ_SLOTS), 0, h->st, SB, 0);
       hipLaunchKernelGGL(k_uout_bw<T>, dim3(1), dim3(RED_THREADS), 0, h->st, A, B, SB, h->im, h->jm,
                          h->km);
     } else if (op == ADJ_OP_RHSAV || op == ADJ_OP_PAV) {
       const int which = op == ADJ_OP_RHSAV ? 0 : 1;
+      hipLaunchKernelGGL(k_scal_fold<T>, dim3(1), dim3(ADJ_SLOTS), 0, h->st, SB, 1 + which);
       hipLaunchKernelGGL(k_wsum_bw<T>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, h->st,
                          A, B, SB, which, P, h->im, h->jm, h->km);
       hipLaunchKernelGGL(k_zero_scal<T>, dim3(1), dim3(1), 0, h->st, SB);
@@ -553,7 +582,7 @@ int zero_temporaries(les_adj* h) {
   for (int q = 0; q < ADJ_NARRAYS; ++q)
     if (h->B[q] && !is_carried(q) && q != ADJ_A_wind && q != ADJ_A_cd)
       ACHK(h, hipMemsetAsync(h->B[q], 0, h->n[q] * tsize(h), h->st));
-  ACHK(h, hipMemsetAsync(h->SB, 0, 3 * tsize(h), h->st));
+  ACHK(h, hipMemsetAsync(h->SB, 0, SB_COUNT * tsize(h), h->st));
   return LES_OK;
 }
 
@@ -659,11 +688,11 @@ extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
     if (r) return bail(r);
   }
   h->npart = 512;
-  if (hipMalloc(&h->S, 3 * sizeof(double)) != hipSuccess || hipMalloc(&h->SB, 3 * sizeof(double)) != hipSuccess ||
+  if (hipMalloc(&h->S, 3 * sizeof(double)) != hipSuccess || hipMalloc(&h->SB, SB_COUNT * sizeof(double)) != hipSuccess ||
       hipMalloc(&h->part, h->npart * sizeof(double)) != hipSuccess ||
       hipMalloc(&h->jacc, sizeof(double)) != hipSuccess)
     return bail(LES_E_NOMEM);
-  if (hipMemset(h->S, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(h->SB, 0, 3 * sizeof(double)) != hipSuccess)
+  if (hipMemset(h->S, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(h->SB, 0, SB_COUNT * sizeof(double)) != hipSuccess)
     return bail(LES_E_HIP);
   if (!h->fp64) {  // without it (les_create failed) the statement sweeps run the SOR
     les_handle* ph = nullptr;
@@ -796,7 +825,7 @@ extern "C" int les_adj_zero_adjoints(les_adj* h) {
   if (!h) return LES_E_INVALID;
   for (int q = 0; q < ADJ_NARRAYS; ++q)
     if (h->B[q]) ACHK(h, hipMemsetAsync(h->B[q], 0, h->n[q] * tsize(h), h->st));
-  ACHK(h, hipMemsetAsync(h->SB, 0, 3 * tsize(h), h->st));
+  ACHK(h, hipMemsetAsync(h->SB, 0, SB_COUNT * tsize(h), h->st));
   return check(h);
 }
 

@@ -712,12 +712,13 @@ __global__ __launch_bounds__(256) void k_split_gather_t(Dims d, float* pb, Split
 // neighbour, c2s from the east, c3l from the south, c3s from the north, cn4l(k-1) from below,
 // cn4s(k+1) from above, each times omega cn1 of x's level), so every cell is written by one
 // thread and no atomics are needed.
+// One thread per split entry ii of a colour-c row (j, k): cell i = first_i + 2 ii.
 __global__ __launch_bounds__(256) void k_split_sweep_t(Dims d, Grid1D g, SplitSOR sp, float om,
                                                        int c, int mode) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
-  if (i > d.ip || j > d.jp || ((i + j + k) & 1) != c) return;
+  const int i = first_i(c, j, k) + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
+  if (i > d.ip || j > d.jp) return;
   const float* po = sp.p[c ^ 1];
   const int e = split_index(sp, i, j, k);
   const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
@@ -771,7 +772,7 @@ void launch_split_gather_t(hipStream_t st, Dims d, float* pb, SplitSOR sp) {
 
 void launch_split_sweep_t(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, float omega, int c,
                           int mode) {
-  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
+  dim3 grid(((d.ip + 1) / 2 + 63) / 64, (d.jp + 3) / 4, d.kp);
   hipLaunchKernelGGL(k_split_sweep_t, grid, dim3(64, 4), 0, st, d, g, sp, omega, c, mode);
 }
 
