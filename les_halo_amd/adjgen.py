@@ -676,6 +676,11 @@ def generate():
     out.append("  }\n  return A.%s;\n}" % ARRAYS[0])
     out.append("constexpr bool ADJ_PASSIVE[ADJ_NARRAYS] = {%s};" %
                ", ".join("true" if a in PASSIVE else "false" for a in ARRAYS))
+    # arrays the step writes (statement targets and the pre-outflow snapshot); the others are
+    # constants of the window
+    written = {st.target.arr for st in stmts} | set(SNAP.values())
+    out.append("constexpr bool ADJ_WRITTEN[ADJ_NARRAYS] = {%s};" %
+               ", ".join("true" if a in written else "false" for a in ARRAYS))
     for s in uniq:
         out += emit_stmt(s, 0)
         out.append("")

@@ -867,22 +867,96 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   if ((e = hipMemcpyAsync(&jsum, h->jacc, sizeof(double), hipMemcpyDeviceToHost, h->st)) != hipSuccess ||
       (e = hipStreamSynchronize(h->st)) != hipSuccess)
     return hipfail(e, "window: read the objective");
-  // reverse: each segment recomputed from its checkpoint, each step recomputed from its state
+  // reverse: each segment recomputed from its checkpoint.  With per-step slots (every array the
+  // step writes and its scalars uout, rhsav, pav, once per step of a segment) the recomputed
+  // steps keep their values and each transposed step reads its own; without them (no memory for the slots) each step is
+  // recomputed once more from its saved state before its transpose.
   if ((r = les_adj_zero_adjoints(h))) return cleanup(), r;
   const double wgt = 1.0 / ((double)nprobes * nsteps);
-  for (int c = nck - 1; c >= 0; --c) {
-    const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
-    if ((r = load_state(h, ck + sb * c))) return cleanup(), r;
-    for (int s = s0; s < s1; ++s) {
-      if ((r = save_state(h, seg + sb * (s - s0)))) return cleanup(), r;
-      if (s + 1 < s1) step_fw(h);
+  const int nseg = std::min(ckpt, nsteps);
+  size_t off[ADJ_NARRAYS] = {}, slot_bytes = 0;
+  for (int q = 0; q < ADJ_NARRAYS; ++q)
+    if (ADJ_WRITTEN[q]) {
+      off[q] = slot_bytes;
+      slot_bytes += (h->n[q] * tsize(h) + 255) / 256 * 256;
     }
-    for (int s = s1 - 1; s >= s0; --s) {
-      if ((r = load_state(h, seg + sb * (s - s0)))) return cleanup(), r;
-      step_fw(h);
-      if ((r = zero_temporaries(h))) return cleanup(), r;
-      seed_probes(h, nprobes, wgt);
-      step_bw(h);
+  const size_t off_s = slot_bytes;  // the step's scalars (AdjScal)
+  slot_bytes += 256;
+  char* slots = nullptr;
+  if (hipMalloc(&slots, slot_bytes * nseg) != hipSuccess) {
+    slots = nullptr;
+    (void)hipGetLastError();  // the failed allocation only selects the recomputing reverse
+  }
+  void* A0[ADJ_NARRAYS];
+  std::memcpy(A0, h->A, sizeof(A0));
+  void* const S0 = h->S;
+  auto slot = [&](int t, int q) { return (void*)(slots + slot_bytes * t + off[q]); };
+  auto point = [&](int t) {
+    for (int q = 0; q < ADJ_NARRAYS; ++q)
+      if (ADJ_WRITTEN[q]) h->A[q] = slot(t, q);
+    h->S = slots + slot_bytes * t + off_s;
+  };
+  auto unslot = [&]() {  // the handle's own arrays back, the slots freed
+    std::memcpy(h->A, A0, sizeof(A0));
+    h->S = S0;
+    if (slots) {
+      hipStreamSynchronize(h->st);
+      hipFree(slots);
+      slots = nullptr;
+    }
+  };
+  auto fail_slots = [&](int code) {
+    unslot();
+    cleanup();
+    return code;
+  };
+  if (slots) {
+    // elements a step reads but never writes keep the values of the handle's arrays
+    for (int t = 0; t < nseg; ++t)
+      for (int q = 0; q < ADJ_NARRAYS; ++q)
+        if (ADJ_WRITTEN[q] &&
+            (e = hipMemcpyAsync(slot(t, q), A0[q], h->n[q] * tsize(h), hipMemcpyDeviceToDevice,
+                                h->st)) != hipSuccess)
+          return unslot(), hipfail(e, "window: step slots");
+    for (int c = nck - 1; c >= 0; --c) {
+      const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+      for (int s = s0; s < s1; ++s) {
+        point(s - s0);
+        if (s == s0) {
+          if ((r = load_state(h, ck + sb * c))) return fail_slots(r);
+        } else {
+          for (int q = 0; q < NCARRIED; ++q) {
+            const int a = CARRIED[q];
+            if ((e = hipMemcpyAsync(slot(s - s0, a), slot(s - s0 - 1, a), h->n[a] * tsize(h),
+                                    hipMemcpyDeviceToDevice, h->st)) != hipSuccess)
+              return unslot(), hipfail(e, "window: carried state");
+          }
+        }
+        step_fw(h);
+      }
+      for (int s = s1 - 1; s >= s0; --s) {
+        point(s - s0);
+        if ((r = zero_temporaries(h))) return fail_slots(r);
+        seed_probes(h, nprobes, wgt);
+        step_bw(h);
+      }
+    }
+    unslot();
+  } else {
+    for (int c = nck - 1; c >= 0; --c) {
+      const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+      if ((r = load_state(h, ck + sb * c))) return cleanup(), r;
+      for (int s = s0; s < s1; ++s) {
+        if ((r = save_state(h, seg + sb * (s - s0)))) return cleanup(), r;
+        if (s + 1 < s1) step_fw(h);
+      }
+      for (int s = s1 - 1; s >= s0; --s) {
+        if ((r = load_state(h, seg + sb * (s - s0)))) return cleanup(), r;
+        step_fw(h);
+        if ((r = zero_temporaries(h))) return cleanup(), r;
+        seed_probes(h, nprobes, wgt);
+        step_bw(h);
+      }
     }
   }
   if ((r = load_state(h, fin))) return cleanup(), r;
