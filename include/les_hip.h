@@ -431,8 +431,11 @@ int les_initial_wind_profile(int jp, int kp, const float* z2, const float* dzn,
  * steps from the current state with a checkpoint every ckpt steps, returns the objective
  * J = mean over the steps and the nprobes (i,j,k) probes of u after each step, and leaves the
  * gradient of J in the adjoints of the controls and of the initial state (recomputing each
- * segment from its checkpoint and each step from its state); the handle's state is the state
- * after the window. */
+ * segment from its checkpoint into per-step copies of every array the step writes, which the
+ * transposed steps then read); the handle's state is the state after the window.
+ * les_adj_set_slots(h, 0) makes the window keep no per-step copies and recompute each step
+ * again from its state before its transpose instead (less memory, one forward pass more, the
+ * same gradient); 1 (the default) keeps them when they fit in device memory. */
 #define LES_ADJ_CD 100
 typedef struct les_adj les_adj;
 int les_adj_create(const les_config* cfg, int fp64, les_adj** out);
@@ -448,6 +451,7 @@ int les_adj_step(les_adj* h, int nsteps);
 int les_adj_reverse_step(les_adj* h);
 int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probes, int nprobes,
                    double* objective);
+int les_adj_set_slots(les_adj* h, int enable);
 
 #ifdef __cplusplus
 }

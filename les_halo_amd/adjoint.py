@@ -140,6 +140,11 @@ class AdjointLES:
         uploaded) become those of the current state; the controls' adjoints accumulate."""
         self._chk(lib().les_adj_reverse_step(self.h), "reverse_step")
 
+    def set_slots(self, enable):
+        """Whether window keeps per-step copies of the segment recomputation (default) or
+        recomputes each step again before its transpose (less memory, the same gradient)."""
+        self._chk(lib().les_adj_set_slots(self.h, 1 if enable else 0), "set_slots")
+
     def window(self, nsteps, ckpt, probes):
         """Runs the window (checkpoint every ckpt steps) and its adjoint; returns J."""
         pr = np.ascontiguousarray(np.asarray(probes, dtype=np.int32).reshape(-1, 3))

@@ -388,6 +388,7 @@ struct les_adj {
   bool ph_tuned = false;
   bool sor_split = false;  // the last forward SOR ran through ph (its transpose does too)
   bool p_mirror = true;  // every halo of p equals its mirror interior cell
+  bool use_slots = true;  // les_adj_window keeps per-step copies (les_adj_set_slots)
 };
 
 namespace {
@@ -797,6 +798,12 @@ extern "C" int les_adj_download_adjoint(les_adj* h, int field, double* host) {
   return xfer(h, field, host, false, true);
 }
 
+extern "C" int les_adj_set_slots(les_adj* h, int enable) {
+  if (!h) return LES_E_INVALID;
+  h->use_slots = enable != 0;
+  return LES_OK;
+}
+
 extern "C" int les_adj_set_sums(les_adj* h, double area, double pco) {
   if (!h) return LES_E_INVALID;
   h->area = area;
@@ -844,8 +851,7 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
     if (seg) hipFree(seg);
     if (fin) hipFree(fin);
   };
-  if (hipMalloc(&ck, sb * nck) != hipSuccess || hipMalloc(&seg, sb * ckpt) != hipSuccess ||
-      hipMalloc(&fin, sb) != hipSuccess) {
+  if (hipMalloc(&ck, sb * nck) != hipSuccess || hipMalloc(&fin, sb) != hipSuccess) {
     cleanup();
     return afail(h, LES_E_NOMEM, "window: checkpoint memory");
   }
@@ -883,9 +889,13 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   const size_t off_s = slot_bytes;  // the step's scalars (AdjScal)
   slot_bytes += 256;
   char* slots = nullptr;
-  if (hipMalloc(&slots, slot_bytes * nseg) != hipSuccess) {
+  if (h->use_slots && hipMalloc(&slots, slot_bytes * nseg) != hipSuccess) {
     slots = nullptr;
     (void)hipGetLastError();  // the failed allocation only selects the recomputing reverse
+  }
+  if (!slots && hipMalloc(&seg, sb * nseg) != hipSuccess) {  // the states of one segment
+    cleanup();
+    return afail(h, LES_E_NOMEM, "window: segment memory");
   }
   void* A0[ADJ_NARRAYS];
   std::memcpy(A0, h->A, sizeof(A0));
