@@ -712,46 +712,82 @@ __global__ __launch_bounds__(256) void k_split_gather_t(Dims d, float* pb, Split
 // neighbour, c2s from the east, c3l from the south, c3s from the north, cn4l(k-1) from below,
 // cn4s(k+1) from above, each times omega cn1 of x's level), so every cell is written by one
 // thread and no atomics are needed.
-// One thread per split entry ii of a colour-c row (j, k): cell i = first_i + 2 ii.
+// A thread takes four consecutive entries 4g..4g+3 of a colour-c row (j, k), cells
+// i = f + 2 (4g + q) with f the row's first i of the colour, through float4 loads of its own
+// row and of the other colour's rows at j, j-1, j+1, k-1, k+1 (the sweep's access pattern);
+// each cell's terms in the same order as a one-cell form would take them.
 __global__ __launch_bounds__(256) void k_split_sweep_t(Dims d, Grid1D g, SplitSOR sp, float om,
                                                        int c, int mode) {
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  const int i = first_i(c, j, k) + 2 * (int)(blockIdx.x * 64 + threadIdx.x);
-  if (i > d.ip || j > d.jp) return;
+  // items: group grp of row (j, k), grp fastest, dealt densely over the workgroups
+  const int ng = sp.nr >> 2;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)ng * d.jp * d.kp) return;
+  const int grp = (int)(t % ng);
+  const int row = (int)(t / ng);
+  const int j = 1 + row % d.jp, k = 1 + row / d.jp;
+  const int f = first_i(c, j, k);
+  const int i0 = f + 8 * grp;  // the group's first cell
+  if (i0 > d.ip) return;
+  auto ld4 = [](const float* p) { return *reinterpret_cast<const float4*>(p); };
   const float* po = sp.p[c ^ 1];
-  const int e = split_index(sp, i, j, k);
+  const int lev = sp.nr * sp.nj;
+  const int base = sp.nr * ((j - 1) + sp.nj * (k - 1)) + 4 * grp;
   const float c2l = g.cn2l[0], c2s = g.cn2s[0], c3l = g.cn3l[0], c3s = g.cn3s[0];
   const float cn1 = sp.cn1k[k];
-  const float own = sp.p[c][e];
-  float out;
-  if (mode == 0) {
-    out = own;
-  } else {
-    float img = 0.0f;
-    if (i == 1) img = img + c2s;
-    if (i == d.ip) img = img + c2l;
-    if (k == 1) img = img + g.cn4s[k - 1];
-    if (k == d.kp) img = img + g.cn4l[k - 1];
-    const float s = (1.0f - om) + om * (cn1 * img);
-    sp.r[c][e] = sp.r[c][e] - om * (cn1 * own);
-    out = s * own;
+  const float4 own4 = ld4(sp.p[c] + base);
+  const float own[4] = {own4.x, own4.y, own4.z, own4.w};
+  float out[4] = {own[0], own[1], own[2], own[3]};
+  if (mode != 0) {  // own coefficient and the rhs adjoint
+    const float4 r4 = ld4(sp.r[c] + base);
+    float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 2 * q;
+      if (i > d.ip) continue;
+      float img = 0.0f;
+      if (i == 1) img = img + c2s;
+      if (i == d.ip) img = img + c2l;
+      if (k == 1) img = img + g.cn4s[k - 1];
+      if (k == d.kp) img = img + g.cn4l[k - 1];
+      const float sc = (1.0f - om) + om * (cn1 * img);
+      rv[q] = rv[q] - om * (cn1 * own[q]);
+      out[q] = sc * own[q];
+    }
+    *reinterpret_cast<float4*>(sp.r[c] + base) = make_float4(rv[0], rv[1], rv[2], rv[3]);
   }
-  if (mode != 2) {
-    const int row = sp.nr * ((j - 1) + sp.nj * (k - 1));
-    const int ii = (i - 1) >> 1;
+  if (mode != 2) {  // the other colour's cells that read these
     const int jn = (j == d.jp) ? 1 : j + 1, js = (j == 1) ? d.jp : j - 1;
-    float gsum = 0.0f;
-    if (i > 1) gsum = gsum + c2l * po[((i - 2) >> 1) + row];
-    if (i < d.ip) gsum = gsum + c2s * po[(i >> 1) + row];
-    gsum = gsum + c3l * po[ii + sp.nr * ((js - 1) + sp.nj * (k - 1))];
-    gsum = gsum + c3s * po[ii + sp.nr * ((jn - 1) + sp.nj * (k - 1))];
-    float v = om * (cn1 * gsum);
-    if (k > 1) v = v + om * (sp.cn1k[k - 1] * (g.cn4l[k - 2] * po[e - sp.nr * sp.nj]));
-    if (k < d.kp) v = v + om * (sp.cn1k[k + 1] * (g.cn4s[k] * po[e + sp.nr * sp.nj]));
-    out = out + v;
+    const float4 xm = ld4(po + base);
+    const float4 xs = ld4(po + base + sp.nr * (js - j));
+    const float4 xn = ld4(po + base + sp.nr * (jn - j));
+    const float4 xd = (k > 1) ? ld4(po + base - lev) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 xu = (k < d.kp) ? ld4(po + base + lev) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // other-colour entry 4g-1 (west of the first cell, f == 1) or 4g+4 (east of the last, f == 2)
+    const float xo = (f == 1) ? (grp > 0 ? po[base - 1] : 0.0f)
+                              : (4 * grp + 4 < sp.nr ? po[base + 4] : 0.0f);
+    const float m[6] = {xo, xm.x, xm.y, xm.z, xm.w, xo};  // m[1 + q]: entry 4g+q
+    const float sv[4] = {xs.x, xs.y, xs.z, xs.w}, nv[4] = {xn.x, xn.y, xn.z, xn.w};
+    const float dv[4] = {xd.x, xd.y, xd.z, xd.w}, uv[4] = {xu.x, xu.y, xu.z, xu.w};
+    const float cdn = (k > 1) ? sp.cn1k[k - 1] : 0.0f, c4d = (k > 1) ? g.cn4l[k - 2] : 0.0f;
+    const float cup = (k < d.kp) ? sp.cn1k[k + 1] : 0.0f, c4u = (k < d.kp) ? g.cn4s[k] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 2 * q;
+      if (i > d.ip) continue;
+      const float pw = (f == 1) ? m[q] : m[1 + q];      // other colour at i - 1
+      const float pe = (f == 1) ? m[1 + q] : m[2 + q];  // other colour at i + 1
+      float gsum = 0.0f;
+      if (i > 1) gsum = gsum + c2l * pw;
+      if (i < d.ip) gsum = gsum + c2s * pe;
+      gsum = gsum + c3l * sv[q];
+      gsum = gsum + c3s * nv[q];
+      float v = om * (cn1 * gsum);
+      if (k > 1) v = v + om * (cdn * (c4d * dv[q]));
+      if (k < d.kp) v = v + om * (cup * (c4u * uv[q]));
+      out[q] = out[q] + v;
+    }
   }
-  sp.p[c][e] = out;
+  *reinterpret_cast<float4*>(sp.p[c] + base) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
 __global__ __launch_bounds__(256) void k_split_scatter_t(Dims d, float* pb, float* rb, SplitSOR sp) {
@@ -772,8 +808,9 @@ void launch_split_gather_t(hipStream_t st, Dims d, float* pb, SplitSOR sp) {
 
 void launch_split_sweep_t(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, float omega, int c,
                           int mode) {
-  dim3 grid(((d.ip + 1) / 2 + 63) / 64, (d.jp + 3) / 4, d.kp);
-  hipLaunchKernelGGL(k_split_sweep_t, grid, dim3(64, 4), 0, st, d, g, sp, omega, c, mode);
+  const long n = (long)(sp.nr / 4) * d.jp * d.kp;
+  hipLaunchKernelGGL(k_split_sweep_t, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, g, sp,
+                     omega, c, mode);
 }
 
 void launch_split_scatter_t(hipStream_t st, Dims d, float* pb, float* rb, SplitSOR sp) {
