@@ -695,9 +695,11 @@ extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
     return bail(LES_E_NOMEM);
   if (hipMemset(h->S, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(h->SB, 0, SB_COUNT * sizeof(double)) != hipSuccess)
     return bail(LES_E_HIP);
-  if (!h->fp64) {  // without it (les_create failed) the statement sweeps run the SOR
+  if (!h->fp64) {  // without it (les_create failed) the statement kernels run everything
+    les_config pc = *cfg;
+    pc.pjuge = 0.0f;  // the statement SOR runs nmaxp iterations, no early exit
     les_handle* ph = nullptr;
-    if (les_create(cfg, &ph) == LES_OK) h->ph = ph;
+    if (les_create(&pc, &ph) == LES_OK) h->ph = ph;
   }
   *out = h;
   return LES_OK;
@@ -740,16 +742,8 @@ static bool p_halos_mirror(const les_adj* h, const std::vector<float>& p) {
   return true;
 }
 
-// grid and coefficient arrays the plain handle's split solve reads
-static bool sor_input(int field) {
-  switch (field) {
-    case LES_DXS: case LES_DYS: case LES_DZS: case LES_CN1: case LES_CN2L: case LES_CN2S:
-    case LES_CN3L: case LES_CN3S: case LES_CN4L: case LES_CN4S:
-      return true;
-    default:
-      return false;
-  }
-}
+// the fields a plain handle holds (every input of the step but the drag field)
+static bool plain_field(int field) { return field >= 0 && field < LES_NFIELDS; }
 
 static int xfer(les_adj* h, int field, double* host, bool up, bool adj) {
   if (!h || !host) return LES_E_INVALID;
@@ -770,7 +764,7 @@ static int xfer(les_adj* h, int field, double* host, bool up, bool adj) {
       for (long q = 0; q < n; ++q) tmp[q] = (float)host[q];
       ACHK(h, hipMemcpy(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
       if (!adj && field == LES_P) h->p_mirror = p_halos_mirror(h, tmp);
-      if (!adj && h->ph && sor_input(field)) {
+      if (!adj && h->ph && plain_field(field)) {
         h->ph_tuned = false;
         if (les_upload(h->ph, field, tmp.data()) != LES_OK) {  // fall back to the statements
           les_destroy(h->ph);
@@ -862,13 +856,66 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   hipError_t e;
   if ((e = hipMemsetAsync(h->jacc, 0, sizeof(double), h->st)) != hipSuccess)
     return hipfail(e, "window: clear the objective");
-  // forward window: a checkpoint every ckpt steps, the objective after every step
-  for (int s = 0; s < nsteps; ++s) {
-    if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return cleanup(), r;
-    step_fw(h);
-    sum_probes(h, nprobes);
+  // forward window: a checkpoint every ckpt steps, the objective after every step.  In FP32 the
+  // plain handle runs it (its fused kernels, feedbf's beta term times the drag field: the same
+  // values as the statement kernels) when it holds the same volume sums; the carried arrays are
+  // its fields meanwhile.
+  bool plain = false;
+  if (h->ph) {
+    les_scalars ps{};
+    plain = les_get_scalars(h->ph, &ps) == LES_OK && ps.area == (float)h->area &&
+            ps.pco == (float)h->pco;
   }
-  if ((r = save_state(h, fin))) return cleanup(), r;
+  void* Ac[NCARRIED];
+  if (plain) {
+    for (int q = 0; q < NCARRIED && plain; ++q) {
+      const int a = CARRIED[q];
+      int fid = -1;
+      for (int f = 0; f < LES_NFIELDS; ++f)
+        if (array_of_field(f) == a) fid = f;
+      Ac[q] = fid >= 0 ? les::field_ptr(h->ph, fid) : nullptr;
+      if (!Ac[q]) plain = false;
+    }
+  }
+  if (plain) {
+    for (int q = 0; q < NCARRIED; ++q) {
+      const int a = CARRIED[q];
+      if ((e = hipMemcpyAsync(Ac[q], h->A[a], h->n[a] * tsize(h), hipMemcpyDeviceToDevice,
+                              h->st)) != hipSuccess)
+        return hipfail(e, "window: state to the plain handle");
+    }
+    if (!h->ph_tuned) {  // the fused SOR iteration's tile for this grid
+      les_tuning t{};
+      if (les_autotune(h->ph, 5, &t) == LES_OK) h->ph_tuned = true;
+    }
+    les::set_drag(h->ph, static_cast<const float*>(h->A[ADJ_A_cd]));
+    les::set_p_consistent(h->ph, h->p_mirror);
+    void* Aown[NCARRIED];
+    for (int q = 0; q < NCARRIED; ++q) {
+      Aown[q] = h->A[CARRIED[q]];
+      h->A[CARRIED[q]] = Ac[q];
+    }
+    auto own_arrays = [&]() {
+      for (int q = 0; q < NCARRIED; ++q) h->A[CARRIED[q]] = Aown[q];
+      les::set_drag(h->ph, nullptr);
+    };
+    for (int s = 0; s < nsteps; ++s) {
+      if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return own_arrays(), cleanup(), r;
+      if ((r = les::step_external(h->ph, h->st, 2)))
+        return own_arrays(), cleanup(), afail(h, r, "window: plain step");
+      sum_probes(h, nprobes);
+    }
+    r = save_state(h, fin);
+    own_arrays();
+    if (r) return cleanup(), r;
+  } else {
+    for (int s = 0; s < nsteps; ++s) {
+      if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return cleanup(), r;
+      step_fw(h);
+      sum_probes(h, nprobes);
+    }
+    if ((r = save_state(h, fin))) return cleanup(), r;
+  }
   double jsum = 0.0;
   if ((e = hipMemcpyAsync(&jsum, h->jacc, sizeof(double), hipMemcpyDeviceToHost, h->st)) != hipSuccess ||
       (e = hipStreamSynchronize(h->st)) != hipSuccess)

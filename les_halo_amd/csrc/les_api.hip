@@ -47,6 +47,7 @@ struct les_handle {
   // second iterate of the fused SOR iteration (ping-pong with sp.p) and its workgroup tile
   float* palt[2] = {nullptr, nullptr};
   SorIterShape iter_shape{0, 0};  // tj == 0: two colour sweeps per iteration (untuned default)
+  const float* drag = nullptr;    // feedbf's drag coefficient field (Fields::cd), null: 1
   bool tuned = false;
   bool p2xy = false;  // see grid_p2xy
   bool split_ok = false;  // the colour-split sweep's index arithmetic covers this grid
@@ -174,6 +175,7 @@ Grid1D grid_of(les_handle* h) {
 
 Fields fields_of(les_handle* h) {
   Fields F;
+  F.cd = h->drag;
   F.p = h->fld[LES_P];
   F.u = h->fld[LES_U];
   F.v = h->fld[LES_V];
@@ -977,6 +979,25 @@ int enqueue_step(les_handle* h, int n) {
   return enqueue_press(h);
 }
 }  // namespace
+
+namespace les {
+float* field_ptr(les_handle* h, int field) {
+  return (h && field >= 0 && field < LES_NFIELDS) ? h->fld[field] : nullptr;
+}
+void set_drag(les_handle* h, const float* cd) {
+  h->drag = cd;
+  h->graph_valid = false;
+}
+void set_p_consistent(les_handle* h, bool consistent) { h->p_consistent = consistent; }
+int step_external(les_handle* h, hipStream_t st, int n) {
+  if (!h) return LES_E_INVALID;
+  const hipStream_t own = h->st;
+  h->st = st;
+  const int r = enqueue_step(h, n);
+  h->st = own;
+  return r;
+}
+}  // namespace les
 
 extern "C" int les_press_sor_solve(les_handle* h) {
   if (!h) return LES_E_INVALID;

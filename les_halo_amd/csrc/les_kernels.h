@@ -13,6 +13,9 @@ struct Fields {
   const uint8_t* mask;  // interior (1:ip,1:jp,1:kp): bit0 bmask1, bit1 cmask1, bit2 dmask1
   const float* cn1;     // cn1(1:ip,1:jp,1:kp)
   const float* wind;    // wind_profile(1:jp,1:kp)
+  // drag coefficient of feedbf's beta term per interior cell (i fastest), beta*cd*u*mask; the
+  // reference's term (cd = 1) when null.  Set by the adjoint form's forward only.
+  const float* cd = nullptr;
 };
 
 // Scalars kept on the device between stages (graph-capturable: no host round trip).
@@ -214,5 +217,12 @@ void launch_halo_write(hipStream_t st, float* const* arrays, int v_dim, int im, 
 int sor_solve_external(::les_handle* h, hipStream_t st, float* p, float* rhs);
 // its transpose (uniform horizontal grid): adjoints pb of p and rb of rhs
 int sor_adjoint_external(::les_handle* h, hipStream_t st, float* pb, float* rb);
+// a plain handle running the adjoint form's FP32 forward window (les_api.hip): its field
+// arrays, the drag field of its feedbf, whether p's halos are mirror images, one eager step of
+// main_ocl.f95:264-299 on stream st
+float* field_ptr(::les_handle* h, int field);
+void set_drag(::les_handle* h, const float* cd);
+void set_p_consistent(::les_handle* h, bool consistent);
+int step_external(::les_handle* h, hipStream_t st, int n);
 
 }  // namespace les

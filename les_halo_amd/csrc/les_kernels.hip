@@ -344,7 +344,8 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
     F.vsum[ifg] = vs;
     F.wsum[ifg] = ws;
     const float f1x = ph.alpha * us * ph.dt, f1y = ph.alpha * vs * ph.dt, f1z = ph.alpha * ws * ph.dt;
-    const float f2x = ph.beta * uu * bm, f2y = ph.beta * vv * cm, f2z = ph.beta * ww * dm;
+    const float bc = F.cd ? ph.beta * F.cd[(i - 1) + d.ip * ((j - 1) + d.jp * (k - 1))] : ph.beta;
+    const float f2x = bc * uu * bm, f2y = bc * vv * cm, f2z = bc * ww * dm;
     fv = fv + (f1x + f2x);
     gv = gv + (f1y + f2y);
     hv = hv + (f1z + f2z);
@@ -482,7 +483,8 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
     bvs.st(ifg, vs);
     bws.st(ifg, ws);
     const float f1x = ph.alpha * us * ph.dt, f1y = ph.alpha * vs * ph.dt, f1z = ph.alpha * ws * ph.dt;
-    const float f2x = ph.beta * uu * bm, f2y = ph.beta * vv * cm, f2z = ph.beta * ww * dm;
+    const float bc = F.cd ? ph.beta * F.cd[(i - 1) + d.ip * ((j - 1) + d.jp * (k - 1))] : ph.beta;
+    const float f2x = bc * uu * bm, f2y = bc * vv * cm, f2z = bc * ww * dm;
     fv = fv + (f1x + f2x);
     gv = gv + (f1y + f2y);
     hv = hv + (f1z + f2z);
