@@ -422,7 +422,9 @@ int les_initial_wind_profile(int jp, int kp, const float* z2, const float* dzn,
  * derivative.  One handle holds the step's arrays in FP32 (fp64 = 0) or FP64 and runs the step
  * of main_ocl.f95:264-299 for n >= 2 (EXTERNAL_WIND_PROFILE) as generated statement kernels,
  * forward (les_adj_step) and transposed in reverse order (les_adj_reverse_step: one step's
- * vector-Jacobian product, the step recomputed from the current state first).  feedbf's beta
+ * vector-Jacobian product, the step recomputed from the current state first).  An FP32 handle
+ * keeps an internal plain handle on the same grid whose kernels run the SOR (forward and
+ * transposed) and the window's first forward pass, with the same values.  feedbf's beta
  * term is beta * cd(i,j,k) * u * mask with a drag coefficient field cd (field LES_ADJ_CD, 1 =
  * the reference).  Fields go in and out as doubles in the reference's Fortran bounds;
  * les_adj_{upload,download}_adjoint address the adjoints of the carried state (u, v, w, p, f, g,
