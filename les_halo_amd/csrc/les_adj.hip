@@ -587,23 +587,53 @@ int zero_temporaries(les_adj* h) {
   return LES_OK;
 }
 
-int save_state(les_adj* h, char* dst) {
-  for (int q = 0; q < NCARRIED; ++q) {
-    const int a = CARRIED[q];
-    ACHK(h, hipMemcpyAsync(dst, h->A[a], h->n[a] * tsize(h), hipMemcpyDeviceToDevice, h->st));
-    dst += h->n[a] * tsize(h);
+// Up to COPY_MAX device arrays copied in one launch (4-byte words; blockIdx.y: the array).
+constexpr int COPY_MAX = 16;
+struct CopyList {
+  const uint32_t* src[COPY_MAX];
+  uint32_t* dst[COPY_MAX];
+  long words[COPY_MAX];
+};
+__global__ __launch_bounds__(256) void k_copy_arrays(CopyList L) {
+  const int a = blockIdx.y;
+  const uint32_t* __restrict__ s = L.src[a];
+  uint32_t* __restrict__ d = L.dst[a];
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < L.words[a]; t += (long)gridDim.x * 256)
+    d[t] = s[t];
+}
+
+// Copies n arrays (bytes[q] each, multiples of 4) on the handle's stream, COPY_MAX per launch.
+int copy_arrays(les_adj* h, int n, void* const* dst, const void* const* src, const size_t* bytes) {
+  for (int q0 = 0; q0 < n; q0 += COPY_MAX) {
+    CopyList L{};
+    const int m = std::min(COPY_MAX, n - q0);
+    for (int q = 0; q < m; ++q) {
+      L.src[q] = static_cast<const uint32_t*>(src[q0 + q]);
+      L.dst[q] = static_cast<uint32_t*>(dst[q0 + q]);
+      L.words[q] = (long)(bytes[q0 + q] / 4);
+    }
+    hipLaunchKernelGGL(k_copy_arrays, dim3(1024, m), dim3(256), 0, h->st, L);
   }
+  ACHK(h, hipGetLastError());
   return LES_OK;
 }
 
-int load_state(les_adj* h, const char* src) {
+// the carried state to / from a packed buffer
+int move_state(les_adj* h, char* buf, bool save) {
+  void* dst[NCARRIED];
+  const void* src[NCARRIED];
+  size_t bytes[NCARRIED];
   for (int q = 0; q < NCARRIED; ++q) {
     const int a = CARRIED[q];
-    ACHK(h, hipMemcpyAsync(h->A[a], src, h->n[a] * tsize(h), hipMemcpyDeviceToDevice, h->st));
-    src += h->n[a] * tsize(h);
+    bytes[q] = h->n[a] * tsize(h);
+    dst[q] = save ? (void*)buf : h->A[a];
+    src[q] = save ? (const void*)h->A[a] : (const void*)buf;
+    buf += bytes[q];
   }
-  return LES_OK;
+  return copy_arrays(h, NCARRIED, dst, src, bytes);
 }
+int save_state(les_adj* h, char* dst) { return move_state(h, dst, true); }
+int load_state(les_adj* h, const char* src) { return move_state(h, const_cast<char*>(src), false); }
 
 size_t state_bytes(const les_adj* h) {
   size_t b = 0;
@@ -878,11 +908,16 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
     }
   }
   if (plain) {
-    for (int q = 0; q < NCARRIED; ++q) {
-      const int a = CARRIED[q];
-      if ((e = hipMemcpyAsync(Ac[q], h->A[a], h->n[a] * tsize(h), hipMemcpyDeviceToDevice,
-                              h->st)) != hipSuccess)
-        return hipfail(e, "window: state to the plain handle");
+    {
+      void* dst[NCARRIED];
+      const void* src[NCARRIED];
+      size_t bytes[NCARRIED];
+      for (int q = 0; q < NCARRIED; ++q) {
+        dst[q] = Ac[q];
+        src[q] = h->A[CARRIED[q]];
+        bytes[q] = h->n[CARRIED[q]] * tsize(h);
+      }
+      if ((r = copy_arrays(h, NCARRIED, dst, src, bytes))) return cleanup(), r;
     }
     if (!h->ph_tuned) {  // the fused SOR iteration's tile for this grid
       les_tuning t{};
@@ -969,12 +1004,19 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   };
   if (slots) {
     // elements a step reads but never writes keep the values of the handle's arrays
-    for (int t = 0; t < nseg; ++t)
+    for (int t = 0; t < nseg; ++t) {
+      void* dst[ADJ_NARRAYS];
+      const void* src[ADJ_NARRAYS];
+      size_t bytes[ADJ_NARRAYS];
+      int m = 0;
       for (int q = 0; q < ADJ_NARRAYS; ++q)
-        if (ADJ_WRITTEN[q] &&
-            (e = hipMemcpyAsync(slot(t, q), A0[q], h->n[q] * tsize(h), hipMemcpyDeviceToDevice,
-                                h->st)) != hipSuccess)
-          return unslot(), hipfail(e, "window: step slots");
+        if (ADJ_WRITTEN[q]) {
+          dst[m] = slot(t, q);
+          src[m] = A0[q];
+          bytes[m++] = h->n[q] * tsize(h);
+        }
+      if ((r = copy_arrays(h, m, dst, src, bytes))) return fail_slots(r);
+    }
     for (int c = nck - 1; c >= 0; --c) {
       const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
       for (int s = s0; s < s1; ++s) {
@@ -982,12 +1024,16 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
         if (s == s0) {
           if ((r = load_state(h, ck + sb * c))) return fail_slots(r);
         } else {
+          void* dst[NCARRIED];
+          const void* src[NCARRIED];
+          size_t bytes[NCARRIED];
           for (int q = 0; q < NCARRIED; ++q) {
             const int a = CARRIED[q];
-            if ((e = hipMemcpyAsync(slot(s - s0, a), slot(s - s0 - 1, a), h->n[a] * tsize(h),
-                                    hipMemcpyDeviceToDevice, h->st)) != hipSuccess)
-              return unslot(), hipfail(e, "window: carried state");
+            dst[q] = slot(s - s0, a);
+            src[q] = slot(s - s0 - 1, a);
+            bytes[q] = h->n[a] * tsize(h);
           }
+          if ((r = copy_arrays(h, NCARRIED, dst, src, bytes))) return fail_slots(r);
         }
         step_fw(h);
       }
