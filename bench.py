@@ -8,12 +8,16 @@ A step is one pass of the whole time step (velnw .. press) over the domain, repl
 captured HIP graph with every input already resident in HBM.
 
 Prints one JSON line: value = M cell-steps/s of the whole job; roofline = the dominant kernel
-(one SOR colour sweep) timed live with HIP events on the library's stream, against the HBM
-peak; cpu_baseline = the reference Fortran time loop (built from the reference sources by
-oracle/ref/build_ref.sh, serial) on a bounded sample of the same workload, or the FP32 CPU
-oracle when the reference build is absent.
+(the fused SOR iteration, or one colour sweep where the grid does not allow it) timed live with
+HIP events on the library's stream, against the HBM peak, priced at the algorithmic bytes of
+one SOR iteration at the stage boundaries of press.f95 (two colour sweeps, 16 B per cell; the
+fused kernel's own compulsory 12 B per cell are reported beside it); ensemble = the batched
+leg: the 64 uneven members of the ensemble configuration advanced together, against the same
+members run one after another; cpu_baseline = the reference Fortran time loop (built from the
+reference sources by oracle/ref/build_ref.sh with the benchmark's SOR settings, serial) on a
+bounded sample of the same workload, or the FP32 CPU oracle when the reference build is absent.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--ensemble 64]
 """
 import argparse
 import json
@@ -27,7 +31,8 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 B_STEP = 1057                  # algorithmic FP32 bytes per interior cell-step
 SOR_SWEEP_B_PER_CELL = 8       # one colour sweep: r p 4 + r rhs 2 + w p 2
-SOR_ITER_B_PER_CELL = 12       # one fused iteration (both colours): r p 4 + r rhs 4 + w p 4
+SOR_ITER_B_PER_CELL = 16       # one SOR iteration at press.f95's stage boundaries: two sweeps
+SOR_ITER_FUSED_B_PER_CELL = 12  # the fused iteration's compulsory bytes: r p 4 + r rhs 4 + w p 4
 METRIC = "M cell-steps/s (50 SOR iters/step) + % HBM peak, 1 GPU; batched vs plain"
 GRID = (300, 300, 90)
 PHYS = dict(dt=0.05, ro=1.1763, vn=1.583e-5, alpha=-10.0, beta=-1.0, cs0=0.1, omega=1.5,
@@ -47,17 +52,34 @@ def traffic_from_profiles(kernel):
         return None
 
 
+def host_cpu():
+    """The host's CPU model and the number of CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(case, nsteps):
-    """The reference's serial time loop on nsteps steps of the same workload."""
+    """The reference's serial time loop on nsteps steps of the same workload (the build with
+    the benchmark's SOR settings: omega 1.5, exactly 50 iterations)."""
     from oracle.ref import run_ref
     ip, jp, kp = case["dims"]
-    exe = os.path.join(HERE, "oracle", "_ref", run_ref.tag_of(ip, jp, kp), "ref_harness")
+    exe = os.path.join(HERE, "oracle", "_ref", run_ref.tag_of(ip, jp, kp, cfg=True), "ref_harness")
     phys = {k: PHYS[k] for k in ("dt", "ro", "vn", "alpha", "beta")}
+    model, ncpu = host_cpu()
     if os.path.exists(exe):
         secs = run_ref.timed_steps(exe, case, phys, nsteps)
         kind = "reference"
-        sample = ("reference Fortran time loop (flang -O2, omega 1, early exit at 1e-4 as "
-                  "hard-coded in press.f95), %d steps after init at %dx%dx%d" % (nsteps, ip, jp, kp))
+        sample = ("reference Fortran time loop (flang -O2, press.f95 with omega 1.5 and no early "
+                  "exit: 50 SOR iterations per step), %d steps after the init sequence at "
+                  "%dx%dx%d, serial" % (nsteps, ip, jp, kp))
     else:
         from oracle.oracle import OracleState
         o = OracleState(ip, jp, kp, **PHYS)
@@ -70,8 +92,9 @@ def cpu_baseline(case, nsteps):
         secs = time.perf_counter() - t0
         kind = "port"
         sample = "FP32 C++ oracle, %d steps at %dx%dx%d" % (nsteps, ip, jp, kp)
-    return {"value": ip * jp * kp * nsteps / secs / 1e6, "unit": "M cell-steps/s", "cores": 1,
-            "kind": kind, "sample": sample, "seconds": round(secs, 3)}
+    return {"value": round(ip * jp * kp * nsteps / secs / 1e6, 3), "unit": "M cell-steps/s",
+            "cores": 1, "kind": kind, "sample": sample, "seconds": round(secs, 3),
+            "host_cpu": model, "host_nproc": ncpu}
 
 
 CALIB_COPY_BYTES = 256 << 20  # calibration copy: 256 MiB, far larger than the 32 MiB of L2
@@ -270,7 +293,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--sor-tile", type=int, nargs=2, default=None, metavar=("TJ", "TK"),
+                    help="set the fused SOR tile by hand instead of autotuning (profiling runs: "
+                         "then every launch of the kernel is a step's or the live timing's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", type=int, default=0, metavar="STEPS",
                     help="also run the mixed-precision 600x600x90 configuration for STEPS steps")
@@ -278,8 +304,8 @@ def main():
                     help="mixed precision corrections: 0 SOR sweeps, 1 preconditioned CG")
     ap.add_argument("--refine-omega", type=float, default=None,
                     help="relaxation of the mixed-precision correction solves")
-    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
-                    help="also run the batched-ensemble configuration with N members")
+    ap.add_argument("--ensemble", type=int, default=64, metavar="N",
+                    help="the batched leg: the ensemble configuration with N members (0: skip)")
     ap.add_argument("--tiled", type=int, default=0, metavar="STEPS",
                     help="also run the 2x2-tiles configuration for STEPS steps")
     ap.add_argument("--adjoint", type=int, default=0, metavar="STEPS",
@@ -300,7 +326,12 @@ def main():
     L = LES(ip, jp, kp, **PHYS)
     L.load_case(case)
     L.ifdata()
-    tuning = L.autotune(20)  # picks the fused SOR iteration's tile; outside the timed region
+    if args.sor_tile:
+        L.set_sor_tile(*args.sor_tile)
+        tuning = {"sor_tile_j": args.sor_tile[0], "sor_tile_k": args.sor_tile[1],
+                  "sor_iter_ms": None}
+    else:
+        tuning = L.autotune(20)  # picks the fused SOR iteration's tile; outside the timed region
     L.run(1, args.warmup)
     L.sync()
     if args.profile_sor:
@@ -318,9 +349,11 @@ def main():
 
     # dominant kernel (the fused SOR iteration when the grid allows it, else the colour
     # sweep), timed live with HIP events on the library's stream
+    compulsory = None
     if tuning["sor_tile_j"] > 0:
         kern_ms = L.time_kernel("sor_iter", 100)
         bytes_launch = SOR_ITER_B_PER_CELL * cells
+        compulsory = SOR_ITER_FUSED_B_PER_CELL * cells
         # the kernel's name as rocprofv3 reports it (tile rows, workgroup size)
         tj = tuning["sor_tile_j"]
         kname = "k_sor_iter<%d, %d>" % (tj, 256 if tj == 4 else 512)
@@ -334,7 +367,15 @@ def main():
                 "traffic": traffic_from_profiles(kname), "kernel": kname,
                 "bytes_per_launch": bytes_launch, "avg_us": round(kern_ms * 1e3, 3),
                 "tile": [tuning["sor_tile_j"], tuning["sor_tile_k"]],
-                "tuned_iter_us": round(tuning["sor_iter_ms"] * 1e3, 3)}
+                "tuned_iter_us": (round(tuning["sor_iter_ms"] * 1e3, 3)
+                                  if tuning["sor_iter_ms"] is not None else None)}
+    if compulsory:
+        # the fused kernel's own compulsory traffic (both colours of p and rhs read once, both
+        # colours of p written once), priced the same way
+        ach_c = compulsory / (kern_ms * 1e-3) / 1e9
+        roofline["compulsory_bytes_per_launch"] = compulsory
+        roofline["compulsory_achieved"] = round(ach_c, 1)
+        roofline["compulsory_frac"] = round(ach_c / HBM_PEAK_GBS, 4)
     step_hbm_frac = B_STEP * cells / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "M cell-steps/s", "n_gpus": 1,

@@ -106,7 +106,8 @@ typedef struct les_config {
   int nmaxp;        /* SOR iterations per step (press.f95:45) */
   int precision;    /* LES_PREC_FP32: reference arithmetic; LES_PREC_MIXED: FP64 reductions and
                        outer refinement of the pressure solve */
-  double refine_tol;  /* LES_PREC_MIXED: target L2 norm of the pressure residual */
+  double refine_tol;  /* LES_PREC_MIXED: target relative L2 norm of the pressure residual,
+                         |rhs - L p|_2 / |rhs|_2 (FP64) */
   int refine_max;     /* LES_PREC_MIXED: maximum outer refinement passes */
   int device;         /* HIP device ordinal */
   int sub_i, sub_j;   /* subdomains along i and j on the one device (1,1 = single domain) */
@@ -303,7 +304,8 @@ typedef struct les_scalars {
   float pco;       /* press.f95:129 */
   float sor;       /* press.f95:97 residual of the last SOR iteration (when computed) */
   int sor_iters;   /* SOR iterations run in the last solve */
-  double resid_l2; /* LES_PREC_MIXED: final L2 norm of the pressure residual */
+  double resid_l2; /* LES_PREC_MIXED: final relative L2 norm of the pressure residual,
+                      |rhs - L p|_2 / |rhs|_2 */
   int refine_passes;
   int cg_iters;    /* LES_PREC_MIXED, refine_solver 1: conjugate-gradient iterations of the last
                       solve, all passes */

@@ -190,10 +190,13 @@ def _torch_function():
             adj.upload("wind_profile", wind.detach().cpu().double().numpy())
             adj.upload("cd", cd.detach().cpu().double().numpy())
             J = adj.window(nsteps, ckpt, probes)
-            gw = torch.from_numpy(adj.gradient("wind_profile")).to(wind.dtype).reshape(wind.shape)
-            gc = torch.from_numpy(adj.gradient("cd")).to(cd.dtype).reshape(cd.shape)
+            # gradients and J live where the inputs live (host or device tensors)
+            gw = torch.from_numpy(adj.gradient("wind_profile")).to(
+                device=wind.device, dtype=wind.dtype).reshape(wind.shape)
+            gc = torch.from_numpy(adj.gradient("cd")).to(
+                device=cd.device, dtype=cd.dtype).reshape(cd.shape)
             ctx.save_for_backward(gw, gc)
-            return torch.tensor(J, dtype=wind.dtype)
+            return torch.tensor(J, dtype=wind.dtype, device=wind.device)
 
         @staticmethod
         def backward(ctx, gJ):

@@ -1432,7 +1432,8 @@ extern "C" int les_initialise_LES_kernel(
     const float* sm, const float* dxs, const float* dys, const float* dzs, const float* dx1,
     const float* dy1, const float* dzn, const float* wind_profile, float dt, int im, int jm,
     int km) {
-  if (g_default) les_destroy(g_default);
+  if (g_default) les_destroy(g_default);  // les_destroy clears g_default
+  g_default = nullptr;
   les_config cfg;
   les_default_config(&cfg, im, jm, km);
   cfg.dt = dt;

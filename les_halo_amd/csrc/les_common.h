@@ -58,16 +58,20 @@ struct Dims {
 // Raw-buffer view of an FP32 device array of n elements: loads and stores address it with a
 // 32-bit byte offset from a descriptor held in scalar registers, so an access costs one VALU
 // add for its row instead of a 64-bit address computation.  The hardware range check makes
-// an access outside [0, n) read 0 or be dropped (arrays are below 2^30 elements).
+// an access outside [0, n) read 0 or be dropped.  Byte counts and offsets are formed in
+// unsigned 32-bit arithmetic: les_create admits arrays below 2^30 elements, so n * 4 and every
+// in-range offset fit, and an index below 0 wraps to an offset beyond the range.
 struct Buf {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ Buf(const float* p, int n)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, n * 4, 0x00020000)) {}
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0,
+                                            (int)((unsigned)n * 4u), 0x00020000)) {}
+  __device__ __forceinline__ static int off(long i) { return (int)((unsigned)i * 4u); }
   __device__ __forceinline__ float operator[](long i) const {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)i * 4, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off(i), 0, 0));
   }
   __device__ __forceinline__ void st(long i, float v) const {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)i * 4, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off(i), 0, 0);
   }
 };
 

@@ -18,22 +18,32 @@ DUMP_ORDER = (["p", "u", "v", "w", "usum", "vsum", "wsum", "f", "g", "h", "fold"
               + ["cn1", "cn2l", "cn2s", "cn3l", "cn3s", "cn4l", "cn4s", "delx1"])
 
 
-def tag_of(ip, jp, kp):
+def tag_of(ip, jp, kp, cfg=False):
+    """Build tag of a grid; cfg selects the build with the benchmark's SOR parameters
+    (omega 1.5, no early exit, 50 iterations) in place of press.f95:44-46's."""
     if (ip, jp, kp) == (150, 150, 90):
-        return "ref150"
-    if (ip, jp, kp) == (300, 300, 90):
-        return "ref300"
-    return "%dx%dx%d" % (ip, jp, kp)
+        tag = "ref150"
+    elif (ip, jp, kp) == (300, 300, 90):
+        tag = "ref300"
+    else:
+        tag = "%dx%dx%d" % (ip, jp, kp)
+    return tag + ("cfg" if cfg else "")
 
 
 def available():
     return os.path.isdir(REF_SRC)
 
 
-def binary(ip, jp, kp):
-    tag = tag_of(ip, jp, kp)
+def binary(ip, jp, kp, cfg=False):
+    """The reference build for this grid; built on demand where the reference sources exist
+    (this container), else whatever was built before (the GPU box gets the built files)."""
+    tag = tag_of(ip, jp, kp, cfg)
     exe = os.path.join(REF_OUT, tag, "ref_harness")
     recipe = [os.path.join(HERE, "ref_harness.f95"), os.path.join(HERE, "build_ref.sh")]
+    if not available():
+        if not os.path.exists(exe):
+            raise FileNotFoundError("reference build %s absent and no reference sources" % exe)
+        return exe
     if not os.path.exists(exe) or max(map(os.path.getmtime, recipe)) > os.path.getmtime(exe):
         subprocess.run([os.path.join(HERE, "build_ref.sh"), tag], check=True,
                        stdout=subprocess.DEVNULL)
@@ -95,11 +105,16 @@ def timed_steps(exe, case, phys, nsteps):
     raise RuntimeError("no timing in reference output: %r" % out[-500:])
 
 
-def trajectory(case, phys, nsteps, dump_steps):
-    """Mode 0: init + ifdata sequence + nsteps steps; returns one state dict per dump step."""
+def trajectory(case, phys, nsteps, dump_steps, cfg=False, names=None):
+    """Mode 0: init + ifdata sequence + nsteps steps; returns one state dict per dump step
+    (only the arrays in names, when given).  cfg: the build with the benchmark's SOR
+    parameters."""
     ip, jp, kp = case["dims"]
     payload = _header(0, ip, jp, kp, phys, case) + _i32(nsteps, len(dump_steps)) + _i32(*dump_steps)
-    return _parse(_run(binary(ip, jp, kp), payload), ip, jp, kp, len(dump_steps))
+    out = _parse(_run(binary(ip, jp, kp, cfg), payload), ip, jp, kp, len(dump_steps))
+    if names is not None:
+        out = [{n: d[n] for n in names} for d in out]
+    return out
 
 
 AVE3 = ["aveu", "avev", "avew", "avep", "avel", "aveuu", "avevv", "aveww", "avesm", "avesmsm",

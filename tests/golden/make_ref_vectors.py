@@ -7,6 +7,12 @@ sampled values and the terrain of the reference's GIS file), so that the tests c
 oracle bit for bit without the reference present.
 
     python tests/golden/make_ref_vectors.py [--big]
+    python tests/golden/make_ref_vectors.py --cfg   (only the benchmark-SOR entries, *_cfg)
+
+The --cfg entries come from the reference built with the benchmark's SOR parameters (omega 1.5,
+no early exit, 50 iterations; build_ref.sh <tag>cfg edits only press.f95:44-46): a 16x12x10
+trajectory, the odd-jp grid, 150x150x90 after 1, 5 and 10 steps, and u, v, w, p of 300x300x90
+after 10 and 100 steps (the grid of the benchmark, the comparison the GPU tests repeat).
 """
 import argparse
 import json
@@ -47,10 +53,58 @@ STAGE_CASES = [("velnw", 0), ("bondv1", 1), ("bondv1", 3), ("velfg", 0), ("feedb
                ("bondfg", 0), ("boundsm", 0), ("step", 1), ("step", 4)]
 
 
+def cfg_vectors(out):
+    """The *_cfg entries (reference built with omega 1.5, pjuge 0, nmaxp 50)."""
+    phys = dict(common.REF_PHYS)
+    case = common.tiny_case()
+    dumps = [0, 1, 2, 5, 12]
+    traj = run_ref.trajectory(case, phys, 12, dumps, cfg=True)
+    out["tiny_trajectory_cfg"] = {"dims": list(case["dims"]), "seed": 7, "hmax": 12.0,
+                                  "coverage": 0.3, "sor": common.CFG_SOR,
+                                  "dumps": {str(n): digest(s) for n, s in zip(dumps, traj)}}
+    print("tiny cfg trajectory done", flush=True)
+    case_odd = common.tiny_case(14, 11, 8, seed=9)
+    traj = run_ref.trajectory(case_odd, phys, 3, [0, 3], cfg=True)
+    out["odd_trajectory_cfg"] = {"dims": list(case_odd["dims"]), "seed": 9, "hmax": 12.0,
+                                 "coverage": 0.3, "sor": common.CFG_SOR,
+                                 "dumps": {str(n): digest(s) for n, s in zip([0, 3], traj)}}
+    print("odd cfg trajectory done", flush=True)
+    keep = ("u", "v", "w", "p", "sm", "rhs", "usum", "f")
+    ip, jp, kp = 150, 150, 90
+    case = inputs.make_case(ip, jp, kp)
+    dumps = [1, 5, 10]
+    traj = run_ref.trajectory(case, phys, 10, dumps, cfg=True, names=keep)
+    out["ref150_trajectory_cfg"] = {
+        "dims": [ip, jp, kp], "seed": 0x4C455331, "sor": common.CFG_SOR,
+        "dumps": {str(n): {k: [common.sha(s[k]), float(np.linalg.norm(s[k].astype(np.float64)))]
+                           for k in keep} for n, s in zip(dumps, traj)}}
+    print("150 cfg trajectory done", flush=True)
+    ip, jp, kp = 300, 300, 90
+    case = inputs.make_case(ip, jp, kp)
+    dumps = [10, 100]
+    keep = ("u", "v", "w", "p")
+    traj = run_ref.trajectory(case, phys, 100, dumps, cfg=True, names=keep)
+    out["ref300_trajectory_cfg"] = {
+        "dims": [ip, jp, kp], "seed": 0x4C455331, "sor": common.CFG_SOR,
+        "dumps": {str(n): {k: [common.sha(s[k]), float(np.linalg.norm(s[k].astype(np.float64)))]
+                           for k in keep} for n, s in zip(dumps, traj)}}
+    print("300 cfg trajectory done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also the 150x150x90 reference trajectory")
+    ap.add_argument("--cfg", action="store_true",
+                    help="only (re)write the entries of the benchmark-SOR reference build")
     args = ap.parse_args()
+    if args.cfg:
+        path = os.path.join(HERE, "ref_vectors.json")
+        out = json.load(open(path))
+        cfg_vectors(out)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        print("wrote", path)
+        return
     phys = dict(common.REF_PHYS)
     out = {"generator": "tests/golden/make_ref_vectors.py",
            "reference_build": "oracle/ref/build_ref.sh (flang -O2 -ffp-contract=off)",

@@ -42,6 +42,39 @@ def test_trajectory_bitwise(key):
             _check(o, dumps[n], "%s after step %d" % (key, n))
 
 
+@pytest.mark.parametrize("key", ["tiny_trajectory_cfg", "odd_trajectory_cfg"])
+def test_trajectory_bitwise_benchmark_sor(key):
+    """The benchmark's SOR settings (omega 1.5, no early exit, 50 iterations): the oracle in
+    sequential-sum mode against the reference built with press.f95:44-46 set to those values."""
+    entry = VEC[key]
+    case = _case_of(entry)
+    o = common.oracle_from_case(case, **common.REF_PHYS, **common.CFG_SOR, sum_mode=SUM_SEQ)
+    o.ifdata()
+    dumps = {int(k): v for k, v in entry["dumps"].items()}
+    if 0 in dumps:
+        _check(o, dumps[0], "%s after init" % key)
+    for n in range(1, max(dumps) + 1):
+        o.step(n)
+        if n in dumps:
+            _check(o, dumps[n], "%s after step %d" % (key, n))
+
+
+@pytest.mark.slow
+def test_reference_grid_150_trajectory_benchmark_sor():
+    """150x150x90 (the CPU configuration's grid, terrain and inflow) with the benchmark's SOR
+    settings: the sequential-sum oracle equals the reference bit for bit after 1, 5, 10 steps."""
+    entry = VEC["ref150_trajectory_cfg"]
+    ip, jp, kp = entry["dims"]
+    case = inputs.make_case(ip, jp, kp, seed=entry["seed"])
+    o = common.oracle_from_case(case, **common.REF_PHYS, **common.CFG_SOR, sum_mode=SUM_SEQ)
+    o.ifdata()
+    dumps = {int(k): v for k, v in entry["dumps"].items()}
+    for n in range(1, max(dumps) + 1):
+        o.step(n)
+        if n in dumps:
+            _check(o, dumps[n], "150x150x90 (omega 1.5) after step %d" % n)
+
+
 STAGE_ORACLE = {"velnw": "velnw", "bondv1": "bondv1", "velfg": "velfg", "feedbf": "feedbf",
                 "les": "les", "adam": "adam", "press": "press", "boundp1": "boundp1",
                 "boundp2": "boundp2", "bondfg": "bondfg", "boundsm": "boundsm", "step": "step"}

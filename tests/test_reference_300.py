@@ -1,40 +1,116 @@
 """The HIP path against the reference's own implementation at 300x300x90 (the benchmark grid and
-inputs), after 10 steps: the reference Fortran (oracle/_ref/ref300, built from the reference
-sources) and the GPU start from the same init sequence; the reference's SOR settings (omega 1,
-early exit at 1e-4, press.f95:44-46) apply to both.
+inputs), after 10 and 100 steps, as one domain and as 2 x 2 tiles.  The reference is the
+Fortran built from the reference sources (oracle/_ref/ref300*), run on the same seeded inputs
+through the same init sequence (ifdata.f95:258-273); its runs start in the background when
+these tests are collected (conftest.py).
 
-The only arithmetic difference is the three sums (rhsav, pav, the SOR residual): the reference
-accumulates them sequentially in FP32, the GPU rounds their exact value once.  A residual near
-the early-exit threshold can then stop the SOR one iteration apart, which moves w and p by a
-few 1e-5 (relative L2); u and v stay within 1e-6.  The FP32 reference itself differs from an
-FP64 run of the same code by 2.5e-5 in w and 6.2e-5 in p at this point.
+Benchmark settings (omega 1.5, exactly 50 SOR iterations, no early exit): the reference build
+ref300cfg differs from the reference only in press.f95:44-46's two values.  The only arithmetic
+difference left between the GPU and the reference is the rounding of two sums, rhsav and pav:
+the reference accumulates them sequentially in FP32 (press.f95:65-75,123-134), the GPU rounds
+their exact value once.  Bars: relative L2 <= 1e-5 after 10 steps, <= 1e-3 after 100 steps,
+for u, v, w and p.
+
+The reference's own settings (omega 1, early exit once the FP32 residual sum is below 1e-4,
+press.f95:44-46,114): the residual's sum order can move the exit by one iteration, which moves
+w and p by 1.6e-5 and 2.3e-5 after 10 steps (u, v stay below 1e-6).  That is above the 1e-5
+bar: the assertion stays at 1e-5 and the test is an expected failure; after 100 steps every
+field is within 1e-3.
 """
+import json
 import os
 
-import numpy as np
 import pytest
 
 import common
-from les_halo_amd import inputs
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-EXE = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "ref300", "ref_harness")
+VEC = json.load(open(os.path.join(HERE, "golden", "ref_vectors.json")))
+FIELDS = ("u", "v", "w", "p")
+BARS = {10: 1e-5, 100: 1e-3}
+_GPU = {}
 
 
-@pytest.mark.skipif(not os.path.exists(EXE), reason="reference build oracle/_ref/ref300 absent")
-def test_gpu_vs_reference_300_after_10_steps():
-    from les_halo_amd import LES
-    from oracle.ref import run_ref
+def _gpu_run(tiled, sor):
+    """u, v, w, p after 10 and 100 steps on the GPU (cached per configuration)."""
+    key = (tiled, tuple(sorted(sor.items())))
+    if key in _GPU:
+        return _GPU[key]
+    from les_halo_amd import LES, TiledLES, inputs
     ip, jp, kp = 300, 300, 90
     case = inputs.make_case(ip, jp, kp)
-    ref = run_ref.trajectory(case, common.REF_PHYS, 10, [10])[0]
-    L = LES(ip, jp, kp, **common.REF_PHYS, **common.REF_SOR)
-    L.load_case(case)
-    L.ifdata()
-    L.run(1, 10)
-    err = {n: common.rel_l2(L.download(n), ref[n]) for n in ("u", "v", "w", "p")}
-    print("rel L2 GPU vs reference after 10 steps:", err)
-    assert err["u"] < 1e-6 and err["v"] < 1e-5, err
-    assert err["w"] < 5e-5 and err["p"] < 5e-5, err
+    if tiled:
+        X = TiledLES(ip, jp, kp, sub_i=2, sub_j=2, **common.REF_PHYS, omega=sor["omega"],
+                     nmaxp=sor["nmaxp"])
+    else:
+        X = LES(ip, jp, kp, **common.REF_PHYS, **sor)
+    X.load_case(case)
+    X.ifdata()
+    out = {}
+    X.run(1, 10)
+    out[10] = {n: X.download(n) for n in FIELDS}
+    X.run(11, 90)
+    out[100] = {n: X.download(n) for n in FIELDS}
+    X.close()
+    _GPU[key] = out
+    return out
+
+
+def _errors(gpu, ref):
+    return {n: common.rel_l2(gpu[n], ref[n]) for n in FIELDS}
+
+
+def _reference_is_pinned(ref):
+    """The reference run on this machine reproduced the committed digests of the same build
+    (tests/golden/ref_vectors.json, written where the reference sources are)."""
+    ent = VEC["ref300_trajectory_cfg"]["dumps"]
+    for idx, n in enumerate((10, 100)):
+        for f in FIELDS:
+            assert common.sha(ref[idx][f]) == ent[str(n)][f][0], (n, f)
+
+
+@pytest.mark.ref300(cfg=True)
+@pytest.mark.timeout(1200)
+@pytest.mark.parametrize("tiled", [False, True], ids=["single", "tiles2x2"])
+def test_vs_reference_benchmark_sor(ref300, tiled):
+    ref = ref300(True)
+    _reference_is_pinned(ref)
+    gpu = _gpu_run(tiled, common.CFG_SOR)
+    for idx, n in enumerate((10, 100)):
+        err = _errors(gpu[n], ref[idx])
+        print("%s, %d steps: rel L2 vs reference %s" % ("2x2 tiles" if tiled else "single domain",
+                                                         n, err))
+        assert all(e <= BARS[n] for e in err.values()), (n, err)
+
+
+@pytest.mark.ref300(cfg=False)
+@pytest.mark.timeout(1200)
+def test_vs_reference_early_exit_100_steps(ref300):
+    """press.f95's own settings, 100 steps: every field within 1e-3."""
+    ref = ref300(False)
+    err = _errors(_gpu_run(False, common.REF_SOR)[100], ref[1])
+    print("reference settings, 100 steps: rel L2 vs reference", err)
+    assert all(e <= 1e-3 for e in err.values()), err
+
+
+@pytest.mark.ref300(cfg=False)
+@pytest.mark.timeout(1200)
+def test_vs_reference_early_exit_10_steps_uv(ref300):
+    """press.f95's own settings, 10 steps: u and v within 1e-5 (in fact 1e-6)."""
+    ref = ref300(False)
+    err = _errors(_gpu_run(False, common.REF_SOR)[10], ref[0])
+    print("reference settings, 10 steps: rel L2 vs reference", err)
+    assert err["u"] <= 1e-6 and err["v"] <= 1e-5, err
+
+
+@pytest.mark.ref300(cfg=False)
+@pytest.mark.timeout(1200)
+@pytest.mark.xfail(strict=True, reason="with the early exit at 1e-4 the residual's sum order "
+                   "moves the exit by one iteration: w and p differ by 1.6e-5 and 2.3e-5")
+def test_vs_reference_early_exit_10_steps_wp(ref300):
+    ref = ref300(False)
+    err = _errors(_gpu_run(False, common.REF_SOR)[10], ref[0])
+    print("reference settings, 10 steps: rel L2 vs reference", err)
+    assert err["w"] <= 1e-5 and err["p"] <= 1e-5, err
