@@ -92,6 +92,12 @@ int64_t les_field_bounds(int field, int ip, int jp, int kp, int lo[3], int hi[3]
 
 /* ---- configuration ------------------------------------------------------------------- */
 enum { LES_PREC_FP32 = 0, LES_PREC_MIXED = 1 };
+/* Order of press.f95's sums (rhsav, pav, the SOR residual).  LES_SUM_EXACT: exact and
+ * order-independent, rounded once to FP32 (the default; at least as accurate as any order).
+ * LES_SUM_REFERENCE: the reference's own sequential FP32 accumulation in its loop order
+ * (press.f95:65-75,89-97,123-134), which makes the results bit-identical to the reference's
+ * CPU code; a parity mode, far slower (each sum is one chain of N dependent additions). */
+enum { LES_SUM_EXACT = 0, LES_SUM_REFERENCE = 1 };
 
 typedef struct les_config {
   int ip, jp, kp;   /* interior cells (params_common_sn.f95:3) */
@@ -116,6 +122,7 @@ typedef struct les_config {
   int refine_solver;  /* LES_PREC_MIXED: 0 = each correction by nmaxp SOR iterations from zero;
                          1 = by nmaxp conjugate-gradient iterations preconditioned by symmetric
                          red-black Gauss-Seidel (the FP32 colour sweeps), FP64 inner products */
+  int sum_order;      /* LES_SUM_EXACT or LES_SUM_REFERENCE (FP32 single-domain handles only) */
 } les_config;
 
 /* Fills the reference's defaults (set.f95, les.f95, press.f95) for an ip x jp x kp grid. */

@@ -13,6 +13,7 @@ P_float = ctypes.POINTER(ctypes.c_float)
 
 LES_OK, LES_E_INVALID, LES_E_HIP, LES_E_NOMEM, LES_E_STATE, LES_E_NODEV = range(6)
 LES_PREC_FP32, LES_PREC_MIXED = 0, 1
+LES_SUM_EXACT, LES_SUM_REFERENCE = 0, 1
 ERR_NAMES = {1: "LES_E_INVALID", 2: "LES_E_HIP", 3: "LES_E_NOMEM", 4: "LES_E_STATE",
              5: "LES_E_NODEV"}
 
@@ -23,7 +24,7 @@ class LesConfig(ctypes.Structure):
                 ("omega", c_float), ("pjuge", c_float), ("nmaxp", c_int), ("precision", c_int),
                 ("refine_tol", c_double), ("refine_max", c_int), ("device", c_int),
                 ("sub_i", c_int), ("sub_j", c_int), ("refine_omega", c_float),
-                ("refine_solver", c_int)]
+                ("refine_solver", c_int), ("sum_order", c_int)]
 
 
 class LesScalars(ctypes.Structure):

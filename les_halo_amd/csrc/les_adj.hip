@@ -685,6 +685,7 @@ void sum_probes(les_adj* h, int np) {
 
 extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
   if (!cfg || !out || cfg->ip < 2 || cfg->jp < 2 || cfg->kp < 2 || cfg->nmaxp < 0) return LES_E_INVALID;
+  if (cfg->sum_order != LES_SUM_EXACT) return LES_E_INVALID;  // the transposes assume exact sums
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_E_NODEV;

@@ -54,6 +54,9 @@ struct les_handle {
   // aveflow statistics, allocated by the first les_aveflow call
   float* ave_mem = nullptr;
   AveStats ave{};
+  // LES_SUM_REFERENCE: per-cell residual terms of one SOR iteration (both colours)
+  float* seq_mem = nullptr;
+  SeqTerms seq_terms{};
 };
 
 namespace {
@@ -363,6 +366,8 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     return LES_E_INVALID;
   if (cfg->sub_i != 1 || cfg->sub_j != 1) return LES_E_INVALID;  // one domain per handle
   if (cfg->precision != LES_PREC_FP32 && cfg->precision != LES_PREC_MIXED) return LES_E_INVALID;
+  if (cfg->sum_order != LES_SUM_EXACT && cfg->sum_order != LES_SUM_REFERENCE) return LES_E_INVALID;
+  if (cfg->sum_order == LES_SUM_REFERENCE && cfg->precision != LES_PREC_FP32) return LES_E_INVALID;
   if (cfg->precision == LES_PREC_MIXED && (!(cfg->refine_tol > 0.0) || cfg->refine_max < 0 ||
                                             cfg->refine_omega < 0.0f || cfg->refine_omega >= 2.0f ||
                                             cfg->refine_solver < 0 || cfg->refine_solver > 1))
@@ -450,6 +455,12 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
       }
     }
   }
+  if (cfg->sum_order == LES_SUM_REFERENCE) {
+    const int nrt = (cfg->ip + 1) / 2;
+    const size_t nt = (size_t)nrt * cfg->jp * cfg->kp;
+    if (hipMalloc(&h->seq_mem, 2 * nt * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
+    h->seq_terms = SeqTerms{{h->seq_mem, h->seq_mem + nt}, nrt};
+  }
   if (hipStreamSynchronize(h->st) != hipSuccess) return bail(LES_E_HIP);
   *out = h;
   return LES_OK;
@@ -470,6 +481,7 @@ extern "C" int les_destroy(les_handle* h) {
   if (h->mx_mem) hipFree(h->mx_mem);
   if (h->cg_mem) hipFree(h->cg_mem);
   if (h->ave_mem) hipFree(h->ave_mem);
+  if (h->seq_mem) hipFree(h->seq_mem);
   if (h->st) hipStreamDestroy(h->st);
   if (h == g_default) g_default = nullptr;
   delete h;
@@ -654,6 +666,25 @@ extern "C" int les_init_coefficients(les_handle* h) {
   const Fields F = fields_of(h);       \
   (void)d; (void)g; (void)F;
 
+namespace {
+// press.f95's sums in the reference's sequential order (les_config::sum_order)
+bool seq_sums(const les_handle* h) { return h->cfg.sum_order == LES_SUM_REFERENCE; }
+
+// rhs and rhsav (press.f95:51-75), the SOR scalars re-armed
+void enqueue_rhs_rhsav(les_handle* h, const Dims& d, const Grid1D& g, const Fields& F) {
+  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc, EDGES_ALL, seq_sums(h) ? 0 : 1);
+  if (seq_sums(h)) launch_seq_rhsav(h->st, d, g, F.rhs, h->sc);
+}
+
+// the residual of one iteration from the sweeps' partials or terms, and the early exit
+void enqueue_resid(les_handle* h, const Dims& d, int nblocks, const Physics& ph) {
+  if (seq_sums(h))
+    launch_seq_resid(h->st, d, h->seq_terms, ph, h->sc);
+  else
+    launch_sor_resid_finalize(h->st, h->rb, nblocks, ph, h->sc);
+}
+}  // namespace
+
 extern "C" int les_velnw(les_handle* h) {
   STAGE_BEGIN(h);
   launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 0);
@@ -749,7 +780,7 @@ extern "C" int les_bondfg(les_handle* h) {
 
 extern "C" int les_press_rhsav(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
+  enqueue_rhs_rhsav(h, d, g, F);
   launch_rhs_adjust(h->st, d, F, h->sc);
   return check_launch(h);
 }
@@ -793,13 +824,14 @@ int enqueue_sor_inplace(les_handle* h) {
   launch_sor_reset(h->st, h->sc);
   const bool resid = h->ph.pjuge > 0.0f;
   const int nb = sor_resid_blocks(d);
+  const SeqTerms tm = seq_sums(h) ? h->seq_terms : SeqTerms{};
   for (int l = 1; l <= h->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd) {
-      launch_sor_colour(h->st, d, g, F, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
+      launch_sor_colour(h->st, d, g, F, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc, tm);
       launch_boundp1(h->st, d, F, h->sc, 1);
     }
     launch_boundp2(h->st, d, F, h->sc, 1);
-    if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, h->ph, h->sc);
+    if (resid) enqueue_resid(h, d, 2 * nb, h->ph);
   }
   return check_launch(h);
 }
@@ -824,10 +856,11 @@ int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true, const Physics*
       launch_sor_iter(h->st, d, g, h->sp, ph, back, h->sc, h->iter_shape);
     }
   }
+  const SeqTerms tm = seq_sums(h) ? h->seq_terms : SeqTerms{};
   for (int l = l0; l <= h->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd)
-      launch_sor_split(h->st, d, g, h->sp, ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc);
-    if (resid) launch_sor_resid_finalize(h->st, h->rb, 2 * nb, ph, h->sc);
+      launch_sor_split(h->st, d, g, h->sp, ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc, tm);
+    if (resid) enqueue_resid(h, d, 2 * nb, ph);
   }
   return check_launch(h);
 }
@@ -951,16 +984,22 @@ int enqueue_press(les_handle* h) {
   STAGE_BEGIN(h);
   if (h->cfg.precision == LES_PREC_MIXED) return enqueue_press_mixed(h);
   int r;
-  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
+  enqueue_rhs_rhsav(h, d, g, F);
   if (use_split(h)) {
     launch_split_scatter(h->st, d, F, h->sp, h->sc, 1);  // rhs -= rhsav on the way
     if ((r = enqueue_sor_split_loop(h, true, nullptr, false))) return r;
-    launch_pav_split(h->st, d, g, h->sp, h->rb, h->sc);
+    if (seq_sums(h))
+      launch_seq_pav(h->st, d, g, nullptr, &h->sp, h->sc);
+    else
+      launch_pav_split(h->st, d, g, h->sp, h->rb, h->sc);
     launch_split_gather_adj(h->st, d, F, h->sp, h->sc, 1);  // p - pav, halos included
   } else {
     launch_rhs_adjust(h->st, d, F, h->sc);
     if ((r = enqueue_sor_inplace(h))) return r;
-    launch_pav(h->st, d, g, F, h->rb, h->sc);
+    if (seq_sums(h))
+      launch_seq_pav(h->st, d, g, F.p, nullptr, h->sc);
+    else
+      launch_pav(h->st, d, g, F, h->rb, h->sc);
     launch_adj(h->st, d, F, h->sc, 1);
   }
   h->p_consistent = true;
@@ -1006,7 +1045,10 @@ extern "C" int les_press_sor_solve(les_handle* h) {
 
 extern "C" int les_press_pav(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_pav(h->st, d, g, F, h->rb, h->sc);
+  if (seq_sums(h))
+    launch_seq_pav(h->st, d, g, F.p, nullptr, h->sc);
+  else
+    launch_pav(h->st, d, g, F, h->rb, h->sc);
   return check_launch(h);
 }
 
@@ -1807,6 +1849,7 @@ extern "C" int les_tiled_create(const les_config* cfg, les_tiled** out) {
   // red-black colours: a tile's local (i+j+k) parity must be the domain's
   if ((si > 1 && ipl % 2) || (sj > 1 && jpl % 2) || ipl < 2 || jpl < 2) return LES_E_INVALID;
   if (cfg->precision != LES_PREC_FP32 || cfg->pjuge > 0.0f) return LES_E_INVALID;
+  if (cfg->sum_order != LES_SUM_EXACT) return LES_E_INVALID;  // tiles combine exact partials
   les_tiled* T = new les_tiled();
   T->cfg = *cfg;
   T->si = si;

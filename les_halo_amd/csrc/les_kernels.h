@@ -54,6 +54,13 @@ struct SplitSOR {
   int jhalo;
   const float *hw, *he;
 };
+// Per-cell residual terms reltmp**2 of one SOR iteration, for the reference-order sum
+// (les_seqsum.hip): colour c's cell (i,j,k) at ((i-1)>>1) + nrt*((j-1) + jp*(k-1)) of t[c],
+// nrt = (ip+1)/2.  t[0] == nullptr: the sweeps accumulate exact partials instead.
+struct SeqTerms {
+  float* t[2];
+  int nrt;
+};
 // The split copies of the tiles of one cut domain and their neighbours, for the face exchange
 // of the split layout (k_split_exchange) after every colour sweep
 constexpr int SPLIT_TILES_MAX = 8;
@@ -139,7 +146,8 @@ void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf r
 void launch_bondfg(hipStream_t st, Dims d, Fields F);
 void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc);
 void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
-                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc);
+                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc,
+                       SeqTerms tm = SeqTerms{});
 void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, int nblocks, Physics ph,
                                DevScalars* sc);
 void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done,
@@ -155,7 +163,15 @@ int sor_resid_blocks(Dims d);
 void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                           int adjust_rhs);
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
-                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc);
+                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc,
+                      SeqTerms tm = SeqTerms{});
+// press.f95's sums in the reference's sequential order (les_seqsum.hip): rhsav over the Fortran
+// rhs; pav over the Fortran p (sp == nullptr) or the split copy; the residual of one iteration
+// over the terms the sweeps stored, with the early exit
+void launch_seq_rhsav(hipStream_t st, Dims d, Grid1D g, const float* rhs, DevScalars* sc);
+void launch_seq_pav(hipStream_t st, Dims d, Grid1D g, const float* p, const SplitSOR* sp,
+                    DevScalars* sc);
+void launch_seq_resid(hipStream_t st, Dims d, SeqTerms tm, Physics ph, DevScalars* sc);
 int sor_split_blocks(Dims d);
 // sets SplitSOR::m_ng, m_jp; false when the grid is too large for the multiply-high quotients
 bool sor_split_divisors(Dims d, SplitSOR& sp);

@@ -969,7 +969,7 @@ void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc) {
 // ==========================================================================================
 __global__ __launch_bounds__(256) void k_sor_colour(Dims d, Grid1D g, Fields F, Physics ph,
                                                     int nrd, int want_resid, int64_t* partials,
-                                                    const DevScalars* sc) {
+                                                    const DevScalars* sc, SeqTerms tm) {
   __shared__ unsigned long long lds[NLIMB];
   if (sc->sor_done) return;
   if (want_resid) block_limbs_init(lds);
@@ -991,7 +991,12 @@ __global__ __launch_bounds__(256) void k_sor_colour(Dims d, Grid1D g, Fields F, 
                          F.rhs[d.R(i, j, k)]) -
                     pc);
     F.p[ip] = pc + reltmp;
-    if (want_resid) acc.add(reltmp * reltmp, lds);
+    if (want_resid) {
+      if (tm.t[0])  // the reference-order sum adds the terms afterwards
+        tm.t[(1 + nrd) & 1][((i - 1) >> 1) + tm.nrt * ((j - 1) + d.jp * (k - 1))] = reltmp * reltmp;
+      else
+        acc.add(reltmp * reltmp, lds);
+    }
   }
   if (want_resid) {
     acc.flush(lds);
@@ -1006,11 +1011,11 @@ int sor_resid_blocks(Dims d) {
 }
 
 void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
-                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc) {
+                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc, SeqTerms tm) {
   int half = (d.ip + 1) / 2;
   dim3 grid((half + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
   hipLaunchKernelGGL(k_sor_colour, grid, dim3(BX, BY), 0, st, d, g, F, ph, nrd, want_resid,
-                     rb.partials + (long)blk_offset * NLIMB, sc);
+                     rb.partials + (long)blk_offset * NLIMB, sc, tm);
 }
 
 // residual of one SOR iteration (both colours) and the early exit of press.f95:114

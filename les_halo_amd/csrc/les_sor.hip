@@ -80,7 +80,8 @@ template <bool UNIFORM, bool RESID>
 __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, const SplitSOR& sp,
                                                const Physics& ph, int nrd, int64_t* partials,
                                                int ng, int nthreads, int L,
-                                               unsigned long long* lds) {
+                                               unsigned long long* lds,
+                                               const SeqTerms& tm = SeqTerms{}) {
   if (RESID) block_limbs_init(lds);
   ThreadAcc acc;
   acc.init();
@@ -162,7 +163,12 @@ __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, c
           ph.omega * (cn1 * (a2l * pe + a2s * pw + c3l * nv[q] + c3s * sv[q] + c4l * uv[q] +
                              c4s * dv[q] - rv[q]) -
                       pc);
-      if (RESID) acc.add(reltmp * reltmp, lds);
+      if (RESID) {
+        if (tm.t[0])  // the reference-order sum adds the terms afterwards
+          tm.t[c][((i - 1) >> 1) + tm.nrt * ((j - 1) + d.jp * (k - 1))] = reltmp * reltmp;
+        else
+          acc.add(reltmp * reltmp, lds);
+      }
       return pc + reltmp;
     };
 #pragma unroll
@@ -184,11 +190,12 @@ __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, c
 template <bool UNIFORM, bool RESID>
 __global__ __launch_bounds__(256) void k_sor_split(Dims d, Grid1D g, SplitSOR sp, Physics ph,
                                                    int nrd, int64_t* partials,
-                                                   const DevScalars* sc, int ng, int nthreads) {
+                                                   const DevScalars* sc, int ng, int nthreads,
+                                                   SeqTerms tm) {
   __shared__ unsigned long long lds[NLIMB];
   if (sc->sor_done) return;
   sor_split_body<UNIFORM, RESID>(d, g, sp, ph, nrd, partials, ng, nthreads,
-                                 xcd_tile(blockIdx.x, gridDim.x), lds);
+                                 xcd_tile(blockIdx.x, gridDim.x), lds, tm);
 }
 
 // one colour sweep of every tile of a cut domain (blockIdx.y = tile), no residual
@@ -504,21 +511,21 @@ bool sor_split_divisors(Dims d, SplitSOR& sp) {
 }
 
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
-                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc) {
+                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc, SeqTerms tm) {
   int ng, nt, nwg;
   split_threads(d, sp, ng, nt, nwg);
   int64_t* part = rb.partials + (long)blk_offset * NLIMB;
   const dim3 grid(nwg), blk(256);
   if (sp.uniform) {
     if (want_resid)
-      hipLaunchKernelGGL((k_sor_split<true, true>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+      hipLaunchKernelGGL((k_sor_split<true, true>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt, tm);
     else
-      hipLaunchKernelGGL((k_sor_split<true, false>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+      hipLaunchKernelGGL((k_sor_split<true, false>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt, tm);
   } else {
     if (want_resid)
-      hipLaunchKernelGGL((k_sor_split<false, true>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+      hipLaunchKernelGGL((k_sor_split<false, true>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt, tm);
     else
-      hipLaunchKernelGGL((k_sor_split<false, false>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt);
+      hipLaunchKernelGGL((k_sor_split<false, false>), grid, blk, 0, st, d, g, sp, ph, nrd, part, sc, ng, nt, tm);
   }
 }
 

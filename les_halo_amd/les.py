@@ -11,7 +11,8 @@ import ctypes
 import numpy as np
 
 from . import fields as fl
-from ._lib import (ERR_NAMES, LES_PREC_FP32, LES_PREC_MIXED, LesConfig, LesScalars, lib, ptr)
+from ._lib import (ERR_NAMES, LES_PREC_FP32, LES_PREC_MIXED, LES_SUM_EXACT, LES_SUM_REFERENCE,
+                   LesConfig, LesScalars, lib, ptr)
 
 
 class LESError(RuntimeError):
@@ -25,7 +26,8 @@ class LES:
 
     def __init__(self, ip, jp, kp, dt=0.2, ro=1.1763, vn=None, alpha=-10.0, beta=-1.0, cs0=0.1,
                  omega=1.0, pjuge=1e-4, nmaxp=50, precision=LES_PREC_FP32, refine_tol=1e-7,
-                 refine_max=8, device=0, refine_omega=0.0, refine_solver=0):
+                 refine_max=8, device=0, refine_omega=0.0, refine_solver=0,
+                 sum_order=LES_SUM_EXACT):
         L = lib()
         cfg = LesConfig()
         L.les_default_config(ctypes.byref(cfg), ip, jp, kp)
@@ -36,6 +38,7 @@ class LES:
         cfg.refine_tol, cfg.refine_max, cfg.device = refine_tol, refine_max, device
         cfg.refine_omega = refine_omega
         cfg.refine_solver = refine_solver
+        cfg.sum_order = sum_order
         self.cfg = cfg
         self.dims = (ip, jp, kp)
         h = ctypes.c_void_p()
@@ -336,7 +339,8 @@ class Ensemble:
             pass
 
 
-__all__ = ["LES", "Ensemble", "LESError", "LES_PREC_FP32", "LES_PREC_MIXED"]
+__all__ = ["LES", "Ensemble", "LESError", "LES_PREC_FP32", "LES_PREC_MIXED", "LES_SUM_EXACT",
+           "LES_SUM_REFERENCE"]
 
 
 class TiledLES:
