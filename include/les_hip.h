@@ -345,6 +345,19 @@ int les_halo_write_arrays(les_handle* h, float* const* dev_arrays, int v_dim, in
  * rhsav, pav and the SOR residual use), result written to *out on the host. */
 int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n, float* out);
 
+/* les_reduce_sum_exact without a handle and without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same exact sum of n device floats
+ * (bit-identical result), written to the device float *dev_out, enqueued on 'stream' (a
+ * hipStream_t; NULL = the null stream) of the current device; returns without waiting.  The
+ * workgroup partials live in a fixed device array of the library (160 KiB per device, held
+ * until the process ends) that every call on a device shares: calls that overlap on one device
+ * race, so serialise them (one stream).  n is at most LES_NOALLOC_SUM_MAX (else
+ * LES_E_INVALID); LES_E_HIP when the array's device address cannot be obtained or the launch
+ * fails.  Calls no hipMalloc/hipFree. */
+#define LES_NOALLOC_SUM_MAX 2147483647LL
+int les_reduce_sum_exact_noalloc(const float* dev_x, int64_t n, float* dev_out, void* stream);
+
 /* ---- measurement (replaces the per-launch exectime of runOcl, TEMPL:366-370,1061-1071) ----
  * Launches one kernel of the step 'reps' times on the handle's stream between two HIP events
  * and returns the average duration per launch in milliseconds.  The kernel runs on the

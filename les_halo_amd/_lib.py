@@ -120,6 +120,7 @@ PROTOS = {
     "les_halo_write_arrays": (c_int, [H, ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int,
                                       c_void_p]),
     "les_reduce_sum_exact": (c_int, [H, c_void_p, c_int64, P_float]),
+    "les_reduce_sum_exact_noalloc": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "les_time_kernel": (c_int, [H, c_int, c_int, P_float]),
     "les_autotune": (c_int, [H, c_int, ctypes.POINTER(LesTuning)]),
     "les_set_sor_tile": (c_int, [H, c_int, c_int]),

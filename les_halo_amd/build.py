@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libles_hip.so")
 
-SOURCES = ["les_api.hip", "les_kernels.hip", "les_sor.hip", "les_seqsum.hip", "les_mixed.hip", "les_ave.hip",
+SOURCES = ["les_api.hip", "les_kernels.hip", "les_sor.hip", "les_seqsum.hip", "les_noalloc.hip", "les_mixed.hip", "les_ave.hip",
            "les_adj.hip", "les_gen.cpp", "les_restart.cpp", "les_output.cpp"]
 
 # -ffp-contract=off: every FP32 operation rounds on its own, as in the reference.
