@@ -354,9 +354,10 @@ def main():
         kern_ms = L.time_kernel("sor_iter", 100)
         bytes_launch = SOR_ITER_B_PER_CELL * cells
         compulsory = SOR_ITER_FUSED_B_PER_CELL * cells
-        # the kernel's name as rocprofv3 reports it (tile rows, workgroup size)
+        # the kernel's name as rocprofv3 reports it (tile rows, workgroup size; the variant
+        # that is not a solve's first iteration)
         tj = tuning["sor_tile_j"]
-        kname = "k_sor_iter<%d, %d>" % (tj, 256 if tj == 4 else 512)
+        kname = "k_sor_iter<%d, %d, false>" % (tj, 256 if tj == 4 else 512)
     else:
         kern_ms = L.time_kernel("sor_sweep", 200)
         bytes_launch = SOR_SWEEP_B_PER_CELL * cells
@@ -364,7 +365,7 @@ def main():
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(kname), "kernel": kname,
+                "traffic": traffic_from_profiles(kname.replace(", false>", ">")), "kernel": kname,
                 "bytes_per_launch": bytes_launch, "avg_us": round(kern_ms * 1e3, 3),
                 "tile": [tuning["sor_tile_j"], tuning["sor_tile_k"]],
                 "tuned_iter_us": (round(tuning["sor_iter_ms"] * 1e3, 3)

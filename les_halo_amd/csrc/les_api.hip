@@ -57,6 +57,17 @@ struct les_handle {
   // LES_SUM_REFERENCE: per-cell residual terms of one SOR iteration (both colours)
   float* seq_mem = nullptr;
   SeqTerms seq_terms{};
+  // Resident split solve (enqueue_press): between steps the split copy keeps the last
+  // iterate, p = iterate - pav_carry, so the next solve needs no scatter of p; k_rhs stores
+  // the raw rhs in the split layout and the first fused iteration writes the adjusted rhs to
+  // ra; F.rhs is filled from ra only when something reads it (sync_rhs).
+  float* ra[2] = {nullptr, nullptr};
+  bool resident_ok = true;    // off once p or rhs are handed out by address, in mixed
+                              // precision and with reference-order sums
+  bool sp_p_current = false;  // the split copy holds the carry state of F.p's interior
+  bool rhs_stale = false;     // F.rhs's interior is behind ra
+  bool graph_resident = false;  // the captured step is a resident solve (its replays leave
+                                // sp_p_current and rhs_stale set)
 };
 
 namespace {
@@ -308,6 +319,26 @@ int check_launch(les_handle* h) {
   return LES_OK;
 }
 
+// F.rhs's interior from the resident solve's adjusted rhs (ra), when it is behind
+void sync_rhs(les_handle* h) {
+  if (!h->rhs_stale) return;
+  SplitSOR s = h->sp;
+  s.r[0] = h->ra[0];
+  s.r[1] = h->ra[1];
+  launch_split_gather_rhs(h->st, h->d, fields_of(h), s);
+  h->rhs_stale = false;
+}
+
+// p or rhs handed out by address: the caller may read or write them at any time, so the
+// resident solve (which keeps them in the split copy between steps) stops for good
+void expose_field(les_handle* h, int field) {
+  if (field != LES_P && field != LES_RHS) return;
+  sync_rhs(h);
+  h->resident_ok = false;
+  h->sp_p_current = false;
+  h->graph_valid = false;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -405,6 +436,10 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   h->rb.max_blocks = std::max(h->rb.max_blocks, 2 * sor_split_blocks(h->d));
   if (hipMalloc(&h->rb.partials, (size_t)h->rb.max_blocks * NLIMB * sizeof(int64_t)) != hipSuccess)
     return bail(LES_E_NOMEM);
+  if (hipMalloc(&h->rb.slots, (size_t)RED_SLOTS * NLIMB * sizeof(int64_t)) != hipSuccess)
+    return bail(LES_E_NOMEM);
+  if (hipMemsetAsync(h->rb.slots, 0, (size_t)RED_SLOTS * NLIMB * sizeof(int64_t), h->st) != hipSuccess)
+    return bail(LES_E_HIP);
   if (hipMalloc(&h->scratch_f, 256) != hipSuccess) return bail(LES_E_NOMEM);
   {  // split SOR copy: 2 colours x (p, rhs) of interior rows padded to nr floats, plus the
      // cn1 factors ax(0:ip+1), by(0:jp+1), cz(0:kp+1) and the table cn1k(0:kp+1)
@@ -412,7 +447,7 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     sp.nr = ((cfg->ip + 1) / 2 + 3) / 4 * 4;
     sp.nj = cfg->jp;
     const size_t ns = (size_t)sp.nr * cfg->jp * cfg->kp;
-    const size_t nf = 6 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + 2 * (cfg->kp + 2);
+    const size_t nf = 8 * ns + (size_t)(cfg->ip + 2) + (cfg->jp + 2) + 2 * (cfg->kp + 2);
     if (hipMalloc(&h->split_mem, nf * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);
     if (hipMemsetAsync(h->split_mem, 0, nf * sizeof(float), h->st) != hipSuccess) return bail(LES_E_HIP);
     sp.p[0] = h->split_mem;
@@ -421,7 +456,9 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     sp.r[1] = sp.r[0] + ns;
     h->palt[0] = sp.r[1] + ns;  // float4-aligned like the four arrays before it
     h->palt[1] = h->palt[0] + ns;
-    float* fac = h->palt[1] + ns;
+    h->ra[0] = h->palt[1] + ns;
+    h->ra[1] = h->ra[0] + ns;
+    float* fac = h->ra[1] + ns;
     sp.ax = fac;
     sp.by = fac + (cfg->ip + 2);
     sp.cz = fac + (cfg->ip + 2) + (cfg->jp + 2);
@@ -455,6 +492,7 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
       }
     }
   }
+  h->resident_ok = cfg->precision == LES_PREC_FP32 && cfg->sum_order == LES_SUM_EXACT;
   if (cfg->sum_order == LES_SUM_REFERENCE) {
     const int nrt = (cfg->ip + 1) / 2;
     const size_t nt = (size_t)nrt * cfg->jp * cfg->kp;
@@ -476,6 +514,7 @@ extern "C" int les_destroy(les_handle* h) {
   if (h->mask) hipFree(h->mask);
   if (h->sc) hipFree(h->sc);
   if (h->rb.partials) hipFree(h->rb.partials);
+  if (h->rb.slots) hipFree(h->rb.slots);
   if (h->scratch_f) hipFree(h->scratch_f);
   if (h->split_mem) hipFree(h->split_mem);
   if (h->mx_mem) hipFree(h->mx_mem);
@@ -535,7 +574,11 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
     h->p2xy = grid_p2xy(h);
     h->graph_valid = false;
   }
-  if (field == LES_P) h->p_consistent = false;
+  if (field == LES_P) {
+    h->p_consistent = false;
+    h->sp_p_current = false;
+  }
+  if (field == LES_RHS) h->rhs_stale = false;  // the upload replaced every value
   if (field == LES_CN1) std::memcpy(h->hcn1.data(), host, n * sizeof(float));
   if (field == LES_CN1 || field == LES_DXS || field == LES_DYS || field == LES_DZS ||
       field == LES_CN2L || field == LES_CN2S || field == LES_CN3L || field == LES_CN3S) {
@@ -599,6 +642,7 @@ extern "C" int les_set_terrain(les_handle* h, const float* zbm) {
 
 extern "C" int les_download(les_handle* h, int field, float* host) {
   if (!h || !host || field < 0 || field >= LES_NFIELDS) return fail(h, LES_E_INVALID, "download: bad argument");
+  if (field == LES_RHS) sync_rhs(h);
   HIPCHK(h, hipStreamSynchronize(h->st));
   HIPCHK(h, hipMemcpy(host, h->fld[field], (size_t)h->cnt[field] * sizeof(float), hipMemcpyDeviceToHost));
   return LES_OK;
@@ -606,6 +650,7 @@ extern "C" int les_download(les_handle* h, int field, float* host) {
 
 extern "C" int les_device_ptr(les_handle* h, int field, void** dptr, int64_t* count) {
   if (!h || field < 0 || field >= LES_NFIELDS) return LES_E_INVALID;
+  expose_field(h, field);
   if (dptr) *dptr = h->fld[field];
   if (count) *count = h->cnt[field];
   return LES_OK;
@@ -766,8 +811,7 @@ extern "C" int les_les_calc_visc__adam(les_handle* h) {
 
 extern "C" int les_les(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_calc_sm(h->st, d, g, F, h->ph);
-  launch_boundsm(h->st, d, F);
+  launch_calc_sm(h->st, d, g, F, h->ph, 1);  // boundsm's halo cells included
   launch_calc_visc_adam(h->st, d, g, F, 1, 0);
   return check_launch(h);
 }
@@ -782,6 +826,7 @@ extern "C" int les_press_rhsav(les_handle* h) {
   STAGE_BEGIN(h);
   enqueue_rhs_rhsav(h, d, g, F);
   launch_rhs_adjust(h->st, d, F, h->sc);
+  h->rhs_stale = false;  // F.rhs holds the adjusted rhs of this call
   return check_launch(h);
 }
 
@@ -800,6 +845,8 @@ extern "C" int les_boundp2(les_handle* h) {
 extern "C" int les_press_sor(les_handle* h, int nrd) {
   STAGE_BEGIN(h);
   if (nrd < 0 || nrd > 2) return fail(h, LES_E_INVALID, "press_sor: nrd must be 0, 1 or 2");
+  sync_rhs(h);
+  h->sp_p_current = false;
   if (nrd == 2) {
     launch_boundp2(h->st, d, F, h->sc, 0);
   } else {
@@ -889,6 +936,7 @@ namespace les {
 // init.f95 form) and, up to the caller, mirror-consistent halos of p on entry.
 int sor_solve_external(les_handle* h, hipStream_t st, float* p, float* rhs) {
   if (!h || !p || !rhs) return LES_E_INVALID;
+  h->sp_p_current = false;  // the split copy takes this solve's arrays
   if (!(h->d.jp % 2 == 0 && h->cn1_separable && h->split_ok))
     return fail(h, LES_E_STATE, "split solve does not apply to this grid");
   const hipStream_t own = h->st;
@@ -908,6 +956,7 @@ int sor_solve_external(les_handle* h, hipStream_t st, float* p, float* rhs) {
 // rhs part.  The split copy of the handle holds the split adjoints meanwhile.
 int sor_adjoint_external(les_handle* h, hipStream_t st, float* pb, float* rb) {
   if (!h || !pb || !rb) return LES_E_INVALID;
+  h->sp_p_current = false;  // the split copy holds the adjoints meanwhile
   if (!(h->d.jp % 2 == 0 && h->cn1_separable && h->split_ok && h->sp.uniform))
     return fail(h, LES_E_STATE, "split solve transpose needs a uniform horizontal grid");
   const Dims d = h->d;
@@ -941,7 +990,7 @@ int enqueue_press_mixed(les_handle* h) {
     launch_boundp1(h->st, d, F, h->sc, 0);
     launch_boundp2(h->st, d, F, h->sc, 0);
   }
-  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc);
+  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc, EDGES_ALL, 0);  // partials for the FP64 sum
   launch_mx_rhsav(h->st, d, h->rb, h->sc);
   launch_mx_scatter(h->st, d, F, h->sp, h->mx, h->sc);
   if ((r = enqueue_sor_split_loop(h, false))) return r;
@@ -979,10 +1028,53 @@ int enqueue_press_mixed(les_handle* h) {
   return check_launch(h);
 }
 
+// The resident split solve applies: fused iterations in pairs, no residual, FP32 with exact
+// sums, p and rhs not handed out by address.
+bool resident_applies(const les_handle* h) {
+  return h->resident_ok && use_split(h) && !(h->ph.pjuge > 0.0f) && h->iter_shape.tj > 0 &&
+         sor_iter_supported(h->d, h->sp) && h->cfg.nmaxp >= 2 && h->cfg.nmaxp % 2 == 0;
+}
+
+// press.f95:49-146 on the resident split copy.  The split copy of p carries over from the last
+// solve (its iterate; F.p = iterate - pav_carry), or is scattered from F.p; k_rhs stores the
+// raw rhs in the split layout; the first fused iteration subtracts rhsav from every rhs value
+// (and pav_carry from every p value) it reads, as the scatter and the gather would, and keeps
+// the adjusted rhs in ra for the other iterations; pav and the gather close the solve without
+// touching the split iterate.  F.rhs is filled from ra on demand (sync_rhs).  Every value is
+// the one the scatter / gather path computes.
+int enqueue_press_resident(les_handle* h) {
+  STAGE_BEGIN(h);
+  const bool carry = h->sp_p_current;
+  if (!carry) launch_split_scatter(h->st, d, F, h->sp, h->sc, 0, 0);  // p only
+  launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc, EDGES_ALL, 1, &h->sp);  // also re-arms SOR
+  SplitSOR sra = h->sp;
+  sra.r[0] = h->ra[0];
+  sra.r[1] = h->ra[1];
+  const SorPair first{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}, {h->ra[0], h->ra[1]},
+                      carry ? 1 : 0};
+  const SorPair fwd{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}, {nullptr, nullptr}, 0};
+  const SorPair back{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}, {nullptr, nullptr}, 0};
+  launch_sor_iter(h->st, d, g, h->sp, h->ph, first, h->sc, h->iter_shape, 1);
+  launch_sor_iter(h->st, d, g, sra, h->ph, back, h->sc, h->iter_shape);
+  for (int l = 3; l + 1 <= h->cfg.nmaxp; l += 2) {
+    launch_sor_iter(h->st, d, g, sra, h->ph, fwd, h->sc, h->iter_shape);
+    launch_sor_iter(h->st, d, g, sra, h->ph, back, h->sc, h->iter_shape);
+  }
+  launch_pav_split(h->st, d, g, h->sp, h->rb, h->sc);
+  launch_split_gather_adj(h->st, d, F, h->sp, h->sc, 1, 1);  // p - pav, halos; pav_carry
+  h->sp_p_current = true;
+  h->rhs_stale = true;
+  h->p_consistent = true;
+  return check_launch(h);
+}
+
 // press.f95:49-146
 int enqueue_press(les_handle* h) {
   STAGE_BEGIN(h);
   if (h->cfg.precision == LES_PREC_MIXED) return enqueue_press_mixed(h);
+  if (resident_applies(h)) return enqueue_press_resident(h);
+  h->sp_p_current = false;
+  h->rhs_stale = false;  // F.rhs is written below
   int r;
   enqueue_rhs_rhsav(h, d, g, F);
   if (use_split(h)) {
@@ -1012,8 +1104,7 @@ int enqueue_step(les_handle* h, int n) {
   launch_uout(h->st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
   launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
-  launch_calc_sm(h->st, d, g, F, h->ph);
-  launch_boundsm(h->st, d, F);
+  launch_calc_sm(h->st, d, g, F, h->ph, 1);  // boundsm's halo cells included
   launch_calc_visc_adam(h->st, d, g, F, 1, 1);
   return enqueue_press(h);
 }
@@ -1021,7 +1112,9 @@ int enqueue_step(les_handle* h, int n) {
 
 namespace les {
 float* field_ptr(les_handle* h, int field) {
-  return (h && field >= 0 && field < LES_NFIELDS) ? h->fld[field] : nullptr;
+  if (!h || field < 0 || field >= LES_NFIELDS) return nullptr;
+  expose_field(h, field);
+  return h->fld[field];
 }
 void set_drag(les_handle* h, const float* cd) {
   h->drag = cd;
@@ -1040,6 +1133,8 @@ int step_external(les_handle* h, hipStream_t st, int n) {
 
 extern "C" int les_press_sor_solve(les_handle* h) {
   if (!h) return LES_E_INVALID;
+  sync_rhs(h);
+  h->sp_p_current = false;
   return enqueue_sor_solve(h);
 }
 
@@ -1054,6 +1149,7 @@ extern "C" int les_press_pav(les_handle* h) {
 
 extern "C" int les_press_adj(les_handle* h) {
   STAGE_BEGIN(h);
+  h->sp_p_current = false;
   launch_adj(h->st, d, F, h->sc, 0);
   h->p_consistent = false;  // interior shifted, halos not yet
   return check_launch(h);
@@ -1096,6 +1192,12 @@ int run_eager_prefix(les_handle* h, int n, int& nsteps, int& r) {
     ++n;
     --nsteps;
   }
+  // the captured step of the resident solve starts from a carried split copy
+  while (nsteps > 0 && resident_applies(h) && !h->sp_p_current) {
+    if ((r = enqueue_step(h, n))) return n;
+    ++n;
+    --nsteps;
+  }
   return n;
 }
 
@@ -1118,7 +1220,17 @@ int ensure_graph(les_handle* h) {
   h->graph = gr;
   HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
   h->graph_valid = true;
+  h->graph_resident = h->sp_p_current;  // set by the captured step iff it is resident
   return LES_OK;
+}
+
+// the host-side state a replay of the captured step leaves
+void after_replays(les_handle* h) {
+  h->p_consistent = true;
+  if (h->graph_resident) {
+    h->sp_p_current = true;
+    h->rhs_stale = true;
+  }
 }
 }  // namespace
 
@@ -1130,6 +1242,7 @@ extern "C" int les_run(les_handle* h, int n0, int nsteps) {
   if (nsteps == 0) return LES_OK;
   if ((r = ensure_graph(h))) return r;
   for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->st));
+  after_replays(h);
   return LES_OK;
 }
 
@@ -1176,6 +1289,8 @@ extern "C" int les_ensemble_run(les_ensemble* e, int n0, int nsteps) {
       if (s < nsteps - left[q]) continue;  // this member ran its first steps eagerly
       HIPCHK(h, hipGraphLaunch(h->gexec, h->st));
     }
+  for (int q = 0; q < nm; ++q)
+    if (left[q] > 0) after_replays(e->m[q]);
   return LES_OK;
 }
 
@@ -1197,8 +1312,7 @@ extern "C" int les_ifdata(les_handle* h) {
   launch_boundp2(h->st, d, F, h->sc, 0);
   h->p_consistent = true;
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
-  launch_calc_sm(h->st, d, g, F, h->ph);
-  launch_boundsm(h->st, d, F);
+  launch_calc_sm(h->st, d, g, F, h->ph, 1);  // boundsm's halo cells included
   launch_calc_visc_adam(h->st, d, g, F, 1, 0);
   return check_launch(h);
 }
@@ -1276,6 +1390,7 @@ extern "C" int les_exchange_halo_read(les_handle* h, const int* fields, int v_di
   int r = halo_common(h, fields, v_dim, h_w, ptrs, im, jm, km);
   if (r) return r;
   if (!dev_buffer) return fail(h, LES_E_INVALID, "halo: null buffer");
+  sync_rhs(h);
   launch_halo_read(h->st, ptrs, v_dim, im, jm, km, h_w, correct ? 1 : 0, dev_buffer);
   return check_launch(h);
 }
@@ -1287,6 +1402,9 @@ extern "C" int les_exchange_halo_write(les_handle* h, const int* fields, int v_d
   int r = halo_common(h, fields, v_dim, h_w, ptrs, im, jm, km);
   if (r) return r;
   if (!dev_buffer) return fail(h, LES_E_INVALID, "halo: null buffer");
+  sync_rhs(h);
+  for (int q = 0; q < v_dim; ++q)
+    if (fields[q] == LES_P) h->sp_p_current = false;
   launch_halo_write(h->st, ptrs, v_dim, im, jm, km, h_w, dev_buffer);
   return check_launch(h);
 }
@@ -1330,6 +1448,8 @@ extern "C" int les_reduce_sum_exact(les_handle* h, const float* dev_x, int64_t n
 extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms) {
   if (!h || !avg_ms || reps < 1) return LES_E_INVALID;
   STAGE_BEGIN(h);
+  sync_rhs(h);
+  h->sp_p_current = false;  // the timed kernels overwrite the split copy
   if (which == LES_KERNEL_STEP && !h->graph_valid) {
     int r = les_run(h, 2, 1);  // captures the graph
     if (r) return r;
@@ -1402,6 +1522,8 @@ extern "C" int les_set_sor_tile(les_handle* h, int tile_j, int tile_k) {
 extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
   if (!h || reps < 1) return LES_E_INVALID;
   STAGE_BEGIN(h);
+  sync_rhs(h);
+  h->sp_p_current = false;  // the trials overwrite the split copy
   les_tuning best{};
   best.sor_iter_ms = -1.0f;
   if (h->split_ok && sor_iter_supported(d, h->sp)) {

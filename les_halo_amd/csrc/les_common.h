@@ -357,6 +357,37 @@ __device__ inline void sum_partials(const int64_t* partials, int nb, int64_t* li
 }
 constexpr int FIN_THREADS = 1024;  // workgroup size of the one-workgroup finalisers
 
+// The same exact sums through a few accumulation slots: block L adds its limbs into slot
+// L % RED_SLOTS with 64-bit integer atomics (integer addition: the order does not matter), and
+// the finaliser reads the RED_SLOTS x NLIMB words instead of one partial per block, clearing
+// each slot word as it reads it (the slots start at zero and are zero after every finalise).
+// A slot limb adds up at most (blocks / RED_SLOTS) block limbs, far inside 64 bits.
+constexpr int RED_SLOTS = 64;
+
+__device__ inline void block_limbs_atomic(unsigned long long* lds, int64_t* slots, int L) {
+  __syncthreads();
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid < NLIMB && lds[tid])
+    atomicAdd(reinterpret_cast<unsigned long long*>(slots) + (L % RED_SLOTS) * NLIMB + tid,
+              lds[tid]);
+}
+
+// Sum the slots into limbs_out and clear them (one workgroup of at least RED_SLOTS * NLIMB
+// threads).
+__device__ inline void sum_slots(int64_t* slots, int64_t* limbs_out) {
+  __shared__ unsigned long long acc[NLIMB];
+  if (threadIdx.x < NLIMB) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  if (threadIdx.x < RED_SLOTS * NLIMB) {
+    const int64_t v = slots[threadIdx.x];
+    slots[threadIdx.x] = 0;
+    if (v) atomicAdd(&acc[threadIdx.x % NLIMB], (unsigned long long)v);
+  }
+  __syncthreads();
+  if (threadIdx.x < NLIMB) limbs_out[threadIdx.x] = (int64_t)acc[threadIdx.x];
+  __syncthreads();
+}
+
 // Workgroups are handed to the 8 XCDs (each with its own L2) round-robin.  With a grid padded
 // to a multiple of 8, this maps dispatch id w to a logical tile so that every XCD works on one
 // contiguous range of tiles: spatial neighbours then share an L2.

@@ -29,6 +29,7 @@ struct DevScalars {
   double rhsav64, area64, pav64, rhs_l2;
   int refine_done;
   float uout_max, uout_min;  // bondv1.f95:71-72 before the average (max seeded and min seeded 0)
+  float pav_carry;  // the pav that k_split_gather_adj took off p while the split copy kept p
 };
 
 struct Physics {
@@ -76,6 +77,13 @@ struct SplitTiles {
 struct SorPair {
   const float* pin[2];
   float* pout[2];
+  // The first iteration of a solve on the resident split copy (k_sor_iter<.., FIRST>): it reads
+  // the raw rhs that k_rhs stored in the split layout and subtracts rhsav (press.f95:76-82)
+  // from every rhs value it reads, writing the result for the cells it owns to rout; with
+  // p_shift it also subtracts DevScalars::pav_carry from every p value it reads (the split
+  // copy then holds the previous solve's iterate before its pav was taken off).
+  float* rout[2];
+  int p_shift;
 };
 // workgroup tile of k_sor_iter: tj rows of j (4, 8 or 16), kc levels of k
 struct SorIterShape {
@@ -88,6 +96,7 @@ constexpr int SOR_ITER_KCMAX = 128;  // most levels per workgroup of the fused k
 struct RedBuf {
   int64_t* partials;  // [max_blocks][NLIMB]
   int max_blocks;
+  int64_t* slots;     // [RED_SLOTS][NLIMB], zero between uses (nullptr: partials only)
 };
 
 // Which faces of a (sub)domain are faces of the whole domain.  A tile of a domain cut into
@@ -137,12 +146,15 @@ void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, D
                        Edges e = EDGES_ALL, int do_outflow = 1);
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf, Edges e = EDGES_ALL);
-void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph);
+// with_bound: also boundsm.f95's halo cells, as k_boundsm would set them afterwards
+void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound = 0);
 void launch_boundsm(hipStream_t st, Dims d, Fields F);
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc,
                            int do_adam);
+// split != nullptr: the raw rhs goes to the split copy's rhs arrays, not to F.rhs
 void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
-                DevScalars* sc, Edges e = EDGES_ALL, int finalize = 1);
+                DevScalars* sc, Edges e = EDGES_ALL, int finalize = 1,
+                const SplitSOR* split = nullptr);
 void launch_bondfg(hipStream_t st, Dims d, Fields F);
 void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc);
 void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
@@ -160,8 +172,9 @@ void launch_adj(hipStream_t st, Dims d, Fields F, DevScalars* sc, int with_bound
 int sor_resid_blocks(Dims d);
 
 // split-layout SOR (jp even; p halos mirror-consistent on entry)
+// adjust_rhs: rhs -= rhsav on the way (also stored back to F.rhs); with_rhs = 0: p only
 void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
-                          int adjust_rhs);
+                          int adjust_rhs, int with_rhs = 1);
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc,
                       SeqTerms tm = SeqTerms{});
@@ -176,8 +189,9 @@ int sor_split_blocks(Dims d);
 // sets SplitSOR::m_ng, m_jp; false when the grid is too large for the multiply-high quotients
 bool sor_split_divisors(Dims d, SplitSOR& sp);
 int sor_iter_supported(Dims d, const SplitSOR& sp);
+// first != 0: the first iteration of a resident solve (see SorPair)
 void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
-                     DevScalars* sc, SorIterShape shape);
+                     DevScalars* sc, SorIterShape shape, int first = 0);
 void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc);
 // the sweep operands of every tile of a cut domain (k_sor_split_tiles)
 struct SorTiles {
@@ -190,8 +204,12 @@ struct SorTiles {
 void launch_sor_split_tiles(hipStream_t st, Dims d, const SorTiles& T, Physics ph, int nrd);
 // fills every tile's halo rows (both colours) and i-halo columns from its neighbours' interior
 void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T);
+// set_carry: also records pav in DevScalars::pav_carry (the split copy keeps p before adj)
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
-                             int subtract_pav);
+                             int subtract_pav, int set_carry = 0);
+// F.rhs over the interior from the split copy's rhs arrays ra (the resident solve's
+// adjusted rhs): the Fortran array materialised on demand
+void launch_split_gather_rhs(hipStream_t st, Dims d, Fields F, SplitSOR sp);
 // Transposes of the colour-split solve (scatter, iterations, gather) for adjoints pb of p and
 // rb of rhs in their Fortran layouts; the split adjoints live in sp.p (of p) and sp.r (of rhs).
 // gather_t: sp.p = the sum of pb over the box cells mirroring each interior cell, those pb

@@ -582,8 +582,40 @@ __device__ __forceinline__ float sm_value(const SmIn& v, float delx1k, float cs0
 #define LES_SM_KPT 1
 #endif
 constexpr int SM_KPT = LES_SM_KPT;
+// boundsm.f95:13-32 composed with the interior value val = sm(i,j,k): every halo cell whose
+// final value boundsm takes from this cell, i.e. (a,b,c) in A x B x C with A = {i, 0 if i = 1,
+// im+1 if i = im}, B likewise in j, C = {k, 0 if k = 1, km+1 if k = km} (sm(.,.,0) = -sm(.,.,1)),
+// and, for the row j = 1, the halo cells of row j = -1, whose sources sm(x,-1,z) no kernel
+// writes.  The three loops of boundsm leave exactly these values (see k_boundsm).
+__device__ inline void boundsm_from(const Dims& d, const Fields& F, int i, int j, int k,
+                                    float val) {
+  const int im = d.ip, jm = d.jp, km = d.kp;
+  const int na = (i == 1) + (i == im), nb = (j == 1) + (j == jm), nc = (k == 1) + (k == km);
+  if (na + nb + nc == 0) return;
+  int A[3] = {i, i == 1 ? 0 : im + 1, im + 1};
+  int B[3] = {j, j == 1 ? 0 : jm + 1, jm + 1};
+  int C[3] = {k, k == 1 ? 0 : km + 1, km + 1};
+  for (int qa = 0; qa <= na; ++qa)
+    for (int qb = 0; qb <= nb; ++qb)
+      for (int qc = 0; qc <= nc; ++qc) {
+        if (qa + qb + qc == 0) continue;
+        const int c = C[qc];
+        F.sm[d.S(A[qa], B[qb], c)] = (c == 0) ? -val : val;
+      }
+  if (j == 1) {  // row j = -1: sm(a,-1,c) for a in A, c in C, from sm(i,-1,.)
+    const float s1 = F.sm[d.S(i, -1, k)];
+    for (int qa = 0; qa <= na; ++qa)
+      for (int qc = 0; qc <= nc; ++qc) {
+        if (qa + qc == 0) continue;
+        const int c = C[qc];
+        F.sm[d.S(A[qa], -1, c)] = (c == 0) ? -s1 : s1;
+      }
+  }
+}
+
+// with_bound: also boundsm's halo cells (boundsm_from), instead of a k_boundsm launch
 __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
-                                                 StripGrid sg) {
+                                                 StripGrid sg, int with_bound) {
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;
   const int i = 1 + bx * BX + threadIdx.x;
@@ -598,14 +630,16 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
   for (int q = 0; q < SM_KPT; ++q) {
     const int k = k0 + q;
     if (k > d.kp) break;
-    bsm.st(d.S(i, j, k), sm_value(v[q], g.delx1[k - 1], ph.cs0));
+    const float val = sm_value(v[q], g.delx1[k - 1], ph.cs0);
+    bsm.st(d.S(i, j, k), val);
+    if (with_bound) boundsm_from(d, F, i, j, k, val);
   }
 }
 
-void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph) {
+void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound) {
   const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY,
                                   (d.kp + SM_KPT - 1) / SM_KPT);
-  hipLaunchKernelGGL(k_calc_sm, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, sg);
+  hipLaunchKernelGGL(k_calc_sm, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, sg, with_bound);
 }
 
 // ==========================================================================================
@@ -851,8 +885,11 @@ void launch_bondfg(hipStream_t st, Dims d, Fields F) {
 template <bool P2>
 // A tile reads f(0) and g(i,0) from its exchanged halo unless its west face is the domain's,
 // and makes bondFG's f and g copies only at the domain's faces (the exchange made the rest).
+// rsplit[0] != nullptr: rhs(i,j,k) goes to rsplit[(i+j+k)&1] at ((i-1)>>1) + nr*((j-1) +
+// jp*(k-1)) (the split copy's layout) instead of F.rhs.
 __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics ph,
-                                             int64_t* partials, StripGrid sg, Edges e) {
+                                             int64_t* partials, StripGrid sg, Edges e,
+                                             float* rs0, float* rs1, int nr, int64_t* slots) {
   __shared__ unsigned long long lds[NLIMB];
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;  // padding: writes no partial
@@ -903,7 +940,10 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
                 (-c.wm + c.wc) / dznk;
       r = xdiv<P2>(c.fc - c.fm, dx1i) + xdiv<P2>(c.gc - c.gm, dy1j) + (c.hc - c.hm) / dznk +
           (r) / dt;
-      brh.st(d.R(i, j, k), r);
+      if (rs0)
+        (((i + j + k) & 1) ? rs1 : rs0)[((i - 1) >> 1) + nr * ((j - 1) + jm * (k - 1))] = r;
+      else
+        brh.st(d.R(i, j, k), r);
       acc.add(dx1i * dy1j * dznk * r, lds);
     }
     // bondFG: f(0,j,k) = f(1,j,k); g(i,0,k) = g(i,jm,k); h(i,j,0) = h(i,j,km) = 0
@@ -915,7 +955,10 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
     }
   }
   acc.flush(lds);
-  block_limbs_store(lds, partials + (long)L * NLIMB);
+  if (slots)
+    block_limbs_atomic(lds, slots, L);
+  else
+    block_limbs_store(lds, partials + (long)L * NLIMB);
 }
 
 // Also re-arms the SOR scalars for the solve that follows (the work of k_sor_reset).
@@ -923,6 +966,18 @@ __global__ __launch_bounds__(FIN_THREADS) void k_rhsav_finalize(const int64_t* p
                                                                 DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
   sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) {
+    sc->rhsav = limbs_to_float(limbs) / sc->area;
+    sc->sor_done = 0;
+    sc->sor_iters = 0;
+    sc->sor = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(RED_SLOTS * NLIMB) void k_rhsav_finalize_slots(int64_t* slots,
+                                                                           DevScalars* sc) {
+  __shared__ int64_t limbs[NLIMB];
+  sum_slots(slots, limbs);
   if (threadIdx.x == 0) {
     sc->rhsav = limbs_to_float(limbs) / sc->area;
     sc->sor_done = 0;
@@ -941,17 +996,25 @@ __global__ __launch_bounds__(256) void k_rhs_adjust(Dims d, Fields F, const DevS
 }
 
 void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
-                DevScalars* sc, Edges e, int finalize) {
+                DevScalars* sc, Edges e, int finalize, const SplitSOR* split) {
   const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, RED_KC);
   const int nb = sg.nbx * sg.nby * sg.nbz;
+  float* rs0 = split ? split->r[0] : nullptr;
+  float* rs1 = split ? split->r[1] : nullptr;
+  const int nr = split ? split->nr : 0;
+  // a finalised sum goes through the slots; partials stay for callers that combine them
+  int64_t* slots = finalize ? rb.slots : nullptr;
   if (g.p2xy)
     hipLaunchKernelGGL(k_rhs<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, rb.partials,
-                       sg, e);
+                       sg, e, rs0, rs1, nr, slots);
   else
     hipLaunchKernelGGL(k_rhs<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
-                       rb.partials, sg, e);
+                       rb.partials, sg, e, rs0, rs1, nr, slots);
   if (!finalize) return;
-  hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
+  if (slots)
+    hipLaunchKernelGGL(k_rhsav_finalize_slots, dim3(1), dim3(RED_SLOTS * NLIMB), 0, st, slots, sc);
+  else
+    hipLaunchKernelGGL(k_rhsav_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nb, sc);
 }
 
 void launch_rhsav_finalize(hipStream_t st, const int64_t* partials, int nb, DevScalars* sc) {

@@ -40,7 +40,8 @@ __device__ __forceinline__ int split_index(const SplitSOR& sp, int i, int j, int
 // i = 1 + 2ii and 2 + 2ii: its split stores are unit-stride across the wavefront, and its two
 // Fortran loads (and rhs stores) fill each other's cache lines.
 __global__ __launch_bounds__(256) void k_split_scatter(Dims d, Fields F, SplitSOR sp,
-                                                       const DevScalars* sc, int adjust_rhs) {
+                                                       const DevScalars* sc, int adjust_rhs,
+                                                       int with_rhs) {
   const int ii = blockIdx.x * 64 + threadIdx.x;
   const int j = 1 + blockIdx.y * 4 + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -52,13 +53,14 @@ __global__ __launch_bounds__(256) void k_split_scatter(Dims d, Fields F, SplitSO
   for (int c = 0; c < 2; ++c) {
     const int i = min(first_i(c, j, k) + 2 * ii, d.ip);  // clamped: loaded, not used
     pv[c] = F.p[d.P(i, j, k)];
-    rv[c] = F.rhs[d.R(i, j, k)];
+    rv[c] = with_rhs ? F.rhs[d.R(i, j, k)] : 0.0f;
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const int i = first_i(c, j, k) + 2 * ii;
     if (i > d.ip) continue;  // padding entries of the split rows stay untouched
     sp.p[c][base] = pv[c];
+    if (!with_rhs) continue;
     float r = rv[c];
     if (adjust_rhs) {
       r = r - sc->rhsav;
@@ -69,9 +71,10 @@ __global__ __launch_bounds__(256) void k_split_scatter(Dims d, Fields F, SplitSO
 }
 
 void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
-                          int adjust_rhs) {
+                          int adjust_rhs, int with_rhs) {
   dim3 grid((sp.nr + 63) / 64, (d.jp + 3) / 4, d.kp);
-  hipLaunchKernelGGL(k_split_scatter, grid, dim3(64, 4), 0, st, d, F, sp, sc, adjust_rhs);
+  hipLaunchKernelGGL(k_split_scatter, grid, dim3(64, 4), 0, st, d, F, sp, sc, adjust_rhs,
+                     with_rhs);
 }
 
 // ---- one colour sweep: 4 consecutive cells of one colour row per thread ----
@@ -235,7 +238,7 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int TJ, int NT>
+template <int TJ, int NT, bool FIRST>
 __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
                                                   SorPair io, const DevScalars* sc, int kc,
                                                   int njt) {
@@ -248,6 +251,14 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
   __shared__ float4 sA4[3][TJ + 2][NW / 4];
   __shared__ float sC[3][SOR_ITER_KCMAX + 3];  // cn1, cn4l, cn4s of levels k0-1 .. k1+1
   if (sc->sor_done) return;
+  // FIRST: the shifts applied to every rhs and p value read (rhs -= rhsav; p -= the pav the
+  // gather took off), each the same FP32 subtraction the scatter and the gather make
+  float rsh = 0.0f, psh = 0.0f;
+  if constexpr (FIRST) {
+    rsh = sc->rhsav;
+    psh = io.p_shift ? sc->pav_carry : 0.0f;
+  }
+  auto sub4 = [](float4 a, float s) { return make_float4(a.x - s, a.y - s, a.z - s, a.w - s); };
   const int L = xcd_tile(blockIdx.x, gridDim.x);
   const int jt = L % njt, kt = L / njt;
   const int j0 = 1 + jt * TJ;
@@ -322,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
   auto put_b = [&](int k, auto sl) {  // into ring slot sl of level k
 #pragma unroll
     for (int q = 0; q < PB; ++q)
-      if (bOk[q]) sB4[sl][bRow[q]][bG[q]] = bq[q];
+      if (bOk[q]) sB4[sl][bRow[q]][bG[q]] = FIRST ? sub4(bq[q], psh) : bq[q];
   };
   auto fetch_a = [&](int k) {
 #pragma unroll
@@ -385,9 +396,15 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
       if (!aOk[q]) continue;
       const int r = aRow[q];
       const int f = ((1 + aJ[q] + k) & 1) == 1 ? 1 : 2;  // first i of colour 1 in this row
-      const float4 v = relax4(ao[q], ar[q], sB4[m][r + 1], sB4[m][r + 2], sB4[m][r],
+      const float4 pc4 = FIRST ? sub4(ao[q], psh) : ao[q];
+      const float4 rr4 = FIRST ? sub4(ar[q], rsh) : ar[q];
+      const float4 v = relax4(pc4, rr4, sB4[m][r + 1], sB4[m][r + 2], sB4[m][r],
                               sB4[u][r + 1], sB4[dn][r + 1], aG[q], f, k, cn1, c4l, c4s, INT);
       sA4[m][r][aG[q]] = v;
+      if constexpr (FIRST) {  // the adjusted rhs of the colour-1 cells this workgroup owns
+        if (k >= k0 && k <= k1 && r >= 1 && r <= nUrow)
+          *reinterpret_cast<float4*>(io.rout[1] + aOff[q] + lev(k)) = rr4;
+      }
     }
   };
   // new colour 0 of level k on the tile rows; writes both colours of level k
@@ -400,9 +417,11 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
       const int r = uRow[q];
       const int f = ((1 + j0 + r + k) & 1) == 0 ? 1 : 2;  // first i of colour 0 in this row
       const float4 pc4 = sB4[m][r + 2][uG[q]];
-      const float4 v = relax4(pc4, br[q], sA4[m][r + 1], sA4[m][r + 2], sA4[m][r],
+      const float4 rr4 = FIRST ? sub4(br[q], rsh) : br[q];
+      const float4 v = relax4(pc4, rr4, sA4[m][r + 1], sA4[m][r + 2], sA4[m][r],
                               sA4[u][r + 1], sA4[dn][r + 1], uG[q], f, k, cn1, c4l, c4s, INT);
       const int o = uOff[q] + lev(k);
+      if constexpr (FIRST) *reinterpret_cast<float4*>(io.rout[0] + o) = rr4;
       *reinterpret_cast<float4*>(io.pout[0] + o) = v;
       *reinterpret_cast<float4*>(io.pout[1] + o) = sA4[m][r + 1][uG[q]];
     }
@@ -463,23 +482,31 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
 
 int sor_iter_supported(Dims d, const SplitSOR& sp) { return sp.uniform && sp.nr <= SOR_ITER_NRMAX; }
 
-void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
-                     DevScalars* sc, SorIterShape shape) {
-  const int kc = shape.kc > 0 ? min(shape.kc, SOR_ITER_KCMAX) : 10;
+namespace {
+template <int TJ, int NT>
+void launch_sor_iter_tj(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
+                        DevScalars* sc, int kc, int first) {
   const int nkt = (d.kp + kc - 1) / kc;
-  if (shape.tj == 4) {
-    const int njt = (d.jp + 3) / 4;
-    const int nwg = (njt * nkt + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_sor_iter<4, 256>), dim3(nwg), dim3(256), 0, st, d, g, sp, ph, io, sc, kc, njt);
-  } else if (shape.tj == 16) {
-    const int njt = (d.jp + 15) / 16;
-    const int nwg = (njt * nkt + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_sor_iter<16, 512>), dim3(nwg), dim3(512), 0, st, d, g, sp, ph, io, sc, kc, njt);
-  } else {
-    const int njt = (d.jp + 7) / 8;
-    const int nwg = (njt * nkt + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_sor_iter<8, 512>), dim3(nwg), dim3(512), 0, st, d, g, sp, ph, io, sc, kc, njt);
-  }
+  const int njt = (d.jp + TJ - 1) / TJ;
+  const int nwg = (njt * nkt + 7) / 8 * 8;
+  if (first)
+    hipLaunchKernelGGL((k_sor_iter<TJ, NT, true>), dim3(nwg), dim3(NT), 0, st, d, g, sp, ph, io,
+                       sc, kc, njt);
+  else
+    hipLaunchKernelGGL((k_sor_iter<TJ, NT, false>), dim3(nwg), dim3(NT), 0, st, d, g, sp, ph, io,
+                       sc, kc, njt);
+}
+}  // namespace
+
+void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
+                     DevScalars* sc, SorIterShape shape, int first) {
+  const int kc = shape.kc > 0 ? min(shape.kc, SOR_ITER_KCMAX) : 10;
+  if (shape.tj == 4)
+    launch_sor_iter_tj<4, 256>(st, d, g, sp, ph, io, sc, kc, first);
+  else if (shape.tj == 16)
+    launch_sor_iter_tj<16, 512>(st, d, g, sp, ph, io, sc, kc, first);
+  else
+    launch_sor_iter_tj<8, 512>(st, d, g, sp, ph, io, sc, kc, first);
 }
 
 namespace {
@@ -546,7 +573,8 @@ void launch_sor_split_tiles(hipStream_t st, Dims d, const SorTiles& T, Physics p
 // thread takes one group of four split entries of row j in both colours over a chunk of k
 // levels (float4 loads along the split rows); padding entries (i > ip) are skipped.
 __global__ __launch_bounds__(256) void k_pav_split(Dims d, Grid1D g, SplitSOR sp,
-                                                   int64_t* partials, int ng, int kchunk) {
+                                                   int64_t* partials, int ng, int kchunk,
+                                                   int64_t* slots) {
   __shared__ unsigned long long lds[NLIMB];
   block_limbs_init(lds);
   const int t = blockIdx.x * 256 + threadIdx.x;
@@ -575,13 +603,24 @@ __global__ __launch_bounds__(256) void k_pav_split(Dims d, Grid1D g, SplitSOR sp
     }
   }
   acc.flush(lds);
-  block_limbs_store(lds, partials + (long)(blockIdx.x + gridDim.x * blockIdx.y) * NLIMB);
+  const int blk = blockIdx.x + gridDim.x * blockIdx.y;
+  if (slots)
+    block_limbs_atomic(lds, slots, blk);
+  else
+    block_limbs_store(lds, partials + (long)blk * NLIMB);
 }
 
 __global__ __launch_bounds__(FIN_THREADS) void k_pav_split_finalize(const int64_t* partials,
                                                                     int nb, DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
   sum_partials(partials, nb, limbs);
+  if (threadIdx.x == 0) sc->pav = limbs_to_float(limbs) / sc->pco;
+}
+
+__global__ __launch_bounds__(RED_SLOTS * NLIMB) void k_pav_split_finalize_slots(int64_t* slots,
+                                                                               DevScalars* sc) {
+  __shared__ int64_t limbs[NLIMB];
+  sum_slots(slots, limbs);
   if (threadIdx.x == 0) sc->pav = limbs_to_float(limbs) / sc->pco;
 }
 
@@ -593,9 +632,13 @@ void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, 
   const int kchunk = (d.kp + nkc - 1) / nkc;
   nkc = (d.kp + kchunk - 1) / kchunk;
   hipLaunchKernelGGL(k_pav_split, dim3(nbx, nkc), dim3(256), 0, st, d, g, sp, rb.partials, ng,
-                     kchunk);
-  hipLaunchKernelGGL(k_pav_split_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials,
-                     nbx * nkc, sc);
+                     kchunk, rb.slots);
+  if (rb.slots)
+    hipLaunchKernelGGL(k_pav_split_finalize_slots, dim3(1), dim3(RED_SLOTS * NLIMB), 0, st,
+                       rb.slots, sc);
+  else
+    hipLaunchKernelGGL(k_pav_split_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials,
+                       nbx * nkc, sc);
 }
 
 // ---- face exchange of the split copies of the tiles of one cut domain (after a colour
@@ -644,10 +687,12 @@ void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T) {
 // (press.f95:135-146): every halo cell takes its mirror interior cell, M(0)=1, M(ip+1)=ip in i,
 // M(0)=jp, M(jp+1)=1 in j, M(0)=1, M(kp+1)=kp in k, as boundp1 then boundp2 leave them ----
 __global__ __launch_bounds__(256) void k_split_gather_adj(Dims d, Fields F, SplitSOR sp,
-                                                          const DevScalars* sc, int subtract_pav) {
+                                                          DevScalars* sc, int subtract_pav,
+                                                          int set_carry) {
   const int a = blockIdx.x * 64 + threadIdx.x;
   const int b = blockIdx.y * 4 + threadIdx.y;
   const int c0 = 2 * blockIdx.z;  // levels c0 and c0 + 1: both loads before the stores
+  if (set_carry && a == 0 && b == 0 && c0 == 0) sc->pav_carry = sc->pav;
   if (a > d.ip + 1 || b > d.jp + 1) return;
   const int i = (a == 0) ? 1 : ((a == d.ip + 1) ? d.ip : a);
   const int j = (b == 0) ? d.jp : ((b == d.jp + 1) ? 1 : b);
@@ -668,9 +713,25 @@ __global__ __launch_bounds__(256) void k_split_gather_adj(Dims d, Fields F, Spli
 }
 
 void launch_split_gather_adj(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
-                             int subtract_pav) {
+                             int subtract_pav, int set_carry) {
   dim3 grid((d.ip + 2 + 63) / 64, (d.jp + 2 + 3) / 4, (d.kp + 3) / 2);
-  hipLaunchKernelGGL(k_split_gather_adj, grid, dim3(64, 4), 0, st, d, F, sp, sc, subtract_pav);
+  hipLaunchKernelGGL(k_split_gather_adj, grid, dim3(64, 4), 0, st, d, F, sp, sc, subtract_pav,
+                     set_carry);
+}
+
+// F.rhs over the interior from the split rhs arrays sp.r (the caller passes the resident
+// solve's adjusted rhs there)
+__global__ __launch_bounds__(256) void k_split_gather_rhs(Dims d, Fields F, SplitSOR sp) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.ip || j > d.jp) return;
+  F.rhs[d.R(i, j, k)] = sp.r[(i + j + k) & 1][split_index(sp, i, j, k)];
+}
+
+void launch_split_gather_rhs(hipStream_t st, Dims d, Fields F, SplitSOR sp) {
+  dim3 grid((d.ip + 63) / 64, (d.jp + 3) / 4, d.kp);
+  hipLaunchKernelGGL(k_split_gather_rhs, grid, dim3(64, 4), 0, st, d, F, sp);
 }
 
 // ---- transposes of the colour-split solve (the adjoint form's FP32 reverse) ----

@@ -330,3 +330,48 @@ def test_aveflow_statistics_bitwise(case):
     got = L.aveflow_stats()
     bad = [common.mismatch_report(got[n], st[n], n) for n in st if not common.bits_equal(got[n], st[n])]
     assert not bad, "\n".join(bad)
+
+
+def test_resident_split_solve_interleavings(case):
+    """The resident split solve (fused iterations: the split copy of p carried from step to
+    step, rhs kept in the split layout and copied to F.rhs only when read) against the oracle
+    through every way of leaving and re-entering it: graph replays, eager steps, reads of rhs
+    between steps, an upload of p, a lone SOR colour, and p handed out by address (which ends
+    the resident solve for the handle)."""
+    L, o = _pair(case, **common.CFG_SOR)
+    L.set_sor_tile(4, 5)
+    L.ifdata()
+    o.ifdata()
+    names = ["u", "v", "w", "p", "rhs", "sm", "f", "usum"]
+
+    def advance(n0, k, eager=False):
+        if eager:
+            for n in range(n0, n0 + k):
+                L.step(n)
+        else:
+            L.run(n0, k)
+        for n in range(n0, n0 + k):
+            o.step(n)
+
+    advance(1, 4)                      # n = 1 eagerly, a scattering step, graph replays
+    _compare(L, o, "resident: first steps", names)
+    advance(5, 2, eager=True)          # eager steps on the carried copy
+    _compare(L, o, "resident: eager steps", names)
+    advance(7, 3)                      # replays again after rhs was read
+    _compare(L, o, "resident: replays after a read of rhs", names)
+    ip, jp, kp = case["dims"]
+    pv = fl.FArray("p", ip, jp, kp, data=o.a["p"].copy())
+    pv[5, 4, 3] = pv[5, 4, 3] + np.float32(0.25)  # an interior value away from the faces
+    p = pv.flat
+    L.upload("p", p)
+    o.set("p", p)
+    advance(10, 3)                     # the upload makes the next solve scatter again
+    _compare(L, o, "resident: after an upload of p", names)
+    L.press_sor(0)                     # a lone colour sweep on F.p, then steps
+    o.call("sor_colour", 0)
+    o.call("boundp1")
+    advance(13, 2)
+    _compare(L, o, "resident: after a lone colour sweep", names)
+    L.device_ptr("p")                  # p handed out: the handle leaves the resident solve
+    advance(15, 3)
+    _compare(L, o, "after p was handed out", names)
