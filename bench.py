@@ -378,6 +378,9 @@ def main():
         roofline["compulsory_achieved"] = round(ach_c, 1)
         roofline["compulsory_frac"] = round(ach_c / HBM_PEAK_GBS, 4)
     step_hbm_frac = B_STEP * cells / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    autotuned = {"sor_tile": [tuning["sor_tile_j"], tuning["sor_tile_k"]]}
+    if "stencil_shapes" in tuning:
+        autotuned["stencil_shapes"] = tuning["stencil_shapes"]
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "M cell-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -388,7 +391,7 @@ def main():
                        "grid": list(GRID), "sor_iters": 50, "omega": 1.5, "dt": 0.05,
                        "parallelism": "single domain, 1 GPU"},
             "step_hbm_frac": round(step_hbm_frac, 4), "step_bytes_per_cell": B_STEP,
-            "roofline": roofline}
+            "autotuned": autotuned, "roofline": roofline}
     if args.tiled:
         line["tiled"] = tiled_bench(args.tiled, case)
     if args.adjoint:

@@ -377,12 +377,19 @@ int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms);
 /* Autotuner (no counterpart in the reference, whose OpenCL launch shapes are fixed): times one
  * SOR iteration without residual as two colour sweeps and as the fused launch with every
  * workgroup tile (rows of j x levels of k), 'reps' iterations each, on this handle's grid, and
- * keeps the fastest for later solves (untuned handles use the two sweeps).  Changes no
- * simulation state.  *out (may be NULL) receives the choice. */
+ * keeps the fastest for later solves (untuned handles use the two sweeps); then times the
+ * velnw, calc_sm and viscous-term kernels with each of their workgroup shapes (64x4, 128x2,
+ * 32x8, 256x1) and keeps the fastest of each.  Changes no simulation state (the arrays the
+ * stencil trials update are saved and restored).  *out (may be NULL) receives the choice. */
 typedef struct les_tuning {
   int sor_tile_j, sor_tile_k;  /* the fused tile, or 0, 0 for the two colour sweeps */
   float sor_iter_ms;           /* average time of one iteration with that choice, or -1 when
                                   the grid allows neither (odd jp, ip > 320 or non-uniform) */
+  /* workgroup shapes (lanes along i, rows of j) chosen for the point-wise stencil kernels
+   * [0] velnw, [1] les calc_sm (with boundsm), [2] les viscous terms + adam, and the average
+   * time of one launch with that shape */
+  int stencil_bx[3], stencil_by[3];
+  float stencil_ms[3];
 } les_tuning;
 int les_autotune(les_handle* h, int reps, les_tuning* out);
 /* Sets the SOR iteration form by hand: tile_j = 0 for two colour sweeps, else the fused launch

@@ -34,7 +34,8 @@ class LesScalars(ctypes.Structure):
 
 
 class LesTuning(ctypes.Structure):
-    _fields_ = [("sor_tile_j", c_int), ("sor_tile_k", c_int), ("sor_iter_ms", c_float)]
+    _fields_ = [("sor_tile_j", c_int), ("sor_tile_k", c_int), ("sor_iter_ms", c_float),
+                ("stencil_bx", c_int * 3), ("stencil_by", c_int * 3), ("stencil_ms", c_float * 3)]
 
 
 # every symbol declared in include/les_hip.h: name -> (restype, argtypes)

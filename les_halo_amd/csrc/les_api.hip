@@ -68,6 +68,8 @@ struct les_handle {
   bool rhs_stale = false;     // F.rhs's interior is behind ra
   bool graph_resident = false;  // the captured step is a resident solve (its replays leave
                                 // sp_p_current and rhs_stale set)
+  // workgroup shapes of the point-wise stencil kernels (les_autotune; 0 = 64 x 4)
+  int shp_velnw = 0, shp_sm = 0, shp_visc = 0;
 };
 
 namespace {
@@ -732,19 +734,19 @@ void enqueue_resid(les_handle* h, const Dims& d, int nblocks, const Physics& ph)
 
 extern "C" int les_velnw(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 0);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 0, h->shp_velnw);
   return check_launch(h);
 }
 
 extern "C" int les_bondv1_init_uvw(les_handle* h, int n) {
   STAGE_BEGIN(h);
-  launch_velnw_inflow(h->st, d, g, F, h->ph, 0, n < 1 ? 2 : n);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 0, n < 1 ? 2 : n, h->shp_velnw);
   return check_launch(h);
 }
 
 extern "C" int les_velnw__bondv1_init_uvw(les_handle* h, int n) {
   STAGE_BEGIN(h);
-  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n, h->shp_velnw);
   return check_launch(h);
 }
 
@@ -774,14 +776,14 @@ extern "C" int les_feedbf(les_handle* h) {
 
 extern "C" int les_les_calc_sm(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_calc_sm(h->st, d, g, F, h->ph);
+  launch_calc_sm(h->st, d, g, F, h->ph, 0, h->shp_sm);
   return check_launch(h);
 }
 
 extern "C" int les_velfg__feedbf__les_calc_sm(les_handle* h) {
   STAGE_BEGIN(h);
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
-  launch_calc_sm(h->st, d, g, F, h->ph);
+  launch_calc_sm(h->st, d, g, F, h->ph, 0, h->shp_sm);
   return check_launch(h);
 }
 
@@ -793,26 +795,26 @@ extern "C" int les_les_bound_sm(les_handle* h) {
 
 extern "C" int les_les_calc_visc(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_calc_visc_adam(h->st, d, g, F, 1, 0);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 0, h->shp_visc);
   return check_launch(h);
 }
 
 extern "C" int les_adam(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_calc_visc_adam(h->st, d, g, F, 0, 1);
+  launch_calc_visc_adam(h->st, d, g, F, 0, 1, h->shp_visc);
   return check_launch(h);
 }
 
 extern "C" int les_les_calc_visc__adam(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_calc_visc_adam(h->st, d, g, F, 1, 1);
+  launch_calc_visc_adam(h->st, d, g, F, 1, 1, h->shp_visc);
   return check_launch(h);
 }
 
 extern "C" int les_les(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_calc_sm(h->st, d, g, F, h->ph, 1);  // boundsm's halo cells included
-  launch_calc_visc_adam(h->st, d, g, F, 1, 0);
+  launch_calc_sm(h->st, d, g, F, h->ph, 1, h->shp_sm);  // boundsm's halo cells included
+  launch_calc_visc_adam(h->st, d, g, F, 1, 0, h->shp_visc);
   return check_launch(h);
 }
 
@@ -1100,12 +1102,12 @@ int enqueue_press(les_handle* h) {
 
 int enqueue_step(les_handle* h, int n) {
   STAGE_BEGIN(h);
-  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 1, n < 1 ? 2 : n, h->shp_velnw);
   launch_uout(h->st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
   launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
-  launch_calc_sm(h->st, d, g, F, h->ph, 1);  // boundsm's halo cells included
-  launch_calc_visc_adam(h->st, d, g, F, 1, 1);
+  launch_calc_sm(h->st, d, g, F, h->ph, 1, h->shp_sm);  // boundsm's halo cells included
+  launch_calc_visc_adam(h->st, d, g, F, 1, 1, h->shp_visc);
   return enqueue_press(h);
 }
 }  // namespace
@@ -1305,15 +1307,15 @@ extern "C" int les_ensemble_sync(les_ensemble* e) {
 
 extern "C" int les_ifdata(les_handle* h) {
   STAGE_BEGIN(h);
-  launch_velnw_inflow(h->st, d, g, F, h->ph, 0, 1);
+  launch_velnw_inflow(h->st, d, g, F, h->ph, 0, 1, h->shp_velnw);
   launch_uout(h->st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
   launch_bondv1_uvw(h->st, d, g, F, h->ph, h->sc);
   launch_boundp1(h->st, d, F, h->sc, 0);
   launch_boundp2(h->st, d, F, h->sc, 0);
   h->p_consistent = true;
   launch_velfg(h->st, d, g, F, h->ph, 1, 1);
-  launch_calc_sm(h->st, d, g, F, h->ph, 1);  // boundsm's halo cells included
-  launch_calc_visc_adam(h->st, d, g, F, 1, 0);
+  launch_calc_sm(h->st, d, g, F, h->ph, 1, h->shp_sm);  // boundsm's halo cells included
+  launch_calc_visc_adam(h->st, d, g, F, 1, 0, h->shp_visc);
   return check_launch(h);
 }
 
@@ -1467,10 +1469,10 @@ extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms
           launch_sor_colour(h->st, d, g, F, h->ph, q & 1, 0, h->rb, 0, h->sc);
         break;
       case LES_KERNEL_VELFG: launch_velfg(h->st, d, g, F, h->ph, 1, 1); break;
-      case LES_KERNEL_VISC_ADAM: launch_calc_visc_adam(h->st, d, g, F, 1, 1); break;
-      case LES_KERNEL_CALC_SM: launch_calc_sm(h->st, d, g, F, h->ph); break;
+      case LES_KERNEL_VISC_ADAM: launch_calc_visc_adam(h->st, d, g, F, 1, 1, h->shp_visc); break;
+      case LES_KERNEL_CALC_SM: launch_calc_sm(h->st, d, g, F, h->ph, 0, h->shp_sm); break;
       case LES_KERNEL_RHS: launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc); break;
-      case LES_KERNEL_VELNW: launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 2); break;
+      case LES_KERNEL_VELNW: launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 2, h->shp_velnw); break;
       case LES_KERNEL_STEP: HIPCHK(h, hipGraphLaunch(h->gexec, h->st)); break;
       case LES_KERNEL_SOR_ITER: {
         if (!sor_iter_supported(d, h->sp)) {
@@ -1519,6 +1521,86 @@ extern "C" int les_set_sor_tile(les_handle* h, int tile_j, int tile_k) {
 // workgroup tile on this handle's grid (the split copy is scratch between solves, so the trial
 // launches change no simulation state) and keeps the fastest; the captured step graph is
 // rebuilt with it.
+namespace {
+// The workgroup shape of each point-wise stencil kernel: every shape timed over 'reps' launches
+// from the same state (the arrays the kernels update are saved first and put back after each
+// trial and at the end).  Without memory for the copy the shapes stay as they are.
+int tune_stencils(les_handle* h, int reps, les_tuning& best) {
+  STAGE_BEGIN(h);
+  const int saved[] = {LES_U, LES_V, LES_W, LES_F, LES_G, LES_H, LES_FOLD, LES_GOLD, LES_HOLD,
+                       LES_SM};
+  size_t tot = 0;
+  for (int f : saved) tot += (size_t)h->cnt[f];
+  int* shp[3] = {&h->shp_velnw, &h->shp_sm, &h->shp_visc};
+  for (int q = 0; q < 3; ++q) {
+    best.stencil_bx[q] = stencil_shape_x(*shp[q]);
+    best.stencil_by[q] = stencil_shape_y(*shp[q]);
+    best.stencil_ms[q] = -1.0f;
+  }
+  float* keep = nullptr;
+  if (hipMalloc(&keep, tot * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    return LES_OK;
+  }
+  auto copy_all = [&](bool to_keep) {
+    size_t off = 0;
+    for (int f : saved) {
+      const size_t n = (size_t)h->cnt[f] * sizeof(float);
+      if (to_keep)
+        (void)hipMemcpyAsync(keep + off, h->fld[f], n, hipMemcpyDeviceToDevice, h->st);
+      else
+        (void)hipMemcpyAsync(h->fld[f], keep + off, n, hipMemcpyDeviceToDevice, h->st);
+      off += (size_t)h->cnt[f];
+    }
+  };
+  auto launch = [&](int q, int shape) {
+    if (q == 0) launch_velnw_inflow(h->st, d, g, F, h->ph, 1, 2, shape);
+    if (q == 1) launch_calc_sm(h->st, d, g, F, h->ph, 1, shape);
+    if (q == 2) launch_calc_visc_adam(h->st, d, g, F, 1, 1, shape);
+  };
+  hipEvent_t e0, e1;
+  int r = LES_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    r = fail(h, LES_E_HIP, "autotune: event");
+  } else {
+    copy_all(true);
+    for (int q = 0; q < 3 && !r; ++q) {
+      float best_ms = -1.0f;
+      int best_shape = *shp[q];
+      for (int shape = 0; shape < STENCIL_SHAPES; ++shape) {
+        copy_all(false);
+        launch(q, shape);  // warm-up
+        (void)hipEventRecord(e0, h->st);
+        for (int t = 0; t < reps; ++t) launch(q, shape);
+        (void)hipEventRecord(e1, h->st);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          r = fail(h, LES_E_HIP, "autotune: stencil trial");
+          break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (best_ms < 0.0f || ms < best_ms) {
+          best_ms = ms;
+          best_shape = shape;
+        }
+      }
+      *shp[q] = best_shape;
+      best.stencil_bx[q] = stencil_shape_x(best_shape);
+      best.stencil_by[q] = stencil_shape_y(best_shape);
+      best.stencil_ms[q] = best_ms;
+    }
+    copy_all(false);
+    (void)hipStreamSynchronize(h->st);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(keep);
+  h->graph_valid = false;
+  return r ? r : check_launch(h);
+}
+}  // namespace
+
 extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
   if (!h || reps < 1) return LES_E_INVALID;
   STAGE_BEGIN(h);
@@ -1575,6 +1657,10 @@ extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
     hipEventDestroy(e1);
     h->iter_shape = SorIterShape{best.sor_tile_j, best.sor_tile_k};
     h->graph_valid = false;
+  }
+  {
+    int r = tune_stencils(h, reps, best);
+    if (r) return r;
   }
   h->tuned = true;
   if (out) *out = best;

@@ -139,18 +139,26 @@ void launch_pav_finalize(hipStream_t st, const int64_t* partials, int nb, DevSca
 void launch_boundsm_tiled(hipStream_t st, Dims d, Fields F, Edges e, int phase);
 
 // ---- launchers (les_kernels.hip); all asynchronous on 'st' ----
+// Workgroup shapes (lanes along i x rows of j, 256 threads) the autotuner chooses from for the
+// point-wise stencil kernels (velnw, calc_sm, visc+adam); shape 0 is the untuned default.
+constexpr int STENCIL_SHAPES = 4;
+__host__ __device__ inline int stencil_shape_x(int s) {
+  return s == 1 ? 128 : (s == 2 ? 32 : (s == 3 ? 256 : 64));
+}
+__host__ __device__ inline int stencil_shape_y(int s) { return 256 / stencil_shape_x(s); }
 void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
-                         int inflow_n);
+                         int inflow_n, int shape = 0);
 void launch_uout(hipStream_t st, Dims d, Fields F, DevScalars* sc, float* part);
 void launch_bondv1_uvw(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, DevScalars* sc,
                        Edges e = EDGES_ALL, int do_outflow = 1);
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf, Edges e = EDGES_ALL);
 // with_bound: also boundsm.f95's halo cells, as k_boundsm would set them afterwards
-void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound = 0);
+void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound = 0,
+                    int shape = 0);
 void launch_boundsm(hipStream_t st, Dims d, Fields F);
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc,
-                           int do_adam);
+                           int do_adam, int shape = 0);
 // split != nullptr: the raw rhs goes to the split copy's rhs arrays, not to F.rhs
 void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf rb,
                 DevScalars* sc, Edges e = EDGES_ALL, int finalize = 1,

@@ -41,8 +41,8 @@ __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F
                                                       StripGrid sg) {
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;
-  const int i = bx * BX + threadIdx.x;
-  const int j = 1 + by * BY + threadIdx.y;
+  const int i = bx * (int)blockDim.x + threadIdx.x;
+  const int j = 1 + by * (int)blockDim.y + threadIdx.y;
   const int k0 = 1 + bz * VN_KPT;
   if (i > imax || j > d.jp) return;
   if (inflow_n > 0 && (i <= 1 || inflow_n == 1)) {
@@ -95,15 +95,16 @@ __global__ __launch_bounds__(256) void k_velnw_inflow(Dims d, Grid1D g, Fields F
 }
 
 void launch_velnw_inflow(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velnw,
-                         int inflow_n) {
+                         int inflow_n, int shape) {
   int imax = (do_velnw || inflow_n == 1) ? d.ip : 1;
-  const StripGrid sg = strip_grid((imax + 1 + BX - 1) / BX, (d.jp + BY - 1) / BY,
+  const int sx = stencil_shape_x(shape), sy = stencil_shape_y(shape);
+  const StripGrid sg = strip_grid((imax + 1 + sx - 1) / sx, (d.jp + sy - 1) / sy,
                                   (d.kp + VN_KPT - 1) / VN_KPT);
   if (g.p2xy)
-    hipLaunchKernelGGL(k_velnw_inflow<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
+    hipLaunchKernelGGL(k_velnw_inflow<true>, strip_dim(sg), dim3(sx, sy), 0, st, d, g, F, ph,
                        do_velnw, inflow_n, imax, sg);
   else
-    hipLaunchKernelGGL(k_velnw_inflow<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph,
+    hipLaunchKernelGGL(k_velnw_inflow<false>, strip_dim(sg), dim3(sx, sy), 0, st, d, g, F, ph,
                        do_velnw, inflow_n, imax, sg);
 }
 
@@ -618,8 +619,8 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
                                                  StripGrid sg, int with_bound) {
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;
-  const int i = 1 + bx * BX + threadIdx.x;
-  const int j = 1 + by * BY + threadIdx.y;
+  const int i = 1 + bx * (int)blockDim.x + threadIdx.x;
+  const int j = 1 + by * (int)blockDim.y + threadIdx.y;
   const int k0 = 1 + bz * SM_KPT;
   if (i > d.ip || j > d.jp) return;
   SmIn v[SM_KPT];
@@ -636,10 +637,12 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
   }
 }
 
-void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound) {
-  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY,
+void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound,
+                    int shape) {
+  const int sx = stencil_shape_x(shape), sy = stencil_shape_y(shape);
+  const StripGrid sg = strip_grid((d.ip + sx - 1) / sx, (d.jp + sy - 1) / sy,
                                   (d.kp + SM_KPT - 1) / SM_KPT);
-  hipLaunchKernelGGL(k_calc_sm, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, ph, sg, with_bound);
+  hipLaunchKernelGGL(k_calc_sm, strip_dim(sg), dim3(sx, sy), 0, st, d, g, F, ph, sg, with_bound);
 }
 
 // ==========================================================================================
@@ -815,8 +818,8 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
                                                    int do_adam, StripGrid sg) {
   int bx, by, bz, L;
   if (!strip_block(sg, bx, by, bz, L)) return;
-  const int i = 1 + bx * BX + threadIdx.x;
-  const int j = 1 + by * BY + threadIdx.y;
+  const int i = 1 + bx * (int)blockDim.x + threadIdx.x;
+  const int j = 1 + by * (int)blockDim.y + threadIdx.y;
   const int k = 1 + bz;
   if (i > d.ip || j > d.jp) return;
   const long ifg = d.F(i, j, k);
@@ -833,13 +836,15 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
   bh.st(ifg, hv);
 }
 
-void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam) {
-  const StripGrid sg = strip_grid((d.ip + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
+void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam,
+                           int shape) {
+  const int sx = stencil_shape_x(shape), sy = stencil_shape_y(shape);
+  const StripGrid sg = strip_grid((d.ip + sx - 1) / sx, (d.jp + sy - 1) / sy, d.kp);
   if (g.p2xy)
-    hipLaunchKernelGGL(k_visc_adam<true>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, do_visc,
+    hipLaunchKernelGGL(k_visc_adam<true>, strip_dim(sg), dim3(sx, sy), 0, st, d, g, F, do_visc,
                        do_adam, sg);
   else
-    hipLaunchKernelGGL(k_visc_adam<false>, strip_dim(sg), dim3(BX, BY), 0, st, d, g, F, do_visc,
+    hipLaunchKernelGGL(k_visc_adam<false>, strip_dim(sg), dim3(sx, sy), 0, st, d, g, F, do_visc,
                        do_adam, sg);
 }
 

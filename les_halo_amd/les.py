@@ -276,13 +276,16 @@ class LES:
                "step": 6, "sor_iter": 7}
 
     def autotune(self, reps=20):
-        """Picks the fused SOR iteration's workgroup tile by timing every candidate on this
-        grid (les_autotune); returns the choice."""
+        """Picks the fused SOR iteration's workgroup tile and the stencil kernels' workgroup
+        shapes by timing every candidate on this grid (les_autotune); returns the choice."""
         from ._lib import LesTuning
         t = LesTuning()
         self._chk(lib().les_autotune(self.h, reps, ctypes.byref(t)), "autotune")
+        names = ("velnw", "calc_sm", "visc_adam")
         return {"sor_tile_j": t.sor_tile_j, "sor_tile_k": t.sor_tile_k,
-                "sor_iter_ms": float(t.sor_iter_ms)}
+                "sor_iter_ms": float(t.sor_iter_ms),
+                "stencil_shapes": {n: [t.stencil_bx[q], t.stencil_by[q]] for q, n in enumerate(names)},
+                "stencil_ms": {n: float(t.stencil_ms[q]) for q, n in enumerate(names)}}
 
     def set_sor_tile(self, tile_j, tile_k=0):
         """SOR iteration form: tile_j = 0 two colour sweeps, else the fused launch's tile."""
