@@ -219,6 +219,20 @@ int les_ensemble_create(les_handle* const* members, int n, les_ensemble** out);
 int les_ensemble_run(les_ensemble* e, int n0, int nsteps);
 int les_ensemble_sync(les_ensemble* e);
 int les_ensemble_destroy(les_ensemble* e);
+/* LES_ENSEMBLE_BATCHED (the default): when every member allows it (FP32, exact sums, no SOR
+ * early exit, an even nmaxp, a tuned fused SOR tile, even jp, the same kind of horizontal grid,
+ * at most 256 members, p and rhs never handed out by address), each les_ensemble_run step is
+ * one captured graph on the ensemble's own stream in which velfg + feedbf, calc_sm + boundsm,
+ * the viscous terms + adam and every fused SOR iteration are one launch each over the
+ * workgroups of all members (a member table per stage deals the launch's workgroups to the
+ * members; the fused SOR runs 4-row tiles with each member's tuned depth; calc_sm and visc use
+ * the workgroup shape most members' autotuning chose), and the other stages are launched per
+ * member inside the same graph.  Members must then outlive the ensemble.  Otherwise, and with
+ * LES_ENSEMBLE_CONCURRENT, every member replays its own graph on its own stream.  The results
+ * are the same bit for bit.  les_ensemble_batched returns 1 when the last run was batched. */
+enum { LES_ENSEMBLE_CONCURRENT = 0, LES_ENSEMBLE_BATCHED = 1 };
+int les_ensemble_set_mode(les_ensemble* e, int mode);
+int les_ensemble_batched(const les_ensemble* e);
 /* ifdata.f95:258-273 on the device: bondv1 (inflow from wind_profile, n=1), boundp1,
  * boundp2, velfg, feedbf, les.  Fields must be zero (les_create) or uploaded. */
 int les_ifdata(les_handle* h);

@@ -126,6 +126,8 @@ PROTOS = {
     "les_autotune": (c_int, [H, c_int, ctypes.POINTER(LesTuning)]),
     "les_set_sor_tile": (c_int, [H, c_int, c_int]),
     "les_ensemble_create": (c_int, [ctypes.POINTER(c_void_p), c_int, ctypes.POINTER(c_void_p)]),
+    "les_ensemble_set_mode": (c_int, [c_void_p, c_int]),
+    "les_ensemble_batched": (c_int, [c_void_p]),
     "les_ensemble_run": (c_int, [c_void_p, c_int, c_int]),
     "les_ensemble_sync": (c_int, [c_void_p]),
     "les_ensemble_destroy": (c_int, [c_void_p]),

@@ -1254,7 +1254,260 @@ extern "C" int les_run(les_handle* h, int n0, int nsteps) {
 struct les_ensemble {
   std::vector<les_handle*> m;
   std::string err;
+  int mode = LES_ENSEMBLE_BATCHED;  // les_ensemble_set_mode
+  int last_batched = 0;             // the last les_ensemble_run used the batched launches
+  // the batched form: one stream and one captured step of every member
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  static constexpr int NAUX = 8;  // streams of the per-member segments inside the graph
+  hipStream_t aux[NAUX] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<int> sig;  // the members' launch settings the graph was captured with
+  BatchStencil* tab_st = nullptr;
+  BatchSor* tab_sor = nullptr;
+  uint8_t* wgm = nullptr;  // member of each logical workgroup: velfg | calc_sm | visc | sor
+  int tot[4] = {0, 0, 0, 0};
+  int shp_sm = 0, shp_visc = 0;
 };
+
+namespace {
+void batch_release(les_ensemble* e) {
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  if (e->st) hipStreamSynchronize(e->st);
+  if (e->tab_st) hipFree(e->tab_st);
+  if (e->tab_sor) hipFree(e->tab_sor);
+  if (e->wgm) hipFree(e->wgm);
+  e->tab_st = nullptr;
+  e->tab_sor = nullptr;
+  e->wgm = nullptr;
+  e->sig.clear();
+}
+
+// The batched step applies when every member takes the resident solve on the same kind of
+// horizontal grid (at most 256 members: the workgroup-to-member table holds bytes).
+bool batch_eligible(const les_ensemble* e) {
+  if (e->mode != LES_ENSEMBLE_BATCHED || e->m.size() > 256) return false;
+  const les_handle* h0 = e->m[0];
+  for (const les_handle* h : e->m) {
+    if (h->cfg.precision != LES_PREC_FP32 || h->cfg.sum_order != LES_SUM_EXACT) return false;
+    if (!h->resident_ok || h->drag || h->p2xy != h0->p2xy || h->d.jp % 2 || !h->cn1_separable ||
+        !h->split_ok || h->ph.pjuge > 0.0f || !sor_iter_supported(h->d, h->sp) ||
+        h->cfg.nmaxp < 2 || h->cfg.nmaxp % 2)
+      return false;
+    if (h->cfg.nmaxp != h0->cfg.nmaxp) return false;
+  }
+  return true;
+}
+
+std::vector<int> batch_signature(const les_ensemble* e) {
+  std::vector<int> v;
+  for (const les_handle* h : e->m) {
+    v.push_back(h->iter_shape.kc);
+    v.push_back(h->shp_velnw);
+    v.push_back(h->shp_sm);
+    v.push_back(h->shp_visc);
+    v.push_back(h->p2xy ? 1 : 0);
+  }
+  return v;
+}
+
+// the workgroup shape most members' autotuning chose for a kernel
+int majority_shape(const les_ensemble* e, int les_handle::*field) {
+  int cnt[STENCIL_SHAPES] = {};
+  for (const les_handle* h : e->m) cnt[h->*field] += 1;
+  int best = 0;
+  for (int q = 1; q < STENCIL_SHAPES; ++q)
+    if (cnt[q] > cnt[best]) best = q;
+  return best;
+}
+
+// Captures one step (n >= 2) of every member on the ensemble's stream: per member velnw and
+// bondv1; velfg + feedbf, calc_sm + boundsm and visc + adam as one launch each over all
+// members' workgroups; per member rhs and rhsav; the fused SOR iterations as one launch each
+// over all members' tiles (the resident solve); per member pav and the gather.
+int batch_capture(les_ensemble* e) {
+  batch_release(e);
+  const int nm = (int)e->m.size();
+  e->shp_sm = majority_shape(e, &les_handle::shp_sm);
+  e->shp_visc = majority_shape(e, &les_handle::shp_visc);
+  std::vector<BatchStencil> ts(nm);
+  std::vector<BatchSor> tq(nm);
+  std::vector<uint8_t> w[4];
+  for (int q = 0; q < nm; ++q) {
+    les_handle* h = e->m[q];
+    BatchStencil& a = ts[q];
+    a.d = h->d;
+    a.g = grid_of(h);
+    a.F = fields_of(h);
+    a.ph = h->ph;
+    a.sg_velfg = velfg_strip(h->d);
+    const int sx = stencil_shape_x(e->shp_sm), sy = stencil_shape_y(e->shp_sm);
+    a.sg_sm = strip_grid((h->d.ip + sx - 1) / sx, (h->d.jp + sy - 1) / sy, h->d.kp);
+    const int vx = stencil_shape_x(e->shp_visc), vy = stencil_shape_y(e->shp_visc);
+    a.sg_visc = strip_grid((h->d.ip + vx - 1) / vx, (h->d.jp + vy - 1) / vy, h->d.kp);
+    const StripGrid* sgs[3] = {&a.sg_velfg, &a.sg_sm, &a.sg_visc};
+    int* bases[3] = {&a.base_velfg, &a.base_sm, &a.base_visc};
+    for (int k = 0; k < 3; ++k) {
+      *bases[k] = (int)w[k].size();
+      w[k].insert(w[k].end(), (size_t)sgs[k]->nbx * sgs[k]->nby * sgs[k]->nbz, (uint8_t)q);
+    }
+    BatchSor& b = tq[q];
+    b.d = h->d;
+    b.g = grid_of(h);
+    b.sp = h->sp;
+    b.sra = h->sp;
+    b.sra.r[0] = h->ra[0];
+    b.sra.r[1] = h->ra[1];
+    b.ph = h->ph;
+    b.first = SorPair{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}, {h->ra[0], h->ra[1]}, 1};
+    b.fwd = SorPair{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}, {nullptr, nullptr}, 0};
+    b.back = SorPair{{h->palt[0], h->palt[1]}, {h->sp.p[0], h->sp.p[1]}, {nullptr, nullptr}, 0};
+    b.sc = h->sc;
+    b.kc = std::min(h->iter_shape.kc > 0 ? h->iter_shape.kc : 10, SOR_ITER_KCMAX);
+    b.njt = (h->d.jp + 3) / 4;
+    b.base = (int)w[3].size();
+    w[3].insert(w[3].end(), (size_t)b.njt * ((h->d.kp + b.kc - 1) / b.kc), (uint8_t)q);
+  }
+  size_t nw = 0;
+  for (int k = 0; k < 4; ++k) {
+    e->tot[k] = (int)w[k].size();
+    nw += w[k].size();
+  }
+  if (hipMalloc(&e->tab_st, nm * sizeof(BatchStencil)) != hipSuccess ||
+      hipMalloc(&e->tab_sor, nm * sizeof(BatchSor)) != hipSuccess ||
+      hipMalloc(&e->wgm, nw) != hipSuccess) {
+    batch_release(e);
+    return LES_E_NOMEM;
+  }
+  std::vector<uint8_t> all;
+  for (int k = 0; k < 4; ++k) all.insert(all.end(), w[k].begin(), w[k].end());
+  if (hipMemcpy(e->tab_st, ts.data(), nm * sizeof(BatchStencil), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->tab_sor, tq.data(), nm * sizeof(BatchSor), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->wgm, all.data(), nw, hipMemcpyHostToDevice) != hipSuccess) {
+    batch_release(e);
+    return LES_E_HIP;
+  }
+  const uint8_t* wg[4];
+  size_t off = 0;
+  for (int k = 0; k < 4; ++k) {
+    wg[k] = e->wgm + off;
+    off += w[k].size();
+  }
+  const int p2 = e->m[0]->p2xy ? 1 : 0;
+  for (int a = 0; a < les_ensemble::NAUX; ++a) {
+    if (!e->aux[a] && hipStreamCreateWithFlags(&e->aux[a], hipStreamNonBlocking) != hipSuccess)
+      return LES_E_HIP;
+    if (!e->join_ev[a] && hipEventCreateWithFlags(&e->join_ev[a], hipEventDisableTiming) != hipSuccess)
+      return LES_E_HIP;
+  }
+  if (!e->fork_ev && hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming) != hipSuccess)
+    return LES_E_HIP;
+  if (hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    batch_release(e);
+    return LES_E_HIP;
+  }
+  // The members' own (small) stages run as NAUX parallel chains of the graph: forked from the
+  // ensemble's stream, member q on chain q % NAUX, joined before the next batched launch.
+  auto per_member = [&](auto&& body) {
+    hipEventRecord(e->fork_ev, e->st);
+    for (int a = 0; a < les_ensemble::NAUX; ++a) hipStreamWaitEvent(e->aux[a], e->fork_ev, 0);
+    for (size_t q = 0; q < e->m.size(); ++q) body(e->m[q], e->aux[q % les_ensemble::NAUX]);
+    for (int a = 0; a < les_ensemble::NAUX; ++a) {
+      hipEventRecord(e->join_ev[a], e->aux[a]);
+      hipStreamWaitEvent(e->st, e->join_ev[a], 0);
+    }
+  };
+  per_member([&](les_handle* h, hipStream_t st) {
+    const Dims d = h->d;
+    const Grid1D g = grid_of(h);
+    const Fields F = fields_of(h);
+    launch_velnw_inflow(st, d, g, F, h->ph, 1, 2, h->shp_velnw);
+    launch_uout(st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
+    launch_bondv1_uvw(st, d, g, F, h->ph, h->sc);
+  });
+  launch_velfg_batch(e->st, e->tab_st, wg[0], e->tot[0], p2);
+  launch_calc_sm_batch(e->st, e->tab_st, wg[1], e->tot[1], e->shp_sm);
+  launch_visc_adam_batch(e->st, e->tab_st, wg[2], e->tot[2], e->shp_visc, p2);
+  per_member([&](les_handle* h, hipStream_t st) {
+    launch_rhs(st, h->d, grid_of(h), fields_of(h), h->ph, h->rb, h->sc, EDGES_ALL, 1, &h->sp);
+  });
+  launch_sor_iter_batch(e->st, e->tab_sor, wg[3], e->tot[3], 0);
+  launch_sor_iter_batch(e->st, e->tab_sor, wg[3], e->tot[3], 2);
+  for (int l = 3; l + 1 <= e->m[0]->cfg.nmaxp; l += 2) {
+    launch_sor_iter_batch(e->st, e->tab_sor, wg[3], e->tot[3], 1);
+    launch_sor_iter_batch(e->st, e->tab_sor, wg[3], e->tot[3], 2);
+  }
+  per_member([&](les_handle* h, hipStream_t st) {
+    launch_pav_split(st, h->d, grid_of(h), h->sp, h->rb, h->sc);
+    launch_split_gather_adj(st, h->d, fields_of(h), h->sp, h->sc, 1, 1);
+  });
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(e->st, &gr);
+  if (ec != hipSuccess || !gr) {
+    if (gr) hipGraphDestroy(gr);
+    batch_release(e);
+    e->err = std::string("batched capture: ") + hipGetErrorString(ec);
+    return LES_E_HIP;
+  }
+  e->graph = gr;
+  if (hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0) != hipSuccess) {
+    batch_release(e);
+    return LES_E_HIP;
+  }
+  e->sig = batch_signature(e);
+  return LES_OK;
+}
+
+// The batched form of les_ensemble_run: eager steps of every member in lockstep until each can
+// take the captured step (n >= 2, mirror-consistent p halos, a carried split copy), then the
+// ensemble's graph, ordered after the members' streams and before them again.
+int batch_run(les_ensemble* e, int n0, int nsteps) {
+  int r;
+  // the batched launches run every member's SOR as fused iterations: a member tuned to the
+  // two colour sweeps takes the fused form with the default depth (the same values)
+  for (les_handle* h : e->m)
+    if (h->iter_shape.tj <= 0) {
+      h->iter_shape = SorIterShape{4, 10};
+      h->graph_valid = false;
+    }
+  auto ready = [&](const les_handle* h) { return h->p_consistent && h->sp_p_current; };
+  while (nsteps > 0) {
+    bool all = n0 >= 2;
+    for (const les_handle* h : e->m) all = all && ready(h);
+    if (all) break;
+    for (les_handle* h : e->m)
+      if ((r = enqueue_step(h, n0))) return r;
+    ++n0;
+    --nsteps;
+  }
+  if (nsteps == 0) return LES_OK;
+  if (!e->st && hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) return LES_E_HIP;
+  if (!e->ev && hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess) return LES_E_HIP;
+  for (les_handle* h : e->m) {  // the ensemble's stream after every member's queued work
+    if (hipEventRecord(e->ev, h->st) != hipSuccess || hipStreamWaitEvent(e->st, e->ev, 0) != hipSuccess)
+      return LES_E_HIP;
+  }
+  if (!e->gexec || e->sig != batch_signature(e)) {
+    if ((r = batch_capture(e))) return r;
+  }
+  for (int s = 0; s < nsteps; ++s)
+    if (hipGraphLaunch(e->gexec, e->st) != hipSuccess) return LES_E_HIP;
+  // the members' state as the captured steps leave it; their streams after the ensemble's
+  if (hipEventRecord(e->ev, e->st) != hipSuccess) return LES_E_HIP;
+  for (les_handle* h : e->m) {
+    h->p_consistent = true;
+    h->sp_p_current = true;
+    h->rhs_stale = true;
+    if (hipStreamWaitEvent(h->st, e->ev, 0) != hipSuccess) return LES_E_HIP;
+  }
+  return LES_OK;
+}
+}  // namespace
 
 extern "C" int les_ensemble_create(les_handle* const* members, int n, les_ensemble** out) {
   if (!members || n < 1 || !out) return LES_E_INVALID;
@@ -1266,15 +1519,38 @@ extern "C" int les_ensemble_create(les_handle* const* members, int n, les_ensemb
   return LES_OK;
 }
 
+extern "C" int les_ensemble_set_mode(les_ensemble* e, int mode) {
+  if (!e || (mode != LES_ENSEMBLE_CONCURRENT && mode != LES_ENSEMBLE_BATCHED)) return LES_E_INVALID;
+  e->mode = mode;
+  return LES_OK;
+}
+
+extern "C" int les_ensemble_batched(const les_ensemble* e) { return e ? e->last_batched : 0; }
+
 extern "C" int les_ensemble_destroy(les_ensemble* e) {
+  if (!e) return LES_OK;
+  batch_release(e);
+  if (e->ev) hipEventDestroy(e->ev);
+  if (e->fork_ev) hipEventDestroy(e->fork_ev);
+  for (int a = 0; a < les_ensemble::NAUX; ++a) {
+    if (e->join_ev[a]) hipEventDestroy(e->join_ev[a]);
+    if (e->aux[a]) hipStreamDestroy(e->aux[a]);
+  }
+  if (e->st) hipStreamDestroy(e->st);
   delete e;
   return LES_OK;
 }
 
-// Every member's stream gets one graph launch per step, step-major, so the members' kernels
+// Batched (when every member allows it): one launch per stage over all members.  Otherwise
+// every member's stream gets one graph launch per step, step-major, so the members' kernels
 // are in flight together and small members fill the device between the large ones.
 extern "C" int les_ensemble_run(les_ensemble* e, int n0, int nsteps) {
   if (!e || nsteps < 0) return LES_E_INVALID;
+  if (batch_eligible(e)) {
+    e->last_batched = 1;
+    return batch_run(e, n0, nsteps);
+  }
+  e->last_batched = 0;
   const int nm = (int)e->m.size();
   std::vector<int> left(nm, nsteps);
   for (int q = 0; q < nm; ++q) {
@@ -1298,6 +1574,7 @@ extern "C" int les_ensemble_run(les_ensemble* e, int n0, int nsteps) {
 
 extern "C" int les_ensemble_sync(les_ensemble* e) {
   if (!e) return LES_E_INVALID;
+  if (e->st && hipStreamSynchronize(e->st) != hipSuccess) return LES_E_HIP;
   for (les_handle* h : e->m) {
     int r = les_sync(h);
     if (r) return r;

@@ -411,6 +411,21 @@ inline dim3 strip_dim(const StripGrid& s) {
   const long n = (long)s.nbx * s.nby * s.nbz;
   return dim3((unsigned)((n + 7) / 8 * 8));
 }
+// Block (bx, by, bz) of logical index L in the strip order; false past the grid's blocks.
+__device__ __forceinline__ bool strip_map(const StripGrid& s, int L, int& bx, int& by, int& bz) {
+  if (L >= s.nbx * s.nby * s.nbz) return false;
+  bx = L % s.nbx;
+  const int R = L / s.nbx;  // row tile, ordered (strip, bz, j within the strip)
+  const int per = s.sj * s.nbz;
+  const int nstrip = (s.nby + s.sj - 1) / s.sj;
+  const int st = min(R / per, nstrip - 1);
+  const int w = min(s.sj, s.nby - st * s.sj);
+  const int r = R - st * per;
+  bz = r / w;
+  by = st * s.sj + (r - bz * w);
+  return true;
+}
+
 // Logical block (bx, by, bz) of this workgroup and its logical index L; false for padding.
 __device__ __forceinline__ bool strip_block(const StripGrid& s, int& bx, int& by, int& bz,
                                             int& L) {

@@ -228,6 +228,43 @@ void launch_split_gather_t(hipStream_t st, Dims d, float* pb, SplitSOR sp);
 void launch_split_sweep_t(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, float omega, int c,
                           int mode);
 void launch_split_scatter_t(hipStream_t st, Dims d, float* pb, float* rb, SplitSOR sp);
+
+// ---- the batched ensemble (one launch per stage over every member's workgroups) ----
+// Per-member operands of the batched stencil launches, and each member's first logical
+// workgroup in the launch (members in order, their workgroups in each kernel's own order).
+struct BatchStencil {
+  Dims d;
+  Grid1D g;
+  Fields F;
+  Physics ph;
+  StripGrid sg_velfg, sg_sm, sg_visc;
+  int base_velfg, base_sm, base_visc;
+};
+// Per-member operands of the batched fused SOR iteration (TJ = 4, 256 threads): the first
+// iteration of the resident solve (reads sp's raw rhs) and the following ones (sra: the
+// adjusted rhs), in the two ping-pong directions.
+struct BatchSor {
+  Dims d;
+  Grid1D g;
+  SplitSOR sp, sra;
+  Physics ph;
+  SorPair first, fwd, back;
+  const DevScalars* sc;
+  int kc, njt, base;
+};
+// wgm[w]: the member of logical workgroup w (at most 256 members per batch)
+void launch_velfg_batch(hipStream_t st, const BatchStencil* tab, const uint8_t* wgm, int total,
+                        int p2xy);
+void launch_calc_sm_batch(hipStream_t st, const BatchStencil* tab, const uint8_t* wgm, int total,
+                          int shape);
+void launch_visc_adam_batch(hipStream_t st, const BatchStencil* tab, const uint8_t* wgm,
+                            int total, int shape, int p2xy);
+// which: 0 = first iteration, 1 = fwd, 2 = back
+void launch_sor_iter_batch(hipStream_t st, const BatchSor* tab, const uint8_t* wgm, int total,
+                           int which);
+// velfg's workgroup grid for a member (the strip order k_velfg_xy uses) and its block shape
+StripGrid velfg_strip(Dims d);
+constexpr int VELFG_THREADS = 512;
 int reduction_blocks(Dims d);
 
 // exact sum of n device floats into *out (device), for tests of the reduction itself

@@ -369,12 +369,14 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
 // thread.  Each value comes from the same formula at the same point as in k_velfg_feedbf, so
 // the results are bit-identical; the i-face and j-face work is halved.
 constexpr int VX = 64, VY = 8;
+// The body of one workgroup with logical index L (k_velfg_xy, and the ensemble's batched launch)
 template <bool P2>
-__global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Physics ph,
-                                                  int do_feedbf, StripGrid sg, Edges e) {
+__device__ __forceinline__ void velfg_xy_body(const Dims& d, const Grid1D& g, const Fields& F,
+                                              const Physics& ph, int do_feedbf,
+                                              const StripGrid& sg, const Edges& e, int L) {
   __shared__ float sy[6][VY][VX];  // cov2 diu2 cov5 diu5 cov8 diu8 of each row
-  int bx, by, bz, L;
-  if (!strip_block(sg, bx, by, bz, L)) return;
+  int bx, by, bz;
+  if (!strip_map(sg, L, bx, by, bz)) return;
   const int x = threadIdx.x, y = threadIdx.y;
   const int im = d.ip, jm = d.jp, km = d.kp;
   const int i = 1 + bx * (VX - 1) + x;
@@ -496,6 +498,16 @@ __global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Ph
   store_diu(d, F, i, j, k, x1, y2, z3, x4, y5, z6, x7, y8, z9, z3n, z6n, e);
 }
 
+template <bool P2>
+__global__ __launch_bounds__(512) void k_velfg_xy(Dims d, Grid1D g, Fields F, Physics ph,
+                                                  int do_feedbf, StripGrid sg, Edges e) {
+  velfg_xy_body<P2>(d, g, F, ph, do_feedbf, sg, e, xcd_tile((int)blockIdx.x, (int)gridDim.x));
+}
+
+StripGrid velfg_strip(Dims d) {
+  return strip_grid((d.ip + VX - 2) / (VX - 1), (d.jp + VY - 2) / (VY - 1), d.kp);
+}
+
 void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do_velfg,
                   int do_feedbf, Edges e) {
   if (do_velfg) {
@@ -615,10 +627,11 @@ __device__ inline void boundsm_from(const Dims& d, const Fields& F, int i, int j
 }
 
 // with_bound: also boundsm's halo cells (boundsm_from), instead of a k_boundsm launch
-__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
-                                                 StripGrid sg, int with_bound) {
-  int bx, by, bz, L;
-  if (!strip_block(sg, bx, by, bz, L)) return;
+__device__ __forceinline__ void calc_sm_body(const Dims& d, const Grid1D& g, const Fields& F,
+                                             const Physics& ph, const StripGrid& sg,
+                                             int with_bound, int L) {
+  int bx, by, bz;
+  if (!strip_map(sg, L, bx, by, bz)) return;
   const int i = 1 + bx * (int)blockDim.x + threadIdx.x;
   const int j = 1 + by * (int)blockDim.y + threadIdx.y;
   const int k0 = 1 + bz * SM_KPT;
@@ -635,6 +648,11 @@ __global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Phy
     bsm.st(d.S(i, j, k), val);
     if (with_bound) boundsm_from(d, F, i, j, k, val);
   }
+}
+
+__global__ __launch_bounds__(256) void k_calc_sm(Dims d, Grid1D g, Fields F, Physics ph,
+                                                 StripGrid sg, int with_bound) {
+  calc_sm_body(d, g, F, ph, sg, with_bound, xcd_tile((int)blockIdx.x, (int)gridDim.x));
 }
 
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound,
@@ -814,10 +832,11 @@ __device__ __forceinline__ void adam_point(const Dims& d, const Fields& F, int i
 }
 
 template <bool P2>
-__global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
-                                                   int do_adam, StripGrid sg) {
-  int bx, by, bz, L;
-  if (!strip_block(sg, bx, by, bz, L)) return;
+__device__ __forceinline__ void visc_adam_body(const Dims& d, const Grid1D& g, const Fields& F,
+                                               int do_visc, int do_adam, const StripGrid& sg,
+                                               int L) {
+  int bx, by, bz;
+  if (!strip_map(sg, L, bx, by, bz)) return;
   const int i = 1 + bx * (int)blockDim.x + threadIdx.x;
   const int j = 1 + by * (int)blockDim.y + threadIdx.y;
   const int k = 1 + bz;
@@ -834,6 +853,12 @@ __global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, i
   bf.st(ifg, fv);
   bg.st(ifg, gv);
   bh.st(ifg, hv);
+}
+
+template <bool P2>
+__global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
+                                                   int do_adam, StripGrid sg) {
+  visc_adam_body<P2>(d, g, F, do_visc, do_adam, sg, xcd_tile((int)blockIdx.x, (int)gridDim.x));
 }
 
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc, int do_adam,
@@ -1500,6 +1525,76 @@ void launch_boundsm_tiled(hipStream_t st, Dims d, Fields F, Edges e, int phase) 
                             : (phase == 1 ? 2L * (d.ip + 2) * d.kp : 2L * (d.ip + 2) * (d.jp + 2));
   hipLaunchKernelGGL(k_boundsm_tiled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, F,
                      e, phase);
+}
+
+// ---- batched launches: workgroup w of the grid (after the XCD remap) belongs to member
+// wgm[w] and is that member's logical workgroup w - base ----
+template <bool P2>
+__global__ __launch_bounds__(512) void k_velfg_batch(const BatchStencil* __restrict__ tab,
+                                                     const uint8_t* __restrict__ wgm, int total) {
+  const int w = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (w >= total) return;
+  // the member's operands into registers once (the body's stores cannot alias them there)
+  const BatchStencil& t = tab[wgm[w]];
+  const Dims d = t.d;
+  const Grid1D g = t.g;
+  const Fields F = t.F;
+  const Physics ph = t.ph;
+  const StripGrid sg = t.sg_velfg;
+  velfg_xy_body<P2>(d, g, F, ph, 1, sg, EDGES_ALL, w - t.base_velfg);
+}
+
+__global__ __launch_bounds__(256) void k_calc_sm_batch(const BatchStencil* __restrict__ tab,
+                                                       const uint8_t* __restrict__ wgm, int total) {
+  const int w = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (w >= total) return;
+  const BatchStencil& t = tab[wgm[w]];
+  const Dims d = t.d;
+  const Grid1D g = t.g;
+  const Fields F = t.F;
+  const Physics ph = t.ph;
+  const StripGrid sg = t.sg_sm;
+  calc_sm_body(d, g, F, ph, sg, 1, w - t.base_sm);
+}
+
+template <bool P2>
+__global__ __launch_bounds__(256) void k_visc_adam_batch(const BatchStencil* __restrict__ tab,
+                                                         const uint8_t* __restrict__ wgm,
+                                                         int total) {
+  const int w = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (w >= total) return;
+  const BatchStencil& t = tab[wgm[w]];
+  const Dims d = t.d;
+  const Grid1D g = t.g;
+  const Fields F = t.F;
+  const StripGrid sg = t.sg_visc;
+  visc_adam_body<P2>(d, g, F, 1, 1, sg, w - t.base_visc);
+}
+
+void launch_velfg_batch(hipStream_t st, const BatchStencil* tab, const uint8_t* wgm, int total,
+                        int p2xy) {
+  const dim3 grid((unsigned)((total + 7) / 8 * 8));
+  if (p2xy)
+    hipLaunchKernelGGL(k_velfg_batch<true>, grid, dim3(VX, VY), 0, st, tab, wgm, total);
+  else
+    hipLaunchKernelGGL(k_velfg_batch<false>, grid, dim3(VX, VY), 0, st, tab, wgm, total);
+}
+
+void launch_calc_sm_batch(hipStream_t st, const BatchStencil* tab, const uint8_t* wgm, int total,
+                          int shape) {
+  const dim3 grid((unsigned)((total + 7) / 8 * 8));
+  hipLaunchKernelGGL(k_calc_sm_batch, grid, dim3(stencil_shape_x(shape), stencil_shape_y(shape)),
+                     0, st, tab, wgm, total);
+}
+
+void launch_visc_adam_batch(hipStream_t st, const BatchStencil* tab, const uint8_t* wgm,
+                            int total, int shape, int p2xy) {
+  const dim3 grid((unsigned)((total + 7) / 8 * 8));
+  const dim3 blk(stencil_shape_x(shape), stencil_shape_y(shape));
+  if (p2xy)
+    hipLaunchKernelGGL(k_visc_adam_batch<true>, grid, blk, 0, st, tab, wgm, total);
+  else
+    hipLaunchKernelGGL(k_visc_adam_batch<false>, grid, blk, 0, st, tab, wgm, total);
 }
 
 }  // namespace les

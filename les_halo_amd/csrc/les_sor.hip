@@ -238,10 +238,12 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// The body of one workgroup with logical index L (k_sor_iter, and the ensemble's batched
+// launch, which deals the tiles of every member to one grid)
 template <int TJ, int NT, bool FIRST>
-__global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
-                                                  SorPair io, const DevScalars* sc, int kc,
-                                                  int njt) {
+__device__ __forceinline__ void sor_iter_body(const Dims& d, const Grid1D& g, const SplitSOR& sp,
+                                              const Physics& ph, const SorPair& io,
+                                              const DevScalars* sc, int kc, int njt, int L) {
   constexpr int NW = SOR_ITER_NRMAX;         // LDS row stride (floats)
   constexpr int NGMAX = SOR_ITER_NRMAX / 4;  // groups of four cells in the longest row
   constexpr int PB = ((TJ + 4) * NGMAX + NT - 1) / NT;  // colour-0 ring groups per thread
@@ -259,7 +261,6 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
     psh = io.p_shift ? sc->pav_carry : 0.0f;
   }
   auto sub4 = [](float4 a, float s) { return make_float4(a.x - s, a.y - s, a.z - s, a.w - s); };
-  const int L = xcd_tile(blockIdx.x, gridDim.x);
   const int jt = L % njt, kt = L / njt;
   const int j0 = 1 + jt * TJ;
   const int k0 = 1 + kt * kc;
@@ -480,7 +481,42 @@ __global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, 
   }
 }
 
+template <int TJ, int NT, bool FIRST>
+__global__ __launch_bounds__(NT) void k_sor_iter(Dims d, Grid1D g, SplitSOR sp, Physics ph,
+                                                  SorPair io, const DevScalars* sc, int kc,
+                                                  int njt) {
+  sor_iter_body<TJ, NT, FIRST>(d, g, sp, ph, io, sc, kc, njt,
+                               xcd_tile((int)blockIdx.x, (int)gridDim.x));
+}
+
 int sor_iter_supported(Dims d, const SplitSOR& sp) { return sp.uniform && sp.nr <= SOR_ITER_NRMAX; }
+
+// the batched fused iteration: workgroup w (after the XCD remap) is tile w - base of member wgm[w]
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_sor_iter_batch(const BatchSor* __restrict__ tab,
+                                                        const uint8_t* __restrict__ wgm,
+                                                        int total, int which) {
+  const int w = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (w >= total) return;
+  // the member's operands into registers once (the body's stores cannot alias them there)
+  const BatchSor& t = tab[wgm[w]];
+  const Dims d = t.d;
+  const Grid1D g = t.g;
+  const SplitSOR sp = which == 0 ? t.sp : t.sra;
+  const Physics ph = t.ph;
+  const SorPair io = which == 0 ? t.first : (which == 1 ? t.fwd : t.back);
+  const DevScalars* sc = t.sc;
+  sor_iter_body<4, 256, FIRST>(d, g, sp, ph, io, sc, t.kc, t.njt, w - t.base);
+}
+
+void launch_sor_iter_batch(hipStream_t st, const BatchSor* tab, const uint8_t* wgm, int total,
+                           int which) {
+  const dim3 grid((unsigned)((total + 7) / 8 * 8));
+  if (which == 0)
+    hipLaunchKernelGGL(k_sor_iter_batch<true>, grid, dim3(256), 0, st, tab, wgm, total, which);
+  else
+    hipLaunchKernelGGL(k_sor_iter_batch<false>, grid, dim3(256), 0, st, tab, wgm, total, which);
+}
 
 namespace {
 template <int TJ, int NT>

@@ -306,11 +306,13 @@ class LES:
 
 
 class Ensemble:
-    """Independent LES members of any sizes advanced together (les_ensemble_*): each step
-    enqueues one graph launch per member on its own stream, so the members run concurrently.
-    The members stay owned by the caller."""
+    """Independent LES members of any sizes advanced together (les_ensemble_*): batched (one
+    launch per stage over every member's workgroups, in one captured graph per step) when the
+    members allow it and batched=True, else one graph launch per member on its own stream,
+    so the members run concurrently.  The members stay owned by the caller and must outlive
+    the ensemble."""
 
-    def __init__(self, members):
+    def __init__(self, members, batched=True):
         self.members = list(members)
         arr = (ctypes.c_void_p * len(self.members))(*[m.h for m in self.members])
         e = ctypes.c_void_p()
@@ -318,6 +320,12 @@ class Ensemble:
         if r != 0:
             raise LESError(r, "les_ensemble_create failed")
         self.e = e
+        lib().les_ensemble_set_mode(self.e, 1 if batched else 0)
+
+    @property
+    def batched(self):
+        """True when the last run used the batched launches."""
+        return bool(lib().les_ensemble_batched(self.e))
 
     def run(self, n0, nsteps):
         r = lib().les_ensemble_run(self.e, n0, nsteps)

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -796,10 +797,35 @@ T press_rhsav(F<T>& a, T dt, int sum_mode, T* area_out) {
   return rhsav;
 }
 
-// press.f95:91-100: one red-black colour sweep; returns the sweep's residual terms via summer
+// press.f95:91-100: one red-black colour sweep; returns the sweep's residual terms via summer.
+// The FP64 instantiation (the high-precision reference, not the bitwise restatement) runs the
+// levels of a colour in parallel: a colour's cells read only the other colour, so the updated p
+// is the same for any order; the residual is summed per level and the levels added in k order,
+// so it does not depend on the thread count either.
 template <typename T>
 void sor_colour(F<T>& a, int nrd, T omega, Summer<T>& sor) {
   const int im = a.im, jm = a.jm, km = a.km;
+  if constexpr (std::is_same<T, double>::value) {
+    std::vector<double> part((size_t)km + 1, 0.0);
+#pragma omp parallel for schedule(static) if ((long)im * jm * km >= (1L << 18))
+    for (int k = 1; k <= km; ++k) {
+      double s = 0.0;
+      for (int j = 1; j <= jm; ++j)
+        for (int i = 1 + (k + j + nrd) % 2; i <= im; i += 2) {
+          T reltmp = omega * (a.cn1(i, j, k) *
+                                  (a.cn2l(i) * a.p(i + 1, j, k) + a.cn2s(i) * a.p(i - 1, j, k) +
+                                   a.cn3l(j) * a.p(i, j + 1, k) + a.cn3s(j) * a.p(i, j - 1, k) +
+                                   a.cn4l(k) * a.p(i, j, k + 1) + a.cn4s(k) * a.p(i, j, k - 1) -
+                                   a.rhs(i, j, k)) -
+                              a.p(i, j, k));
+          a.p(i, j, k) = a.p(i, j, k) + reltmp;
+          s += reltmp * reltmp;
+        }
+      part[k] = s;
+    }
+    for (int k = 1; k <= km; ++k) sor.add(part[k]);
+    return;
+  }
   for (int k = 1; k <= km; ++k)
     for (int j = 1; j <= jm; ++j)
       for (int i = 1 + (k + j + nrd) % 2; i <= im; i += 2) {
@@ -861,6 +887,30 @@ T press_pav_adj(F<T>& a, int sum_mode, T* pco_out) {
 template <typename T>
 double rel_residual(F<T>& a) {
   double rr = 0.0, bb = 0.0;
+  if constexpr (std::is_same<T, double>::value) {  // per-level sums added in k order
+    std::vector<double> prr((size_t)a.km + 1, 0.0), pbb((size_t)a.km + 1, 0.0);
+#pragma omp parallel for schedule(static) if ((long)a.im * a.jm * a.km >= (1L << 18))
+    for (int k = 1; k <= a.km; ++k) {
+      double r2 = 0.0, b2 = 0.0;
+      for (int j = 1; j <= a.jm; ++j)
+        for (int i = 1; i <= a.im; ++i) {
+          const double lp = a.cn2l(i) * a.p(i + 1, j, k) + a.cn2s(i) * a.p(i - 1, j, k) +
+                            a.cn3l(j) * a.p(i, j + 1, k) + a.cn3s(j) * a.p(i, j - 1, k) +
+                            a.cn4l(k) * a.p(i, j, k + 1) + a.cn4s(k) * a.p(i, j, k - 1) -
+                            a.p(i, j, k) / a.cn1(i, j, k);
+          const double b = a.rhs(i, j, k), r = b - lp;
+          r2 += r * r;
+          b2 += b * b;
+        }
+      prr[k] = r2;
+      pbb[k] = b2;
+    }
+    for (int k = 1; k <= a.km; ++k) {
+      rr += prr[k];
+      bb += pbb[k];
+    }
+    return bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
+  }
   for (int k = 1; k <= a.km; ++k)
     for (int j = 1; j <= a.jm; ++j)
       for (int i = 1; i <= a.im; ++i) {
