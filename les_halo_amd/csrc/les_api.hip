@@ -2019,10 +2019,10 @@ extern "C" int les_finalise_LES_kernel(void) {
 // arrays (the halo exchange, here a device copy since every tile is on the same GPU), the
 // domain's own faces keep the reference's boundary rules (Edges), and the sums (uout, rhsav,
 // pav) combine the tiles' exact partials, so the tiled step is bitwise the single-domain step.
-// Pressure sweeps each tile's colour-split copy (k_sor_split with halo rows and columns) and
-// exchanges the split faces after every colour (k_split_exchange); tiles whose grid the split
-// sweep does not cover use the in-place colour sweep (k_sor_colour) with Fortran-layout
-// exchanges.
+// Pressure sweeps each tile's colour-split copy (k_sor_split_tiles with halo rows and
+// columns, which also stores each tile's updated face cells into its neighbours' halos); tiles
+// whose grid the split sweep does not cover use the in-place colour sweep (k_sor_colour) with
+// Fortran-layout exchanges.
 // ------------------------------------------------------------------------------------------
 struct les_tiled {
   les_config cfg;  // the whole domain
@@ -2250,9 +2250,10 @@ SplitTiles split_tiles(les_tiled* T) {
   return x;
 }
 
-// press.f95:87-115 on the tiles' split copies: the colour sweeps of every tile, then the face
-// exchange of the split layout, per colour; the k faces need no images (a cell reads itself
-// there), and the gather leaves the interior of p for pav, adj and the boundary images
+// press.f95:87-115 on the tiles' split copies: one face exchange of the split layout after the
+// scatter, then the colour sweeps of every tile, each storing its updated face cells into the
+// neighbours' halos as well; the k faces need no images (a cell reads itself there), and the
+// gather leaves the interior of p for pav, adj and the boundary images
 void tiled_sor_split(les_tiled* T, hipStream_t st) {
   const SplitTiles x = split_tiles(T);
   for (size_t q = 0; q < T->t.size(); ++q) {
@@ -2266,12 +2267,16 @@ void tiled_sor_split(les_tiled* T, hipStream_t st) {
     sw.sp[q] = tile_split(T, q);
     sw.g[q] = grid_of(T->t[q]);
     sw.sc[q] = T->t[q]->sc;
+    SplitPush& pu = sw.push[q];
+    for (int c = 0; c < 2; ++c) {
+      pu.n[c] = x.p[x.n[q]][c];
+      pu.s[c] = x.p[x.s[q]][c];
+    }
+    pu.e = x.e[q] >= 0 ? x.hw[x.e[q]] : nullptr;
+    pu.w = x.w[q] >= 0 ? x.he[x.w[q]] : nullptr;
   }
   for (int l = 1; l <= T->cfg.nmaxp; ++l)
-    for (int nrd = 0; nrd <= 1; ++nrd) {
-      launch_sor_split_tiles(st, T->t[0]->d, sw, T->t[0]->ph, nrd);
-      launch_split_exchange(st, T->t[0]->d, x);
-    }
+    for (int nrd = 0; nrd <= 1; ++nrd) launch_sor_split_tiles(st, T->t[0]->d, sw, T->t[0]->ph, nrd);
   for (size_t q = 0; q < T->t.size(); ++q) {
     les_handle* h = T->t[q];
     launch_split_gather_adj(st, h->d, fields_of(h), tile_split(T, (int)q), h->sc, 0);

@@ -63,7 +63,7 @@ struct SeqTerms {
   int nrt;
 };
 // The split copies of the tiles of one cut domain and their neighbours, for the face exchange
-// of the split layout (k_split_exchange) after every colour sweep
+// of the split layout (k_split_exchange) after the scatter
 constexpr int SPLIT_TILES_MAX = 8;
 struct SplitTiles {
   float* p[SPLIT_TILES_MAX][2];
@@ -201,14 +201,27 @@ int sor_iter_supported(Dims d, const SplitSOR& sp);
 void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, SorPair io,
                      DevScalars* sc, SorIterShape shape, int first = 0);
 void launch_pav_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, RedBuf rb, DevScalars* sc);
+// Where a tile's sweep stores the cells it updates that are a neighbour's halo: its row j = 1
+// goes to row index jp (j = jp+1) of the south tile's s[c], its row j = jp to row index jp+1
+// (j = 0) of the north tile's n[c], its p(ip,j,k) to the east tile's hw, its p(1,j,k) to the
+// west tile's he (nullptr: no neighbour on that face).  During a sweep of colour c a tile reads
+// only colour 1-c halo entries, so these stores never meet a read of the same sweep.
+struct SplitPush {
+  float* n[2];
+  float* s[2];
+  float *e, *w;
+};
 // the sweep operands of every tile of a cut domain (k_sor_split_tiles)
 struct SorTiles {
   SplitSOR sp[SPLIT_TILES_MAX];
   Grid1D g[SPLIT_TILES_MAX];
   const DevScalars* sc[SPLIT_TILES_MAX];
+  SplitPush push[SPLIT_TILES_MAX];
   int ntiles;
 };
-// one colour sweep (no residual) of every tile, all tiles of the same size d
+// one colour sweep (no residual) of every tile, all tiles of the same size d; the sweep stores
+// the tiles' updated face cells into their neighbours' halos (SplitPush), so one exchange after
+// the scatter keeps every halo current for the whole solve
 void launch_sor_split_tiles(hipStream_t st, Dims d, const SorTiles& T, Physics ph, int nrd);
 // fills every tile's halo rows (both colours) and i-halo columns from its neighbours' interior
 void launch_split_exchange(hipStream_t st, Dims d, const SplitTiles& T);

@@ -84,7 +84,8 @@ __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, c
                                                const Physics& ph, int nrd, int64_t* partials,
                                                int ng, int nthreads, int L,
                                                unsigned long long* lds,
-                                               const SeqTerms& tm = SeqTerms{}) {
+                                               const SeqTerms& tm = SeqTerms{},
+                                               const SplitPush* push = nullptr) {
   if (RESID) block_limbs_init(lds);
   ThreadAcc acc;
   acc.init();
@@ -182,7 +183,19 @@ __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, c
       const float pw = (i == 1) ? (sp.hw ? hwv : pc) : xw[q];
       out[q] = (i <= d.ip) ? relax(q, i, pe, pw) : pc;
     }
-    *reinterpret_cast<float4*>(sp.p[c] + base) = make_float4(out[0], out[1], out[2], out[3]);
+    const float4 o4 = make_float4(out[0], out[1], out[2], out[3]);
+    *reinterpret_cast<float4*>(sp.p[c] + base) = o4;
+    if (push) {  // this tile's face cells are its neighbours' halos (same split position)
+      const int lev = pl * (k - 1) + 4 * grp;
+      if (j == 1 && push->s[c])
+        *reinterpret_cast<float4*>(push->s[c] + lev + sp.nr * d.jp) = o4;
+      if (j == d.jp && push->n[c])
+        *reinterpret_cast<float4*>(push->n[c] + lev + sp.nr * (d.jp + 1)) = o4;
+      if (i0 == 1 && push->w) push->w[ih] = out[0];
+      const int qe = (d.ip - i0) >> 1;  // the group's cell i = ip, if it has one
+      if (push->e && i0 + 6 >= d.ip && i0 + 2 * qe == d.ip)
+        push->e[ih] = qe == 0 ? out[0] : (qe == 1 ? out[1] : (qe == 2 ? out[2] : out[3]));
+    }
   }
   if (RESID) {
     acc.flush(lds);
@@ -208,7 +221,7 @@ __global__ __launch_bounds__(256) void k_sor_split_tiles(Dims d, SorTiles T, Phy
   const int q = blockIdx.y;
   if (T.sc[q]->sor_done) return;
   sor_split_body<UNIFORM, false>(d, T.g[q], T.sp[q], ph, nrd, nullptr, ng, nthreads,
-                                 xcd_tile(blockIdx.x, gridDim.x), nullptr);
+                                 xcd_tile(blockIdx.x, gridDim.x), nullptr, SeqTerms{}, &T.push[q]);
 }
 
 // ---- one whole SOR iteration (colour 1 then colour 0, press.f95:90-100 with nrd = 0, 1) per
