@@ -994,7 +994,7 @@ int enqueue_press_mixed(les_handle* h) {
   }
   launch_rhs(h->st, d, g, F, h->ph, h->rb, h->sc, EDGES_ALL, 0);  // partials for the FP64 sum
   launch_mx_rhsav(h->st, d, h->rb, h->sc);
-  launch_mx_scatter(h->st, d, F, h->sp, h->mx, h->sc);
+  launch_mx_scatter(h->st, d, g, F, h->sp, h->mx, h->sc);
   if ((r = enqueue_sor_split_loop(h, false))) return r;
   launch_mx_promote(h->st, d, h->sp, h->mx);
   Physics ph_corr = h->ph;  // the correction solves may relax with their own omega

@@ -16,12 +16,15 @@ struct Mixed {
   float* dir[2];      // search direction (two buffers: the next is formed from the last)
   float* dir2[2];
   float* q[2];        // A * dir
-  double* cg;         // rz, pq, alpha, beta, stopped flag, rz at the start, iterations
+  double* cg;         // rz, pq, alpha, beta, stopped flag, rz at the start, iterations, w-mean
+                      // of the rhs
 };
 
 int mixed_blocks(Dims d);
 void launch_mx_rhsav(hipStream_t st, Dims d, RedBuf rb, DevScalars* sc);
-void launch_mx_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, Mixed mx, DevScalars* sc);
+// p -> split copy; rhs64 = rhs - rhsav64 minus its w-mean (the solvable right-hand side)
+void launch_mx_scatter(hipStream_t st, Dims d, Grid1D g, Fields F, SplitSOR sp, Mixed mx,
+                       DevScalars* sc);
 void launch_mx_promote(hipStream_t st, Dims d, SplitSOR sp, Mixed mx);
 void launch_mx_resid(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int first,
                      double tol, DevScalars* sc);

@@ -7,6 +7,7 @@
 // cn4l/cn4s along k, init.f95:82-107), mirror images across the i and k faces and periodic j
 // (boundp.f95).  The solve:
 //   rhs64 = rhs - rhsav64                 (FP64 mean of vol*rhs, exact sum rounded to FP64)
+//   rhs64 = rhs64 - sum(w rhs64)/sum(w)   (the solvable part; w below, vol())
 //   p32   = SOR_nmaxp(p, rhs64)           (the FP32 colour-split sweeps)
 //   p64   = p32
 //   repeat: r64 = rhs64 - L p64 (FP64);  stop when |r64|_2 <= tol * |rhs64|_2
@@ -56,6 +57,15 @@ __device__ __forceinline__ int flat_block() {
   return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 }
 
+// The weight w = (dxs(i-1)+dxs(i)) (dys(j-1)+dys(j)) (dzs(k-1)+dzs(k)) in which L (boundary
+// images folded) is symmetric: w(i) cn2l(i) = w(i+1) cn2s(i+1) = 2/dxs(i) with init.f95:82-107's
+// coefficients, alike along j and k.  The constants are L's null space and w its left null
+// vector, so L p = rhs has a solution only if sum w rhs = 0.
+__device__ __forceinline__ double vol(const Grid1D& g, int i, int j, int k) {
+  return ((double)g.DXS(i - 1) + (double)g.DXS(i)) * ((double)g.DYS(j - 1) + (double)g.DYS(j)) *
+         ((double)g.DZS(k - 1) + (double)g.DZS(k));
+}
+
 }  // namespace
 
 // ---- rhsav in FP64 from the exact partial sums of k_rhs (press.f95:65-75) ----
@@ -68,21 +78,55 @@ __global__ __launch_bounds__(256) void k_mx_rhsav(const int64_t* partials, int n
   }
 }
 
-// ---- p -> split FP32; rhs64 = rhs - rhsav64 (split FP64 and its FP32 rounding for the sweeps) ----
-__global__ __launch_bounds__(256) void k_mx_scatter(Dims d, Fields F, SplitSOR sp, Mixed mx,
-                                                    const DevScalars* sc) {
+// ---- p -> split FP32; rhs64 = rhs - rhsav64 in FP64; block sums of w*rhs64 and w ----
+__global__ __launch_bounds__(256) void k_mx_scatter(Dims d, Grid1D g, Fields F, SplitSOR sp,
+                                                    Mixed mx, const DevScalars* sc) {
+  __shared__ double lds[4];
+  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
+  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  double a = 0.0, b = 0.0;
+  if (i <= d.ip && j <= d.jp) {
+    const int par = (i + j + k) & 1;
+    const int is = split_index(sp, i, j, k);
+    sp.p[par][is] = F.p[d.P(i, j, k)];
+    const double r = (double)F.rhs[d.R(i, j, k)] - sc->rhsav64;
+    mx.r64[par][is] = r;
+    b = vol(g, i, j, k);
+    a = b * r;
+  }
+  const double sa = block_sum(a, lds);
+  if (tid() == 0) mx.part[flat_block()] = sa;
+  __syncthreads();
+  const double sb = block_sum(b, lds);
+  if (tid() == 0) mx.part[mx.nblk + flat_block()] = sb;
+}
+
+// ---- the solvable rhs: rhs64 -= sum(w rhs64) / sum(w) (cg[7]); the sweeps get its FP32
+// rounding, and so does rhs (the right-hand side the step solved) ----
+__global__ __launch_bounds__(256) void k_mx_wmean(Mixed mx) {
+  __shared__ double lds[4];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < mx.nblk; q += 256) {
+    a += mx.part[q];
+    b += mx.part[mx.nblk + q];
+  }
+  const double sa = block_sum(a, lds);
+  __syncthreads();
+  const double sb = block_sum(b, lds);
+  if (threadIdx.x == 0) mx.cg[7] = (sb > 0.0) ? sa / sb : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mx_project_rhs(Dims d, Fields F, SplitSOR sp, Mixed mx) {
   const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
   const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
   const int k = 1 + blockIdx.z;
   if (i > d.ip || j > d.jp) return;
-  const int par = (i + j + k) & 1;
-  const int is = split_index(sp, i, j, k);
-  sp.p[par][is] = F.p[d.P(i, j, k)];
-  const long ir = d.R(i, j, k);
-  const double r = (double)F.rhs[ir] - sc->rhsav64;
+  const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
+  const double r = mx.r64[par][is] - mx.cg[7];
   mx.r64[par][is] = r;
   sp.r[par][is] = (float)r;
-  F.rhs[ir] = (float)r;
+  F.rhs[d.R(i, j, k)] = (float)r;
 }
 
 // ---- p64 = p32 ----
@@ -96,7 +140,12 @@ __global__ __launch_bounds__(256) void k_mx_promote(Dims d, SplitSOR sp, Mixed m
 }
 
 // ---- r = rhs64 - L p64 in FP64; sweeps get (float)r and a zero correction; block sums of
-// r^2 (and of rhs64^2 when first != 0) ----
+// r^2 (and of rhs64^2 when first != 0).  L in FP64 in flux form, L p = sum over the real
+// neighbours of c_nb (p_nb - p_c), with init.f95:82-107's coefficients evaluated in FP64 from the
+// grid spacings: in exact arithmetic 1/cn1 is the sum of the c_nb, so this is the operator whose
+// fixed point the sweeps relax, with the constants exactly in its null space and w exactly its
+// left null vector (the FP32 coefficients and cn1 break both at rounding level, which left a
+// residual of (constant part of p) x (FP32 row sums) no correction could remove). ----
 __global__ __launch_bounds__(256) void k_mx_resid(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
                                                   int first) {
   __shared__ double lds[4];
@@ -107,16 +156,16 @@ __global__ __launch_bounds__(256) void k_mx_resid(Dims d, Grid1D g, SplitSOR sp,
   if (i <= d.ip && j <= d.jp) {
     const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
     const double pc = mx.p64[par][is];
-    const double pe = (i == d.ip) ? pc : p64_at(sp, mx, i + 1, j, k);
-    const double pw = (i == 1) ? pc : p64_at(sp, mx, i - 1, j, k);
-    const double pn = p64_at(sp, mx, i, j == d.jp ? 1 : j + 1, k);
-    const double ps = p64_at(sp, mx, i, j == 1 ? d.jp : j - 1, k);
-    const double pu = (k == d.kp) ? pc : p64_at(sp, mx, i, j, k + 1);
-    const double pd = (k == 1) ? pc : p64_at(sp, mx, i, j, k - 1);
-    const double cn1 = (double)mx.cn1[d.O(i, j, k)];
-    const double lp = (double)g.cn2l[i - 1] * pe + (double)g.cn2s[i - 1] * pw +
-                      (double)g.cn3l[j - 1] * pn + (double)g.cn3s[j - 1] * ps +
-                      (double)g.cn4l[k - 1] * pu + (double)g.cn4s[k - 1] * pd - pc / cn1;
+    const double xm = g.DXS(i - 1), xp = g.DXS(i), ym = g.DYS(j - 1), yp = g.DYS(j);
+    const double zm = g.DZS(k - 1), zp = g.DZS(k);
+    // mirror images across the i and k faces contribute p_c - p_c = 0
+    double lp = 0.0;
+    if (i < d.ip) lp += 2.0 / (xp * (xm + xp)) * (p64_at(sp, mx, i + 1, j, k) - pc);
+    if (i > 1) lp += 2.0 / (xm * (xm + xp)) * (p64_at(sp, mx, i - 1, j, k) - pc);
+    lp += 2.0 / (yp * (ym + yp)) * (p64_at(sp, mx, i, j == d.jp ? 1 : j + 1, k) - pc);
+    lp += 2.0 / (ym * (ym + yp)) * (p64_at(sp, mx, i, j == 1 ? d.jp : j - 1, k) - pc);
+    if (k < d.kp) lp += 2.0 / (zp * (zm + zp)) * (p64_at(sp, mx, i, j, k + 1) - pc);
+    if (k > 1) lp += 2.0 / (zm * (zm + zp)) * (p64_at(sp, mx, i, j, k - 1) - pc);
     const double b = mx.r64[par][is];
     const double r = b - lp;
     sp.r[par][is] = (float)r;
@@ -205,8 +254,9 @@ __global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp
 
 // ---- conjugate gradients for the correction (refine_solver 1) ----
 // A = -L with the boundary images folded into the matrix (a face neighbour across an i or k face
-// is the cell itself, so its coefficient joins the diagonal): (A x)_c = x_c * dg_c - sum over the
-// real neighbours of c_nb * x_nb, dg_c = 1/cn1 - (image coefficients).  With cn2l(i) =
+// is the cell itself, so its coefficient joins the diagonal): (A x)_c = sum over the real
+// neighbours of c_nb * (x_c - x_nb); the preconditioner's diagonal is dg_c = 1/cn1 - (image
+// coefficients), the same sum up to rounding.  With cn2l(i) =
 // 2/(dxs(i)(dxs(i-1)+dxs(i))) and cn2s(i) = 2/(dxs(i-1)(dxs(i-1)+dxs(i))) (init.f95:82-107, and
 // alike along j and k), w*cn2l(i) = w(i+1)*cn2s(i+1) for the weight w = (dxs(i-1)+dxs(i)) *
 // (dys(j-1)+dys(j)) * (dzs(k-1)+dzs(k)): A is symmetric in the inner product weighted by w (the
@@ -241,11 +291,6 @@ __device__ __forceinline__ float offdiag_sum(Dims d, const Grid1D& g, const Spli
   return s;
 }
 
-__device__ __forceinline__ double vol(const Grid1D& g, int i, int j, int k) {
-  return ((double)g.DXS(i - 1) + (double)g.DXS(i)) * ((double)g.DYS(j - 1) + (double)g.DYS(j)) *
-         ((double)g.DZS(k - 1) + (double)g.DZS(k));
-}
-
 }  // namespace
 
 // dnew = z + beta dold (beta = 0 on the first iteration: dnew = z), q = A dnew, block partials of
@@ -266,19 +311,19 @@ __global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp,
       return first ? mx.z[par][is] : mx.z[par][is] + beta * dd.old_[par][is];
     };
     const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
-    float dg = 1.0f / mx.cn1[d.O(i, j, k)];
-    float s = 0.0f;
     const float c2l = g.cn2l[i - 1], c2s = g.cn2s[i - 1], c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
     const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-    if (i < d.ip) s += c2l * dn(i + 1, j, k); else dg -= c2l;
-    if (i > 1) s += c2s * dn(i - 1, j, k); else dg -= c2s;
-    s += c3l * dn(i, j == d.jp ? 1 : j + 1, k);
-    s += c3s * dn(i, j == 1 ? d.jp : j - 1, k);
-    if (k < d.kp) s += c4l * dn(i, j, k + 1); else dg -= c4l;
-    if (k > 1) s += c4s * dn(i, j, k - 1); else dg -= c4s;
     const float xv = dn(i, j, k);
+    // flux form, sum c_nb (x_c - x_nb): A annihilates the constants exactly, and a smooth x
+    // loses no digits to the cancellation of a large diagonal against its neighbours
+    float qv = 0.0f;
+    if (i < d.ip) qv += c2l * (xv - dn(i + 1, j, k));
+    if (i > 1) qv += c2s * (xv - dn(i - 1, j, k));
+    qv += c3l * (xv - dn(i, j == d.jp ? 1 : j + 1, k));
+    qv += c3s * (xv - dn(i, j == 1 ? d.jp : j - 1, k));
+    if (k < d.kp) qv += c4l * (xv - dn(i, j, k + 1));
+    if (k > 1) qv += c4s * (xv - dn(i, j, k - 1));
     dd.new_[par][is] = xv;
-    const float qv = dg * xv - s;
     mx.q[par][is] = qv;
     acc = vol(g, i, j, k) * (double)xv * (double)qv;
   }
@@ -440,8 +485,11 @@ void launch_mx_rhsav(hipStream_t st, Dims d, RedBuf rb, DevScalars* sc) {
   hipLaunchKernelGGL(k_mx_rhsav, dim3(1), dim3(256), 0, st, rb.partials, reduction_blocks(d), sc);
 }
 
-void launch_mx_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, Mixed mx, DevScalars* sc) {
-  hipLaunchKernelGGL(k_mx_scatter, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, F, sp, mx, sc);
+void launch_mx_scatter(hipStream_t st, Dims d, Grid1D g, Fields F, SplitSOR sp, Mixed mx,
+                       DevScalars* sc) {
+  hipLaunchKernelGGL(k_mx_scatter, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, g, F, sp, mx, sc);
+  hipLaunchKernelGGL(k_mx_wmean, dim3(1), dim3(256), 0, st, mx);
+  hipLaunchKernelGGL(k_mx_project_rhs, mx_grid(d), dim3(MX_BX, MX_BY), 0, st, d, F, sp, mx);
 }
 
 void launch_mx_promote(hipStream_t st, Dims d, SplitSOR sp, Mixed mx) {

@@ -925,10 +925,43 @@ double rel_residual(F<T>& a) {
   return bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
 }
 
+// Weight of cell (i,j,k) in which the Poisson operator of rel_residual (boundary images folded:
+// mirror in i and k, periodic in j) is symmetric: w = (dxs(i-1)+dxs(i)) (dys(j-1)+dys(j))
+// (dzs(k-1)+dzs(k)), since w(i) cn2l(i) = w(i+1) cn2s(i+1) = 2/dxs(i) with init.f95:82-107's
+// coefficients (alike along j and k).  w is the operator's left null vector (its right null
+// vector is the constants): L p = rhs has a solution only if sum w rhs = 0.
+template <typename T>
+double null_weight(F<T>& a, int i, int j, int k) {
+  return ((double)a.dxs(i - 1) + (double)a.dxs(i)) * ((double)a.dys(j - 1) + (double)a.dys(j)) *
+         ((double)a.dzs(k - 1) + (double)a.dzs(k));
+}
+
+// rhs -= sum(w rhs) / sum(w): the rhs made solvable.  press.f95:65-82 removes the volume mean
+// (dx1 dy1 dzn); on a stretched vertical grid that leaves sum w rhs != 0, a part of rhs no p
+// can produce, so a solve to a tight residual first removes it.  Returns the shift.
+template <typename T>
+double project_rhs(F<T>& a) {
+  double sw = 0.0, swr = 0.0;
+  for (int k = 1; k <= a.km; ++k)
+    for (int j = 1; j <= a.jm; ++j)
+      for (int i = 1; i <= a.im; ++i) {
+        const double w = null_weight(a, i, j, k);
+        sw += w;
+        swr += w * (double)a.rhs(i, j, k);
+      }
+  const double c = swr / sw;
+  for (int k = 1; k <= a.km; ++k)
+    for (int j = 1; j <= a.jm; ++j)
+      for (int i = 1; i <= a.im; ++i) a.rhs(i, j, k) = (T)((double)a.rhs(i, j, k) - c);
+  return c;
+}
+
 template <typename T>
 void press(F<T>& a, const orc_params& q, orc_scalars* out) {
   T area, pco, sorv;
   T rhsav = press_rhsav(a, (T)q.dt, q.sum_mode, &area);
+  // a solve to a tolerance needs a solvable problem
+  if (q.refine_tol > 0.0) project_rhs(a);
   int it = press_sor(a, (T)q.omega, (T)q.pjuge, q.nmaxp, q.sum_mode, &sorv);
   int passes = 0;
   double res = 0.0;

@@ -286,3 +286,73 @@ def test_refinement_converges_and_continues_sor():
     q2 = o.copy()
     r2, _ = _press_residual(q2, nmaxp=10 * (passes + 1), refine_tol=1e-300, refine_max=0)
     assert abs(r2 - r) <= 1e-6 * r
+
+
+def _folded_laplacian(o, p, ip, jp, kp):
+    """L p = sum_nb c_nb p_nb - p/cn1 (init.f95:82-107's coefficients from the oracle), with
+    press.f95's boundary images: mirror across the i and k faces, periodic in j."""
+    pe = np.concatenate([p[1:], p[-1:]], 0)
+    pw = np.concatenate([p[:1], p[:-1]], 0)
+    pn = np.roll(p, -1, 1)
+    ps = np.roll(p, 1, 1)
+    pu = np.concatenate([p[:, :, 1:], p[:, :, -1:]], 2)
+    pd = np.concatenate([p[:, :, :1], p[:, :, :-1]], 2)
+    a = o.a
+    cn1 = a["cn1"].reshape((ip, jp, kp), order="F")
+    return (a["cn2l"][:, None, None] * pe + a["cn2s"][:, None, None] * pw
+            + a["cn3l"][None, :, None] * pn + a["cn3s"][None, :, None] * ps
+            + a["cn4l"][None, None, :] * pu + a["cn4s"][None, None, :] * pd - p / cn1)
+
+
+def _null_weight(o, ip, jp, kp):
+    """(dxs(i-1)+dxs(i)) (dys(j-1)+dys(j)) (dzs(k-1)+dzs(k)); dxs(0:), dys(0:), dzs(-1:)."""
+    dxs, dys, dzs = o.a["dxs"], o.a["dys"], o.a["dzs"]
+    wx = dxs[0:ip] + dxs[1:ip + 1]
+    wy = dys[0:jp] + dys[1:jp + 1]
+    wz = dzs[1:kp + 1] + dzs[2:kp + 2]
+    return wx[:, None, None] * wy[None, :, None] * wz[None, None, :]
+
+
+def test_poisson_solvability_weight_and_converged_solve():
+    """On a vertically stretched grid (kp = 30: spacing 2 m, x1.05 above 40 m).
+    * w = (dxs(i-1)+dxs(i))(dys(j-1)+dys(j))(dzs(k-1)+dzs(k)) is the left null vector of the
+      folded Poisson operator: sum w L p = 0 for any p (a wrong weight, e.g. the cell volume
+      dx1 dy1 dzn, leaves a sum of the size of the terms).
+    * press.f95's volume-mean removal leaves sum w rhs != 0 there, so no p solves L p = rhs;
+      the converged-solve mode first removes rhs's w-mean and then reaches a tight residual
+      (it stalls without that shift)."""
+    ip, jp, kp = 16, 12, 30
+    case = inputs.make_case(ip, jp, kp, seed=5, hmax=12.0, coverage=0.3)
+    o = common.oracle_from_case(case, np.float64, **common.REF_PHYS, **common.CFG_SOR)
+    dz = np.diff(o.a["dzs"][1:kp + 2])
+    assert np.abs(dz).max() > 0.01  # the grid is stretched
+    w = _null_weight(o, ip, jp, kp)
+    rng = np.random.default_rng(4)
+    for _ in range(3):
+        p = rng.standard_normal((ip, jp, kp))
+        lp = _folded_laplacian(o, p, ip, jp, kp)
+        assert abs((w * lp).sum()) <= 1e-12 * np.abs(w * lp).sum()
+        vol = (o.a["dx1"][2:ip + 2, None, None] * o.a["dy1"][None, 1:jp + 1, None]
+               * o.a["dzn"][None, None, 2:kp + 2])
+        assert abs((vol * lp).sum()) > 1e-6 * np.abs(vol * lp).sum()
+    # the constants are the right null vector
+    assert np.abs(_folded_laplacian(o, np.full((ip, jp, kp), 2.5), ip, jp, kp)).max() < 1e-12
+    o.ifdata()
+    o.step(1)
+    base = o.copy()
+    q = base.copy()
+    q.call("press")  # press.f95 as written: the volume mean removed
+    rhs = common.interior("rhs", q.a["rhs"], ip, jp, kp)
+    # no p gets |rhs - L p| below rhs's distance to the range of L (the w-orthogonal vectors),
+    # here above the tolerance reached below
+    floor = abs((w * rhs).sum()) / np.linalg.norm(w) / np.linalg.norm(rhs)
+    assert floor > 1e-10, floor
+    q = base.copy()
+    r, passes = _press_residual(q, nmaxp=50, omega=1.8, refine_tol=1e-10, refine_max=400)
+    rhs = common.interior("rhs", q.a["rhs"], ip, jp, kp)
+    assert abs((w * rhs).sum()) <= 1e-13 * np.abs(w * rhs).sum()
+    assert r <= 1e-10, (r, passes)
+    # the residual the oracle reports is that of the shifted rhs and the returned p
+    p = common.interior("p", q.a["p"], ip, jp, kp)
+    res = rhs - _folded_laplacian(o, p, ip, jp, kp)
+    assert np.linalg.norm(res) <= 2e-10 * np.linalg.norm(rhs)
