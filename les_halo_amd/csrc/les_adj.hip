@@ -389,6 +389,10 @@ struct les_adj {
   bool sor_split = false;  // the last forward SOR ran through ph (its transpose does too)
   bool p_mirror = true;  // every halo of p equals its mirror interior cell
   bool use_slots = true;  // les_adj_window keeps per-step copies (les_adj_set_slots)
+  // The recomputation of a step skips the SOR solve (its p comes from the forward window's
+  // per-step copy instead): the transposes never read the SOR's iterates, the solve being
+  // linear, and a recomputed step is bitwise the window's step.
+  bool skip_sor = false;
 };
 
 namespace {
@@ -506,7 +510,9 @@ void run_ops_fw(les_adj* h, const int* ops, int nops) {
     } else if (op == ADJ_OP_PAV) {
       run_sum<T>(h, 1);
     } else if (op == ADJ_OP_SOR) {
-      if (!split_sor_fw<T>(h))
+      if (h->skip_sor)  // the transpose tries the split form (statement sweeps if it fails)
+        h->sor_split = sizeof(T) == sizeof(float) && h->ph && h->p_mirror && h->cfg.nmaxp >= 1;
+      else if (!split_sor_fw<T>(h))
         for (int l = 0; l < h->cfg.nmaxp; ++l) run_ops_fw<T>(h, ADJ_SOR_OPS, ADJ_SOR_NOPS);
     }
   }
@@ -577,13 +583,38 @@ bool is_carried(int a) {
   return false;
 }
 
+// Up to ZERO_MAX device arrays cleared in one launch (4-byte words; blockIdx.y: the array).
+constexpr int ZERO_MAX = 64;
+struct ZeroList {
+  uint32_t* dst[ZERO_MAX];
+  long words[ZERO_MAX];
+};
+__global__ __launch_bounds__(256) void k_zero_arrays(ZeroList L) {
+  const int a = blockIdx.y;
+  uint32_t* __restrict__ d = L.dst[a];
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < L.words[a]; t += (long)gridDim.x * 256)
+    d[t] = 0u;
+}
+
 // the adjoints of everything but the carried state and the controls (wind, cd) start each
-// reversed step at zero
+// reversed step at zero (a fill kernel over the list of arrays, not one memset per array)
 int zero_temporaries(les_adj* h) {
+  ZeroList L{};
+  int m = 0;
+  auto flush = [&]() {
+    if (m) hipLaunchKernelGGL(k_zero_arrays, dim3(256, m), dim3(256), 0, h->st, L);
+    m = 0;
+  };
+  auto add = [&](void* p, size_t bytes) {
+    L.dst[m] = static_cast<uint32_t*>(p);
+    L.words[m++] = (long)(bytes / 4);
+    if (m == ZERO_MAX) flush();
+  };
   for (int q = 0; q < ADJ_NARRAYS; ++q)
-    if (h->B[q] && !is_carried(q) && q != ADJ_A_wind && q != ADJ_A_cd)
-      ACHK(h, hipMemsetAsync(h->B[q], 0, h->n[q] * tsize(h), h->st));
-  ACHK(h, hipMemsetAsync(h->SB, 0, SB_COUNT * tsize(h), h->st));
+    if (h->B[q] && !is_carried(q) && q != ADJ_A_wind && q != ADJ_A_cd) add(h->B[q], h->n[q] * tsize(h));
+  add(h->SB, SB_COUNT * tsize(h));
+  flush();
+  ACHK(h, hipGetLastError());
   return LES_OK;
 }
 
@@ -869,12 +900,13 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   if ((r = set_probes(h, probes, nprobes))) return r;
   const size_t sb = state_bytes(h);
   const int nck = (nsteps + ckpt - 1) / ckpt;
-  char *ck = nullptr, *seg = nullptr, *fin = nullptr;
+  char *ck = nullptr, *seg = nullptr, *fin = nullptr, *pst = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(h->st);
     if (ck) hipFree(ck);
     if (seg) hipFree(seg);
     if (fin) hipFree(fin);
+    if (pst) hipFree(pst);
   };
   if (hipMalloc(&ck, sb * nck) != hipSuccess || hipMalloc(&fin, sb) != hipSuccess) {
     cleanup();
@@ -887,6 +919,32 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   hipError_t e;
   if ((e = hipMemsetAsync(h->jacc, 0, sizeof(double), h->st)) != hipSuccess)
     return hipfail(e, "window: clear the objective");
+  // p after every step of the forward window (8 MB per step at 150x150x90): the recomputed
+  // steps take it instead of solving again.  Without the memory they solve.
+  const size_t pbytes = h->n[ADJ_A_p] * tsize(h);
+  if (hipMalloc(&pst, pbytes * nsteps) != hipSuccess) {
+    pst = nullptr;
+    (void)hipGetLastError();
+  }
+  auto keep_p = [&](int s) -> int {
+    if (!pst) return LES_OK;
+    void* dst = pst + pbytes * s;
+    const void* src = h->A[ADJ_A_p];
+    return copy_arrays(h, 1, &dst, &src, &pbytes);
+  };
+  // step s again from its state, p from the forward window's copy when there is one
+  auto recompute = [&](int s) -> int {
+    if (!pst) {
+      step_fw(h);
+      return LES_OK;
+    }
+    h->skip_sor = true;
+    step_fw(h);
+    h->skip_sor = false;
+    void* dst = h->A[ADJ_A_p];
+    const void* src = pst + pbytes * s;
+    return copy_arrays(h, 1, &dst, &src, &pbytes);
+  };
   // forward window: a checkpoint every ckpt steps, the objective after every step.  In FP32 the
   // plain handle runs it (its fused kernels, feedbf's beta term times the drag field: the same
   // values as the statement kernels) when it holds the same volume sums; the carried arrays are
@@ -939,6 +997,7 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
       if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return own_arrays(), cleanup(), r;
       if ((r = les::step_external(h->ph, h->st, 2)))
         return own_arrays(), cleanup(), afail(h, r, "window: plain step");
+      if ((r = keep_p(s))) return own_arrays(), cleanup(), r;
       sum_probes(h, nprobes);
     }
     r = save_state(h, fin);
@@ -948,6 +1007,7 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
     for (int s = 0; s < nsteps; ++s) {
       if (s % ckpt == 0 && (r = save_state(h, ck + sb * (s / ckpt)))) return cleanup(), r;
       step_fw(h);
+      if ((r = keep_p(s))) return cleanup(), r;
       sum_probes(h, nprobes);
     }
     if ((r = save_state(h, fin))) return cleanup(), r;
@@ -1036,7 +1096,7 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
           }
           if ((r = copy_arrays(h, NCARRIED, dst, src, bytes))) return fail_slots(r);
         }
-        step_fw(h);
+        if ((r = recompute(s))) return fail_slots(r);
       }
       for (int s = s1 - 1; s >= s0; --s) {
         point(s - s0);
@@ -1052,11 +1112,11 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
       if ((r = load_state(h, ck + sb * c))) return cleanup(), r;
       for (int s = s0; s < s1; ++s) {
         if ((r = save_state(h, seg + sb * (s - s0)))) return cleanup(), r;
-        if (s + 1 < s1) step_fw(h);
+        if (s + 1 < s1 && (r = recompute(s))) return cleanup(), r;
       }
       for (int s = s1 - 1; s >= s0; --s) {
         if ((r = load_state(h, seg + sb * (s - s0)))) return cleanup(), r;
-        step_fw(h);
+        if ((r = recompute(s))) return cleanup(), r;
         if ((r = zero_temporaries(h))) return cleanup(), r;
         seed_probes(h, nprobes, wgt);
         step_bw(h);
