@@ -480,7 +480,11 @@ int les_initial_wind_profile(int jp, int kp, const float* z2, const float* dzn,
  * transposed steps then read); the handle's state is the state after the window.
  * les_adj_set_slots(h, 0) makes the window keep no per-step copies and recompute each step
  * again from its state before its transpose instead (less memory, one forward pass more, the
- * same gradient); 1 (the default) keeps them when they fit in device memory. */
+ * same gradient); 1 (the default) keeps them when they fit in device memory.  The window keeps
+ * p after every step of its first forward pass, so the recomputed steps do not solve again.
+ * les_adj_set_transposes(h, 1) (the default) transposes the blocks of statements that have
+ * hand-written fused kernels (vel2 + velFG) with those; 0 runs every statement's generated
+ * transpose (the same values in exact arithmetic; the check of the hand-written kernels). */
 #define LES_ADJ_CD 100
 typedef struct les_adj les_adj;
 int les_adj_create(const les_config* cfg, int fp64, les_adj** out);
@@ -497,6 +501,7 @@ int les_adj_reverse_step(les_adj* h);
 int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probes, int nprobes,
                    double* objective);
 int les_adj_set_slots(les_adj* h, int enable);
+int les_adj_set_transposes(les_adj* h, int hand);
 
 #ifdef __cplusplus
 }

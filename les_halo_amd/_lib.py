@@ -80,6 +80,7 @@ PROTOS = {
     "les_adj_reverse_step": (c_int, [c_void_p]),
     "les_adj_window": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, ctypes.POINTER(c_double)]),
     "les_adj_set_slots": (c_int, [c_void_p, c_int]),
+    "les_adj_set_transposes": (c_int, [c_void_p, c_int]),
     "les_anime_arrays": (c_int, [c_int] * 6 + [c_void_p] * 7 + [ctypes.c_char_p] * 2),
     "les_upload": (c_int, [H, c_int, c_void_p]),
     "les_download": (c_int, [H, c_int, c_void_p]),

@@ -145,6 +145,11 @@ class AdjointLES:
         recomputes each step again before its transpose (less memory, the same gradient)."""
         self._chk(lib().les_adj_set_slots(self.h, 1 if enable else 0), "set_slots")
 
+    def set_transposes(self, hand):
+        """hand True (default): the hand-written fused transposes where they exist; False: every
+        statement's generated transpose."""
+        self._chk(lib().les_adj_set_transposes(self.h, 1 if hand else 0), "set_transposes")
+
     def window(self, nsteps, ckpt, probes):
         """Runs the window (checkpoint every ckpt steps) and its adjoint; returns J."""
         pr = np.ascontiguousarray(np.asarray(probes, dtype=np.int32).reshape(-1, 3))

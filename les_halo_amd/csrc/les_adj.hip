@@ -63,6 +63,7 @@ __global__ __launch_bounds__(ADJ_SLOTS) void k_scal_fold(AdjScal<T>* SB, int q) 
 }
 
 #include "les_adj_gen.inc"
+#include "les_adj_hand.h"
 
 // ---- reductions of the step and their adjoints -----------------------------------------------
 constexpr int RED_THREADS = 1024;
@@ -348,6 +349,13 @@ __global__ __launch_bounds__(256) void k_bp2(AdjArr<T> A, AdjArr<T> B, int adjoi
   }
 }
 
+// ops[q0 ..] holds the statements first, first + 1, .., last in this order
+bool is_block(const int* ops, int q0, int first, int last) {
+  for (int s = first; s <= last; ++s)
+    if (ops[q0 + (s - first)] != s) return false;
+  return true;
+}
+
 bool is_seq(const int* ops, int n, int q, std::initializer_list<int> want) {
   if (q + (int)want.size() > n) return false;
   int z = q;
@@ -393,6 +401,9 @@ struct les_adj {
   // per-step copy instead): the transposes never read the SOR's iterates, the solve being
   // linear, and a recomputed step is bitwise the window's step.
   bool skip_sor = false;
+  // the hand-written fused transposes replace the generated ones of the blocks they cover
+  // (les_adj_set_transposes)
+  bool hand = true;
 };
 
 namespace {
@@ -529,7 +540,11 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
   const long nbp2 = 2L * (h->im + 2) * (h->jm + 2);
   for (int q = nops - 1; q >= 0; --q) {
     const int op = ops[q];
-    if (q >= 3 && is_seq(ops, nops, q - 3, {ADJ_bp1_i0, ADJ_bp1_i1, ADJ_bp1_j0, ADJ_bp1_j1})) {
+    if (h->hand && op == VEL_BLOCK_LAST && q >= VEL_BLOCK_LAST - VEL_BLOCK_FIRST &&
+        is_block(ops, q - (VEL_BLOCK_LAST - VEL_BLOCK_FIRST), VEL_BLOCK_FIRST, VEL_BLOCK_LAST)) {
+      launch_vel_block_bw<T>(h->st, A, B, P, h->im, h->jm, h->km);
+      q -= VEL_BLOCK_LAST - VEL_BLOCK_FIRST;
+    } else if (q >= 3 && is_seq(ops, nops, q - 3, {ADJ_bp1_i0, ADJ_bp1_i1, ADJ_bp1_j0, ADJ_bp1_j1})) {
       hipLaunchKernelGGL(k_bp1_bw<T>, dim3((unsigned)((nbp1 + 255) / 256)), dim3(256), 0, h->st, B,
                          h->im, h->jm, h->km);
       q -= 3;
@@ -882,6 +897,12 @@ extern "C" int les_adj_reverse_step(les_adj* h) {
   if ((r = zero_temporaries(h))) return r;
   step_bw(h);
   return check(h);
+}
+
+extern "C" int les_adj_set_transposes(les_adj* h, int hand) {
+  if (!h) return LES_E_INVALID;
+  h->hand = hand != 0;
+  return LES_OK;
 }
 
 extern "C" int les_adj_zero_adjoints(les_adj* h) {
