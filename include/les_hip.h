@@ -483,7 +483,7 @@ int les_initial_wind_profile(int jp, int kp, const float* z2, const float* dzn,
  * same gradient); 1 (the default) keeps them when they fit in device memory.  The window keeps
  * p after every step of its first forward pass, so the recomputed steps do not solve again.
  * les_adj_set_transposes(h, 1) (the default) transposes the blocks of statements that have
- * hand-written fused kernels (vel2 + velFG) with those; 0 runs every statement's generated
+ * hand-written fused kernels (vel2 + velFG; les: calc_sm, boundsm, visc) with those; 0 runs every statement's generated
  * transpose (the same values in exact arithmetic; the check of the hand-written kernels). */
 #define LES_ADJ_CD 100
 typedef struct les_adj les_adj;

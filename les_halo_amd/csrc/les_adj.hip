@@ -12,6 +12,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "les_hip.h"
@@ -402,8 +403,10 @@ struct les_adj {
   // linear, and a recomputed step is bitwise the window's step.
   bool skip_sor = false;
   // the hand-written fused transposes replace the generated ones of the blocks they cover
-  // (les_adj_set_transposes)
-  bool hand = true;
+  // (les_adj_set_transposes; bit 0 the vel2 + velFG block, bit 1 the les block)
+  int hand = 3;
+  // scratch of the hand-written les transpose: 4 arrays of im*jm*km and the weight tables (T)
+  void* hs = nullptr;
 };
 
 namespace {
@@ -540,10 +543,15 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
   const long nbp2 = 2L * (h->im + 2) * (h->jm + 2);
   for (int q = nops - 1; q >= 0; --q) {
     const int op = ops[q];
-    if (h->hand && op == VEL_BLOCK_LAST && q >= VEL_BLOCK_LAST - VEL_BLOCK_FIRST &&
+    if ((h->hand & 1) && op == VEL_BLOCK_LAST && q >= VEL_BLOCK_LAST - VEL_BLOCK_FIRST &&
         is_block(ops, q - (VEL_BLOCK_LAST - VEL_BLOCK_FIRST), VEL_BLOCK_FIRST, VEL_BLOCK_LAST)) {
       launch_vel_block_bw<T>(h->st, A, B, P, h->im, h->jm, h->km);
       q -= VEL_BLOCK_LAST - VEL_BLOCK_FIRST;
+    } else if ((h->hand & 2) && op == LES_BLOCK_LAST && q >= LES_BLOCK_LAST - LES_BLOCK_FIRST &&
+               is_block(ops, q - (LES_BLOCK_LAST - LES_BLOCK_FIRST), LES_BLOCK_FIRST,
+                        LES_BLOCK_LAST)) {
+      launch_les_block_bw<T>(h->st, A, B, P, static_cast<T*>(h->hs), h->im, h->jm, h->km);
+      q -= LES_BLOCK_LAST - LES_BLOCK_FIRST;
     } else if (q >= 3 && is_seq(ops, nops, q - 3, {ADJ_bp1_i0, ADJ_bp1_i1, ADJ_bp1_j0, ADJ_bp1_j1})) {
       hipLaunchKernelGGL(k_bp1_bw<T>, dim3((unsigned)((nbp1 + 255) / 256)), dim3(256), 0, h->st, B,
                          h->im, h->jm, h->km);
@@ -766,6 +774,9 @@ extern "C" int les_adj_create(const les_config* cfg, int fp64, les_adj** out) {
     if (r) return bail(r);
   }
   h->npart = 512;
+  if (hipMalloc(&h->hs, (4 * (size_t)h->im * h->jm * h->km + 5 * (size_t)(h->im + h->jm + h->km + 6)) *
+                           tsize(h)) != hipSuccess)
+    return bail(LES_E_NOMEM);
   if (hipMalloc(&h->S, 3 * sizeof(double)) != hipSuccess || hipMalloc(&h->SB, SB_COUNT * sizeof(double)) != hipSuccess ||
       hipMalloc(&h->part, h->npart * sizeof(double)) != hipSuccess ||
       hipMalloc(&h->jacc, sizeof(double)) != hipSuccess)
@@ -790,7 +801,7 @@ extern "C" int les_adj_destroy(les_adj* h) {
     if (h->A[q]) hipFree(h->A[q]);
     if (h->B[q]) hipFree(h->B[q]);
   }
-  for (void* p : {h->S, h->SB, (void*)h->part, (void*)h->jacc, (void*)h->probes})
+  for (void* p : {h->S, h->SB, (void*)h->part, (void*)h->jacc, (void*)h->probes, h->hs})
     if (p) hipFree(p);
   if (h->st) hipStreamDestroy(h->st);
   delete h;
@@ -901,7 +912,7 @@ extern "C" int les_adj_reverse_step(les_adj* h) {
 
 extern "C" int les_adj_set_transposes(les_adj* h, int hand) {
   if (!h) return LES_E_INVALID;
-  h->hand = hand != 0;
+  h->hand = hand ? 3 : 0;
   return LES_OK;
 }
 
@@ -1150,3 +1161,4 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
   cleanup();
   return check(h);
 }
+

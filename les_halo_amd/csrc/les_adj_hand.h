@@ -283,3 +283,359 @@ void launch_vel_block_bw(hipStream_t st, const AdjArr<T>& A, const AdjArr<T>& B,
   hipLaunchKernelGGL(k_vel2_bw<T>, dim3((im + 2 + 63) / 64, (jm + 2 + 3) / 4, km + 2),
                      dim3(64, 4), 0, st, A, B, im, jm, km);
 }
+
+// ---- les.f95:33-182 (calc_sm, boundsm, visc) transposed ----------------------------------------
+// The block: calc_sm (sm from the diu), boundsm's six copies, then f, g, h += the viscous
+// terms.  Each viscous term is (flux2 - flux1) / h along x, y, z with flux = evs * dsum: evs an
+// interpolation of sm (one value, or four with the products of two linear weights), dsum one or
+// two diu values (2 diu for the own direction).  Its transpose:
+//   k_visc_sm_bw    sm's adjoint gathered from every flux that reads sm (alpha * dsum * weight)
+//   k_bsm_bw        boundsm's copies transposed (k faces, then j, then i: the reverse order)
+//   k_calc_sm_bw    per cell G = smb cd^2 / (2 sqrt(s)) and the three cross terms of s, kept in
+//                   a scratch array; sm's adjoint zeroed where calc_sm writes it
+//   k_les_diu_bw    the diu adjoints gathered from the fluxes (alpha * evs * coefficient) and
+//                   from calc_sm (through the scratch array)
+// with alpha = +-(f, g or h adjoint) / h.
+
+__device__ __forceinline__ Lay lay_sm(int im, int jm) { return Lay{-1, -1, 0, im + 3, jm + 3}; }
+
+// one viscous flux of les.f95:78-180: statement s (0 f, 1 g, 2 h), face t (0 x2, 1 x1, 2 y2,
+// 3 y1, 4 z2, 5 z1); term n = 6 s + t
+struct ViscTerm {
+  int quad;          // 0: evs = sm at (oi, oj, ok); 1: sum over e, e' of W_outer(e) W_inner(e') sm
+  int oi, oj, ok;    // offset of the single sm value
+  int wo, wi;        // weight kinds: 0 xi, 1 xm (along i), 2 yi, 3 ym (j), 4 zi (k), 5 zm (k)
+  int oo[2], io[2];  // offsets of the outer / inner weights' sm values along their axes
+  int nd;            // diu values in dsum
+  int dq[2], di[2], dj[2], dk[2];
+  int dc;            // coefficient of dsum's diu (2 for the own direction, else 1)
+};
+
+__host__ __device__ constexpr ViscTerm visc_term(int n) {
+  switch (n) {
+    // visc_f: visux2 = sm(i+1) 2 diu1(i+1), visux1 = sm 2 diu1, visuy2/1 = evsy (diu2 + diu4),
+    // visuz2/1 = evsz (diu3 + diu7)
+    case 0: return {0, 1, 0, 0, 0, 0, {0, 0}, {0, 0}, 1, {1, 0}, {1, 0}, {0, 0}, {0, 0}, 2};
+    case 1: return {0, 0, 0, 0, 0, 0, {0, 0}, {0, 0}, 1, {1, 0}, {0, 0}, {0, 0}, {0, 0}, 2};
+    case 2: return {1, 0, 0, 0, 2, 0, {0, 1}, {0, 1}, 2, {2, 4}, {0, 1}, {1, 0}, {0, 0}, 1};
+    case 3: return {1, 0, 0, 0, 2, 0, {-1, 0}, {0, 1}, 2, {2, 4}, {0, 1}, {0, -1}, {0, 0}, 1};
+    case 4: return {1, 0, 0, 0, 4, 0, {0, 1}, {0, 1}, 2, {3, 7}, {0, 1}, {0, 0}, {1, 0}, 1};
+    case 5: return {1, 0, 0, 0, 5, 0, {-1, 0}, {0, 1}, 2, {3, 7}, {0, 1}, {0, 0}, {0, -1}, 1};
+    // visc_g: evsx (diu2 + diu4) with xi / xm, sm(j+1) 2 diu5(j+1), sm 2 diu5, evsz (diu6 + diu8)
+    case 6: return {1, 0, 0, 0, 2, 0, {0, 1}, {0, 1}, 2, {2, 4}, {0, 1}, {1, 0}, {0, 0}, 1};
+    case 7: return {1, 0, 0, 0, 2, 1, {0, 1}, {-1, 0}, 2, {2, 4}, {-1, 0}, {1, 0}, {0, 0}, 1};
+    case 8: return {0, 0, 1, 0, 0, 0, {0, 0}, {0, 0}, 1, {5, 0}, {0, 0}, {1, 0}, {0, 0}, 2};
+    case 9: return {0, 0, 0, 0, 0, 0, {0, 0}, {0, 0}, 1, {5, 0}, {0, 0}, {0, 0}, {0, 0}, 2};
+    case 10: return {1, 0, 0, 0, 4, 0, {0, 1}, {0, 1}, 2, {6, 8}, {0, 0}, {0, 1}, {1, 0}, 1};
+    case 11: return {1, 0, 0, 0, 5, 0, {-1, 0}, {0, 1}, 2, {6, 8}, {0, 0}, {0, 1}, {0, -1}, 1};
+    // visc_h: evsx (diu3 + diu7) with zi x xi / xm, evsy (diu6 + diu8) with zi x yi / ym,
+    // sm(k+1) 2 diu9(k+1), sm 2 diu9
+    case 12: return {1, 0, 0, 0, 4, 0, {0, 1}, {0, 1}, 2, {3, 7}, {0, 1}, {0, 0}, {1, 0}, 1};
+    case 13: return {1, 0, 0, 0, 4, 1, {0, 1}, {-1, 0}, 2, {3, 7}, {-1, 0}, {0, 0}, {1, 0}, 1};
+    case 14: return {1, 0, 0, 0, 4, 2, {0, 1}, {0, 1}, 2, {6, 8}, {0, 0}, {0, 1}, {1, 0}, 1};
+    case 15: return {1, 0, 0, 0, 4, 3, {0, 1}, {-1, 0}, 2, {6, 8}, {0, 0}, {-1, 0}, {1, 0}, 1};
+    case 16: return {0, 0, 0, 1, 0, 0, {0, 0}, {0, 0}, 1, {9, 0}, {0, 0}, {0, 0}, {1, 0}, 2};
+    default: return {0, 0, 0, 0, 0, 0, {0, 0}, {0, 0}, 1, {9, 0}, {0, 0}, {0, 0}, {0, 0}, 2};
+  }
+}
+
+// The interpolation weights and reciprocal spacings the fluxes use, per index of their axis
+// (k_les_tab): for i in 1..im, xi0 = dx1(i+1)/(dx1(i)+dx1(i+1)), xi1 = dx1(i)/(dx1(i)+dx1(i+1))
+// (the weights of xi's first and second value), xm0 = dx1(i)/(dx1(i-1)+dx1(i)), xm1 =
+// dx1(i-1)/(dx1(i-1)+dx1(i)), 1/dx1(i); the same along j with dy1 and along k with dzn.
+// Table (axis a, kind c) holds n_a = (im, jm or km) + 2 entries at ((a-base) + c n_a).
+template <typename T>
+struct LesTab {
+  const T* w;
+  int im, jm, km;
+  __device__ __forceinline__ T at(int axis, int c, int idx) const {
+    const int nx = im + 2, ny = jm + 2, nz = km + 2;
+    const int base = axis == 0 ? 0 : (axis == 1 ? 5 * nx : 5 * (nx + ny));
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    return w[base + c * n + idx];
+  }
+  // weight e of interpolation kind w at point (i, j, k)
+  __device__ __forceinline__ T weight(int w, int e, int i, int j, int k) const {
+    return at(w / 2, 2 * (w % 2) + e, w < 2 ? i : (w < 4 ? j : k));
+  }
+  __device__ __forceinline__ T rcp(int axis, int idx) const { return at(axis, 4, idx); }
+  __device__ __forceinline__ bool in(int i, int j, int k) const { return in_box(i, j, k, im, jm, 1, km); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_les_tab(T* w, const T* dx1, const T* dy1, const T* dzn,
+                                                 int im, int jm, int km) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int nx = im + 2, ny = jm + 2, nz = km + 2;
+  int axis, idx, n, m;
+  const T* d;
+  int off;  // array index of element 0 of d
+  if (t < nx) {
+    axis = 0, idx = t, n = nx, m = im, d = dx1, off = 1;
+  } else if (t < nx + ny) {
+    axis = 1, idx = t - nx, n = ny, m = jm, d = dy1, off = 0;
+  } else if (t < nx + ny + nz) {
+    axis = 2, idx = t - nx - ny, n = nz, m = km, d = dzn, off = 1;
+  } else {
+    return;
+  }
+  const int base = axis == 0 ? 0 : (axis == 1 ? 5 * nx : 5 * (nx + ny));
+  T v[5] = {T(0), T(0), T(0), T(0), T(0)};
+  if (idx >= 1 && idx <= m) {
+    const T dm = d[idx - 1 + off], dc = d[idx + off], dp = d[idx + 1 + off];
+    v[0] = dp / (dc + dp);
+    v[1] = dc / (dc + dp);
+    v[2] = dc / (dm + dc);
+    v[3] = dm / (dm + dc);
+    v[4] = T(1) / dc;
+  }
+  for (int c = 0; c < 5; ++c) w[base + c * n + idx] = v[c];
+}
+
+template <int Q, typename T>
+__device__ __forceinline__ const T* diu_ptr(const AdjArr<T>& X) {
+  if constexpr (Q == 1) return X.diu1;
+  else if constexpr (Q == 2) return X.diu2;
+  else if constexpr (Q == 3) return X.diu3;
+  else if constexpr (Q == 4) return X.diu4;
+  else if constexpr (Q == 5) return X.diu5;
+  else if constexpr (Q == 6) return X.diu6;
+  else if constexpr (Q == 7) return X.diu7;
+  else if constexpr (Q == 8) return X.diu8;
+  else return X.diu9;
+}
+
+// alpha of term N at point (a, b, c): +-(f, g or h adjoint) / (dx1(a), dy1(b) or dzn(c))
+template <int N, typename T>
+__device__ __forceinline__ T visc_alpha(const AdjArr<T>& B, const LesTab<T>& W, const Lay& F,
+                                        int a, int b, int c) {
+  constexpr int s = N / 6, t = N % 6;
+  const long e = F.at(a, b, c);
+  const T y = s == 0 ? B.f[e] : (s == 1 ? B.g[e] : B.h[e]);
+  const T r = t < 2 ? W.rcp(0, a) : (t < 4 ? W.rcp(1, b) : W.rcp(2, c));
+  return (t % 2 == 0 ? y : -y) * r;
+}
+
+// the offset of a weight kind's sm value o along its axis, as (di, dj, dk)
+__host__ __device__ constexpr int axis_di(int w, int o) { return w <= 1 ? o : 0; }
+__host__ __device__ constexpr int axis_dj(int w, int o) { return (w == 2 || w == 3) ? o : 0; }
+__host__ __device__ constexpr int axis_dk(int w, int o) { return w >= 4 ? o : 0; }
+
+// term N's share of sm(i, j, k)'s adjoint: alpha * dc * dsum * weight at every point reading it
+template <int N, typename T>
+__device__ __forceinline__ void visc_sm_term(T& acc, const AdjArr<T>& A, const AdjArr<T>& B,
+                                             const LesTab<T>& W, const Lay& F, int i, int j,
+                                             int k) {
+  constexpr ViscTerm v = visc_term(N);
+  const int im = W.im, jm = W.jm;
+  auto dsum = [&](int a, int b, int c) {
+    T ds = diu_ptr<v.dq[0], T>(A)[lay_vel(v.dq[0], im, jm).at(a + v.di[0], b + v.dj[0], c + v.dk[0])];
+    if constexpr (v.nd == 2)
+      ds += diu_ptr<v.dq[1], T>(A)[lay_vel(v.dq[1], im, jm).at(a + v.di[1], b + v.dj[1], c + v.dk[1])];
+    return ds;
+  };
+  if constexpr (!v.quad) {
+    const int a = i - v.oi, b = j - v.oj, c = k - v.ok;
+    if (W.in(a, b, c)) acc += visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * dsum(a, b, c);
+  } else {
+#pragma unroll
+    for (int eo = 0; eo < 2; ++eo)
+#pragma unroll
+      for (int ei = 0; ei < 2; ++ei) {
+        const int a = i - axis_di(v.wo, v.oo[eo]) - axis_di(v.wi, v.io[ei]);
+        const int b = j - axis_dj(v.wo, v.oo[eo]) - axis_dj(v.wi, v.io[ei]);
+        const int c = k - axis_dk(v.wo, v.oo[eo]) - axis_dk(v.wi, v.io[ei]);
+        if (W.in(a, b, c))
+          acc += visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * dsum(a, b, c) *
+                 W.weight(v.wo, eo, a, b, c) * W.weight(v.wi, ei, a, b, c);
+      }
+  }
+}
+
+template <typename T, int... N>
+__device__ __forceinline__ void visc_sm_terms(std::integer_sequence<int, N...>, T& acc,
+                                              const AdjArr<T>& A, const AdjArr<T>& B,
+                                              const LesTab<T>& W, const Lay& F, int i, int j,
+                                              int k) {
+  (visc_sm_term<N, T>(acc, A, B, W, F, i, j, k), ...);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_visc_sm_bw(AdjArr<T> A, AdjArr<T> B, LesTab<T> W) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  const int im = W.im, jm = W.jm, km = W.km;
+  if (i > im + 1 || j > jm + 1 || k > km + 1) return;
+  T acc = T(0);
+  visc_sm_terms(std::make_integer_sequence<int, 18>{}, acc, A, B, W, lay_fgh(im, jm), i, j, k);
+  B.sm[lay_sm(im, jm).at(i, j, k)] += acc;
+}
+
+// boundsm.f95:13-32 transposed, one phase per launch (0: k faces, 1: j faces, 2: i faces), both
+// faces of the phase together (blockIdx.y)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bsm_bw(T* smb, int im, int jm, int km, int phase) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int side = blockIdx.y;
+  const Lay L = lay_sm(im, jm);
+  long dst, src;
+  T sgn = T(1);
+  if (phase == 0) {  // sm(i, j, 0) = -sm(i, j, 1), sm(i, j, km+1) = sm(i, j, km): i 0..im+1, j -1..jm+1
+    const int ni = im + 2;
+    if (t >= ni * (jm + 3)) return;
+    const int a = t % ni, b = -1 + t / ni;
+    dst = L.at(a, b, side == 0 ? 0 : km + 1);
+    src = L.at(a, b, side == 0 ? 1 : km);
+    if (side == 0) sgn = T(-1);
+  } else if (phase == 1) {  // sm(i, 0, k) = sm(i, 1, k), sm(i, jm+1, k) = sm(i, jm, k): i 0..im+1, k 0..km+1
+    const int ni = im + 2;
+    if (t >= ni * (km + 2)) return;
+    const int a = t % ni, c = t / ni;
+    dst = L.at(a, side == 0 ? 0 : jm + 1, c);
+    src = L.at(a, side == 0 ? 1 : jm, c);
+  } else {  // sm(0, j, k) = sm(1, j, k), sm(im+1, j, k) = sm(im, j, k): j -1..jm+1, k 0..km+1
+    const int nj = jm + 3;
+    if (t >= nj * (km + 2)) return;
+    const int b = -1 + t % nj, c = t / nj;
+    dst = L.at(side == 0 ? 0 : im + 1, b, c);
+    src = L.at(side == 0 ? 1 : im, b, c);
+  }
+  smb[src] += sgn * smb[dst];
+  smb[dst] = T(0);
+}
+
+// calc_sm (les.f95:40-56) at each interior cell: G = smb (cs0 delx1(k))^2 / (2 sqrt(s)) (0 if
+// s = 0) and T12, T23, T13 = G (S12, S23, S13) / 2, the sums of the averaged cross derivatives
+// whose squares s holds; stored per cell (scratch: G, T12, T23, T13 arrays of im*jm*km)
+template <typename T>
+__global__ __launch_bounds__(256) void k_calc_sm_bw(AdjArr<T> A, AdjArr<T> B, AdjPar<T> P,
+                                                    T* scratch, int im, int jm, int km) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > im || j > jm) return;
+  const VelPtrs<T> D = diu_of(A);
+  auto d = [&](int q, int a, int b, int c) { return D.a[q][lay_vel(q, im, jm).at(a, b, c)]; };
+  const T q4(0.25);
+  const T d1 = d(1, i, j, k), d5 = d(5, i, j, k), d9 = d(9, i, j, k);
+  const T s12 = (d(2, i - 1, j, k) + d(2, i - 1, j + 1, k) + d(2, i, j, k) + d(2, i, j + 1, k)) * q4 +
+                (d(4, i, j, k) + d(4, i, j - 1, k) + d(4, i + 1, j, k) + d(4, i + 1, j - 1, k)) * q4;
+  const T s23 = (d(8, i, j, k) + d(8, i, j, k - 1) + d(8, i, j + 1, k) + d(8, i, j + 1, k - 1)) * q4 +
+                (d(6, i, j - 1, k) + d(6, i, j - 1, k + 1) + d(6, i, j, k) + d(6, i, j, k + 1)) * q4;
+  const T s13 = (d(3, i - 1, j, k) + d(3, i - 1, j, k + 1) + d(3, i, j, k) + d(3, i, j, k + 1)) * q4 +
+                (d(7, i, j, k) + d(7, i, j, k - 1) + d(7, i + 1, j, k) + d(7, i + 1, j, k - 1)) * q4;
+  const T s = T(2) * (d1 * d1 + d5 * d5 + d9 * d9) + s12 * s12 + s23 * s23 + s13 * s13;
+  const long e = lay_sm(im, jm).at(i, j, k);
+  const T cd = P.cs0 * A.delx1[k - 1];
+  const T g = s > T(0) ? B.sm[e] * (cd * cd) / (T(2) * sqrt(s)) : T(0);
+  B.sm[e] = T(0);
+  const long n = (long)im * jm * km, c = (i - 1) + (long)im * ((j - 1) + (long)jm * (k - 1));
+  scratch[c] = g;
+  scratch[n + c] = T(0.5) * g * s12;
+  scratch[2 * n + c] = T(0.5) * g * s23;
+  scratch[3 * n + c] = T(0.5) * g * s13;
+}
+
+// term N's share of the diu adjoints at (i, j, k): alpha * dc * evs at every point whose dsum
+// reads diu dq[m] there
+template <int N, typename T>
+__device__ __forceinline__ void visc_diu_term(T* acc, const AdjArr<T>& A, const AdjArr<T>& B,
+                                              const LesTab<T>& W, const Lay& F, const Lay& S,
+                                              int i, int j, int k) {
+  constexpr ViscTerm v = visc_term(N);
+#pragma unroll
+  for (int m = 0; m < v.nd; ++m) {
+    const int a = i - v.di[m], b = j - v.dj[m], c = k - v.dk[m];
+    if (!W.in(a, b, c)) continue;
+    T evs;
+    if constexpr (!v.quad) {
+      evs = A.sm[S.at(a + v.oi, b + v.oj, c + v.ok)];
+    } else {
+      evs = T(0);
+#pragma unroll
+      for (int eo = 0; eo < 2; ++eo)
+#pragma unroll
+        for (int ei = 0; ei < 2; ++ei)
+          evs += W.weight(v.wo, eo, a, b, c) * W.weight(v.wi, ei, a, b, c) *
+                 A.sm[S.at(a + axis_di(v.wo, v.oo[eo]) + axis_di(v.wi, v.io[ei]),
+                           b + axis_dj(v.wo, v.oo[eo]) + axis_dj(v.wi, v.io[ei]),
+                           c + axis_dk(v.wo, v.oo[eo]) + axis_dk(v.wi, v.io[ei]))];
+    }
+    acc[v.dq[m]] += visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * evs;
+  }
+}
+
+template <typename T, int... N>
+__device__ __forceinline__ void visc_diu_terms(std::integer_sequence<int, N...>, T* acc,
+                                               const AdjArr<T>& A, const AdjArr<T>& B,
+                                               const LesTab<T>& W, const Lay& F, const Lay& S,
+                                               int i, int j, int k) {
+  (visc_diu_term<N, T>(acc, A, B, W, F, S, i, j, k), ...);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_les_diu_bw(AdjArr<T> A, AdjArr<T> B, LesTab<T> W,
+                                                    const T* scratch) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  const int im = W.im, jm = W.jm, km = W.km;
+  if (i > im + 1 || j > jm + 1 || k > km + 1) return;
+  T acc[10];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) acc[q] = T(0);
+  visc_diu_terms(std::make_integer_sequence<int, 18>{}, acc, A, B, W, lay_fgh(im, jm),
+                 lay_sm(im, jm), i, j, k);
+  // calc_sm: diu1, 5, 9 at their own cell (4 G diu), the cross derivatives through T12, T23, T13
+  const long n3 = (long)im * jm * km;
+  auto sc = [&](int w, int a, int b, int c) {
+    if (!W.in(a, b, c)) return T(0);
+    return scratch[w * n3 + (a - 1) + (long)im * ((b - 1) + (long)jm * (c - 1))];
+  };
+  if (W.in(i, j, k)) {
+    const T g4 = T(4) * sc(0, i, j, k);
+    acc[1] += g4 * A.diu1[lay_vel(1, im, jm).at(i, j, k)];
+    acc[5] += g4 * A.diu5[lay_vel(5, im, jm).at(i, j, k)];
+    acc[9] += g4 * A.diu9[lay_vel(9, im, jm).at(i, j, k)];
+  }
+  acc[2] += sc(1, i + 1, j, k) + sc(1, i + 1, j - 1, k) + sc(1, i, j, k) + sc(1, i, j - 1, k);
+  acc[4] += sc(1, i, j, k) + sc(1, i, j + 1, k) + sc(1, i - 1, j, k) + sc(1, i - 1, j + 1, k);
+  acc[8] += sc(2, i, j, k) + sc(2, i, j, k + 1) + sc(2, i, j - 1, k) + sc(2, i, j - 1, k + 1);
+  acc[6] += sc(2, i, j + 1, k) + sc(2, i, j + 1, k - 1) + sc(2, i, j, k) + sc(2, i, j, k - 1);
+  acc[3] += sc(3, i + 1, j, k) + sc(3, i + 1, j, k - 1) + sc(3, i, j, k) + sc(3, i, j, k - 1);
+  acc[7] += sc(3, i, j, k) + sc(3, i, j, k + 1) + sc(3, i - 1, j, k) + sc(3, i - 1, j, k + 1);
+  B.diu1[lay_vel(1, im, jm).at(i, j, k)] += acc[1];
+  B.diu2[lay_vel(2, im, jm).at(i, j, k)] += acc[2];
+  B.diu3[lay_vel(3, im, jm).at(i, j, k)] += acc[3];
+  B.diu4[lay_vel(4, im, jm).at(i, j, k)] += acc[4];
+  B.diu5[lay_vel(5, im, jm).at(i, j, k)] += acc[5];
+  B.diu6[lay_vel(6, im, jm).at(i, j, k)] += acc[6];
+  B.diu7[lay_vel(7, im, jm).at(i, j, k)] += acc[7];
+  B.diu8[lay_vel(8, im, jm).at(i, j, k)] += acc[8];
+  B.diu9[lay_vel(9, im, jm).at(i, j, k)] += acc[9];
+}
+
+// The les block of the statement list (ADJ_calc_sm .. ADJ_visc_h, in this order)
+constexpr int LES_BLOCK_FIRST = ADJ_calc_sm, LES_BLOCK_LAST = ADJ_visc_h;
+static_assert(LES_BLOCK_LAST - LES_BLOCK_FIRST == 9, "the les block has 10 statements");
+
+template <typename T>
+void launch_les_block_bw(hipStream_t st, const AdjArr<T>& A, const AdjArr<T>& B,
+                         const AdjPar<T>& P, T* scratch, int im, int jm, int km) {
+  // scratch: G, T12, T23, T13 (4 im*jm*km), then the weight tables
+  T* tab = scratch + 4 * (long)im * jm * km;
+  const int ntab = (im + 2) + (jm + 2) + (km + 2);
+  hipLaunchKernelGGL(k_les_tab<T>, dim3((ntab + 255) / 256), dim3(256), 0, st, tab, A.dx1, A.dy1,
+                     A.dzn, im, jm, km);
+  const LesTab<T> W{tab, im, jm, km};
+  const dim3 gx((im + 2 + 63) / 64, (jm + 2 + 3) / 4, km + 2), blk(64, 4);
+  hipLaunchKernelGGL(k_visc_sm_bw<T>, gx, blk, 0, st, A, B, W);
+  const int nk = (im + 2) * (jm + 3), nj = (im + 2) * (km + 2), ni = (jm + 3) * (km + 2);
+  hipLaunchKernelGGL(k_bsm_bw<T>, dim3((nk + 255) / 256, 2), dim3(256), 0, st, B.sm, im, jm, km, 0);
+  hipLaunchKernelGGL(k_bsm_bw<T>, dim3((nj + 255) / 256, 2), dim3(256), 0, st, B.sm, im, jm, km, 1);
+  hipLaunchKernelGGL(k_bsm_bw<T>, dim3((ni + 255) / 256, 2), dim3(256), 0, st, B.sm, im, jm, km, 2);
+  hipLaunchKernelGGL(k_calc_sm_bw<T>, dim3((im + 63) / 64, (jm + 3) / 4, km), blk, 0, st, A, B, P,
+                     scratch, im, jm, km);
+  hipLaunchKernelGGL(k_les_diu_bw<T>, gx, blk, 0, st, A, B, W, scratch);
+}
