@@ -357,6 +357,18 @@ bool is_block(const int* ops, int q0, int first, int last) {
   return true;
 }
 
+// the first statement of the hand-transposed point-wise block that ends at ops[q], or -1
+int point_block(const int* ops, int q) {
+  static const int blocks[4][2] = {{ADJ_velnw_u, ADJ_velnw_w},
+                                   {ADJ_fb_usum, ADJ_fb_h},
+                                   {ADJ_adam_save_f, ADJ_adam_old_h},
+                                   {ADJ_rhs_uvw, ADJ_rhs_fgh}};
+  for (const auto& b : blocks)
+    if (ops[q] == b[1] && q >= b[1] - b[0] && is_block(ops, q - (b[1] - b[0]), b[0], b[1]))
+      return b[0];
+  return -1;
+}
+
 bool is_seq(const int* ops, int n, int q, std::initializer_list<int> want) {
   if (q + (int)want.size() > n) return false;
   int z = q;
@@ -547,6 +559,10 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
         is_block(ops, q - (VEL_BLOCK_LAST - VEL_BLOCK_FIRST), VEL_BLOCK_FIRST, VEL_BLOCK_LAST)) {
       launch_vel_block_bw<T>(h->st, A, B, P, h->im, h->jm, h->km);
       q -= VEL_BLOCK_LAST - VEL_BLOCK_FIRST;
+    } else if ((h->hand & 2) && point_block(ops, q) >= 0) {
+      const int first = point_block(ops, q);
+      launch_point_block_bw<T>(first, h->st, A, B, P, h->im, h->jm, h->km);
+      q -= op - first;
     } else if ((h->hand & 2) && op == LES_BLOCK_LAST && q >= LES_BLOCK_LAST - LES_BLOCK_FIRST &&
                is_block(ops, q - (LES_BLOCK_LAST - LES_BLOCK_FIRST), LES_BLOCK_FIRST,
                         LES_BLOCK_LAST)) {

@@ -639,3 +639,169 @@ void launch_les_block_bw(hipStream_t st, const AdjArr<T>& A, const AdjArr<T>& B,
                      scratch, im, jm, km);
   hipLaunchKernelGGL(k_les_diu_bw<T>, gx, blk, 0, st, A, B, W, scratch);
 }
+
+// ---- the point-wise and two-point statements around the stencils, transposed ------------------
+__device__ __forceinline__ Lay lay_p(int im, int jm) { return Lay{0, 0, 0, im + 3, jm + 3}; }
+__device__ __forceinline__ Lay lay_in(int im, int jm) { return Lay{1, 1, 1, im, jm}; }  // fold, fd, cd
+__device__ __forceinline__ Lay lay_rhs(int im, int jm) { return Lay{0, 0, 0, im + 2, jm + 2}; }
+
+// velnw.f95:27-59: u += dt (f - (p(i+1) - p(i)) / ro / dxs(i)) on IN, v alike along j, w along k
+// for k <= km-1.  Transposed: f, g, h adjoints += dt times u's, v's, w's; p's adjoint gathers
+// the pressure-gradient terms of the points that read it.  Thread: p element (i, j, k), 1..im+1,
+// 1..jm+1, 1..km.
+template <typename T>
+__global__ __launch_bounds__(256) void k_velnw_bw(AdjArr<T> A, AdjArr<T> B, AdjPar<T> P, int im,
+                                                  int jm, int km) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > im + 1 || j > jm + 1 || k > km) return;
+  const Lay U = lay_uv(im, jm), W = lay_w(im, jm), F = lay_fgh(im, jm);
+  auto dxs = [&](int a) { return A.dxs[a]; };
+  auto dys = [&](int b) { return A.dys[b]; };
+  auto dzs = [&](int c) { return A.dzs[c + 1]; };
+  const T c = P.dt / P.ro;
+  T pb = T(0);
+  if (in_box(i, j, k, im, jm, 1, km)) {
+    const T ub = B.u[U.at(i, j, k)], vb = B.v[U.at(i, j, k)];
+    pb += c * ub / dxs(i) + c * vb / dys(j);
+    const long e = F.at(i, j, k);
+    B.f[e] += P.dt * ub;
+    B.g[e] += P.dt * vb;
+    if (k <= km - 1) {
+      const T wb = B.w[W.at(i, j, k)];
+      pb += c * wb / dzs(k);
+      B.h[e] += P.dt * wb;
+    }
+  }
+  if (in_box(i - 1, j, k, im, jm, 1, km)) pb -= c * B.u[U.at(i - 1, j, k)] / dxs(i - 1);
+  if (in_box(i, j - 1, k, im, jm, 1, km)) pb -= c * B.v[U.at(i, j - 1, k)] / dys(j - 1);
+  if (in_box(i, j, k - 1, im, jm, 1, km - 1)) pb -= c * B.w[W.at(i, j, k - 1)] / dzs(k - 1);
+  B.p[lay_p(im, jm).at(i, j, k)] += pb;
+}
+
+// feedbf.f95:41-65 with the drag field: usum = (usum + u) bmask1, then f += alpha usum dt +
+// beta cd u bmask1 (v, vsum, cmask1, g and w, wsum, dmask1, h alike), on IN.  Transposed per
+// cell: the f adjoint stays; usum's gains alpha dt f_b, cd's beta u bmask1 f_b, u's beta cd
+// bmask1 f_b; then usum's adjoint passes through the mask to the old usum and to u.
+template <typename T>
+__global__ __launch_bounds__(256) void k_feedbf_bw(AdjArr<T> A, AdjArr<T> B, AdjPar<T> P, int im,
+                                                   int jm, int km) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > im || j > jm) return;
+  const Lay U = lay_uv(im, jm), W = lay_w(im, jm), F = lay_fgh(im, jm), C = lay_in(im, jm);
+  const long e = F.at(i, j, k), ec = C.at(i, j, k);
+  const T cd = A.cd[ec];
+  const T m[3] = {A.bmask1[Lay{-1, 0, 0, im + 3, jm + 2}.at(i, j, k)],
+                  A.cmask1[Lay{0, -1, 0, im + 2, jm + 3}.at(i, j, k)],
+                  A.dmask1[Lay{0, 0, 0, im + 2, jm + 2}.at(i, j, k)]};
+  T* tb[3] = {B.f, B.g, B.h};
+  T* sb[3] = {B.usum, B.vsum, B.wsum};
+  const T* a[3] = {A.u, A.v, A.w};
+  T* ab[3] = {B.u, B.v, B.w};
+  T cdb = T(0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const long ea = q < 2 ? U.at(i, j, k) : W.at(i, j, k);
+    const T y = tb[q][e];
+    const T s_b = sb[q][e] + P.alpha * P.dt * y;
+    cdb += P.beta * a[q][ea] * m[q] * y;
+    ab[q][ea] += P.beta * cd * m[q] * y + m[q] * s_b;
+    sb[q][e] = m[q] * s_b;
+  }
+  B.cd[ec] += cdb;
+}
+
+// adam.f95:35-43: fd = f; f = 1.5 f - 0.5 fold; fold = fd (g, h alike), on IN.  Transposed per
+// cell: f_b' = 1.5 f_b + fold_b + fd_b, fold_b' = -0.5 f_b, fd_b' = 0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adam_bw(AdjArr<T> B, int im, int jm, int km) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > im || j > jm) return;
+  const long e = lay_fgh(im, jm).at(i, j, k), eo = lay_in(im, jm).at(i, j, k);
+  T* fb[3] = {B.f, B.g, B.h};
+  T* ob[3] = {B.fold, B.gold, B.hold};
+  T* db[3] = {B.fd, B.gd, B.hd};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T y = fb[q][e];
+    fb[q][e] = T(1.5) * y + ob[q][eo] + db[q][eo];
+    ob[q][eo] = T(-0.5) * y;
+    db[q][eo] = T(0);
+  }
+}
+
+// press.f95:59-64: rhs = div(u, v, w) on IN, then rhs = div(f, g, h) + rhs / dt.  Transposed:
+// with r the rhs adjoint, f's gathers r(i)/dx1(i) - r(i+1)/dx1(i+1) (g, h alike along j, k) and
+// u's the same of r / dt.  Thread: element (i, j, k), 0..im, 0..jm, 0..km.  The rhs adjoint is
+// left as it is: no statement before these reads rhs.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_bw(AdjArr<T> A, AdjArr<T> B, AdjPar<T> P, int im,
+                                                int jm, int km) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > im || j > jm || k > km) return;
+  const Lay R = lay_rhs(im, jm), U = lay_uv(im, jm), W = lay_w(im, jm), F = lay_fgh(im, jm);
+  auto dx1 = [&](int a) { return A.dx1[a + 1]; };
+  auto dy1 = [&](int b) { return A.dy1[b]; };
+  auto dzn = [&](int c) { return A.dzn[c + 1]; };
+  auto r = [&](int a, int b, int c) {
+    return in_box(a, b, c, im, jm, 1, km) ? B.rhs[R.at(a, b, c)] : T(0);
+  };
+  const T rc = r(i, j, k);
+  const bool in = in_box(i, j, k, im, jm, 1, km);
+  T xf = (in ? rc / dx1(i) : T(0)), yf = (in ? rc / dy1(j) : T(0)), zf = (in ? rc / dzn(k) : T(0));
+  T xu = (in ? rc / P.dt / dx1(i) : T(0)), yu = (in ? rc / P.dt / dy1(j) : T(0)),
+    zu = (in ? rc / P.dt / dzn(k) : T(0));
+  if (in_box(i + 1, j, k, im, jm, 1, km)) {
+    const T rr = r(i + 1, j, k);
+    xf -= rr / dx1(i + 1);
+    xu -= rr / P.dt / dx1(i + 1);
+  }
+  if (in_box(i, j + 1, k, im, jm, 1, km)) {
+    const T rr = r(i, j + 1, k);
+    yf -= rr / dy1(j + 1);
+    yu -= rr / P.dt / dy1(j + 1);
+  }
+  if (in_box(i, j, k + 1, im, jm, 1, km)) {
+    const T rr = r(i, j, k + 1);
+    zf -= rr / dzn(k + 1);
+    zu -= rr / P.dt / dzn(k + 1);
+  }
+  const long e = F.at(i, j, k);
+  B.f[e] += xf;
+  B.g[e] += yf;
+  B.h[e] += zf;
+  B.u[U.at(i, j, k)] += xu;
+  B.v[U.at(i, j, k)] += yu;
+  B.w[W.at(i, j, k)] += zu;
+}
+
+// blocks of consecutive statements with these transposes
+static_assert(ADJ_velnw_w - ADJ_velnw_u == 2, "velnw block");
+static_assert(ADJ_fb_h - ADJ_fb_usum == 5, "feedbf block");
+static_assert(ADJ_adam_old_h - ADJ_adam_save_f == 8, "adam block");
+static_assert(ADJ_rhs_fgh - ADJ_rhs_uvw == 1, "rhs block");
+
+template <typename T>
+void launch_point_block_bw(int first, hipStream_t st, const AdjArr<T>& A, const AdjArr<T>& B,
+                           const AdjPar<T>& P, int im, int jm, int km) {
+  const dim3 blk(64, 4);
+  if (first == ADJ_velnw_u)
+    hipLaunchKernelGGL(k_velnw_bw<T>, dim3((im + 1 + 63) / 64, (jm + 1 + 3) / 4, km), blk, 0, st, A,
+                       B, P, im, jm, km);
+  else if (first == ADJ_fb_usum)
+    hipLaunchKernelGGL(k_feedbf_bw<T>, dim3((im + 63) / 64, (jm + 3) / 4, km), blk, 0, st, A, B, P,
+                       im, jm, km);
+  else if (first == ADJ_adam_save_f)
+    hipLaunchKernelGGL(k_adam_bw<T>, dim3((im + 63) / 64, (jm + 3) / 4, km), blk, 0, st, B, im, jm,
+                       km);
+  else if (first == ADJ_rhs_uvw)
+    hipLaunchKernelGGL(k_rhs_bw<T>, dim3((im + 1 + 63) / 64, (jm + 1 + 3) / 4, km + 1), blk, 0, st,
+                       A, B, P, im, jm, km);
+}
