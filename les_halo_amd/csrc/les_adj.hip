@@ -1126,7 +1126,74 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
         }
       if ((r = copy_arrays(h, m, dst, src, bytes))) return fail_slots(r);
     }
-    for (int c = nck - 1; c >= 0; --c) {
+    // FP32 with the hand-written transposes: the segment is recomputed by the plain handle's
+    // fused kernels (the stages before the pressure solve; p from the forward window's copy),
+    // and each step's slot takes the arrays those transposes read: u, v, w, the snapshot
+    // before the outflow, uout, diu1..9 and sm.  Its values are the statement kernels' (the
+    // window's forward runs the same kernels).
+    const bool plain_rc = plain && h->hand && pst;
+    const int kept[] = {ADJ_A_u, ADJ_A_v, ADJ_A_w, ADJ_A_sm, ADJ_A_diu1, ADJ_A_diu2, ADJ_A_diu3,
+                        ADJ_A_diu4, ADJ_A_diu5, ADJ_A_diu6, ADJ_A_diu7, ADJ_A_diu8, ADJ_A_diu9};
+    constexpr int NKEPT = sizeof(kept) / sizeof(kept[0]);
+    void* kept_plain[NKEPT] = {};
+    if (plain_rc) {
+      const int fid[NKEPT] = {LES_U, LES_V, LES_W, LES_SM, LES_DIU1, LES_DIU2, LES_DIU3,
+                              LES_DIU4, LES_DIU5, LES_DIU6, LES_DIU7, LES_DIU8, LES_DIU9};
+      for (int q = 0; q < NKEPT; ++q) kept_plain[q] = les::field_ptr(h->ph, fid[q]);
+    }
+    for (int c = nck - 1; c >= 0 && plain_rc; --c) {
+      const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+      {  // the checkpoint into the plain handle's fields
+        void* dst[NCARRIED];
+        const void* src[NCARRIED];
+        size_t bytes[NCARRIED];
+        const char* b = ck + sb * c;
+        for (int q = 0; q < NCARRIED; ++q) {
+          dst[q] = Ac[q];
+          src[q] = b;
+          bytes[q] = h->n[CARRIED[q]] * tsize(h);
+          b += bytes[q];
+        }
+        if ((r = copy_arrays(h, NCARRIED, dst, src, bytes))) return fail_slots(r);
+      }
+      les::set_drag(h->ph, static_cast<const float*>(h->A[ADJ_A_cd]));
+      for (int s = s0; s < s1; ++s) {
+        point(s - s0);
+        if ((r = les::stages_external(h->ph, h->st, 2, static_cast<float*>(h->A[ADJ_A_u_pre]),
+                                      static_cast<float*>(h->A[ADJ_A_v_pre]),
+                                      static_cast<float*>(h->A[ADJ_A_w_pre]),
+                                      &static_cast<AdjScal<float>*>(h->S)->uout))) {
+          les::set_drag(h->ph, nullptr);
+          return fail_slots(afail(h, r, "window: plain stages"));
+        }
+        void* dst[NKEPT + 1];
+        const void* src[NKEPT + 1];
+        size_t bytes[NKEPT + 1];
+        for (int q = 0; q < NKEPT; ++q) {
+          dst[q] = h->A[kept[q]];
+          src[q] = kept_plain[q];
+          bytes[q] = h->n[kept[q]] * tsize(h);
+        }
+        dst[NKEPT] = Ac[3];  // p after the step, from the forward window
+        src[NKEPT] = pst + pbytes * s;
+        bytes[NKEPT] = pbytes;
+        if ((r = copy_arrays(h, NKEPT + 1, dst, src, bytes))) {
+          les::set_drag(h->ph, nullptr);
+          return fail_slots(r);
+        }
+      }
+      les::set_drag(h->ph, nullptr);
+      // the SOR's transpose through the split form, as after a skipped solve (the statement
+      // sweeps if it does not apply)
+      h->sor_split = h->p_mirror && h->cfg.nmaxp >= 1;
+      for (int s = s1 - 1; s >= s0; --s) {
+        point(s - s0);
+        if ((r = zero_temporaries(h))) return fail_slots(r);
+        seed_probes(h, nprobes, wgt);
+        step_bw(h);
+      }
+    }
+    for (int c = nck - 1; c >= 0 && !plain_rc; --c) {
       const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);
       for (int s = s0; s < s1; ++s) {
         point(s - s0);

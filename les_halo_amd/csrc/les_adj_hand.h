@@ -13,7 +13,8 @@
 //   k_vel2_bw       cov adjoints -> nou, diu adjoints (pointwise, in registers) -> u, v, w
 //                   adjoints; f, g, h adjoints zeroed on the points velFG writes
 // The nou adjoints other than through cov are zero (no statement reads nou but cov), so
-// k_vel2_bw forms them from the cov adjoints alone.
+// k_vel2_bw forms them from the cov adjoints alone, and it recomputes nou from u, v, w: the
+// transposes read no nou array.
 
 // Fortran layout of a 3-D array: element (i, j, k) at (i - i0) + ni * ((j - j0) + nj * (k - k0))
 struct Lay {
@@ -193,7 +194,30 @@ __global__ __launch_bounds__(256) void k_vel2_bw(AdjArr<T> A, AdjArr<T> B, int i
   auto dy1 = [&](int b) { return A.dy1[b]; };
   auto dzn = [&](int c) { return A.dzn[c + 1]; };
   auto dzs = [&](int c) { return A.dzs[c + 1]; };
-  const VelPtrs<T> Cb = cov_of(B), Db = diu_of(B), Nf = nou_of(A), Df = diu_of(A);
+  const T two(2), half(0.5);
+  const VelPtrs<T> Cb = cov_of(B), Db = diu_of(B), Df = diu_of(A);
+  const Lay U = lay_uv(im, jm), W = lay_w(im, jm), F = lay_fgh(im, jm);
+  // nou q at (a, b, c) from u, v, w (vel2.f95:48-123; the statements' expressions), so the
+  // recomputation need not keep the nou arrays
+  auto nou = [&](int q, int a, int b, int c) {
+    switch (q) {
+      case 1: return (A.u[U.at(a - 1, b, c)] + A.u[U.at(a, b, c)]) / two;
+      case 5: return (A.v[U.at(a, b - 1, c)] + A.v[U.at(a, b, c)]) / two;
+      case 9: return (A.w[W.at(a, b, c - 1)] + A.w[W.at(a, b, c)]) / two;
+      case 2: return (dx1(a + 1) * A.v[U.at(a, b - 1, c)] + dx1(a) * A.v[U.at(a + 1, b - 1, c)]) /
+                     (dx1(a) + dx1(a + 1));
+      case 3: return (dx1(a + 1) * A.w[W.at(a, b, c - 1)] + dx1(a) * A.w[W.at(a + 1, b, c - 1)]) /
+                     (dx1(a) + dx1(a + 1));
+      case 4: return (dy1(b + 1) * A.u[U.at(a - 1, b, c)] + dy1(b) * A.u[U.at(a - 1, b + 1, c)]) /
+                     (dy1(b) + dy1(b + 1));
+      case 6: return (dy1(b + 1) * A.w[W.at(a, b, c - 1)] + dy1(b) * A.w[W.at(a, b + 1, c - 1)]) /
+                     (dy1(b) + dy1(b + 1));
+      case 7: return (dzn(c + 1) * A.u[U.at(a - 1, b, c)] + dzn(c) * A.u[U.at(a - 1, b, c + 1)]) /
+                     (dzn(c) + dzn(c + 1));
+      default: return (dzn(c + 1) * A.v[U.at(a, b - 1, c)] + dzn(c) * A.v[U.at(a, b - 1, c + 1)]) /
+                      (dzn(c) + dzn(c + 1));
+    }
+  };
   // the adjoints of nou q and diu q at point (a, b, c) of their box (0 outside it)
   auto noub = [&](int q, int a, int b, int c) {
     if (!in_box(a, b, c, im, jm, 1, vel_khi(q, km))) return T(0);
@@ -203,10 +227,9 @@ __global__ __launch_bounds__(256) void k_vel2_bw(AdjArr<T> A, AdjArr<T> B, int i
   auto diub = [&](int q, int a, int b, int c) {
     if (!in_box(a, b, c, im, jm, 1, vel_khi(q, km))) return T(0);
     const long e = lay_vel(q, im, jm).at(a, b, c);
-    return Db.a[q][e] + Cb.a[q][e] * Nf.a[q][e];
+    return Db.a[q][e] + Cb.a[q][e] * nou(q, a, b, c);
   };
   auto inq = [&](int q, int a, int b, int c) { return in_box(a, b, c, im, jm, 1, vel_khi(q, km)); };
-  const T two(2), half(0.5);
   // u: nou1 = (u(i-1) + u(i))/2, diu1 = (u(i) - u(i-1))/dx1(i), diu2 = 2 (u(j) - u(j-1)) /
   // (dy1(j-1) + dy1(j)), diu3 = (u(k) - u(k-1))/dzs(k-1), nou4 = (dy1(j+1) u(i-1,j) + dy1(j)
   // u(i-1,j+1)) / (dy1(j) + dy1(j+1)), nou7 = (dzn(k+1) u(i-1,k) + dzn(k) u(i-1,k+1)) /
@@ -252,7 +275,6 @@ __global__ __launch_bounds__(256) void k_vel2_bw(AdjArr<T> A, AdjArr<T> B, int i
   if (inq(7, i + 1, j, k)) wb -= two * diub(7, i + 1, j, k) / (dx1(i) + dx1(i + 1));
   if (inq(8, i, j, k)) wb += two * diub(8, i, j, k) / (dy1(j - 1) + dy1(j));
   if (inq(8, i, j + 1, k)) wb -= two * diub(8, i, j + 1, k) / (dy1(j) + dy1(j + 1));
-  const Lay U = lay_uv(im, jm), W = lay_w(im, jm), F = lay_fgh(im, jm);
   B.u[U.at(i, j, k)] += ub;
   B.v[U.at(i, j, k)] += vb;
   B.w[W.at(i, j, k)] += wb;

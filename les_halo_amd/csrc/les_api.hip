@@ -1131,6 +1131,32 @@ int step_external(les_handle* h, hipStream_t st, int n) {
   h->st = own;
   return r;
 }
+int stages_external(les_handle* h, hipStream_t st, int n, float* u_pre, float* v_pre, float* w_pre,
+                    float* uout_dst) {
+  if (!h || !u_pre || !v_pre || !w_pre || !uout_dst) return LES_E_INVALID;
+  const hipStream_t own = h->st;
+  h->st = st;
+  const Dims d = h->d;
+  const Grid1D g = grid_of(h);
+  const Fields F = fields_of(h);
+  launch_velnw_inflow(st, d, g, F, h->ph, 1, n < 1 ? 2 : n, h->shp_velnw);
+  hipError_t e = hipMemcpyAsync(u_pre, F.u, (size_t)h->cnt[LES_U] * sizeof(float),
+                                hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(v_pre, F.v, (size_t)h->cnt[LES_V] * sizeof(float), hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w_pre, F.w, (size_t)h->cnt[LES_W] * sizeof(float), hipMemcpyDeviceToDevice, st);
+  launch_uout(st, d, F, h->sc, reinterpret_cast<float*>(h->rb.partials));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(uout_dst, &h->sc->uout, sizeof(float), hipMemcpyDeviceToDevice, st);
+  launch_bondv1_uvw(st, d, g, F, h->ph, h->sc);
+  launch_velfg(st, d, g, F, h->ph, 1, 1);
+  launch_calc_sm(st, d, g, F, h->ph, 1, h->shp_sm);
+  launch_calc_visc_adam(st, d, g, F, 1, 1, h->shp_visc);
+  h->st = own;
+  if (e != hipSuccess) return fail(h, LES_E_HIP, std::string("stages: ") + hipGetErrorString(e));
+  return check_launch(h);
+}
 }  // namespace les
 
 extern "C" int les_press_sor_solve(les_handle* h) {

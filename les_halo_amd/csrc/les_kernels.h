@@ -316,5 +316,10 @@ float* field_ptr(::les_handle* h, int field);
 void set_drag(::les_handle* h, const float* cd);
 void set_p_consistent(::les_handle* h, bool consistent);
 int step_external(::les_handle* h, hipStream_t st, int n);
+// the stages of that step before the pressure solve (velnw to visc/adam), for the adjoint
+// form's recomputation: u, v, w after velnw and the inflow (before the outflow) are copied to
+// u_pre, v_pre, w_pre, and uout to uout_dst (device pointers)
+int stages_external(::les_handle* h, hipStream_t st, int n, float* u_pre, float* v_pre,
+                    float* w_pre, float* uout_dst);
 
 }  // namespace les
