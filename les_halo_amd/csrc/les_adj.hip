@@ -369,6 +369,19 @@ int point_block(const int* ops, int q) {
   return -1;
 }
 
+// the first statement of the hand-transposed boundary / scalar block that ends at ops[q], or -1
+int bc_block(const int* ops, int q) {
+  static const int blocks[5][2] = {{ADJ_inflow_u, ADJ_inflow_w},
+                                   {ADJ_perj0_u, ADJ_top_w},
+                                   {ADJ_bfg_f, ADJ_bfg_h1},
+                                   {ADJ_rhs_adj, ADJ_rhs_adj},
+                                   {ADJ_pav_adj, ADJ_pav_adj}};
+  for (const auto& b : blocks)
+    if (ops[q] == b[1] && q >= b[1] - b[0] && is_block(ops, q - (b[1] - b[0]), b[0], b[1]))
+      return b[0];
+  return -1;
+}
+
 bool is_seq(const int* ops, int n, int q, std::initializer_list<int> want) {
   if (q + (int)want.size() > n) return false;
   int z = q;
@@ -559,6 +572,10 @@ void run_ops_bw(les_adj* h, const int* ops, int nops) {
         is_block(ops, q - (VEL_BLOCK_LAST - VEL_BLOCK_FIRST), VEL_BLOCK_FIRST, VEL_BLOCK_LAST)) {
       launch_vel_block_bw<T>(h->st, A, B, P, h->im, h->jm, h->km);
       q -= VEL_BLOCK_LAST - VEL_BLOCK_FIRST;
+    } else if ((h->hand & 2) && bc_block(ops, q) >= 0) {
+      const int first = bc_block(ops, q);
+      launch_bc_block_bw<T>(first, h->st, B, SB, h->im, h->jm, h->km);
+      q -= op - first;
     } else if ((h->hand & 2) && point_block(ops, q) >= 0) {
       const int first = point_block(ops, q);
       launch_point_block_bw<T>(first, h->st, A, B, P, h->im, h->jm, h->km);

@@ -25,14 +25,14 @@ struct Lay {
 };
 
 // nou, diu, cov q: (lo:im+2, 0:jm+2, 0:km+2), lo = -1 for q = 1, 5 and 0 otherwise
-__device__ __forceinline__ Lay lay_vel(int q, int im, int jm) {
+__host__ __device__ inline Lay lay_vel(int q, int im, int jm) {
   const int lo = (q == 1 || q == 5) ? -1 : 0;
   return Lay{lo, 0, 0, im + 3 - lo, jm + 3};
 }
 // u, v: (0:im+1, -1:jm+1, 0:km+1); w: (0:im+1, -1:jm+1, -1:km+1); f, g, h: (0:im, 0:jm, 0:km)
-__device__ __forceinline__ Lay lay_uv(int im, int jm) { return Lay{0, -1, 0, im + 2, jm + 3}; }
-__device__ __forceinline__ Lay lay_w(int im, int jm) { return Lay{0, -1, -1, im + 2, jm + 3}; }
-__device__ __forceinline__ Lay lay_fgh(int im, int jm) { return Lay{0, 0, 0, im + 1, jm + 1}; }
+__host__ __device__ inline Lay lay_uv(int im, int jm) { return Lay{0, -1, 0, im + 2, jm + 3}; }
+__host__ __device__ inline Lay lay_w(int im, int jm) { return Lay{0, -1, -1, im + 2, jm + 3}; }
+__host__ __device__ inline Lay lay_fgh(int im, int jm) { return Lay{0, 0, 0, im + 1, jm + 1}; }
 
 // the boxes of the block's statements: IN (1:im, 1:jm, 1:km), K1 (k to km+1), KL (k to km-1)
 __device__ __forceinline__ bool in_box(int i, int j, int k, int im, int jm, int klo, int khi) {
@@ -319,7 +319,7 @@ void launch_vel_block_bw(hipStream_t st, const AdjArr<T>& A, const AdjArr<T>& B,
 //                   from calc_sm (through the scratch array)
 // with alpha = +-(f, g or h adjoint) / h.
 
-__device__ __forceinline__ Lay lay_sm(int im, int jm) { return Lay{-1, -1, 0, im + 3, jm + 3}; }
+__host__ __device__ inline Lay lay_sm(int im, int jm) { return Lay{-1, -1, 0, im + 3, jm + 3}; }
 
 // one viscous flux of les.f95:78-180: statement s (0 f, 1 g, 2 h), face t (0 x2, 1 x1, 2 y2,
 // 3 y1, 4 z2, 5 z1); term n = 6 s + t
@@ -663,9 +663,9 @@ void launch_les_block_bw(hipStream_t st, const AdjArr<T>& A, const AdjArr<T>& B,
 }
 
 // ---- the point-wise and two-point statements around the stencils, transposed ------------------
-__device__ __forceinline__ Lay lay_p(int im, int jm) { return Lay{0, 0, 0, im + 3, jm + 3}; }
-__device__ __forceinline__ Lay lay_in(int im, int jm) { return Lay{1, 1, 1, im, jm}; }  // fold, fd, cd
-__device__ __forceinline__ Lay lay_rhs(int im, int jm) { return Lay{0, 0, 0, im + 2, jm + 2}; }
+__host__ __device__ inline Lay lay_p(int im, int jm) { return Lay{0, 0, 0, im + 3, jm + 3}; }
+__host__ __device__ inline Lay lay_in(int im, int jm) { return Lay{1, 1, 1, im, jm}; }  // fold, fd, cd
+__host__ __device__ inline Lay lay_rhs(int im, int jm) { return Lay{0, 0, 0, im + 2, jm + 2}; }
 
 // velnw.f95:27-59: u += dt (f - (p(i+1) - p(i)) / ro / dxs(i)) on IN, v alike along j, w along k
 // for k <= km-1.  Transposed: f, g, h adjoints += dt times u's, v's, w's; p's adjoint gathers
@@ -826,4 +826,128 @@ void launch_point_block_bw(int first, hipStream_t st, const AdjArr<T>& A, const 
   else if (first == ADJ_rhs_uvw)
     hipLaunchKernelGGL(k_rhs_bw<T>, dim3((im + 1 + 63) / 64, (jm + 1 + 3) / 4, km + 1), blk, 0, st,
                        A, B, P, im, jm, km);
+}
+
+// ---- boundary copies and the scalar subtractions, transposed -----------------------------------
+// bondv1_ext_wind_profile.f95's inflow (u = wind at i = 0, 1; v = w = 0 there): the wind
+// adjoint gathers u's two inflow adjoints, which (and v's, w's) are consumed.  Thread (j, k).
+template <typename T>
+__global__ __launch_bounds__(256) void k_inflow_bw(AdjArr<T> B, int im, int jm, int km) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= jm * km) return;
+  const int j = 1 + t % jm, k = 1 + t / jm;
+  const Lay U = lay_uv(im, jm), W = lay_w(im, jm);
+  T s = T(0);
+  for (int i = 0; i <= 1; ++i) {
+    s += B.u[U.at(i, j, k)];
+    B.u[U.at(i, j, k)] = T(0);
+    B.v[U.at(i, j, k)] = T(0);
+    B.w[W.at(i, j, k)] = T(0);
+  }
+  B.wind[(j - 1) + (long)jm * (k - 1)] += s;
+}
+
+// bondv1.f95:81-137's faces: phase 0 the k faces (u, v at k = 0 the negated, at km+1 the copied
+// level; w = 0 at k = 0 and km), phase 1 the periodic j faces (u, v for k 0..km+1, w for 0..km),
+// each transposed (the k faces first: they read j faces the periodic copies wrote).
+// blockIdx.y: the array and face.  w's k-face statements cover i = -1..im+1 and j = -1..jm+1,
+// i = -1 meaning the element before w(0, j, k) in memory, as in the reference's loops.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_uvw_bw(AdjArr<T> B, int im, int jm, int km, int phase) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int a = blockIdx.y >> 1, side = blockIdx.y & 1;  // a: 0 u, 1 v, 2 w
+  T* y = a == 0 ? B.u : (a == 1 ? B.v : B.w);
+  const Lay L = a == 2 ? lay_w(im, jm) : lay_uv(im, jm);
+  if (phase == 0) {
+    if (a < 2) {  // i 0..im+1, j 0..jm+1
+      const int ni = im + 2;
+      if (t >= ni * (jm + 2)) return;
+      const int i = t % ni, j = t / ni;
+      const long dst = L.at(i, j, side == 0 ? 0 : km + 1), src = L.at(i, j, side == 0 ? 1 : km);
+      y[src] += side == 0 ? -y[dst] : y[dst];
+      y[dst] = T(0);
+    } else {  // i -1..im+1, j -1..jm+1
+      const int ni = im + 3;
+      if (t >= ni * (jm + 3)) return;
+      const int i = -1 + t % ni, j = -1 + t / ni;
+      y[L.at(i, j, side == 0 ? 0 : km)] = T(0);
+    }
+  } else {  // i 0..im+1, k 0..km+1 (w: 0..km); side 0: j = 0 from jm, 1: j = jm+1 from 1
+    const int ni = im + 2, nk = a == 2 ? km + 1 : km + 2;
+    if (t >= ni * nk) return;
+    const int i = t % ni, k = t / ni;
+    const long dst = L.at(i, side == 0 ? 0 : jm + 1, k), src = L.at(i, side == 0 ? jm : 1, k);
+    y[src] += y[dst];
+    y[dst] = T(0);
+  }
+}
+
+// bondFG.f95:15-32: f(0) = f(1), g(j = 0) = g(jm), h(k = 0) = h(km) = 0, transposed
+// (blockIdx.y: the statement)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bfg_bw(AdjArr<T> B, int im, int jm, int km) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const Lay F = lay_fgh(im, jm);
+  switch (blockIdx.y) {
+    case 0: {  // j 1..jm, k 1..km
+      if (t >= jm * km) return;
+      const int j = 1 + t % jm, k = 1 + t / jm;
+      B.f[F.at(1, j, k)] += B.f[F.at(0, j, k)];
+      B.f[F.at(0, j, k)] = T(0);
+      return;
+    }
+    case 1: {  // i 1..im, k 1..km
+      if (t >= im * km) return;
+      const int i = 1 + t % im, k = 1 + t / im;
+      B.g[F.at(i, jm, k)] += B.g[F.at(i, 0, k)];
+      B.g[F.at(i, 0, k)] = T(0);
+      return;
+    }
+    default: {  // i 1..im, j 1..jm
+      if (t >= im * jm) return;
+      const int i = 1 + t % im, j = 1 + t / im;
+      B.h[F.at(i, j, blockIdx.y == 2 ? 0 : km)] = T(0);
+      return;
+    }
+  }
+}
+
+// press.f95:82 (rhs -= rhsav) and :139 (p -= pav) transposed: the adjoint stays, and the
+// scalar's adjoint (slot q of the scalar adjoints) takes minus its sum over the interior.
+// Thread: (i, j) with a chunk of 8 levels (one-dimensional workgroups, as scal_add needs).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_scalar_bw(const T* y, Lay L, AdjScal<T>* SB, int q,
+                                                       int im, int jm, int km) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int k0 = 1 + blockIdx.y * 8;
+  T s = T(0);
+  if (t < im * jm) {
+    const int i = 1 + t % im, j = 1 + t / im;
+    for (int k = k0; k < k0 + 8 && k <= km; ++k) s += y[L.at(i, j, k)];
+  }
+  scal_add(SB, q, -s);
+}
+
+static_assert(ADJ_inflow_w - ADJ_inflow_u == 2, "inflow block");
+static_assert(ADJ_top_w - ADJ_perj0_u == 11, "bondv1 face block");
+static_assert(ADJ_bfg_h1 - ADJ_bfg_f == 3, "bondfg block");
+
+template <typename T>
+void launch_bc_block_bw(int first, hipStream_t st, const AdjArr<T>& B, AdjScal<T>* SB, int im,
+                        int jm, int km) {
+  if (first == ADJ_inflow_u) {
+    hipLaunchKernelGGL(k_inflow_bw<T>, dim3((jm * km + 255) / 256), dim3(256), 0, st, B, im, jm, km);
+  } else if (first == ADJ_perj0_u) {
+    const int n0 = (im + 3) * (jm + 3), n1 = (im + 2) * (km + 2);
+    hipLaunchKernelGGL(k_bc_uvw_bw<T>, dim3((n0 + 255) / 256, 6), dim3(256), 0, st, B, im, jm, km, 0);
+    hipLaunchKernelGGL(k_bc_uvw_bw<T>, dim3((n1 + 255) / 256, 6), dim3(256), 0, st, B, im, jm, km, 1);
+  } else if (first == ADJ_bfg_f) {
+    const int n = std::max(std::max(jm * km, im * km), im * jm);
+    hipLaunchKernelGGL(k_bfg_bw<T>, dim3((n + 255) / 256, 4), dim3(256), 0, st, B, im, jm, km);
+  } else if (first == ADJ_rhs_adj || first == ADJ_pav_adj) {
+    const bool r = first == ADJ_rhs_adj;
+    hipLaunchKernelGGL(k_sub_scalar_bw<T>, dim3((im * jm + 255) / 256, (km + 7) / 8), dim3(256), 0,
+                       st, r ? B.rhs : B.p, r ? lay_rhs(im, jm) : lay_p(im, jm), SB, r ? 1 : 2, im,
+                       jm, km);
+  }
 }
