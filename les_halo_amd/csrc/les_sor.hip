@@ -834,14 +834,16 @@ __global__ __launch_bounds__(256) void k_split_gather_t(Dims d, float* pb, Split
 // row and of the other colour's rows at j, j-1, j+1, k-1, k+1 (the sweep's access pattern);
 // each cell's terms in the same order as a one-cell form would take them.
 __global__ __launch_bounds__(256) void k_split_sweep_t(Dims d, Grid1D g, SplitSOR sp, float om,
-                                                       int c, int mode) {
-  // items: group grp of row (j, k), grp fastest, dealt densely over the workgroups
+                                                       int c, int mode, int nthreads) {
+  // items: group grp of row (j, k), grp fastest, dealt densely over the workgroups in the
+  // XCD-aware order of the forward sweep; quotients by multiply-high (sor_split_divisors)
   const int ng = sp.nr >> 2;
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)ng * d.jp * d.kp) return;
-  const int grp = (int)(t % ng);
-  const int row = (int)(t / ng);
-  const int j = 1 + row % d.jp, k = 1 + row / d.jp;
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (t >= nthreads) return;
+  const int row = (int)__umulhi((unsigned)t, sp.m_ng);
+  const int grp = t - row * ng;
+  const int kk = (int)__umulhi((unsigned)row, sp.m_jp);
+  const int j = 1 + row - kk * d.jp, k = 1 + kk;
   const int f = first_i(c, j, k);
   const int i0 = f + 8 * grp;  // the group's first cell
   if (i0 > d.ip) return;
@@ -925,9 +927,9 @@ void launch_split_gather_t(hipStream_t st, Dims d, float* pb, SplitSOR sp) {
 
 void launch_split_sweep_t(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, float omega, int c,
                           int mode) {
-  const long n = (long)(sp.nr / 4) * d.jp * d.kp;
-  hipLaunchKernelGGL(k_split_sweep_t, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, g, sp,
-                     omega, c, mode);
+  int ng, nt, nwg;
+  split_threads(d, sp, ng, nt, nwg);
+  hipLaunchKernelGGL(k_split_sweep_t, dim3(nwg), dim3(256), 0, st, d, g, sp, omega, c, mode, nt);
 }
 
 void launch_split_scatter_t(hipStream_t st, Dims d, float* pb, float* rb, SplitSOR sp) {
