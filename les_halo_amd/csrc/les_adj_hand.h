@@ -75,9 +75,12 @@ __global__ __launch_bounds__(256) void k_velfg_bw(AdjArr<T> A, AdjArr<T> B, AdjP
   auto dy1 = [&](int b) { return A.dy1[b]; };
   auto dzn = [&](int c) { return A.dzn[c + 1]; };
   auto dzs = [&](int c) { return A.dzs[c + 1]; };
-  // the f, g (box IN) and h (box KL) adjoints at (i, j, k) and at its -1 neighbours
+  // the f, g (box IN) and h (box KL) adjoints at (i, j, k) and at its -1 neighbours, 0 outside
+  // the box (loaded at the nearest point of the box either way, so no load waits on a branch)
   auto ld = [&](const T* y, int a, int b, int c, int khi) {
-    return in_box(a, b, c, im, jm, 1, khi) ? y[F.at(a, b, c)] : T(0);
+    const bool ok = in_box(a, b, c, im, jm, 1, khi);
+    const T v = y[F.at(min(max(a, 1), im), min(max(b, 1), jm), min(max(c, 1), khi))];
+    return ok ? v : T(0);
   };
   const T fc = ld(B.f, i, j, k, km), fw = ld(B.f, i - 1, j, k, km), fs = ld(B.f, i, j - 1, k, km),
           fd = ld(B.f, i, j, k - 1, km);
@@ -85,42 +88,38 @@ __global__ __launch_bounds__(256) void k_velfg_bw(AdjArr<T> A, AdjArr<T> B, AdjP
           gd = ld(B.g, i, j, k - 1, km);
   const T hc = ld(B.h, i, j, k, km - 1), hw = ld(B.h, i - 1, j, k, km - 1),
           hs = ld(B.h, i, j - 1, k, km - 1), hd = ld(B.h, i, j, k - 1, km - 1);
-  // a term whose point lies outside the box reads no grid value beyond the arrays' bounds
-  const bool ic = i <= im, jc = j <= jm, kc = k <= km, kh = k <= km - 1;
+  // a term whose point lies outside the box has a zero adjoint; its grid values are read at
+  // indices inside the arrays' bounds (i+1, j+1 clamped)
+  const int ic = min(i, im), jc = min(j, jm);
   const T two(2), half(0.5), vn = P.vn;
   T cb[10], db[10];
   // f: covx1 = (dx1(i+1) cov1(i) + dx1(i) cov1(i+1)) / (dx1(i) + dx1(i+1)), covy1, covz1 the
   // means of cov2 along j and cov3 along k; dfu1 = 2 (diu1(i+1) - diu1(i)) / (dx1(i) + dx1(i+1))
   // + (diu2(j+1) - diu2(j)) / dy1(j) + (diu3(k+1) - diu3(k)) / dzn(k)
-  cb[1] = -((ic ? fc * dx1(i + 1) / (dx1(i) + dx1(i + 1)) : T(0)) +
-            (i > 1 ? fw * dx1(i - 1) / (dx1(i - 1) + dx1(i)) : T(0)));
+  cb[1] = -(fc * dx1(ic + 1) / (dx1(ic) + dx1(ic + 1)) + fw * dx1(i - 1) / (dx1(i - 1) + dx1(i)));
   cb[2] = -half * (fc + fs);
   cb[3] = -half * (fc + fd);
-  db[1] = vn * ((i > 1 ? two * fw / (dx1(i - 1) + dx1(i)) : T(0)) -
-                (ic ? two * fc / (dx1(i) + dx1(i + 1)) : T(0)));
-  db[2] = vn * ((j > 1 ? fs / dy1(j - 1) : T(0)) - (jc ? fc / dy1(j) : T(0)));
-  db[3] = vn * ((k > 1 ? fd / dzn(k - 1) : T(0)) - (kc ? fc / dzn(k) : T(0)));
+  db[1] = vn * (two * fw / (dx1(i - 1) + dx1(i)) - two * fc / (dx1(ic) + dx1(ic + 1)));
+  db[2] = vn * (fs / dy1(j - 1) - fc / dy1(j));
+  db[3] = vn * (fd / dzn(k - 1) - fc / dzn(k));
   // g: covx1 the mean of cov4 along i, covy1 = (dy1(j+1) cov5(j) + dy1(j) cov5(j+1)) /
   // (dy1(j) + dy1(j+1)), covz1 the mean of cov6 along k; dfv1 = (diu4(i+1) - diu4(i)) / dx1(i)
   // + 2 (diu5(j+1) - diu5(j)) / (dy1(j) + dy1(j+1)) + (diu6(k+1) - diu6(k)) / dzn(k)
   cb[4] = -half * (gc + gw);
-  cb[5] = -((jc ? gc * dy1(j + 1) / (dy1(j) + dy1(j + 1)) : T(0)) +
-            (j > 1 ? gs * dy1(j - 1) / (dy1(j - 1) + dy1(j)) : T(0)));
+  cb[5] = -(gc * dy1(jc + 1) / (dy1(jc) + dy1(jc + 1)) + gs * dy1(j - 1) / (dy1(j - 1) + dy1(j)));
   cb[6] = -half * (gc + gd);
-  db[4] = vn * ((i > 1 ? gw / dx1(i - 1) : T(0)) - (ic ? gc / dx1(i) : T(0)));
-  db[5] = vn * ((j > 1 ? two * gs / (dy1(j - 1) + dy1(j)) : T(0)) -
-                (jc ? two * gc / (dy1(j) + dy1(j + 1)) : T(0)));
-  db[6] = vn * ((k > 1 ? gd / dzn(k - 1) : T(0)) - (kc ? gc / dzn(k) : T(0)));
+  db[4] = vn * (gw / dx1(i - 1) - gc / dx1(i));
+  db[5] = vn * (two * gs / (dy1(j - 1) + dy1(j)) - two * gc / (dy1(jc) + dy1(jc + 1)));
+  db[6] = vn * (gd / dzn(k - 1) - gc / dzn(k));
   // h (k <= km-1): covx1, covy1 the means of cov7 along i and cov8 along j, covz1 =
   // (dzn(k+1) cov9(k) + dzn(k) cov9(k+1)) / (dzn(k) + dzn(k+1)); dfw1 = (diu7(i+1) - diu7(i)) /
   // dx1(i) + (diu8(j+1) - diu8(j)) / dy1(j) + (diu9(k+1) - diu9(k)) / dzs(k)
   cb[7] = -half * (hc + hw);
   cb[8] = -half * (hc + hs);
-  cb[9] = -((kh ? hc * dzn(k + 1) / (dzn(k) + dzn(k + 1)) : T(0)) +
-            (k > 1 ? hd * dzn(k - 1) / (dzn(k - 1) + dzn(k)) : T(0)));
-  db[7] = vn * ((i > 1 ? hw / dx1(i - 1) : T(0)) - (ic ? hc / dx1(i) : T(0)));
-  db[8] = vn * ((j > 1 ? hs / dy1(j - 1) : T(0)) - (jc ? hc / dy1(j) : T(0)));
-  db[9] = vn * ((k > 1 ? hd / dzs(k - 1) : T(0)) - (kh ? hc / dzs(k) : T(0)));
+  cb[9] = -(hc * dzn(k + 1) / (dzn(k) + dzn(k + 1)) + hd * dzn(k - 1) / (dzn(k - 1) + dzn(k)));
+  db[7] = vn * (hw / dx1(i - 1) - hc / dx1(i));
+  db[8] = vn * (hs / dy1(j - 1) - hc / dy1(j));
+  db[9] = vn * (hd / dzs(k - 1) - hc / dzs(k));
   const VelPtrs<T> C = cov_of(B), D = diu_of(B);
 #pragma unroll
   for (int q = 1; q <= 9; ++q) {
@@ -218,63 +217,66 @@ __global__ __launch_bounds__(256) void k_vel2_bw(AdjArr<T> A, AdjArr<T> B, int i
                       (dzn(c) + dzn(c + 1));
     }
   };
-  // the adjoints of nou q and diu q at point (a, b, c) of their box (0 outside it)
-  auto noub = [&](int q, int a, int b, int c) {
-    if (!in_box(a, b, c, im, jm, 1, vel_khi(q, km))) return T(0);
+  // term: the adjoint of nou q (isn = 1) or diu q at the statement point (a, b, c) times the
+  // coefficient the statement gives the element there (kind, at that point), 0 if the point is
+  // outside the statement's box.  The loads are made at the nearest point of the box whatever
+  // the test says, so the terms' loads issue without waiting on branches.
+  auto term = [&](int q, bool isn, int a, int b, int c, int kind) {
+    const bool ok = in_box(a, b, c, im, jm, 1, vel_khi(q, km));
+    a = min(max(a, 1), im);
+    b = min(max(b, 1), jm);
+    c = min(max(c, 1), vel_khi(q, km));
     const long e = lay_vel(q, im, jm).at(a, b, c);
-    return Cb.a[q][e] * Df.a[q][e];
+    const T cb = Cb.a[q][e];
+    const T adj = isn ? cb * Df.a[q][e] : Db.a[q][e] + cb * nou(q, a, b, c);
+    T co;
+    switch (kind) {
+      case 0: co = half; break;
+      case 1: co = T(1) / dx1(a); break;
+      case 2: co = T(1) / dy1(b); break;
+      case 3: co = T(1) / dzn(c); break;
+      case 4: co = T(1) / dzs(c - 1); break;
+      case 5: co = two / (dx1(a - 1) + dx1(a)); break;
+      case 6: co = two / (dy1(b - 1) + dy1(b)); break;
+      case 7: co = dx1(a + 1) / (dx1(a) + dx1(a + 1)); break;
+      case 8: co = dx1(a) / (dx1(a) + dx1(a + 1)); break;
+      case 9: co = dy1(b + 1) / (dy1(b) + dy1(b + 1)); break;
+      case 10: co = dy1(b) / (dy1(b) + dy1(b + 1)); break;
+      case 11: co = dzn(c + 1) / (dzn(c) + dzn(c + 1)); break;
+      default: co = dzn(c) / (dzn(c) + dzn(c + 1)); break;
+    }
+    return ok ? adj * co : T(0);
   };
-  auto diub = [&](int q, int a, int b, int c) {
-    if (!in_box(a, b, c, im, jm, 1, vel_khi(q, km))) return T(0);
-    const long e = lay_vel(q, im, jm).at(a, b, c);
-    return Db.a[q][e] + Cb.a[q][e] * nou(q, a, b, c);
-  };
-  auto inq = [&](int q, int a, int b, int c) { return in_box(a, b, c, im, jm, 1, vel_khi(q, km)); };
   // u: nou1 = (u(i-1) + u(i))/2, diu1 = (u(i) - u(i-1))/dx1(i), diu2 = 2 (u(j) - u(j-1)) /
   // (dy1(j-1) + dy1(j)), diu3 = (u(k) - u(k-1))/dzs(k-1), nou4 = (dy1(j+1) u(i-1,j) + dy1(j)
   // u(i-1,j+1)) / (dy1(j) + dy1(j+1)), nou7 = (dzn(k+1) u(i-1,k) + dzn(k) u(i-1,k+1)) /
-  // (dzn(k) + dzn(k+1))
-  T ub = half * (noub(1, i, j, k) + noub(1, i + 1, j, k));
-  if (inq(1, i, j, k)) ub += diub(1, i, j, k) / dx1(i);
-  if (inq(1, i + 1, j, k)) ub -= diub(1, i + 1, j, k) / dx1(i + 1);
-  if (inq(2, i, j, k)) ub += two * diub(2, i, j, k) / (dy1(j - 1) + dy1(j));
-  if (inq(2, i, j + 1, k)) ub -= two * diub(2, i, j + 1, k) / (dy1(j) + dy1(j + 1));
-  if (inq(3, i, j, k)) ub += diub(3, i, j, k) / dzs(k - 1);
-  if (inq(3, i, j, k + 1)) ub -= diub(3, i, j, k + 1) / dzs(k);
-  if (inq(4, i + 1, j, k)) ub += noub(4, i + 1, j, k) * dy1(j + 1) / (dy1(j) + dy1(j + 1));
-  if (inq(4, i + 1, j - 1, k)) ub += noub(4, i + 1, j - 1, k) * dy1(j - 1) / (dy1(j - 1) + dy1(j));
-  if (inq(7, i + 1, j, k)) ub += noub(7, i + 1, j, k) * dzn(k + 1) / (dzn(k) + dzn(k + 1));
-  if (inq(7, i + 1, j, k - 1)) ub += noub(7, i + 1, j, k - 1) * dzn(k - 1) / (dzn(k - 1) + dzn(k));
+  // (dzn(k) + dzn(k+1)); each coefficient at the statement point
+  const T ub = term(1, true, i, j, k, 0) + term(1, true, i + 1, j, k, 0) +
+               term(1, false, i, j, k, 1) - term(1, false, i + 1, j, k, 1) +
+               term(2, false, i, j, k, 6) - term(2, false, i, j + 1, k, 6) +
+               term(3, false, i, j, k, 4) - term(3, false, i, j, k + 1, 4) +
+               term(4, true, i + 1, j, k, 9) + term(4, true, i + 1, j - 1, k, 10) +
+               term(7, true, i + 1, j, k, 11) + term(7, true, i + 1, j, k - 1, 12);
   // v: nou5 = (v(j-1) + v(j))/2, diu5 = (v(j) - v(j-1))/dy1(j), nou2 = (dx1(i+1) v(i,j-1) +
   // dx1(i) v(i+1,j-1)) / (dx1(i) + dx1(i+1)), diu4 = 2 (v(i) - v(i-1)) / (dx1(i-1) + dx1(i)),
   // diu6 = (v(k) - v(k-1))/dzs(k-1), nou8 = (dzn(k+1) v(j-1,k) + dzn(k) v(j-1,k+1)) /
   // (dzn(k) + dzn(k+1))
-  T vb = half * (noub(5, i, j, k) + noub(5, i, j + 1, k));
-  if (inq(5, i, j, k)) vb += diub(5, i, j, k) / dy1(j);
-  if (inq(5, i, j + 1, k)) vb -= diub(5, i, j + 1, k) / dy1(j + 1);
-  if (inq(2, i, j + 1, k)) vb += noub(2, i, j + 1, k) * dx1(i + 1) / (dx1(i) + dx1(i + 1));
-  if (inq(2, i - 1, j + 1, k)) vb += noub(2, i - 1, j + 1, k) * dx1(i - 1) / (dx1(i - 1) + dx1(i));
-  if (inq(4, i, j, k)) vb += two * diub(4, i, j, k) / (dx1(i - 1) + dx1(i));
-  if (inq(4, i + 1, j, k)) vb -= two * diub(4, i + 1, j, k) / (dx1(i) + dx1(i + 1));
-  if (inq(6, i, j, k)) vb += diub(6, i, j, k) / dzs(k - 1);
-  if (inq(6, i, j, k + 1)) vb -= diub(6, i, j, k + 1) / dzs(k);
-  if (inq(8, i, j + 1, k)) vb += noub(8, i, j + 1, k) * dzn(k + 1) / (dzn(k) + dzn(k + 1));
-  if (inq(8, i, j + 1, k - 1)) vb += noub(8, i, j + 1, k - 1) * dzn(k - 1) / (dzn(k - 1) + dzn(k));
+  const T vb = term(5, true, i, j, k, 0) + term(5, true, i, j + 1, k, 0) +
+               term(5, false, i, j, k, 2) - term(5, false, i, j + 1, k, 2) +
+               term(2, true, i, j + 1, k, 7) + term(2, true, i - 1, j + 1, k, 8) +
+               term(4, false, i, j, k, 5) - term(4, false, i + 1, j, k, 5) +
+               term(6, false, i, j, k, 4) - term(6, false, i, j, k + 1, 4) +
+               term(8, true, i, j + 1, k, 11) + term(8, true, i, j + 1, k - 1, 12);
   // w: nou9 = (w(k-1) + w(k))/2, diu9 = (w(k) - w(k-1))/dzn(k), nou3 = (dx1(i+1) w(i,k-1) +
   // dx1(i) w(i+1,k-1)) / (dx1(i) + dx1(i+1)), nou6 = (dy1(j+1) w(j,k-1) + dy1(j) w(j+1,k-1)) /
   // (dy1(j) + dy1(j+1)), diu7 = 2 (w(i) - w(i-1)) / (dx1(i-1) + dx1(i)), diu8 = 2 (w(j) -
   // w(j-1)) / (dy1(j-1) + dy1(j))
-  T wb = half * (noub(9, i, j, k) + noub(9, i, j, k + 1));
-  if (inq(9, i, j, k)) wb += diub(9, i, j, k) / dzn(k);
-  if (inq(9, i, j, k + 1)) wb -= diub(9, i, j, k + 1) / dzn(k + 1);
-  if (inq(3, i, j, k + 1)) wb += noub(3, i, j, k + 1) * dx1(i + 1) / (dx1(i) + dx1(i + 1));
-  if (inq(3, i - 1, j, k + 1)) wb += noub(3, i - 1, j, k + 1) * dx1(i - 1) / (dx1(i - 1) + dx1(i));
-  if (inq(6, i, j, k + 1)) wb += noub(6, i, j, k + 1) * dy1(j + 1) / (dy1(j) + dy1(j + 1));
-  if (inq(6, i, j - 1, k + 1)) wb += noub(6, i, j - 1, k + 1) * dy1(j - 1) / (dy1(j - 1) + dy1(j));
-  if (inq(7, i, j, k)) wb += two * diub(7, i, j, k) / (dx1(i - 1) + dx1(i));
-  if (inq(7, i + 1, j, k)) wb -= two * diub(7, i + 1, j, k) / (dx1(i) + dx1(i + 1));
-  if (inq(8, i, j, k)) wb += two * diub(8, i, j, k) / (dy1(j - 1) + dy1(j));
-  if (inq(8, i, j + 1, k)) wb -= two * diub(8, i, j + 1, k) / (dy1(j) + dy1(j + 1));
+  const T wb = term(9, true, i, j, k, 0) + term(9, true, i, j, k + 1, 0) +
+               term(9, false, i, j, k, 3) - term(9, false, i, j, k + 1, 3) +
+               term(3, true, i, j, k + 1, 7) + term(3, true, i - 1, j, k + 1, 8) +
+               term(6, true, i, j, k + 1, 9) + term(6, true, i, j - 1, k + 1, 10) +
+               term(7, false, i, j, k, 5) - term(7, false, i + 1, j, k, 5) +
+               term(8, false, i, j, k, 6) - term(8, false, i, j + 1, k, 6);
   B.u[U.at(i, j, k)] += ub;
   B.v[U.at(i, j, k)] += vb;
   B.w[W.at(i, j, k)] += wb;
@@ -382,6 +384,13 @@ struct LesTab {
   }
   __device__ __forceinline__ T rcp(int axis, int idx) const { return at(axis, 4, idx); }
   __device__ __forceinline__ bool in(int i, int j, int k) const { return in_box(i, j, k, im, jm, 1, km); }
+  // the nearest cell of IN: the gathers load at it unconditionally and keep the term only for a
+  // point inside IN, so the loads of all terms issue without waiting on branches
+  __device__ __forceinline__ void clamp(int& i, int& j, int& k) const {
+    i = min(max(i, 1), im);
+    j = min(max(j, 1), jm);
+    k = min(max(k, 1), km);
+  }
 };
 
 template <typename T>
@@ -457,19 +466,24 @@ __device__ __forceinline__ void visc_sm_term(T& acc, const AdjArr<T>& A, const A
     return ds;
   };
   if constexpr (!v.quad) {
-    const int a = i - v.oi, b = j - v.oj, c = k - v.ok;
-    if (W.in(a, b, c)) acc += visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * dsum(a, b, c);
+    int a = i - v.oi, b = j - v.oj, c = k - v.ok;
+    const bool ok = W.in(a, b, c);
+    W.clamp(a, b, c);
+    const T t = visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * dsum(a, b, c);
+    acc += ok ? t : T(0);
   } else {
 #pragma unroll
     for (int eo = 0; eo < 2; ++eo)
 #pragma unroll
       for (int ei = 0; ei < 2; ++ei) {
-        const int a = i - axis_di(v.wo, v.oo[eo]) - axis_di(v.wi, v.io[ei]);
-        const int b = j - axis_dj(v.wo, v.oo[eo]) - axis_dj(v.wi, v.io[ei]);
-        const int c = k - axis_dk(v.wo, v.oo[eo]) - axis_dk(v.wi, v.io[ei]);
-        if (W.in(a, b, c))
-          acc += visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * dsum(a, b, c) *
-                 W.weight(v.wo, eo, a, b, c) * W.weight(v.wi, ei, a, b, c);
+        int a = i - axis_di(v.wo, v.oo[eo]) - axis_di(v.wi, v.io[ei]);
+        int b = j - axis_dj(v.wo, v.oo[eo]) - axis_dj(v.wi, v.io[ei]);
+        int c = k - axis_dk(v.wo, v.oo[eo]) - axis_dk(v.wi, v.io[ei]);
+        const bool ok = W.in(a, b, c);
+        W.clamp(a, b, c);
+        const T t = visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * dsum(a, b, c) *
+                    W.weight(v.wo, eo, a, b, c) * W.weight(v.wi, ei, a, b, c);
+        acc += ok ? t : T(0);
       }
   }
 }
@@ -568,8 +582,9 @@ __device__ __forceinline__ void visc_diu_term(T* acc, const AdjArr<T>& A, const 
   constexpr ViscTerm v = visc_term(N);
 #pragma unroll
   for (int m = 0; m < v.nd; ++m) {
-    const int a = i - v.di[m], b = j - v.dj[m], c = k - v.dk[m];
-    if (!W.in(a, b, c)) continue;
+    int a = i - v.di[m], b = j - v.dj[m], c = k - v.dk[m];
+    const bool ok = W.in(a, b, c);
+    W.clamp(a, b, c);
     T evs;
     if constexpr (!v.quad) {
       evs = A.sm[S.at(a + v.oi, b + v.oj, c + v.ok)];
@@ -584,7 +599,8 @@ __device__ __forceinline__ void visc_diu_term(T* acc, const AdjArr<T>& A, const 
                            b + axis_dj(v.wo, v.oo[eo]) + axis_dj(v.wi, v.io[ei]),
                            c + axis_dk(v.wo, v.oo[eo]) + axis_dk(v.wi, v.io[ei]))];
     }
-    acc[v.dq[m]] += visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * evs;
+    const T t = visc_alpha<N>(B, W, F, a, b, c) * T(v.dc) * evs;
+    acc[v.dq[m]] += ok ? t : T(0);
   }
 }
 
@@ -612,8 +628,10 @@ __global__ __launch_bounds__(256) void k_les_diu_bw(AdjArr<T> A, AdjArr<T> B, Le
   // calc_sm: diu1, 5, 9 at their own cell (4 G diu), the cross derivatives through T12, T23, T13
   const long n3 = (long)im * jm * km;
   auto sc = [&](int w, int a, int b, int c) {
-    if (!W.in(a, b, c)) return T(0);
-    return scratch[w * n3 + (a - 1) + (long)im * ((b - 1) + (long)jm * (c - 1))];
+    const bool ok = W.in(a, b, c);
+    W.clamp(a, b, c);
+    const T t = scratch[w * n3 + (a - 1) + (long)im * ((b - 1) + (long)jm * (c - 1))];
+    return ok ? t : T(0);
   };
   if (W.in(i, j, k)) {
     const T g4 = T(4) * sc(0, i, j, k);
