@@ -1128,9 +1128,25 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
     cleanup();
     return code;
   };
+  // FP32 with the hand-written transposes: the segment is recomputed by the plain handle's
+  // fused kernels (the stages before the pressure solve; p from the forward window's copy),
+  // and each step's slot takes the arrays those transposes read: u, v, w, the snapshot
+  // before the outflow, uout, diu1..9 and sm.  Its values are the statement kernels' (the
+  // window's forward runs the same kernels).
+  const bool plain_rc = plain && h->hand && pst && slots;
+  const int kept[] = {ADJ_A_u, ADJ_A_v, ADJ_A_w, ADJ_A_sm, ADJ_A_diu1, ADJ_A_diu2, ADJ_A_diu3,
+                      ADJ_A_diu4, ADJ_A_diu5, ADJ_A_diu6, ADJ_A_diu7, ADJ_A_diu8, ADJ_A_diu9};
+  constexpr int NKEPT = sizeof(kept) / sizeof(kept[0]);
+  void* kept_plain[NKEPT] = {};
+  if (plain_rc) {
+    const int fid[NKEPT] = {LES_U, LES_V, LES_W, LES_SM, LES_DIU1, LES_DIU2, LES_DIU3,
+                            LES_DIU4, LES_DIU5, LES_DIU6, LES_DIU7, LES_DIU8, LES_DIU9};
+    for (int q = 0; q < NKEPT; ++q) kept_plain[q] = les::field_ptr(h->ph, fid[q]);
+  }
   if (slots) {
-    // elements a step reads but never writes keep the values of the handle's arrays
-    for (int t = 0; t < nseg; ++t) {
+    // elements a step reads but never writes keep the values of the handle's arrays (the
+    // plain recomputation overwrites every array the hand-written transposes read)
+    for (int t = 0; t < nseg && !plain_rc; ++t) {
       void* dst[ADJ_NARRAYS];
       const void* src[ADJ_NARRAYS];
       size_t bytes[ADJ_NARRAYS];
@@ -1142,21 +1158,6 @@ extern "C" int les_adj_window(les_adj* h, int nsteps, int ckpt, const int* probe
           bytes[m++] = h->n[q] * tsize(h);
         }
       if ((r = copy_arrays(h, m, dst, src, bytes))) return fail_slots(r);
-    }
-    // FP32 with the hand-written transposes: the segment is recomputed by the plain handle's
-    // fused kernels (the stages before the pressure solve; p from the forward window's copy),
-    // and each step's slot takes the arrays those transposes read: u, v, w, the snapshot
-    // before the outflow, uout, diu1..9 and sm.  Its values are the statement kernels' (the
-    // window's forward runs the same kernels).
-    const bool plain_rc = plain && h->hand && pst;
-    const int kept[] = {ADJ_A_u, ADJ_A_v, ADJ_A_w, ADJ_A_sm, ADJ_A_diu1, ADJ_A_diu2, ADJ_A_diu3,
-                        ADJ_A_diu4, ADJ_A_diu5, ADJ_A_diu6, ADJ_A_diu7, ADJ_A_diu8, ADJ_A_diu9};
-    constexpr int NKEPT = sizeof(kept) / sizeof(kept[0]);
-    void* kept_plain[NKEPT] = {};
-    if (plain_rc) {
-      const int fid[NKEPT] = {LES_U, LES_V, LES_W, LES_SM, LES_DIU1, LES_DIU2, LES_DIU3,
-                              LES_DIU4, LES_DIU5, LES_DIU6, LES_DIU7, LES_DIU8, LES_DIU9};
-      for (int q = 0; q < NKEPT; ++q) kept_plain[q] = les::field_ptr(h->ph, fid[q]);
     }
     for (int c = nck - 1; c >= 0 && plain_rc; --c) {
       const int s0 = c * ckpt, s1 = std::min(nsteps, s0 + ckpt);

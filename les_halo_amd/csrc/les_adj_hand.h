@@ -701,22 +701,26 @@ __global__ __launch_bounds__(256) void k_velnw_bw(AdjArr<T> A, AdjArr<T> B, AdjP
   auto dys = [&](int b) { return A.dys[b]; };
   auto dzs = [&](int c) { return A.dzs[c + 1]; };
   const T c = P.dt / P.ro;
-  T pb = T(0);
+  // u, v (box IN) and w (box KL) adjoints at a point, 0 outside the box; loaded at the nearest
+  // point of the box either way, so no load waits on a branch
+  auto ld = [&](const T* y, const Lay& L, int a, int b, int cc, int khi) {
+    const bool ok = in_box(a, b, cc, im, jm, 1, khi);
+    const T v = y[L.at(min(max(a, 1), im), min(max(b, 1), jm), min(max(cc, 1), khi))];
+    return ok ? v : T(0);
+  };
+  const T ub = ld(B.u, U, i, j, k, km), vb = ld(B.v, U, i, j, k, km), wb = ld(B.w, W, i, j, k, km - 1);
+  const T uw = ld(B.u, U, i - 1, j, k, km), vs = ld(B.v, U, i, j - 1, k, km),
+          wd = ld(B.w, W, i, j, k - 1, km - 1);
+  const int ic = min(i, im), jc = min(j, jm), kc = min(k, km - 1);
+  // the gradient terms: p(i) in u(i) (+) and in u(i-1) (-), alike along j and k
+  T pb = c * ub / dxs(ic) + c * vb / dys(jc) + c * wb / dzs(kc);
+  pb -= c * uw / dxs(i - 1) + c * vs / dys(j - 1) + c * wd / dzs(k - 1);
   if (in_box(i, j, k, im, jm, 1, km)) {
-    const T ub = B.u[U.at(i, j, k)], vb = B.v[U.at(i, j, k)];
-    pb += c * ub / dxs(i) + c * vb / dys(j);
     const long e = F.at(i, j, k);
     B.f[e] += P.dt * ub;
     B.g[e] += P.dt * vb;
-    if (k <= km - 1) {
-      const T wb = B.w[W.at(i, j, k)];
-      pb += c * wb / dzs(k);
-      B.h[e] += P.dt * wb;
-    }
+    if (k <= km - 1) B.h[e] += P.dt * wb;
   }
-  if (in_box(i - 1, j, k, im, jm, 1, km)) pb -= c * B.u[U.at(i - 1, j, k)] / dxs(i - 1);
-  if (in_box(i, j - 1, k, im, jm, 1, km)) pb -= c * B.v[U.at(i, j - 1, k)] / dys(j - 1);
-  if (in_box(i, j, k - 1, im, jm, 1, km - 1)) pb -= c * B.w[W.at(i, j, k - 1)] / dzs(k - 1);
   B.p[lay_p(im, jm).at(i, j, k)] += pb;
 }
 
@@ -790,29 +794,20 @@ __global__ __launch_bounds__(256) void k_rhs_bw(AdjArr<T> A, AdjArr<T> B, AdjPar
   auto dx1 = [&](int a) { return A.dx1[a + 1]; };
   auto dy1 = [&](int b) { return A.dy1[b]; };
   auto dzn = [&](int c) { return A.dzn[c + 1]; };
+  // the rhs adjoint at a point of IN, 0 outside (loaded at the nearest point of IN either way)
   auto r = [&](int a, int b, int c) {
-    return in_box(a, b, c, im, jm, 1, km) ? B.rhs[R.at(a, b, c)] : T(0);
+    const bool ok = in_box(a, b, c, im, jm, 1, km);
+    const T v = B.rhs[R.at(min(max(a, 1), im), min(max(b, 1), jm), min(max(c, 1), km))];
+    return ok ? v : T(0);
   };
-  const T rc = r(i, j, k);
-  const bool in = in_box(i, j, k, im, jm, 1, km);
-  T xf = (in ? rc / dx1(i) : T(0)), yf = (in ? rc / dy1(j) : T(0)), zf = (in ? rc / dzn(k) : T(0));
-  T xu = (in ? rc / P.dt / dx1(i) : T(0)), yu = (in ? rc / P.dt / dy1(j) : T(0)),
-    zu = (in ? rc / P.dt / dzn(k) : T(0));
-  if (in_box(i + 1, j, k, im, jm, 1, km)) {
-    const T rr = r(i + 1, j, k);
-    xf -= rr / dx1(i + 1);
-    xu -= rr / P.dt / dx1(i + 1);
-  }
-  if (in_box(i, j + 1, k, im, jm, 1, km)) {
-    const T rr = r(i, j + 1, k);
-    yf -= rr / dy1(j + 1);
-    yu -= rr / P.dt / dy1(j + 1);
-  }
-  if (in_box(i, j, k + 1, im, jm, 1, km)) {
-    const T rr = r(i, j, k + 1);
-    zf -= rr / dzn(k + 1);
-    zu -= rr / P.dt / dzn(k + 1);
-  }
+  const T rc = r(i, j, k), re = r(i + 1, j, k), rn = r(i, j + 1, k), ru = r(i, j, k + 1);
+  // grid values at indices inside the arrays (a zero adjoint multiplies the clamped ones)
+  const int ia = max(i, 1), ja = max(j, 1), ka = max(k, 1);
+  const int ib = min(i + 1, im), jb = min(j + 1, jm), kb = min(k + 1, km);
+  const T xf = rc / dx1(ia) - re / dx1(ib), yf = rc / dy1(ja) - rn / dy1(jb),
+          zf = rc / dzn(ka) - ru / dzn(kb);
+  const T xu = rc / P.dt / dx1(ia) - re / P.dt / dx1(ib), yu = rc / P.dt / dy1(ja) - rn / P.dt / dy1(jb),
+          zu = rc / P.dt / dzn(ka) - ru / P.dt / dzn(kb);
   const long e = F.at(i, j, k);
   B.f[e] += xf;
   B.g[e] += yf;
