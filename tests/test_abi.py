@@ -166,3 +166,15 @@ def test_invalid_sizes_rejected():
     lib().les_default_config(ctypes.byref(cfg), 1, 8, 8)
     h = ctypes.c_void_p()
     assert lib().les_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+
+
+def test_size_limit_of_32_bit_indexing():
+    """les_create admits a grid only if its largest array, diu1(-1:ip+2, 0:jp+2, 0:kp+2), has
+    fewer than 2^30 elements (device indices and buffer byte offsets are 32-bit): 2048 x 2048 x
+    256 has (2052)(2051)(259) >= 2^30 and is rejected as invalid before any device work."""
+    from les_halo_amd._lib import LesConfig
+    assert 2052 * 2051 * 259 >= 1 << 30
+    cfg = LesConfig()
+    lib().les_default_config(ctypes.byref(cfg), 2048, 2048, 256)
+    h = ctypes.c_void_p()
+    assert lib().les_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # LES_E_INVALID
