@@ -183,9 +183,9 @@ def ensemble_bench(nmembers, steps, warmup):
 
 def tiled_bench(steps, case):
     """The plain step as 2 x 2 tiles of 150 x 150 x 90 on the one GPU (les_tiled: face
-    exchanges between the tiles after every stage; SOR colour sweeps on each tile's split copy
-    with split-layout face exchanges after every colour), timed as graph replays, then checked
-    bitwise against one handle on the whole domain after the same steps."""
+    exchanges between the tiles after every stage; SOR colour sweeps on each tile's split copy,
+    each storing its tiles' face cells into the neighbours' halos), timed as graph replays, then
+    checked bitwise against one handle on the whole domain after the same steps."""
     import numpy as np
     import torch
     from les_halo_amd import LES, TiledLES
