@@ -471,7 +471,8 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
     if (cfg->precision == LES_PREC_MIXED) {
       Mixed& mx = h->mx;
       mx.nblk = mixed_blocks(h->d);
-      const size_t nd = 4 * ns + 2 * (size_t)mx.nblk + 8;
+      const size_t np = (size_t)mixed_part_len(h->d, sp.nr);
+      const size_t nd = 4 * ns + np + 8;
       if (hipMalloc(&h->mx_mem, nd * sizeof(double)) != hipSuccess) return bail(LES_E_NOMEM);
       if (hipMemsetAsync(h->mx_mem, 0, nd * sizeof(double), h->st) != hipSuccess)
         return bail(LES_E_HIP);
@@ -480,7 +481,7 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
       mx.r64[0] = mx.p64[1] + ns;
       mx.r64[1] = mx.r64[0] + ns;
       mx.part = mx.r64[1] + ns;
-      mx.cg = mx.part + 2 * (size_t)mx.nblk;
+      mx.cg = mx.part + np;
       mx.cn1 = h->fld[LES_CN1];
       // FP32 vectors of the conjugate-gradient correction solve
       if (hipMalloc(&h->cg_mem, 8 * ns * sizeof(float)) != hipSuccess) return bail(LES_E_NOMEM);

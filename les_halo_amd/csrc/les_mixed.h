@@ -8,7 +8,7 @@ namespace les {
 struct Mixed {
   double* p64[2];     // pressure iterate
   double* r64[2];     // rhs - rhsav64
-  double* part;       // per-block partial sums, 2 * nblk
+  double* part;       // per-block partial sums, mixed_part_len entries
   const float* cn1;   // cn1(1:ip,1:jp,1:kp)
   int nblk;
   // conjugate-gradient correction solve (refine_solver 1): FP32 split vectors and FP64 scalars
@@ -21,6 +21,8 @@ struct Mixed {
 };
 
 int mixed_blocks(Dims d);
+// entries of Mixed::part: two per block of the cell kernels, or one per conjugate-gradient block
+int mixed_part_len(Dims d, int nr);
 void launch_mx_rhsav(hipStream_t st, Dims d, RedBuf rb, DevScalars* sc);
 // p -> split copy; rhs64 = rhs - rhsav64 minus its w-mean (the solvable right-hand side)
 void launch_mx_scatter(hipStream_t st, Dims d, Grid1D g, Fields F, SplitSOR sp, Mixed mx,

@@ -19,6 +19,9 @@
 // in the colour-split layout of les_sor.hip (interior cells, colour (i+j+k)&1).
 #include "les_mixed.h"
 
+#include <algorithm>
+#include <utility>
+
 namespace les {
 
 namespace {
@@ -254,15 +257,22 @@ __global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp
 
 // ---- conjugate gradients for the correction (refine_solver 1) ----
 // A = -L with the boundary images folded into the matrix (a face neighbour across an i or k face
-// is the cell itself, so its coefficient joins the diagonal): (A x)_c = sum over the real
-// neighbours of c_nb * (x_c - x_nb); the preconditioner's diagonal is dg_c = 1/cn1 - (image
-// coefficients), the same sum up to rounding.  With cn2l(i) =
-// 2/(dxs(i)(dxs(i-1)+dxs(i))) and cn2s(i) = 2/(dxs(i-1)(dxs(i-1)+dxs(i))) (init.f95:82-107, and
-// alike along j and k), w*cn2l(i) = w(i+1)*cn2s(i+1) for the weight w = (dxs(i-1)+dxs(i)) *
-// (dys(j-1)+dys(j)) * (dzs(k-1)+dzs(k)): A is symmetric in the inner product weighted by w (the
-// periodic j coupling as well when dys(0) = dys(jp), as on a uniform horizontal grid).  CG runs
-// in that inner product, preconditioned by symmetric Gauss-Seidel (colour 0, colour 1, colour 0,
-// from zero) on the same matrix.  Vectors FP32, inner products FP64.
+// is the cell itself, so it drops out): (A x)_c = sum over the real neighbours of
+// c_nb * (x_c - x_nb), and the preconditioner's diagonal dg_c is the sum of those c_nb.  With
+// cn2l(i) = 2/(dxs(i)(dxs(i-1)+dxs(i))) and cn2s(i) = 2/(dxs(i-1)(dxs(i-1)+dxs(i)))
+// (init.f95:82-107, and alike along j and k), w*cn2l(i) = w(i+1)*cn2s(i+1) for the weight
+// w = (dxs(i-1)+dxs(i)) (dys(j-1)+dys(j)) (dzs(k-1)+dzs(k)): A is symmetric in the inner product
+// weighted by w (the periodic j coupling as well when dys(0) = dys(jp), as on a uniform
+// horizontal grid).  CG runs in that inner product, preconditioned by symmetric Gauss-Seidel
+// (colour 0, colour 1, colour 0, from zero) on the same matrix.  Vectors FP32, inner products
+// FP64.
+//
+// A work item is group grp (entries 4 grp .. 4 grp + 3) of row (j, k) of the colour-split
+// layout, dealt over the workgroups in the XCD-aware order of the SOR sweeps: every load and
+// store is a float4 of one colour row, and no lane idles on the other colour.  Cell i of colour
+// c sits at entry (i - 1) / 2 of its row; its j and k neighbours at the same entry of the other
+// colour's rows j +- 1 and k +- 1, its i neighbours at entries e - 1, e (first cell of the row of
+// colour c at i = 1) or e, e + 1 (at i = 2).
 namespace {
 
 struct CgDir {  // the search direction of the last iteration and the one being formed
@@ -270,121 +280,236 @@ struct CgDir {  // the search direction of the last iteration and the one being 
   float* new_[2];
 };
 
-__device__ __forceinline__ float xs(const float* const* v, const SplitSOR& sp, int i, int j, int k) {
-  return v[(i + j + k) & 1][split_index(sp, i, j, k)];
+struct Item {
+  int j, k, grp, base;
+  bool ok;
+};
+
+__device__ __forceinline__ Item cg_item(Dims d, const SplitSOR& sp, int nthreads) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  Item it;
+  it.ok = t < nthreads;
+  const int tc = min(t, nthreads - 1);
+  const int row = (int)__umulhi((unsigned)tc, sp.m_ng);
+  it.grp = tc - row * (sp.nr >> 2);
+  const int kk = (int)__umulhi((unsigned)row, sp.m_jp);
+  it.j = 1 + row - kk * d.jp;
+  it.k = 1 + kk;
+  it.base = sp.nr * ((it.j - 1) + sp.nj * (it.k - 1)) + 4 * it.grp;
+  return it;
 }
 
-// the matrix row of cell (i,j,k) applied to v (its own value excluded)
-__device__ __forceinline__ float offdiag_sum(Dims d, const Grid1D& g, const SplitSOR& sp,
-                                             const float* const* v, int i, int j, int k,
-                                             float& dg, float cn1) {
-  float s = 0.0f;
-  dg = 1.0f / cn1;
-  const float c2l = g.cn2l[i - 1], c2s = g.cn2s[i - 1], c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
-  const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-  if (i < d.ip) s += c2l * xs(v, sp, i + 1, j, k); else dg -= c2l;
-  if (i > 1) s += c2s * xs(v, sp, i - 1, j, k); else dg -= c2s;
-  s += c3l * xs(v, sp, i, j == d.jp ? 1 : j + 1, k);
-  s += c3s * xs(v, sp, i, j == 1 ? d.jp : j - 1, k);
-  if (k < d.kp) s += c4l * xs(v, sp, i, j, k + 1); else dg -= c4l;
-  if (k > 1) s += c4s * xs(v, sp, i, j, k - 1); else dg -= c4s;
-  return s;
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st4(float* p, const float v[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void unpack(float4 a, float v[4]) {
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
 }
 
 }  // namespace
 
-// dnew = z + beta dold (beta = 0 on the first iteration: dnew = z), q = A dnew, block partials of
-// <dnew, q>.  The neighbours' dnew values are formed from their z and dold the same way, so no
-// thread reads what another writes (dnew and dold are different arrays).
-__global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
-                                                  CgDir dd, int first) {
+// workgroups of the four-entry kernels (a multiple of the 8 XCDs), each writing one partial
+int cg_blocks(Dims d, int nr) {
+  const int nt = nr / 4 * d.jp * d.kp;
+  return ((nt + 255) / 256 + 7) / 8 * 8;
+}
+
+// q = A dnew over both colours (dnew = z + beta dold formed for every value read), stores dnew
+// and q, block partials of w dnew q
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_cg_apply4(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
+                                                   CgDir dd, int nthreads) {
   __shared__ double lds[4];
-  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
-  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  if (mx.cg[4] != 0.0) return;
+  const Item it = cg_item(d, sp, nthreads);
+  const int j = it.j, k = it.k, grp = it.grp, base = it.base, nr = sp.nr;
+  const float beta = FIRST ? 0.0f : (float)mx.cg[3];
+  const int lev = nr * sp.nj;
+  const int dn_ = (j == d.jp) ? -nr * (d.jp - 1) : nr;  // row j+1 (periodic)
+  const int ds_ = (j == 1) ? nr * (d.jp - 1) : -nr;     // row j-1
+  auto xv = [&](int c, int off, float v[4]) {  // dnew of colour c at base + off
+    float a[4];
+    unpack(ld4(mx.z[c] + base + off), a);
+    if (!FIRST) {
+      float b[4];
+      unpack(ld4(dd.old_[c] + base + off), b);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = a[q] + beta * b[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = a[q];
+  };
+  auto xs = [&](int c, int off) {  // one dnew value
+    const float a = mx.z[c][base + off];
+    return FIRST ? a : a + beta * dd.old_[c][base + off];
+  };
+  float x[2][4], xn[2][4], xsn[2][4], xu[2][4], xd[2][4];
+  float xe[2];  // per colour: its entry just before (f == 1 cells' west) or after the group
+  for (int c = 0; c < 2; ++c) {
+    xv(c, 0, x[c]);
+    xv(c, dn_, xn[c]);
+    xv(c, ds_, xsn[c]);
+    if (k < d.kp) xv(c, lev, xu[c]);
+    if (k > 1) xv(c, -lev, xd[c]);
+  }
   double acc = 0.0;
-  if (i <= d.ip && j <= d.jp) {
-    const float beta = first ? 0.0f : (float)mx.cg[3];
-    auto dn = [&](int a, int b, int c) {
-      const int par = (a + b + c) & 1, is = split_index(sp, a, b, c);
-      return first ? mx.z[par][is] : mx.z[par][is] + beta * dd.old_[par][is];
-    };
-    const int par = (i + j + k) & 1, is = split_index(sp, i, j, k);
-    const float c2l = g.cn2l[i - 1], c2s = g.cn2s[i - 1], c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
+  float qv[2][4];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int o = c ^ 1;
+    const int f = ((1 + j + k) & 1) == c ? 1 : 2;  // first i of colour c in this row
+    // the other colour's entry before the group (f == 1) or after it (f == 2)
+    float edge = 0.0f;
+    if (f == 1 && grp > 0) edge = xs(o, -1);
+    if (f == 2 && 4 * grp + 4 < nr) edge = xs(o, 4);
+    const float m[6] = {edge, x[o][0], x[o][1], x[o][2], x[o][3], edge};
+    const float c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
     const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-    const float xv = dn(i, j, k);
-    // flux form, sum c_nb (x_c - x_nb): A annihilates the constants exactly, and a smooth x
-    // loses no digits to the cancellation of a large diagonal against its neighbours
-    float qv = 0.0f;
-    if (i < d.ip) qv += c2l * (xv - dn(i + 1, j, k));
-    if (i > 1) qv += c2s * (xv - dn(i - 1, j, k));
-    qv += c3l * (xv - dn(i, j == d.jp ? 1 : j + 1, k));
-    qv += c3s * (xv - dn(i, j == 1 ? d.jp : j - 1, k));
-    if (k < d.kp) qv += c4l * (xv - dn(i, j, k + 1));
-    if (k > 1) qv += c4s * (xv - dn(i, j, k - 1));
-    dd.new_[par][is] = xv;
-    mx.q[par][is] = qv;
-    acc = vol(g, i, j, k) * (double)xv * (double)qv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = f + 2 * (4 * grp + q);
+      const float xc = x[c][q];
+      const float pw = (f == 1) ? m[q] : m[1 + q];      // other colour at i - 1
+      const float pe = (f == 1) ? m[1 + q] : m[2 + q];  // other colour at i + 1
+      float qq = 0.0f;
+      if (i <= d.ip) {
+        if (i < d.ip) qq += g.cn2l[i - 1] * (xc - pe);
+        if (i > 1) qq += g.cn2s[i - 1] * (xc - pw);
+        qq += c3l * (xc - xn[o][q]);
+        qq += c3s * (xc - xsn[o][q]);
+        if (k < d.kp) qq += c4l * (xc - xu[o][q]);
+        if (k > 1) qq += c4s * (xc - xd[o][q]);
+        acc += vol(g, i, j, k) * (double)xc * (double)qq;
+      }
+      qv[c][q] = qq;
+    }
   }
+  if (it.ok) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      st4(dd.new_[c] + base, x[c]);
+      st4(mx.q[c] + base, qv[c]);
+    }
+  }
+  if (!it.ok) acc = 0.0;
   const double t = block_sum(acc, lds);
-  if (tid() == 0) mx.part[flat_block()] = t;
+  if (tid() == 0) mx.part[blockIdx.x] = t;
 }
 
-__device__ __forceinline__ void cg_relax_body(Dims d, const Grid1D& g, SplitSOR sp, Mixed mx,
-                                              int i, int j, int k, int is, int colour, int first,
-                                              int fresh, float om) {
-  float dg;
-  const float s = first ? 0.0f : offdiag_sum(d, g, sp, mx.z, i, j, k, dg, mx.cn1[d.O(i, j, k)]);
-  if (first) {
-    dg = 1.0f / mx.cn1[d.O(i, j, k)];
-    if (i == d.ip) dg -= g.cn2l[i - 1];
-    if (i == 1) dg -= g.cn2s[i - 1];
-    if (k == d.kp) dg -= g.cn4l[k - 1];
-    if (k == 1) dg -= g.cn4s[k - 1];
-  }
-  const float gs = (-sp.r[colour][is] + s) / dg;
-  mx.z[colour][is] = (first || fresh) ? om * gs : (1.0f - om) * mx.z[colour][is] + om * gs;
-}
-
-// one colour of the symmetric SOR preconditioner z = M^-1 rr (rr in sp.r, negated: the system
-// is -L e = -r), relaxation om: z_c <- (1 - om) z_c + om (rr_c + sum c_nb z_nb) / dg_c.  The
-// application starts from z = 0: first (colour 0 forward) ignores the neighbours and its own
-// value, fresh (colour 1 forward) its own value.
-// update (first pass of an iteration): every cell first takes e += alpha dnew and rr -= alpha q.
-// rz (last pass): block partials of <rr, z> over both colours (colour 1 is final by then).
-__global__ __launch_bounds__(256) void k_cg_sgs(Dims d, Grid1D g, SplitSOR sp, Mixed mx, int colour,
-                                                int first, int fresh, float om, CgDir dd,
-                                                int update, int rz) {
+// one colour of the symmetric SOR preconditioner (as k_cg_sgs) on four entries of a colour-c
+// row; update: first e += alpha dnew, rr -= alpha q on both colours' entries of the group;
+// rz: block partials of <rr, z> over both colours' entries
+__global__ __launch_bounds__(256) void k_cg_sgs4(Dims d, Grid1D g, SplitSOR sp, Mixed mx, int c,
+                                                 int first, int fresh, float om, CgDir dd,
+                                                 int update, int rz, int nthreads) {
   __shared__ double lds[4];
-  const int i = 1 + blockIdx.x * MX_BX + threadIdx.x;
-  const int j = 1 + blockIdx.y * MX_BY + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
-  const bool in = i <= d.ip && j <= d.jp;
-  const int par = (i + j + k) & 1;
-  const int is = in ? split_index(sp, i, j, k) : 0;
-  if (in && update) {
+  if (mx.cg[4] != 0.0) return;
+  const Item it = cg_item(d, sp, nthreads);
+  const int j = it.j, k = it.k, grp = it.grp, base = it.base, nr = sp.nr;
+  const int o = c ^ 1;
+  const int lev = nr * sp.nj;
+  float rr[2][4];
+  for (int cc = 0; cc < 2; ++cc) unpack(ld4(sp.r[cc] + base), rr[cc]);
+  if (update) {
     const float a = (float)mx.cg[2];
-    sp.p[par][is] = sp.p[par][is] + a * dd.new_[par][is];
-    sp.r[par][is] = sp.r[par][is] + a * mx.q[par][is];
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      float e[4], dn4[4], q4[4];
+      unpack(ld4(sp.p[cc] + base), e);
+      unpack(ld4(dd.new_[cc] + base), dn4);
+      unpack(ld4(mx.q[cc] + base), q4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        e[q] = e[q] + a * dn4[q];
+        rr[cc][q] = rr[cc][q] + a * q4[q];
+      }
+      if (it.ok) {
+        st4(sp.p[cc] + base, e);
+        st4(sp.r[cc] + base, rr[cc]);
+      }
+    }
   }
-  if (in && par == colour) cg_relax_body(d, g, sp, mx, i, j, k, is, colour, first, fresh, om);
+  const int f = ((1 + j + k) & 1) == c ? 1 : 2;
+  float zc[4], zo[4];
+  unpack(ld4(mx.z[c] + base), zc);
+  unpack(ld4(mx.z[o] + base), zo);
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}, dg[4];
+  const float c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
+  const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+  if (!first) {
+    const int dn_ = (j == d.jp) ? -nr * (d.jp - 1) : nr;
+    const int ds_ = (j == 1) ? nr * (d.jp - 1) : -nr;
+    float zn[4], zs[4], zu[4] = {0, 0, 0, 0}, zd[4] = {0, 0, 0, 0};
+    unpack(ld4(mx.z[o] + base + dn_), zn);
+    unpack(ld4(mx.z[o] + base + ds_), zs);
+    if (k < d.kp) unpack(ld4(mx.z[o] + base + lev), zu);
+    if (k > 1) unpack(ld4(mx.z[o] + base - lev), zd);
+    float edge = 0.0f;
+    if (f == 1 && grp > 0) edge = mx.z[o][base - 1];
+    if (f == 2 && 4 * grp + 4 < nr) edge = mx.z[o][base + 4];
+    const float m[6] = {edge, zo[0], zo[1], zo[2], zo[3], edge};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = f + 2 * (4 * grp + q);
+      const float pw = (f == 1) ? m[q] : m[1 + q];
+      const float pe = (f == 1) ? m[1 + q] : m[2 + q];
+      float t = c3l * zn[q] + c3s * zs[q];
+      if (i < d.ip) t += g.cn2l[min(i, d.ip) - 1] * pe;
+      if (i > 1) t += g.cn2s[min(i, d.ip) - 1] * pw;
+      if (k < d.kp) t += c4l * zu[q];
+      if (k > 1) t += c4s * zd[q];
+      s[q] = t;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = min(f + 2 * (4 * grp + q), d.ip);
+    float t = c3l + c3s;
+    if (i < d.ip) t += g.cn2l[i - 1];
+    if (i > 1) t += g.cn2s[i - 1];
+    if (k < d.kp) t += c4l;
+    if (k > 1) t += c4s;
+    dg[q] = t;
+  }
+  float zn4[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float gs = (-rr[c][q] + s[q]) / dg[q];
+    zn4[q] = (first || fresh) ? om * gs : (1.0f - om) * zc[q] + om * gs;
+    const int i = f + 2 * (4 * grp + q);
+    if (i > d.ip) zn4[q] = zc[q];  // padding entries keep their value
+  }
+  if (it.ok) st4(mx.z[c] + base, zn4);
   if (rz) {
     double acc = 0.0;
-    if (in) acc = vol(g, i, j, k) * -(double)sp.r[par][is] * (double)mx.z[par][is];
+    if (it.ok) {
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const int fc = ((1 + j + k) & 1) == cc ? 1 : 2;
+        const float* zz = cc == c ? zn4 : zo;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = fc + 2 * (4 * grp + q);
+          if (i <= d.ip) acc += vol(g, i, j, k) * -(double)rr[cc][q] * (double)zz[q];
+        }
+      }
+    }
     const double t = block_sum(acc, lds);
-    if (tid() == 0) mx.part[flat_block()] = t;
+    if (tid() == 0) mx.part[blockIdx.x] = t;
   }
 }
 
 // which 0: rz = rz0 = sum (start); 1: alpha = rz / sum(pq); 2: beta = sum / rz, rz = sum, and
 // stop (cg[4] = 1) once rz <= stop * rz0.  cg: rz, pq, alpha, beta, stopped, rz0
-__global__ __launch_bounds__(256) void k_cg_scalar(Mixed mx, int which, double stop) {
+__global__ __launch_bounds__(256) void k_cg_scalar(Mixed mx, int which, double stop, int nb) {
   __shared__ double lds[4];
   if (mx.cg[4] != 0.0) return;
   double a = 0.0;
-  for (int q = threadIdx.x; q < mx.nblk; q += 256) a += mx.part[q];
+  for (int q = threadIdx.x; q < nb; q += 256) a += mx.part[q];
   const double s = block_sum(a, lds);
   if (threadIdx.x == 0) {
     if (which == 0) {
@@ -454,32 +579,39 @@ void launch_mx_cg(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int i
                   double stop) {
   const dim3 grid = mx_grid(d), blk(MX_BX, MX_BY);
   CgDir dd{{mx.dir[0], mx.dir[1]}, {mx.dir2[0], mx.dir2[1]}};
+  // four entries of a colour row per thread (the split sweeps' work items)
+  const int nt = sp.nr / 4 * d.jp * d.kp, nwg = cg_blocks(d, sp.nr);
   // symmetric SOR from zero: colour 0, colour 1 forward, colour 1, colour 0 backward; the first
   // pass may take the iteration's e and rr update, the last adds up <rr, z>
   auto precondition = [&](int update) {
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 1, 1, om, dd, update, 0);
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 1, om, dd, 0, 0);
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 1, 0, 0, om, dd, 0, 0);
-    hipLaunchKernelGGL(k_cg_sgs, grid, blk, 0, st, d, g, sp, mx, 0, 0, 0, om, dd, 0, 1);
+    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 0, 1, 1, om, dd, update, 0, nt);
+    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 1, 0, 1, om, dd, 0, 0, nt);
+    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 1, 0, 0, om, dd, 0, 0, nt);
+    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 0, 0, 0, om, dd, 0, 1, nt);
   };
   hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, st, mx);
   hipLaunchKernelGGL(k_cg_mean_part, grid, blk, 0, st, d, g, sp, mx);
   hipLaunchKernelGGL(k_cg_mean, dim3(1), dim3(256), 0, st, mx);
   hipLaunchKernelGGL(k_cg_project, grid, blk, 0, st, d, sp, mx);
   precondition(0);
-  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 0, stop);
+  hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 0, stop, nwg);
   for (int it = 0; it < iters; ++it) {
     // dnew = z + beta dold into the other buffer; q = A dnew; alpha
-    hipLaunchKernelGGL(k_cg_apply, grid, blk, 0, st, d, g, sp, mx, dd, it == 0 ? 1 : 0);
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 1, stop);
+    if (it == 0)
+      hipLaunchKernelGGL(k_cg_apply4<true>, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, dd, nt);
+    else
+      hipLaunchKernelGGL(k_cg_apply4<false>, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, dd, nt);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 1, stop, nwg);
     precondition(1);  // e += alpha dnew, rr -= alpha q, z = M^-1 rr, <rr, z>
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 2, stop);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 2, stop, nwg);
     std::swap(dd.old_[0], dd.new_[0]);
     std::swap(dd.old_[1], dd.new_[1]);
   }
 }
 
 int mixed_blocks(Dims d) { return mx_blocks(d); }
+
+int mixed_part_len(Dims d, int nr) { return std::max(2 * mx_blocks(d), cg_blocks(d, nr)); }
 
 void launch_mx_rhsav(hipStream_t st, Dims d, RedBuf rb, DevScalars* sc) {
   hipLaunchKernelGGL(k_mx_rhsav, dim3(1), dim3(256), 0, st, rb.partials, reduction_blocks(d), sc);
