@@ -267,35 +267,40 @@ __global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp
 // (colour 0, colour 1, colour 0, from zero) on the same matrix.  Vectors FP32, inner products
 // FP64.
 //
-// A work item is group grp (entries 4 grp .. 4 grp + 3) of row (j, k) of the colour-split
-// layout, dealt over the workgroups in the XCD-aware order of the SOR sweeps: every load and
-// store is a float4 of one colour row, and no lane idles on the other colour.  Cell i of colour
-// c sits at entry (i - 1) / 2 of its row; its j and k neighbours at the same entry of the other
-// colour's rows j +- 1 and k +- 1, its i neighbours at entries e - 1, e (first cell of the row of
-// colour c at i = 1) or e, e + 1 (at i = 2).
+// A work item is group grp (entries 4 grp .. 4 grp + 3) of colour row j over a chunk of CG_KC
+// levels, marching up in k, with the items dealt over the workgroups in the XCD-aware order of
+// the SOR sweeps: every load and store is a float4 of one colour row, no lane idles on the other
+// colour, and the other colour's levels k - 1, k, k + 1 stay in registers, so each array is read
+// once per chunk (plus two levels).  Cell i of colour c sits at entry (i - 1) / 2 of its row;
+// its j and k neighbours at the same entry of the other colour's rows j +- 1 and levels k +- 1,
+// its i neighbours at entries e - 1, e (the row's first cell of colour c at i = 1) or e, e + 1
+// (at i = 2).
 namespace {
+
+constexpr int CG_KC = 10;  // levels per work item
 
 struct CgDir {  // the search direction of the last iteration and the one being formed
   float* old_[2];
   float* new_[2];
 };
 
-struct Item {
-  int j, k, grp, base;
+struct Col {
+  int j, grp, row, k0, k1;  // row: the group's offset in level 1
   bool ok;
 };
 
-__device__ __forceinline__ Item cg_item(Dims d, const SplitSOR& sp, int nthreads) {
+__device__ __forceinline__ Col cg_col(Dims d, const SplitSOR& sp, int nthreads) {
   const int t = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  Item it;
+  Col it;
   it.ok = t < nthreads;
-  const int tc = min(t, nthreads - 1);
-  const int row = (int)__umulhi((unsigned)tc, sp.m_ng);
-  it.grp = tc - row * (sp.nr >> 2);
-  const int kk = (int)__umulhi((unsigned)row, sp.m_jp);
-  it.j = 1 + row - kk * d.jp;
-  it.k = 1 + kk;
-  it.base = sp.nr * ((it.j - 1) + sp.nj * (it.k - 1)) + 4 * it.grp;
+  const int tc = min(t, nthreads - 1);  // past the end: the last item's loads, nothing stored
+  const int r = (int)__umulhi((unsigned)tc, sp.m_ng);  // chunk * jp + j - 1
+  it.grp = tc - r * (sp.nr >> 2);
+  const int ch = (int)__umulhi((unsigned)r, sp.m_jp);
+  it.j = 1 + r - ch * d.jp;
+  it.k0 = 1 + ch * CG_KC;
+  it.k1 = min(d.kp, it.k0 + CG_KC - 1);
+  it.row = sp.nr * (it.j - 1) + 4 * it.grp;
   return it;
 }
 
@@ -309,90 +314,161 @@ __device__ __forceinline__ void unpack(float4 a, float v[4]) {
   v[2] = a.z;
   v[3] = a.w;
 }
+__device__ __forceinline__ void copy4(float* a, const float* b) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] = b[q];
+}
+
+// The weights of the item's cells without their k factor: (dxs(i-1)+dxs(i)) (dys(j-1)+dys(j))
+// for the four cells of a row whose first cell is i = 1 (f = 1) or i = 2 (f = 2); the padding
+// entries' index is clamped (their terms are never added)
+struct Wxy {
+  double a[2][4];
+};
+
+__device__ __forceinline__ Wxy cg_wxy(Dims d, const Grid1D& g, int j, int grp) {
+  Wxy w;
+  const double wy = (double)g.DYS(j - 1) + (double)g.DYS(j);
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = min(1 + f + 2 * (4 * grp + q), d.ip);
+      w.a[f][q] = ((double)g.DXS(i - 1) + (double)g.DXS(i)) * wy;
+    }
+  return w;
+}
+
+__device__ __forceinline__ double cg_wz(const Grid1D& g, int k) {
+  return (double)g.DZS(k - 1) + (double)g.DZS(k);
+}
+
+// The i coefficients of the item's cells, held for all its levels: cn2l and cn2s of the four
+// cells of a row whose first cell is i = 1 (index 0) or i = 2 (index 1), zero where the
+// neighbour is absent (i = ip, i = 1, padding), and the preconditioner's diagonal less its k
+// terms, c3l + c3s + l + s.  A zero term adds exactly nothing: every value it multiplies is
+// finite (the padding entries of z, dold and dnew stay zero).
+struct Cx {
+  float l[2][4], s[2][4], dxy[2][4];
+};
+
+__device__ __forceinline__ Cx cg_cx(Dims d, const Grid1D& g, int grp, float c3l, float c3s) {
+  Cx x;
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = 1 + f + 2 * (4 * grp + q);
+      const int ic = min(i, d.ip);
+      x.l[f][q] = i < d.ip ? g.cn2l[ic - 1] : 0.0f;
+      x.s[f][q] = (i > 1 && i <= d.ip) ? g.cn2s[ic - 1] : 0.0f;
+      x.dxy[f][q] = ((c3l + c3s) + x.l[f][q]) + x.s[f][q];
+    }
+  return x;
+}
+
+// the row's first cell of colour c at level k is i = 1 (0) or i = 2 (1)
+__device__ __forceinline__ int cg_f(int c, int j, int k) { return ((1 + j + k) & 1) == c ? 0 : 1; }
 
 }  // namespace
 
-// workgroups of the four-entry kernels (a multiple of the 8 XCDs), each writing one partial
+// workgroups of the CG kernels (a multiple of the 8 XCDs), each writing one partial
 int cg_blocks(Dims d, int nr) {
-  const int nt = nr / 4 * d.jp * d.kp;
+  const int nt = nr / 4 * d.jp * ((d.kp + CG_KC - 1) / CG_KC);
   return ((nt + 255) / 256 + 7) / 8 * 8;
 }
 
-// q = A dnew over both colours (dnew = z + beta dold formed for every value read), stores dnew
-// and q, block partials of w dnew q
+// q = A dnew over both colours (dnew = z + beta dold, formed for every value read), stores dnew
+// and q, block partials of <dnew, q>.  No thread reads what another writes (dnew and dold are
+// different arrays).
 template <bool FIRST>
-__global__ __launch_bounds__(256) void k_cg_apply4(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
-                                                   CgDir dd, int nthreads) {
+__global__ __launch_bounds__(256) void k_cg_apply(Dims d, Grid1D g, SplitSOR sp, Mixed mx,
+                                                  CgDir dd, int nthreads) {
   __shared__ double lds[4];
-  if (mx.cg[4] != 0.0) return;
-  const Item it = cg_item(d, sp, nthreads);
-  const int j = it.j, k = it.k, grp = it.grp, base = it.base, nr = sp.nr;
-  const float beta = FIRST ? 0.0f : (float)mx.cg[3];
+  if (mx.cg[4] != 0.0) return;  // converged (uniform: every block returns)
+  const Col it = cg_col(d, sp, nthreads);
+  const int j = it.j, grp = it.grp, nr = sp.nr;
   const int lev = nr * sp.nj;
+  const float beta = FIRST ? 0.0f : (float)mx.cg[3];
   const int dn_ = (j == d.jp) ? -nr * (d.jp - 1) : nr;  // row j+1 (periodic)
   const int ds_ = (j == 1) ? nr * (d.jp - 1) : -nr;     // row j-1
-  auto xv = [&](int c, int off, float v[4]) {  // dnew of colour c at base + off
-    float a[4];
-    unpack(ld4(mx.z[c] + base + off), a);
+  auto dnew4 = [&](int c, int o, float v[4]) {  // dnew of colour c at entries o .. o+3
+    unpack(ld4(mx.z[c] + o), v);
     if (!FIRST) {
       float b[4];
-      unpack(ld4(dd.old_[c] + base + off), b);
+      unpack(ld4(dd.old_[c] + o), b);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) a[q] = a[q] + beta * b[q];
+      for (int q = 0; q < 4; ++q) v[q] = v[q] + beta * b[q];
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = a[q];
   };
-  auto xs = [&](int c, int off) {  // one dnew value
-    const float a = mx.z[c][base + off];
-    return FIRST ? a : a + beta * dd.old_[c][base + off];
+  auto dnew1 = [&](int c, int o) {
+    const float a = mx.z[c][o];
+    return FIRST ? a : a + beta * dd.old_[c][o];
   };
-  float x[2][4], xn[2][4], xsn[2][4], xu[2][4], xd[2][4];
-  float xe[2];  // per colour: its entry just before (f == 1 cells' west) or after the group
-  for (int c = 0; c < 2; ++c) {
-    xv(c, 0, x[c]);
-    xv(c, dn_, xn[c]);
-    xv(c, ds_, xsn[c]);
-    if (k < d.kp) xv(c, lev, xu[c]);
-    if (k > 1) xv(c, -lev, xd[c]);
+  const Wxy wxy = cg_wxy(d, g, j, grp);
+  const float c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
+  const Cx cx = cg_cx(d, g, grp, c3l, c3s);
+  // dnew of both colours at levels k-1, k, k+1
+  float xm[2][4] = {}, xc[2][4], xp[2][4] = {};  // zero where a level is absent
+  const int b0 = it.row + lev * (it.k0 - 1);
+  if (it.k0 > 1) {
+    dnew4(0, b0 - lev, xm[0]);
+    dnew4(1, b0 - lev, xm[1]);
   }
+  dnew4(0, b0, xc[0]);
+  dnew4(1, b0, xc[1]);
   double acc = 0.0;
-  float qv[2][4];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int o = c ^ 1;
-    const int f = ((1 + j + k) & 1) == c ? 1 : 2;  // first i of colour c in this row
-    // the other colour's entry before the group (f == 1) or after it (f == 2)
-    float edge = 0.0f;
-    if (f == 1 && grp > 0) edge = xs(o, -1);
-    if (f == 2 && 4 * grp + 4 < nr) edge = xs(o, 4);
-    const float m[6] = {edge, x[o][0], x[o][1], x[o][2], x[o][3], edge};
-    const float c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
-    const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = f + 2 * (4 * grp + q);
-      const float xc = x[c][q];
-      const float pw = (f == 1) ? m[q] : m[1 + q];      // other colour at i - 1
-      const float pe = (f == 1) ? m[1 + q] : m[2 + q];  // other colour at i + 1
-      float qq = 0.0f;
-      if (i <= d.ip) {
-        if (i < d.ip) qq += g.cn2l[i - 1] * (xc - pe);
-        if (i > 1) qq += g.cn2s[i - 1] * (xc - pw);
-        qq += c3l * (xc - xn[o][q]);
-        qq += c3s * (xc - xsn[o][q]);
-        if (k < d.kp) qq += c4l * (xc - xu[o][q]);
-        if (k > 1) qq += c4s * (xc - xd[o][q]);
-        acc += vol(g, i, j, k) * (double)xc * (double)qq;
-      }
-      qv[c][q] = qq;
+  for (int k = it.k0; k <= it.k1; ++k) {
+    const int b = it.row + lev * (k - 1);
+    if (k < d.kp) {
+      dnew4(0, b + lev, xp[0]);
+      dnew4(1, b + lev, xp[1]);
     }
-  }
-  if (it.ok) {
+    const float c4l = k < d.kp ? g.cn4l[k - 1] : 0.0f, c4s = k > 1 ? g.cn4s[k - 1] : 0.0f;
+    const double wz = cg_wz(g, k);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      st4(dd.new_[c] + base, x[c]);
-      st4(mx.q[c] + base, qv[c]);
+      const int o = c ^ 1;
+      const int f = cg_f(c, j, k);
+      float xn[4], xs[4];
+      dnew4(o, b + dn_, xn);
+      dnew4(o, b + ds_, xs);
+      // the other colour's entry before the group (f == 0) or after it (f == 1)
+      float edge = 0.0f;
+      if (f == 0 && grp > 0) edge = dnew1(o, b - 1);
+      if (f == 1 && 4 * grp + 4 < nr) edge = dnew1(o, b + 4);
+      const float m[6] = {edge, xc[o][0], xc[o][1], xc[o][2], xc[o][3], edge};
+      float qv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float x = xc[c][q];
+        const float pw = f == 0 ? m[q] : m[1 + q];      // other colour at i - 1
+        const float pe = f == 0 ? m[1 + q] : m[2 + q];  // other colour at i + 1
+        const float l = f == 0 ? cx.l[0][q] : cx.l[1][q];
+        const float sw = f == 0 ? cx.s[0][q] : cx.s[1][q];
+        // flux form, sum c_nb (x_c - x_nb): A annihilates the constants exactly, and a smooth x
+        // loses no digits to the cancellation of a large diagonal against its neighbours
+        float qq = l * (x - pe);
+        qq += sw * (x - pw);
+        qq += c3l * (x - xn[q]);
+        qq += c3s * (x - xs[q]);
+        qq += c4l * (x - xp[o][q]);
+        qq += c4s * (x - xm[o][q]);
+        const bool real = f + 1 + 2 * (4 * grp + q) <= d.ip;
+        if (!real) qq = 0.0f;
+        const double w = f == 0 ? wxy.a[0][q] : wxy.a[1][q];
+        if (real) acc += w * wz * (double)x * (double)qq;
+        qv[q] = qq;
+      }
+      if (it.ok) {
+        st4(dd.new_[c] + b, xc[c]);
+        st4(mx.q[c] + b, qv);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      copy4(xm[c], xc[c]);
+      copy4(xc[c], xp[c]);
     }
   }
   if (!it.ok) acc = 0.0;
@@ -400,104 +476,120 @@ __global__ __launch_bounds__(256) void k_cg_apply4(Dims d, Grid1D g, SplitSOR sp
   if (tid() == 0) mx.part[blockIdx.x] = t;
 }
 
-// one colour of the symmetric SOR preconditioner (as k_cg_sgs) on four entries of a colour-c
-// row; update: first e += alpha dnew, rr -= alpha q on both colours' entries of the group;
-// rz: block partials of <rr, z> over both colours' entries
-__global__ __launch_bounds__(256) void k_cg_sgs4(Dims d, Grid1D g, SplitSOR sp, Mixed mx, int c,
-                                                 int first, int fresh, float om, CgDir dd,
-                                                 int update, int rz, int nthreads) {
+// The preconditioner z = M^-1 rr (rr in sp.r, negated: the system is -L e = -r): symmetric SOR
+// from z = 0 with relaxation om, z_c <- (1 - om) z_c + om (-rr_c + sum c_nb z_nb) / dg_c, over
+// colour 0, colour 1, colour 1, colour 0.  Colour 1's two passes see the same colour-0 values,
+// so they are one: z_1 = om (2 - om) gs_1.  Three passes:
+//   PASS 0: colour 0 from zero, z_0 = om (-rr_0) / dg (no neighbours); UPDATE first takes the
+//           iteration's e += alpha dnew, rr += alpha q on both colours (q = A dnew = -L dnew);
+//   PASS 1: colour 1 from the colour-0 neighbours, z_1 = om (2 - om) gs;
+//   PASS 2: colour 0, z_0 = (1 - om) z_0 + om gs, and block partials of <rr, z>.
+template <int PASS, bool UPDATE>
+__global__ __launch_bounds__(256) void k_cg_sgs(Dims d, Grid1D g, SplitSOR sp, Mixed mx, float om,
+                                                CgDir dd, int nthreads) {
+  constexpr int C = PASS == 1 ? 1 : 0, O = C ^ 1;
   __shared__ double lds[4];
   if (mx.cg[4] != 0.0) return;
-  const Item it = cg_item(d, sp, nthreads);
-  const int j = it.j, k = it.k, grp = it.grp, base = it.base, nr = sp.nr;
-  const int o = c ^ 1;
+  const Col it = cg_col(d, sp, nthreads);
+  const int j = it.j, grp = it.grp, nr = sp.nr;
   const int lev = nr * sp.nj;
-  float rr[2][4];
-  for (int cc = 0; cc < 2; ++cc) unpack(ld4(sp.r[cc] + base), rr[cc]);
-  if (update) {
-    const float a = (float)mx.cg[2];
-#pragma unroll
-    for (int cc = 0; cc < 2; ++cc) {
-      float e[4], dn4[4], q4[4];
-      unpack(ld4(sp.p[cc] + base), e);
-      unpack(ld4(dd.new_[cc] + base), dn4);
-      unpack(ld4(mx.q[cc] + base), q4);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        e[q] = e[q] + a * dn4[q];
-        rr[cc][q] = rr[cc][q] + a * q4[q];
-      }
-      if (it.ok) {
-        st4(sp.p[cc] + base, e);
-        st4(sp.r[cc] + base, rr[cc]);
-      }
-    }
-  }
-  const int f = ((1 + j + k) & 1) == c ? 1 : 2;
-  float zc[4], zo[4];
-  unpack(ld4(mx.z[c] + base), zc);
-  unpack(ld4(mx.z[o] + base), zo);
-  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}, dg[4];
+  const int dn_ = (j == d.jp) ? -nr * (d.jp - 1) : nr;
+  const int ds_ = (j == 1) ? nr * (d.jp - 1) : -nr;
   const float c3l = g.cn3l[j - 1], c3s = g.cn3s[j - 1];
-  const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-  if (!first) {
-    const int dn_ = (j == d.jp) ? -nr * (d.jp - 1) : nr;
-    const int ds_ = (j == 1) ? nr * (d.jp - 1) : -nr;
-    float zn[4], zs[4], zu[4] = {0, 0, 0, 0}, zd[4] = {0, 0, 0, 0};
-    unpack(ld4(mx.z[o] + base + dn_), zn);
-    unpack(ld4(mx.z[o] + base + ds_), zs);
-    if (k < d.kp) unpack(ld4(mx.z[o] + base + lev), zu);
-    if (k > 1) unpack(ld4(mx.z[o] + base - lev), zd);
-    float edge = 0.0f;
-    if (f == 1 && grp > 0) edge = mx.z[o][base - 1];
-    if (f == 2 && 4 * grp + 4 < nr) edge = mx.z[o][base + 4];
-    const float m[6] = {edge, zo[0], zo[1], zo[2], zo[3], edge};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = f + 2 * (4 * grp + q);
-      const float pw = (f == 1) ? m[q] : m[1 + q];
-      const float pe = (f == 1) ? m[1 + q] : m[2 + q];
-      float t = c3l * zn[q] + c3s * zs[q];
-      if (i < d.ip) t += g.cn2l[min(i, d.ip) - 1] * pe;
-      if (i > 1) t += g.cn2s[min(i, d.ip) - 1] * pw;
-      if (k < d.kp) t += c4l * zu[q];
-      if (k > 1) t += c4s * zd[q];
-      s[q] = t;
-    }
+  const float om1 = PASS == 1 ? om * (2.0f - om) : om;
+  const Cx cx = cg_cx(d, g, grp, c3l, c3s);
+  Wxy wxy;
+  if constexpr (PASS == 2) wxy = cg_wxy(d, g, j, grp);
+  // the other colour's z at levels k-1, k, k+1 (passes 1 and 2), zero where a level is absent
+  float zm[4] = {0, 0, 0, 0}, zc[4] = {0, 0, 0, 0}, zp[4] = {0, 0, 0, 0};
+  const int b0 = it.row + lev * (it.k0 - 1);
+  if constexpr (PASS != 0) {
+    if (it.k0 > 1) unpack(ld4(mx.z[O] + b0 - lev), zm);
+    unpack(ld4(mx.z[O] + b0), zc);
   }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int i = min(f + 2 * (4 * grp + q), d.ip);
-    float t = c3l + c3s;
-    if (i < d.ip) t += g.cn2l[i - 1];
-    if (i > 1) t += g.cn2s[i - 1];
-    if (k < d.kp) t += c4l;
-    if (k > 1) t += c4s;
-    dg[q] = t;
-  }
-  float zn4[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float gs = (-rr[c][q] + s[q]) / dg[q];
-    zn4[q] = (first || fresh) ? om * gs : (1.0f - om) * zc[q] + om * gs;
-    const int i = f + 2 * (4 * grp + q);
-    if (i > d.ip) zn4[q] = zc[q];  // padding entries keep their value
-  }
-  if (it.ok) st4(mx.z[c] + base, zn4);
-  if (rz) {
-    double acc = 0.0;
-    if (it.ok) {
+  double acc = 0.0;
+  for (int k = it.k0; k <= it.k1; ++k) {
+    const int b = it.row + lev * (k - 1);
+    float rr[2][4];
+    unpack(ld4(sp.r[C] + b), rr[C]);
+    if constexpr (PASS == 2) unpack(ld4(sp.r[O] + b), rr[O]);
+    if constexpr (UPDATE) {
+      const float a = (float)mx.cg[2];
+      unpack(ld4(sp.r[O] + b), rr[O]);
 #pragma unroll
       for (int cc = 0; cc < 2; ++cc) {
-        const int fc = ((1 + j + k) & 1) == cc ? 1 : 2;
-        const float* zz = cc == c ? zn4 : zo;
+        float e[4], dn4[4], q4[4];
+        unpack(ld4(sp.p[cc] + b), e);
+        unpack(ld4(dd.new_[cc] + b), dn4);
+        unpack(ld4(mx.q[cc] + b), q4);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int i = fc + 2 * (4 * grp + q);
-          if (i <= d.ip) acc += vol(g, i, j, k) * -(double)rr[cc][q] * (double)zz[q];
+          e[q] = e[q] + a * dn4[q];
+          rr[cc][q] = rr[cc][q] + a * q4[q];
+        }
+        if (it.ok) {
+          st4(sp.p[cc] + b, e);
+          st4(sp.r[cc] + b, rr[cc]);
         }
       }
     }
+    const float c4l = k < d.kp ? g.cn4l[k - 1] : 0.0f, c4s = k > 1 ? g.cn4s[k - 1] : 0.0f;
+    const int f = cg_f(C, j, k);
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (PASS != 0) {
+      if (k < d.kp) unpack(ld4(mx.z[O] + b + lev), zp);
+      float zn[4], zs[4];
+      unpack(ld4(mx.z[O] + b + dn_), zn);
+      unpack(ld4(mx.z[O] + b + ds_), zs);
+      float edge = 0.0f;
+      if (f == 0 && grp > 0) edge = mx.z[O][b - 1];
+      if (f == 1 && 4 * grp + 4 < nr) edge = mx.z[O][b + 4];
+      const float m[6] = {edge, zc[0], zc[1], zc[2], zc[3], edge};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float pw = f == 0 ? m[q] : m[1 + q];
+        const float pe = f == 0 ? m[1 + q] : m[2 + q];
+        const float l = f == 0 ? cx.l[0][q] : cx.l[1][q];
+        const float sw = f == 0 ? cx.s[0][q] : cx.s[1][q];
+        float t = c3l * zn[q] + c3s * zs[q];
+        t += l * pe;
+        t += sw * pw;
+        t += c4l * zp[q];
+        t += c4s * zm[q];
+        s[q] = t;
+      }
+    }
+    float zo[4] = {0, 0, 0, 0};
+    if constexpr (PASS == 2) unpack(ld4(mx.z[C] + b), zo);
+    float zn4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float dg = ((f == 0 ? cx.dxy[0][q] : cx.dxy[1][q]) + c4l) + c4s;
+      const float gs = (-rr[C][q] + s[q]) / dg;
+      zn4[q] = PASS == 2 ? (1.0f - om) * zo[q] + om * gs : om1 * gs;
+      if (f + 1 + 2 * (4 * grp + q) > d.ip) zn4[q] = 0.0f;  // padding entries stay zero
+    }
+    if (it.ok) st4(mx.z[C] + b, zn4);
+    if constexpr (PASS == 2) {
+      const double wz = cg_wz(g, k);
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const int fc = cg_f(cc, j, k);
+        const float* zz = cc == C ? zn4 : zc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double w = fc == 0 ? wxy.a[0][q] : wxy.a[1][q];
+          if (fc + 1 + 2 * (4 * grp + q) <= d.ip) acc += w * wz * (-(double)rr[cc][q] * (double)zz[q]);
+        }
+      }
+    }
+    if constexpr (PASS != 0) {
+      copy4(zm, zc);
+      copy4(zc, zp);
+    }
+  }
+  if constexpr (PASS == 2) {
+    if (!it.ok) acc = 0.0;
     const double t = block_sum(acc, lds);
     if (tid() == 0) mx.part[blockIdx.x] = t;
   }
@@ -579,30 +671,32 @@ void launch_mx_cg(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Mixed mx, int i
                   double stop) {
   const dim3 grid = mx_grid(d), blk(MX_BX, MX_BY);
   CgDir dd{{mx.dir[0], mx.dir[1]}, {mx.dir2[0], mx.dir2[1]}};
-  // four entries of a colour row per thread (the split sweeps' work items)
-  const int nt = sp.nr / 4 * d.jp * d.kp, nwg = cg_blocks(d, sp.nr);
-  // symmetric SOR from zero: colour 0, colour 1 forward, colour 1, colour 0 backward; the first
-  // pass may take the iteration's e and rr update, the last adds up <rr, z>
-  auto precondition = [&](int update) {
-    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 0, 1, 1, om, dd, update, 0, nt);
-    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 1, 0, 1, om, dd, 0, 0, nt);
-    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 1, 0, 0, om, dd, 0, 0, nt);
-    hipLaunchKernelGGL(k_cg_sgs4, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, 0, 0, 0, om, dd, 0, 1, nt);
+  // four entries of a colour row over CG_KC levels per thread
+  const int nt = sp.nr / 4 * d.jp * ((d.kp + CG_KC - 1) / CG_KC), nwg = cg_blocks(d, sp.nr);
+  const dim3 cg(nwg), cb(256);
+  // the last pass adds up <rr, z>
+  auto precondition = [&](bool update) {
+    if (update)
+      hipLaunchKernelGGL((k_cg_sgs<0, true>), cg, cb, 0, st, d, g, sp, mx, om, dd, nt);
+    else
+      hipLaunchKernelGGL((k_cg_sgs<0, false>), cg, cb, 0, st, d, g, sp, mx, om, dd, nt);
+    hipLaunchKernelGGL((k_cg_sgs<1, false>), cg, cb, 0, st, d, g, sp, mx, om, dd, nt);
+    hipLaunchKernelGGL((k_cg_sgs<2, false>), cg, cb, 0, st, d, g, sp, mx, om, dd, nt);
   };
   hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(64), 0, st, mx);
   hipLaunchKernelGGL(k_cg_mean_part, grid, blk, 0, st, d, g, sp, mx);
   hipLaunchKernelGGL(k_cg_mean, dim3(1), dim3(256), 0, st, mx);
   hipLaunchKernelGGL(k_cg_project, grid, blk, 0, st, d, sp, mx);
-  precondition(0);
+  precondition(false);
   hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 0, stop, nwg);
   for (int it = 0; it < iters; ++it) {
     // dnew = z + beta dold into the other buffer; q = A dnew; alpha
     if (it == 0)
-      hipLaunchKernelGGL(k_cg_apply4<true>, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, dd, nt);
+      hipLaunchKernelGGL(k_cg_apply<true>, cg, cb, 0, st, d, g, sp, mx, dd, nt);
     else
-      hipLaunchKernelGGL(k_cg_apply4<false>, dim3(nwg), dim3(256), 0, st, d, g, sp, mx, dd, nt);
+      hipLaunchKernelGGL(k_cg_apply<false>, cg, cb, 0, st, d, g, sp, mx, dd, nt);
     hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 1, stop, nwg);
-    precondition(1);  // e += alpha dnew, rr -= alpha q, z = M^-1 rr, <rr, z>
+    precondition(true);  // e += alpha dnew, rr += alpha q, z = M^-1 rr, <rr, z>
     hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(256), 0, st, mx, 2, stop, nwg);
     std::swap(dd.old_[0], dd.new_[0]);
     std::swap(dd.old_[1], dd.new_[1]);
