@@ -52,8 +52,8 @@ def bits_equal(a, b):
 
 
 def mismatch_report(a, b, name):
-    a = np.asarray(a, np.float32)
-    b = np.asarray(b, np.float32)
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    b = np.ascontiguousarray(b, np.float32).ravel()
     bad = a.view(np.uint32) != b.view(np.uint32)
     idx = np.flatnonzero(bad)[:5]
     return "%s: %d/%d differ, first flat idx %s, a=%s b=%s, max|d|=%g" % (

@@ -21,9 +21,12 @@ def _interior(name, a, dims):
 
 # even tile sizes sweep the tiles' colour-split copies with split-layout face exchanges; an
 # odd jp (32 x 21: the colour flips across the periodic seam) sweeps in place, as the single
-# domain does there
+# domain does there.  18 columns per tile give colour rows of 9 entries (a float4 group of the
+# split rows holds the tile's last entry and padding); 3 x 2 tiles have middle tiles with
+# neighbours on both i faces.
 @pytest.mark.parametrize("tiles,grid", [((2, 2), (32, 24)), ((2, 1), (32, 24)),
-                                        ((1, 2), (32, 24)), ((2, 1), (32, 21))])
+                                        ((1, 2), (32, 24)), ((2, 1), (32, 21)),
+                                        ((2, 2), (36, 24)), ((3, 2), (54, 24))])
 def test_tiled_equals_single_domain(tiles, grid):
     from les_halo_amd import LES, TiledLES
     case = common.tiny_case(grid[0], grid[1], 16, seed=21)
