@@ -1374,9 +1374,9 @@ int batch_capture(les_ensemble* e) {
     a.ph = h->ph;
     a.sg_velfg = velfg_strip(h->d);
     const int sx = stencil_shape_x(e->shp_sm), sy = stencil_shape_y(e->shp_sm);
-    a.sg_sm = strip_grid((h->d.ip + sx - 1) / sx, (h->d.jp + sy - 1) / sy, h->d.kp);
+    a.sg_sm = strip_grid((h->d.ip + sx - 1) / sx, (h->d.jp + sy - 1) / sy, calc_sm_kblocks(h->d.kp));
     const int vx = stencil_shape_x(e->shp_visc), vy = stencil_shape_y(e->shp_visc);
-    a.sg_visc = strip_grid((h->d.ip + vx - 1) / vx, (h->d.jp + vy - 1) / vy, h->d.kp);
+    a.sg_visc = strip_grid((h->d.ip + vx - 1) / vx, (h->d.jp + vy - 1) / vy, visc_kblocks(h->d.kp));
     const StripGrid* sgs[3] = {&a.sg_velfg, &a.sg_sm, &a.sg_visc};
     int* bases[3] = {&a.base_velfg, &a.base_sm, &a.base_visc};
     for (int k = 0; k < 3; ++k) {

@@ -156,7 +156,11 @@ void launch_velfg(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int do
 // with_bound: also boundsm.f95's halo cells, as k_boundsm would set them afterwards
 void launch_calc_sm(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int with_bound = 0,
                     int shape = 0);
+// workgroup rows along k of k_calc_sm (a thread takes consecutive levels)
+int calc_sm_kblocks(int kp);
 void launch_boundsm(hipStream_t st, Dims d, Fields F);
+// workgroup rows along k of k_visc_adam (a thread takes two levels)
+int visc_kblocks(int kp);
 void launch_calc_visc_adam(hipStream_t st, Dims d, Grid1D g, Fields F, int do_visc,
                            int do_adam, int shape = 0);
 // split != nullptr: the raw rhs goes to the split copy's rhs arrays, not to F.rhs
