@@ -78,8 +78,16 @@ void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScal
 }
 
 // ---- one colour sweep: 4 consecutive cells of one colour row per thread ----
+// A thread takes group grp (entries 4 grp .. 4 grp + 3) of colour row j over KC consecutive
+// levels, marching up in k: the other colour's rows at k-1 and k stay in registers from the
+// levels before, so a level loads the other colour at k+1 and at j +- 1 only.  A sweep that
+// sums the residual takes SPLIT_KC_RESID levels a thread (its exact-sum flush then serves four
+// times the cells: at 300x300x90 with the residual every iteration, 2.95 ms per step against
+// 3.57 with one level, 3.18 with two, 3.41 with eight); a sweep without it one level (2x2 tiles
+// of 150x150x90: 2.43 ms per step against 2.54, 2.60 and 2.88 with two, four and eight).
+constexpr int SPLIT_KC_RESID = 4;
 // L: the workgroup's index in the sweep (after the XCD remap); lds: its limb scratch
-template <bool UNIFORM, bool RESID>
+template <bool UNIFORM, bool RESID, int KC = RESID ? SPLIT_KC_RESID : 1>
 __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, const SplitSOR& sp,
                                                const Physics& ph, int nrd, int64_t* partials,
                                                int ng, int nthreads, int L,
@@ -91,110 +99,118 @@ __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, c
   acc.init();
   const int t = L * 256 + threadIdx.x;
   const int tc = min(t, nthreads - 1);  // tail lanes index a valid group
-  // tc = grp + ng * row, row = (j-1) + jp * (k-1); quotients by multiply-high (exact for the
+  // tc = grp + ng * row, row = (j-1) + jp * chunk; quotients by multiply-high (exact for the
   // sizes sor_split_divisors admits)
   const int row = (int)__umulhi((unsigned)tc, sp.m_ng);
   const int grp = tc - row * ng;
-  const int kk = (int)__umulhi((unsigned)row, sp.m_jp);
-  const int j = 1 + row - kk * d.jp;
-  const int k = 1 + kk;
+  const int ch = (int)__umulhi((unsigned)row, sp.m_jp);
+  const int j = 1 + row - ch * d.jp;
+  const int k0 = 1 + ch * KC, k1 = min(d.kp, k0 + KC - 1);
   const int c = (1 + nrd) & 1;  // nrd = 0 updates i = 1 + mod(k+j,2): odd i+j+k
-  const int i0 = first_i(c, j, k) + 8 * grp;  // i of the group's first cell
-  const int base = sp.nr * ((j - 1) + sp.nj * (k - 1)) + 4 * grp;
   const float* po = sp.p[c ^ 1];
-  if (t < nthreads && i0 <= d.ip) {
-    const float4 oc = *reinterpret_cast<const float4*>(po + base);
-    // neighbour rows: periodic in j (a tile: its halo rows, j = jp+1 at row index jp and
-    // j = 0 at jp+1), the cell itself across k faces
-    const int rn = (j == d.jp && !sp.jhalo) ? -sp.nr * (d.jp - 1) : sp.nr;     // j+1
-    const int rsj = (j == 1) ? sp.nr * (sp.jhalo ? d.jp + 1 : d.jp - 1) : -sp.nr;  // j-1
-    // i faces shared with a neighbour tile: its cells p(0,j,k) and p(ip+1,j,k)
-    const int ih = (j - 1) + d.jp * (k - 1);
-    const float hwv = (sp.hw && i0 == 1) ? sp.hw[ih] : 0.0f;
-    const float hev = (sp.he && i0 + 6 >= d.ip) ? sp.he[ih] : 0.0f;
-    const int pl = sp.nr * sp.nj;
-    const float4 own = *reinterpret_cast<const float4*>(sp.p[c] + base);
-    const float4 rhs = *reinterpret_cast<const float4*>(sp.r[c] + base);
-    const float4 on = *reinterpret_cast<const float4*>(po + base + rn);
-    const float4 os = *reinterpret_cast<const float4*>(po + base + rsj);
-    float4 ou, od;
-    if (k < d.kp) ou = *reinterpret_cast<const float4*>(po + base + pl);
-    else ou = own;
-    if (k > 1) od = *reinterpret_cast<const float4*>(po + base - pl);
-    else od = own;
-    // i neighbours: odd i -> east o[ii], west o[ii-1]; even i -> east o[ii+1], west o[ii]
-    float xe[4], xw[4];
-    const float ov[4] = {oc.x, oc.y, oc.z, oc.w};
-    if (i0 & 1) {
-      const float prev = (grp > 0) ? po[base - 1] : 0.0f;
-      xe[0] = ov[0]; xe[1] = ov[1]; xe[2] = ov[2]; xe[3] = ov[3];
-      xw[0] = prev; xw[1] = ov[0]; xw[2] = ov[1]; xw[3] = ov[2];
-    } else {
-      const float next = (i0 + 7 <= d.ip) ? po[base + 4] : 0.0f;
-      xw[0] = ov[0]; xw[1] = ov[1]; xw[2] = ov[2]; xw[3] = ov[3];
-      xe[0] = ov[1]; xe[1] = ov[2]; xe[2] = ov[3]; xe[3] = next;
-    }
-    const float pv[4] = {own.x, own.y, own.z, own.w};
-    const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
-    const float nv[4] = {on.x, on.y, on.z, on.w};
-    const float sv[4] = {os.x, os.y, os.z, os.w};
-    const float uv[4] = {ou.x, ou.y, ou.z, ou.w};
-    const float dv[4] = {od.x, od.y, od.z, od.w};
-    float out[4];
-    float c2l = 0.0f, c2s = 0.0f, c3l, c3s, cn1k = 0.0f;
-    const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
-    if (UNIFORM) {
-      c2l = g.cn2l[0];
-      c2s = g.cn2s[0];
-      c3l = g.cn3l[0];
-      c3s = g.cn3s[0];
-      cn1k = sp.cn1k[k];
-    } else {
-      c3l = g.cn3l[j - 1];
-      c3s = g.cn3s[j - 1];
-    }
-    auto relax = [&](int q, int i, float pe, float pw) {
-      const float pc = pv[q];
-      float cn1, a2l = c2l, a2s = c2s;
-      if (UNIFORM) {
-        cn1 = cn1k;
-      } else {
-        cn1 = 1.0f / (sp.ax[i] + sp.by[j] + sp.cz[k]);
-        a2l = g.cn2l[i - 1];
-        a2s = g.cn2s[i - 1];
-      }
-      const float reltmp =
-          ph.omega * (cn1 * (a2l * pe + a2s * pw + c3l * nv[q] + c3s * sv[q] + c4l * uv[q] +
-                             c4s * dv[q] - rv[q]) -
-                      pc);
-      if (RESID) {
-        if (tm.t[0])  // the reference-order sum adds the terms afterwards
-          tm.t[c][((i - 1) >> 1) + tm.nrt * ((j - 1) + d.jp * (k - 1))] = reltmp * reltmp;
-        else
-          acc.add(reltmp * reltmp, lds);
-      }
-      return pc + reltmp;
-    };
+  const int pl = sp.nr * sp.nj;
+  const int rowoff = sp.nr * (j - 1) + 4 * grp;  // the group in level 1
+  // neighbour rows: periodic in j (a tile: its halo rows, j = jp+1 at row index jp and j = 0 at
+  // jp+1), the cell itself across k faces
+  const int rn = (j == d.jp && !sp.jhalo) ? -sp.nr * (d.jp - 1) : sp.nr;     // j+1
+  const int rsj = (j == 1) ? sp.nr * (sp.jhalo ? d.jp + 1 : d.jp - 1) : -sp.nr;  // j-1
+  float c2l = 0.0f, c2s = 0.0f, c3l, c3s;
+  if (UNIFORM) {
+    c2l = g.cn2l[0];
+    c2s = g.cn2s[0];
+    c3l = g.cn3l[0];
+    c3s = g.cn3s[0];
+  } else {
+    c3l = g.cn3l[j - 1];
+    c3s = g.cn3s[j - 1];
+  }
+  auto ld4 = [](const float* p) { return *reinterpret_cast<const float4*>(p); };
+  if (t < nthreads) {
+    // the other colour's group at levels k-1 (od) and k (oc); od is not read at k = 1
+    float4 oc = ld4(po + rowoff + pl * (k0 - 1));
+    float4 od = ld4(po + rowoff + pl * max(k0 - 2, 0));
+    for (int k = k0; k <= k1; ++k) {
+      const int base = rowoff + pl * (k - 1);
+      const float4 ou = ld4(po + base + (k < d.kp ? pl : 0));  // k+1 (k itself at the top)
+      const int i0 = first_i(c, j, k) + 8 * grp;  // i of the group's first cell
+      if (i0 <= d.ip) {
+        // i faces shared with a neighbour tile: its cells p(0,j,k) and p(ip+1,j,k)
+        const int ih = (j - 1) + d.jp * (k - 1);
+        const float hwv = (sp.hw && i0 == 1) ? sp.hw[ih] : 0.0f;
+        const float hev = (sp.he && i0 + 6 >= d.ip) ? sp.he[ih] : 0.0f;
+        const float4 own = ld4(sp.p[c] + base);
+        const float4 rhs = ld4(sp.r[c] + base);
+        const float4 on = ld4(po + base + rn);
+        const float4 os = ld4(po + base + rsj);
+        const float4 uu = (k < d.kp) ? ou : own;
+        const float4 dd = (k > 1) ? od : own;
+        // i neighbours: odd i -> east o[ii], west o[ii-1]; even i -> east o[ii+1], west o[ii]
+        float xe[4], xw[4];
+        const float ov[4] = {oc.x, oc.y, oc.z, oc.w};
+        if (i0 & 1) {
+          const float prev = (grp > 0) ? po[base - 1] : 0.0f;
+          xe[0] = ov[0]; xe[1] = ov[1]; xe[2] = ov[2]; xe[3] = ov[3];
+          xw[0] = prev; xw[1] = ov[0]; xw[2] = ov[1]; xw[3] = ov[2];
+        } else {
+          const float next = (i0 + 7 <= d.ip) ? po[base + 4] : 0.0f;
+          xw[0] = ov[0]; xw[1] = ov[1]; xw[2] = ov[2]; xw[3] = ov[3];
+          xe[0] = ov[1]; xe[1] = ov[2]; xe[2] = ov[3]; xe[3] = next;
+        }
+        const float pv[4] = {own.x, own.y, own.z, own.w};
+        const float rv[4] = {rhs.x, rhs.y, rhs.z, rhs.w};
+        const float nv[4] = {on.x, on.y, on.z, on.w};
+        const float sv[4] = {os.x, os.y, os.z, os.w};
+        const float uv[4] = {uu.x, uu.y, uu.z, uu.w};
+        const float dv[4] = {dd.x, dd.y, dd.z, dd.w};
+        float out[4];
+        const float c4l = g.cn4l[k - 1], c4s = g.cn4s[k - 1];
+        const float cn1k = UNIFORM ? sp.cn1k[k] : 0.0f;
+        auto relax = [&](int q, int i, float pe, float pw) {
+          const float pc = pv[q];
+          float cn1, a2l = c2l, a2s = c2s;
+          if (UNIFORM) {
+            cn1 = cn1k;
+          } else {
+            cn1 = 1.0f / (sp.ax[i] + sp.by[j] + sp.cz[k]);
+            a2l = g.cn2l[i - 1];
+            a2s = g.cn2s[i - 1];
+          }
+          const float reltmp =
+              ph.omega * (cn1 * (a2l * pe + a2s * pw + c3l * nv[q] + c3s * sv[q] + c4l * uv[q] +
+                                 c4s * dv[q] - rv[q]) -
+                          pc);
+          if (RESID) {
+            if (tm.t[0])  // the reference-order sum adds the terms afterwards
+              tm.t[c][((i - 1) >> 1) + tm.nrt * ((j - 1) + d.jp * (k - 1))] = reltmp * reltmp;
+            else
+              acc.add(reltmp * reltmp, lds);
+          }
+          return pc + reltmp;
+        };
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = i0 + 2 * q;
-      const float pc = pv[q];
-      const float pe = (i == d.ip) ? (sp.he ? hev : pc) : xe[q];
-      const float pw = (i == 1) ? (sp.hw ? hwv : pc) : xw[q];
-      out[q] = (i <= d.ip) ? relax(q, i, pe, pw) : pc;
-    }
-    const float4 o4 = make_float4(out[0], out[1], out[2], out[3]);
-    *reinterpret_cast<float4*>(sp.p[c] + base) = o4;
-    if (push) {  // this tile's face cells are its neighbours' halos (same split position)
-      const int lev = pl * (k - 1) + 4 * grp;
-      if (j == 1 && push->s[c])
-        *reinterpret_cast<float4*>(push->s[c] + lev + sp.nr * d.jp) = o4;
-      if (j == d.jp && push->n[c])
-        *reinterpret_cast<float4*>(push->n[c] + lev + sp.nr * (d.jp + 1)) = o4;
-      if (i0 == 1 && push->w) push->w[ih] = out[0];
-      const int qe = (d.ip - i0) >> 1;  // the group's cell i = ip, if it has one
-      if (push->e && i0 + 6 >= d.ip && i0 + 2 * qe == d.ip)
-        push->e[ih] = qe == 0 ? out[0] : (qe == 1 ? out[1] : (qe == 2 ? out[2] : out[3]));
+        for (int q = 0; q < 4; ++q) {
+          const int i = i0 + 2 * q;
+          const float pc = pv[q];
+          const float pe = (i == d.ip) ? (sp.he ? hev : pc) : xe[q];
+          const float pw = (i == 1) ? (sp.hw ? hwv : pc) : xw[q];
+          out[q] = (i <= d.ip) ? relax(q, i, pe, pw) : pc;
+        }
+        const float4 o4 = make_float4(out[0], out[1], out[2], out[3]);
+        *reinterpret_cast<float4*>(sp.p[c] + base) = o4;
+        if (push) {  // this tile's face cells are its neighbours' halos (same split position)
+          const int lev = pl * (k - 1) + 4 * grp;
+          if (j == 1 && push->s[c])
+            *reinterpret_cast<float4*>(push->s[c] + lev + sp.nr * d.jp) = o4;
+          if (j == d.jp && push->n[c])
+            *reinterpret_cast<float4*>(push->n[c] + lev + sp.nr * (d.jp + 1)) = o4;
+          if (i0 == 1 && push->w) push->w[ih] = out[0];
+          const int qe = (d.ip - i0) >> 1;  // the group's cell i = ip, if it has one
+          if (push->e && i0 + 6 >= d.ip && i0 + 2 * qe == d.ip)
+            push->e[ih] = qe == 0 ? out[0] : (qe == 1 ? out[1] : (qe == 2 ? out[2] : out[3]));
+        }
+      }
+      od = oc;
+      oc = ou;
     }
   }
   if (RESID) {
@@ -559,9 +575,10 @@ void launch_sor_iter(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, 
 }
 
 namespace {
-void split_threads(Dims d, const SplitSOR& sp, int& ng, int& nthreads, int& nwg) {
+// kc levels per thread (the forward sweep's; one in the transposed sweep)
+void split_threads(Dims d, const SplitSOR& sp, int& ng, int& nthreads, int& nwg, int kc = 1) {
   ng = sp.nr / 4;
-  nthreads = ng * d.jp * d.kp;
+  nthreads = ng * d.jp * ((d.kp + kc - 1) / kc);
   const int nb = (nthreads + 255) / 256;
   nwg = (nb + 7) / 8 * 8;
 }
@@ -571,7 +588,7 @@ int sor_split_blocks(Dims d) {
   SplitSOR sp{};
   sp.nr = ((d.ip + 1) / 2 + 3) / 4 * 4;
   int ng, nt, nwg;
-  split_threads(d, sp, ng, nt, nwg);
+  split_threads(d, sp, ng, nt, nwg, SPLIT_KC_RESID);  // the sweeps that write partials
   return nwg;
 }
 
@@ -589,7 +606,7 @@ bool sor_split_divisors(Dims d, SplitSOR& sp) {
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc, SeqTerms tm) {
   int ng, nt, nwg;
-  split_threads(d, sp, ng, nt, nwg);
+  split_threads(d, sp, ng, nt, nwg, want_resid ? SPLIT_KC_RESID : 1);
   int64_t* part = rb.partials + (long)blk_offset * NLIMB;
   const dim3 grid(nwg), blk(256);
   if (sp.uniform) {
