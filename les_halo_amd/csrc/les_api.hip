@@ -435,7 +435,6 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   if (hipMalloc(&h->sc, sizeof(DevScalars)) != hipSuccess) return bail(LES_E_NOMEM);
   if (hipMemsetAsync(h->sc, 0, sizeof(DevScalars), h->st) != hipSuccess) return bail(LES_E_HIP);
   h->rb.max_blocks = std::max(std::max(reduction_blocks(h->d), 2 * sor_resid_blocks(h->d)), 4096);
-  h->rb.max_blocks = std::max(h->rb.max_blocks, 2 * sor_split_blocks(h->d));
   if (hipMalloc(&h->rb.partials, (size_t)h->rb.max_blocks * NLIMB * sizeof(int64_t)) != hipSuccess)
     return bail(LES_E_NOMEM);
   if (hipMalloc(&h->rb.slots, (size_t)RED_SLOTS * NLIMB * sizeof(int64_t)) != hipSuccess)
@@ -725,11 +724,11 @@ void enqueue_rhs_rhsav(les_handle* h, const Dims& d, const Grid1D& g, const Fiel
 }
 
 // the residual of one iteration from the sweeps' partials or terms, and the early exit
-void enqueue_resid(les_handle* h, const Dims& d, int nblocks, const Physics& ph) {
+void enqueue_resid(les_handle* h, const Dims& d, const Physics& ph) {
   if (seq_sums(h))
     launch_seq_resid(h->st, d, h->seq_terms, ph, h->sc);
   else
-    launch_sor_resid_finalize(h->st, h->rb, nblocks, ph, h->sc);
+    launch_sor_resid_finalize(h->st, h->rb, ph, h->sc);
 }
 }  // namespace
 
@@ -853,7 +852,7 @@ extern "C" int les_press_sor(les_handle* h, int nrd) {
   if (nrd == 2) {
     launch_boundp2(h->st, d, F, h->sc, 0);
   } else {
-    launch_sor_colour(h->st, d, g, F, h->ph, nrd, 0, h->rb, 0, h->sc);
+    launch_sor_colour(h->st, d, g, F, h->ph, nrd, 0, h->rb, h->sc);
     launch_boundp1(h->st, d, F, h->sc, 0);
   }
   h->p_consistent = false;  // a lone colour or boundp2 leaves an intermediate halo state
@@ -873,15 +872,14 @@ int enqueue_sor_inplace(les_handle* h) {
   STAGE_BEGIN(h);
   launch_sor_reset(h->st, h->sc);
   const bool resid = h->ph.pjuge > 0.0f;
-  const int nb = sor_resid_blocks(d);
   const SeqTerms tm = seq_sums(h) ? h->seq_terms : SeqTerms{};
   for (int l = 1; l <= h->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd) {
-      launch_sor_colour(h->st, d, g, F, h->ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc, tm);
+      launch_sor_colour(h->st, d, g, F, h->ph, nrd, resid ? 1 : 0, h->rb, h->sc, tm);
       launch_boundp1(h->st, d, F, h->sc, 1);
     }
     launch_boundp2(h->st, d, F, h->sc, 1);
-    if (resid) enqueue_resid(h, d, 2 * nb, h->ph);
+    if (resid) enqueue_resid(h, d, h->ph);
   }
   return check_launch(h);
 }
@@ -896,7 +894,6 @@ int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true, const Physics*
   const Physics ph = phys ? *phys : h->ph;
   if (reset) launch_sor_reset(h->st, h->sc);
   const bool resid = allow_exit && h->ph.pjuge > 0.0f;
-  const int nb = sor_split_blocks(d);
   int l0 = 1;
   if (!resid && h->iter_shape.tj > 0 && sor_iter_supported(d, h->sp)) {
     const SorPair fwd{{h->sp.p[0], h->sp.p[1]}, {h->palt[0], h->palt[1]}};
@@ -909,8 +906,8 @@ int enqueue_sor_split_loop(les_handle* h, bool allow_exit = true, const Physics*
   const SeqTerms tm = seq_sums(h) ? h->seq_terms : SeqTerms{};
   for (int l = l0; l <= h->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd)
-      launch_sor_split(h->st, d, g, h->sp, ph, nrd, resid ? 1 : 0, h->rb, nrd * nb, h->sc, tm);
-    if (resid) enqueue_resid(h, d, 2 * nb, ph);
+      launch_sor_split(h->st, d, g, h->sp, ph, nrd, resid ? 1 : 0, h->rb, h->sc, tm);
+    if (resid) enqueue_resid(h, d, ph);
   }
   return check_launch(h);
 }
@@ -1768,9 +1765,9 @@ extern "C" int les_time_kernel(les_handle* h, int which, int reps, float* avg_ms
     switch (which) {
       case LES_KERNEL_SOR_SWEEP:
         if (use_split(h))
-          launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, 0, h->sc);
+          launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, h->sc);
         else
-          launch_sor_colour(h->st, d, g, F, h->ph, q & 1, 0, h->rb, 0, h->sc);
+          launch_sor_colour(h->st, d, g, F, h->ph, q & 1, 0, h->rb, h->sc);
         break;
       case LES_KERNEL_VELFG: launch_velfg(h->st, d, g, F, h->ph, 1, 1); break;
       case LES_KERNEL_VISC_ADAM: launch_calc_visc_adam(h->st, d, g, F, 1, 1, h->shp_visc); break;
@@ -1919,10 +1916,10 @@ extern "C" int les_autotune(les_handle* h, int reps, les_tuning* out) {
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
     {  // the two colour sweeps of one iteration
-      for (int q = 0; q < 4; ++q) launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, 0, h->sc);
+      for (int q = 0; q < 4; ++q) launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, h->sc);
       HIPCHK(h, hipEventRecord(e0, h->st));
       for (int q = 0; q < 2 * reps; ++q)
-        launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, 0, h->sc);
+        launch_sor_split(h->st, d, g, h->sp, h->ph, q & 1, 0, h->rb, h->sc);
       HIPCHK(h, hipEventRecord(e1, h->st));
       HIPCHK(h, hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -2328,7 +2325,7 @@ void tiled_press(les_tiled* T, hipStream_t st) {
   for (int l = 1; !split && l <= T->cfg.nmaxp; ++l) {
     for (int nrd = 0; nrd <= 1; ++nrd) {
       for (les_handle* h : T->t)
-        launch_sor_colour(st, h->d, grid_of(h), fields_of(h), h->ph, nrd, 0, h->rb, 0, h->sc);
+        launch_sor_colour(st, h->d, grid_of(h), fields_of(h), h->ph, nrd, 0, h->rb, h->sc);
       for (size_t q = 0; q < T->t.size(); ++q) {
         les_handle* h = T->t[q];
         launch_boundp1(st, h->d, fields_of(h), h->sc, 1, T->e[q]);

@@ -169,11 +169,12 @@ void launch_rhs(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, RedBuf r
                 const SplitSOR* split = nullptr);
 void launch_bondfg(hipStream_t st, Dims d, Fields F);
 void launch_rhs_adjust(hipStream_t st, Dims d, Fields F, DevScalars* sc);
+// want_resid: the sweep adds its residual terms' exact sum into the slots rb.slots (or, with
+// tm.t[0] set, stores each term for the reference-order sum)
 void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
-                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc,
-                       SeqTerms tm = SeqTerms{});
-void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, int nblocks, Physics ph,
-                               DevScalars* sc);
+                       int want_resid, RedBuf rb, DevScalars* sc, SeqTerms tm = SeqTerms{});
+// the iteration's residual from rb.slots (clearing them) and the early exit
+void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, Physics ph, DevScalars* sc);
 void launch_boundp1(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done,
                     Edges e = EDGES_ALL);
 void launch_boundp2(hipStream_t st, Dims d, Fields F, DevScalars* sc, int check_done);
@@ -188,8 +189,7 @@ int sor_resid_blocks(Dims d);
 void launch_split_scatter(hipStream_t st, Dims d, Fields F, SplitSOR sp, DevScalars* sc,
                           int adjust_rhs, int with_rhs = 1);
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
-                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc,
-                      SeqTerms tm = SeqTerms{});
+                      int want_resid, RedBuf rb, DevScalars* sc, SeqTerms tm = SeqTerms{});
 // press.f95's sums in the reference's sequential order (les_seqsum.hip): rhsav over the Fortran
 // rhs; pav over the Fortran p (sp == nullptr) or the split copy; the residual of one iteration
 // over the terms the sweeps stored, with the early exit
@@ -197,7 +197,6 @@ void launch_seq_rhsav(hipStream_t st, Dims d, Grid1D g, const float* rhs, DevSca
 void launch_seq_pav(hipStream_t st, Dims d, Grid1D g, const float* p, const SplitSOR* sp,
                     DevScalars* sc);
 void launch_seq_resid(hipStream_t st, Dims d, SeqTerms tm, Physics ph, DevScalars* sc);
-int sor_split_blocks(Dims d);
 // sets SplitSOR::m_ng, m_jp; false when the grid is too large for the multiply-high quotients
 bool sor_split_divisors(Dims d, SplitSOR& sp);
 int sor_iter_supported(Dims d, const SplitSOR& sp);

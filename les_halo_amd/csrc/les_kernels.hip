@@ -1117,10 +1117,10 @@ __global__ __launch_bounds__(256) void k_sor_colour(Dims d, Grid1D g, Fields F, 
         acc.add(reltmp * reltmp, lds);
     }
   }
-  if (want_resid) {
+  if (want_resid) {  // into the accumulation slots (partials: RedBuf::slots)
     acc.flush(lds);
     const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    block_limbs_store(lds, partials + (long)blk * NLIMB);
+    block_limbs_atomic(lds, partials, blk);
   }
 }
 
@@ -1130,19 +1130,21 @@ int sor_resid_blocks(Dims d) {
 }
 
 void launch_sor_colour(hipStream_t st, Dims d, Grid1D g, Fields F, Physics ph, int nrd,
-                       int want_resid, RedBuf rb, int blk_offset, DevScalars* sc, SeqTerms tm) {
+                       int want_resid, RedBuf rb, DevScalars* sc, SeqTerms tm) {
   int half = (d.ip + 1) / 2;
   dim3 grid((half + BX - 1) / BX, (d.jp + BY - 1) / BY, d.kp);
   hipLaunchKernelGGL(k_sor_colour, grid, dim3(BX, BY), 0, st, d, g, F, ph, nrd, want_resid,
-                     rb.partials + (long)blk_offset * NLIMB, sc, tm);
+                     rb.slots, sc, tm);
 }
 
 // residual of one SOR iteration (both colours) and the early exit of press.f95:114
-__global__ __launch_bounds__(FIN_THREADS) void k_sor_resid_finalize(const int64_t* partials, int nb,
-                                                            Physics ph, DevScalars* sc) {
+// the residual of the iteration's two colour sweeps from the accumulation slots (cleared as
+// they are read), and the early exit of press.f95:114
+__global__ __launch_bounds__(FIN_THREADS) void k_sor_resid_finalize(int64_t* slots, Physics ph,
+                                                                    DevScalars* sc) {
   __shared__ int64_t limbs[NLIMB];
-  if (sc->sor_done) return;
-  sum_partials(partials, nb, limbs);
+  if (sc->sor_done) return;  // (the sweeps returned too: the slots are zero)
+  sum_slots(slots, limbs);
   if (threadIdx.x == 0) {
     float s = limbs_to_float(limbs);
     sc->sor = s;
@@ -1151,10 +1153,8 @@ __global__ __launch_bounds__(FIN_THREADS) void k_sor_resid_finalize(const int64_
   }
 }
 
-void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, int nblocks, Physics ph,
-                               DevScalars* sc) {
-  hipLaunchKernelGGL(k_sor_resid_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.partials, nblocks, ph,
-                     sc);
+void launch_sor_resid_finalize(hipStream_t st, RedBuf rb, Physics ph, DevScalars* sc) {
+  hipLaunchKernelGGL(k_sor_resid_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rb.slots, ph, sc);
 }
 
 __global__ void k_sor_reset(DevScalars* sc) {

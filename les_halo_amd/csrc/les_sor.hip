@@ -213,9 +213,9 @@ __device__ __forceinline__ void sor_split_body(const Dims& d, const Grid1D& g, c
       oc = ou;
     }
   }
-  if (RESID) {
+  if (RESID) {  // into the accumulation slots (partials: RedBuf::slots)
     acc.flush(lds);
-    block_limbs_store(lds, partials + (long)blockIdx.x * NLIMB);
+    block_limbs_atomic(lds, partials, (int)blockIdx.x);
   }
 }
 
@@ -584,13 +584,6 @@ void split_threads(Dims d, const SplitSOR& sp, int& ng, int& nthreads, int& nwg,
 }
 }  // namespace
 
-int sor_split_blocks(Dims d) {
-  SplitSOR sp{};
-  sp.nr = ((d.ip + 1) / 2 + 3) / 4 * 4;
-  int ng, nt, nwg;
-  split_threads(d, sp, ng, nt, nwg, SPLIT_KC_RESID);  // the sweeps that write partials
-  return nwg;
-}
 
 // floor(2^32 / n) + 1: umulhi(t, m) == t / n for every t with t * n < 2^32
 uint32_t magic_div(uint32_t n) { return (uint32_t)(0xffffffffu / n) + 1u; }
@@ -604,10 +597,10 @@ bool sor_split_divisors(Dims d, SplitSOR& sp) {
 }
 
 void launch_sor_split(hipStream_t st, Dims d, Grid1D g, SplitSOR sp, Physics ph, int nrd,
-                      int want_resid, RedBuf rb, int blk_offset, DevScalars* sc, SeqTerms tm) {
+                      int want_resid, RedBuf rb, DevScalars* sc, SeqTerms tm) {
   int ng, nt, nwg;
   split_threads(d, sp, ng, nt, nwg, want_resid ? SPLIT_KC_RESID : 1);
-  int64_t* part = rb.partials + (long)blk_offset * NLIMB;
+  int64_t* part = rb.slots;
   const dim3 grid(nwg), blk(256);
   if (sp.uniform) {
     if (want_resid)
