@@ -149,6 +149,21 @@ def test_trajectory_odd_jm(case_odd):
     _compare(L, o, "odd jm step 3")
 
 
+@pytest.mark.parametrize("sor", ["ref", "cfg"])
+def test_trajectory_odd_km(sor):
+    """An odd number of levels: kernels that take two or four levels a thread (visc+adam, the
+    residual-summing SOR sweeps) end on a lone level whose partner lies past kp."""
+    case = common.tiny_case(16, 12, 9, seed=13)
+    sorp = common.REF_SOR if sor == "ref" else common.CFG_SOR
+    L, o = _pair(case, **sorp)
+    L.ifdata()
+    o.ifdata()
+    for n in range(1, 5):
+        L.step(n)
+        o.step(n)
+    _compare(L, o, "odd km step 4")
+
+
 def test_graph_replay_equals_eager(case):
     """les_run replays a captured HIP graph; several replays with eager work in between."""
     A, o = _pair(case, **common.CFG_SOR)
