@@ -29,7 +29,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-B_STEP = 1057                  # algorithmic FP32 bytes per interior cell-step
+# algorithmic FP32 bytes per interior cell-step: velnw 40 + velfg/feedbf/calc_sm 89 + visc/adam 88
+# + rhs 28 + 50 SOR iterations at 16 B (two 8 B colour sweeps) + pav/adj 12
+B_STEP = 1057
 SOR_SWEEP_B_PER_CELL = 8       # one colour sweep: r p 4 + r rhs 2 + w p 2
 SOR_ITER_B_PER_CELL = 16       # one SOR iteration at press.f95's stage boundaries: two sweeps
 SOR_ITER_FUSED_B_PER_CELL = 12  # the fused iteration's compulsory bytes: r p 4 + r rhs 4 + w p 4
