@@ -277,7 +277,12 @@ __global__ __launch_bounds__(256) void k_mx_gather(Dims d, Fields F, SplitSOR sp
 // (at i = 2).
 namespace {
 
-constexpr int CG_KC = 10;  // levels per work item
+// levels per work item: at 600x600x90, 10 levels gave 509 us per CG iteration, 5 levels 510
+// and 20 levels 520
+#ifndef LES_CG_KC
+#define LES_CG_KC 10
+#endif
+constexpr int CG_KC = LES_CG_KC;
 
 struct CgDir {  // the search direction of the last iteration and the one being formed
   float* old_[2];
