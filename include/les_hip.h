@@ -147,7 +147,10 @@ int les_sync(les_handle* h);
 int les_upload(les_handle* h, int field, const float* host);
 int les_download(les_handle* h, int field, float* host);
 
-/* Device address and element count of a field (for callers that keep data on the device). */
+/* Device address and element count of a field (for callers that keep data on the device).
+ * The grid (dx1..dzs) is read through tables that les_upload derives from it (power-of-two
+ * flags, the FP64 reciprocals of the vertical spacings): change grid fields with les_upload,
+ * not through this address. */
 int les_device_ptr(les_handle* h, int field, void** dptr, int64_t* count);
 
 /* init.f95:82-107 (cn1, cn2l/s, cn3l/s, cn4l/s) and les.f95:33-37 (delx1) from the uploaded

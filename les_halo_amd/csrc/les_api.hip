@@ -26,6 +26,7 @@ struct les_handle {
   float* fld[LES_NFIELDS] = {};
   int64_t cnt[LES_NFIELDS] = {};
   uint8_t* mask = nullptr;  // interior bits: 1 b, 2 c, 4 d, 8 a
+  double2* rz = nullptr;     // Grid1D::rzn, rzs, rzp, each cnt[LES_DZN] long (upload_zrecip)
   DevScalars* sc = nullptr;
   RedBuf rb{};
   std::vector<float> hgrid[LES_NFIELDS];  // host copies of the 1-D grid/coefficient arrays
@@ -186,6 +187,10 @@ Grid1D grid_of(les_handle* h) {
   g.cn4l = h->fld[LES_CN4L];
   g.cn4s = h->fld[LES_CN4S];
   g.delx1 = h->fld[LES_DELX1];
+  const int64_t nz = h->cnt[LES_DZN];
+  g.rzn = h->rz;
+  g.rzs = h->rz + nz;
+  g.rzp = h->rz + 2 * nz;
   return g;
 }
 
@@ -239,6 +244,26 @@ void build_mask_bits(const les_handle* h, const std::vector<float>* hm, std::vec
         if (hm[3].size() && hm[3][ia] != 0.0f) m |= 4;
         out[(size_t)(i - 1) + (size_t)ip * ((j - 1) + (size_t)jp * (k - 1))] = m;
       }
+}
+
+// The vertical divisors of velfg, visc and rhs with their FP64 reciprocals (Grid1D::rzn, rzs,
+// rzp; zdiv): (b, RN64(1/b)) for b = dzn(k), dzs(k) and the FP32 sum dzn(k) + dzn(k+1), from
+// the host copies of the grid (a zero spacing gives an infinite reciprocal, which zdiv's fixup
+// turns into the division's result).
+int upload_zrecip(les_handle* h) {
+  const auto& dzn = h->hgrid[LES_DZN];
+  const auto& dzs = h->hgrid[LES_DZS];
+  const size_t n = (size_t)h->cnt[LES_DZN];
+  std::vector<double2> t(3 * n);
+  auto pair = [](float b) { return make_double2((double)b, 1.0 / (double)b); };
+  for (size_t q = 0; q < n; ++q) {
+    t[q] = pair(dzn[q]);
+    t[n + q] = pair(dzs[q]);
+    t[2 * n + q] = pair(q + 1 < n ? dzn[q] + dzn[q + 1] : dzn[q]);  // last entry: never read
+  }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(h->rz, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return LES_OK;
 }
 
 int upload_scalar_area(les_handle* h) {
@@ -442,6 +467,9 @@ extern "C" int les_create(const les_config* cfg, les_handle** out) {
   if (hipMemsetAsync(h->rb.slots, 0, (size_t)RED_SLOTS * NLIMB * sizeof(int64_t), h->st) != hipSuccess)
     return bail(LES_E_HIP);
   if (hipMalloc(&h->scratch_f, 256) != hipSuccess) return bail(LES_E_NOMEM);
+  if (hipMalloc(&h->rz, 3 * (size_t)h->cnt[LES_DZN] * sizeof(double2)) != hipSuccess)
+    return bail(LES_E_NOMEM);
+  if (upload_zrecip(h) != LES_OK) return bail(LES_E_HIP);
   {  // split SOR copy: 2 colours x (p, rhs) of interior rows padded to nr floats, plus the
      // cn1 factors ax(0:ip+1), by(0:jp+1), cz(0:kp+1) and the table cn1k(0:kp+1)
     SplitSOR& sp = h->sp;
@@ -514,6 +542,7 @@ extern "C" int les_destroy(les_handle* h) {
   for (int f = 0; f < LES_NFIELDS; ++f)
     if (h->fld[f]) hipFree(h->fld[f]);
   if (h->mask) hipFree(h->mask);
+  if (h->rz) hipFree(h->rz);
   if (h->sc) hipFree(h->sc);
   if (h->rb.partials) hipFree(h->rb.partials);
   if (h->rb.slots) hipFree(h->rb.slots);
@@ -570,6 +599,10 @@ extern "C" int les_upload(les_handle* h, int field, const float* host) {
   }
   if (field == LES_DX1 || field == LES_DY1 || field == LES_DZN) {
     int r = upload_scalar_area(h);
+    if (r) return r;
+  }
+  if (field == LES_DZN || field == LES_DZS) {
+    int r = upload_zrecip(h);
     if (r) return r;
   }
   if (field == LES_DX1 || field == LES_DY1 || field == LES_DXS || field == LES_DYS) {

@@ -80,6 +80,9 @@ struct Buf {
 struct Grid1D {
   const float *dx1, *dxs, *dy1, *dys, *dzn, *dzs;
   const float *cn2l, *cn2s, *cn3l, *cn3s, *cn4l, *cn4s, *delx1;
+  // per-level divisors along k as (b, RN64(1/b)) in FP64, for zdiv: dzn(k), dzs(k) and
+  // dzn(k) + dzn(k+1) (the FP32 sum), indexed like dzn (set on the host from the uploaded grid)
+  const double2 *rzn, *rzs, *rzp;
   int p2xy;  // every dx1, dy1, dxs, dys and neighbour sum dx1(i)+dx1(i+1), dy1(j)+dy1(j+1) is a
              // power of two (set on the host; selects xdiv<true>)
   __device__ float DX1(int i) const { return dx1[i + 1]; }
@@ -88,6 +91,9 @@ struct Grid1D {
   __device__ float DYS(int j) const { return dys[j]; }
   __device__ float DZN(int k) const { return dzn[k + 1]; }
   __device__ float DZS(int k) const { return dzs[k + 1]; }
+  __device__ double2 RZN(int k) const { return rzn[k + 1]; }
+  __device__ double2 RZS(int k) const { return rzs[k + 1]; }
+  __device__ double2 RZP(int k) const { return rzp[k + 1]; }  // dzn(k) + dzn(k+1)
 };
 
 // a / b with IEEE rounding, faster when b is a power of two.  For a positive normal b = 2^e
@@ -107,6 +113,29 @@ template <bool P2>
 __device__ __forceinline__ float xdiv(float a, float b) {
   if constexpr (P2) return a * pow2_recip(b);
   return a / b;
+}
+
+// a / b, bit for bit the FP32 division, for a divisor that comes with z = (b, y = RN64(1/b))
+// in FP64 (the vertical spacings, one per level).  For finite nonzero a and b:
+//  * q0 = RN64(a * y) is within 2^-52 (relative) of a/b.  A quotient of two floats is either
+//    a midpoint between floats or at least 2^-49 (relative) away from every midpoint (write
+//    a = A 2^s, b = B 2^t, m = M 2^u with 24-bit A, B and 25-bit odd M: a - m b is a nonzero
+//    multiple of 2^min(s, u+t)), so RN32(q0) = RN32(a/b) unless a/b is a midpoint, which it
+//    can only be when it is subnormal.
+//  * q1 = q0 + (a - b q0) y (Markstein's correction; the FMA remainder is exact) is within
+//    2^-104 of a/b, so on a midpoint it is that midpoint exactly and the final rounding ties to
+//    even like the division's.
+// v_div_fixup_f32, the last instruction of the division sequence, then gives the division's
+// results for zero, infinite and NaN operands (and the sign); for other operands it passes
+// its first operand with the quotient's sign, which is RN32(a/b) here as in the division.
+// Six VALU instructions (two conversions, three FP64 FMA-class, the fixup) against eleven,
+// one of them the quarter-rate reciprocal.
+__device__ __forceinline__ float zdiv(float a, float b, double2 z) {
+  const double x = (double)a;
+  const double q0 = x * z.y;
+  const double r = __builtin_fma(-z.x, q0, x);
+  const double q1 = __builtin_fma(r, z.y, q0);
+  return __builtin_amdgcn_div_fixupf((float)q1, b, a);
 }
 
 // ------------------------------------------------------------------------------------------

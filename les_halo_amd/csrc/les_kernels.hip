@@ -292,6 +292,7 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
   if (do_velfg) {
     const float dx1i = c.dx1(i), dx1ip = c.dx1(i + 1), dy1j = c.dy1(j), dy1jp = c.dy1(j + 1);
     const float dznk = c.dzn(k), dznkp = c.dzn(k + 1);
+    const double2 rznk = c.rzn(k);
     // u momentum (velFG.f95:59-72)
     {
       float covx1 = xdiv<P2>(dx1ip * c.cov1(i, j, k) + dx1i * c.cov1(ip1, j, k), dx1i + dx1ip);
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
       float covz1 = (c.cov3(i, j, k) + c.cov3(i, j, k + 1)) / two;
       float covc = covx1 + covy1 + covz1;
       float dfu1 = xdiv<P2>(two * (-d1 + c.diu1(ip1, j, k)), dx1i + dx1ip) +
-                   xdiv<P2>(-d2 + c.diu2(i, jp1, k), dy1j) + (-d3 + c.diu3(i, j, k + 1)) / dznk;
+                   xdiv<P2>(-d2 + c.diu2(i, jp1, k), dy1j) + zdiv(-d3 + c.diu3(i, j, k + 1), dznk, rznk);
       float df = ph.vn * dfu1;
       fv = (-covc + df);
     }
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
       float covc = covx1 + covy1 + covz1;
       float dfv1 = xdiv<P2>(-d4 + c.diu4(ip1, j, k), dx1i) +
                    xdiv<P2>(two * (-d5 + c.diu5(i, jp1, k)), dy1j + dy1jp) +
-                   (-d6 + c.diu6(i, j, k + 1)) / dznk;
+                   zdiv(-d6 + c.diu6(i, j, k + 1), dznk, rznk);
       float df = ph.vn * dfv1;
       gv = (-covc + df);
     }
@@ -319,10 +320,10 @@ __global__ __launch_bounds__(256) void k_velfg_feedbf(Dims d, Grid1D g, Fields F
     if (hk) {
       float covx1 = (c.cov7(i, j, k) + c.cov7(ip1, j, k)) / two;
       float covy1 = (c.cov8(i, j, k) + c.cov8(i, jp1, k)) / two;
-      float covz1 = (dznkp * c.cov9(i, j, k) + dznk * c.cov9(i, j, k + 1)) / (dznk + dznkp);
+      float covz1 = zdiv(dznkp * c.cov9(i, j, k) + dznk * c.cov9(i, j, k + 1), dznk + dznkp, c.rzp(k));
       float covc = covx1 + covy1 + covz1;
       float dfw1 = xdiv<P2>(-d7 + c.diu7(ip1, j, k), dx1i) + xdiv<P2>(-d8 + c.diu8(i, jp1, k), dy1j) +
-                   (-d9 + c.diu9(i, j, k + 1)) / c.dzs(k);
+                   zdiv(-d9 + c.diu9(i, j, k + 1), c.dzs(k), c.rzs(k));
       float df = ph.vn * dfw1;
       hv = (-covc + df);
     } else {
@@ -437,6 +438,7 @@ __device__ __forceinline__ void velfg_xy_body(const Dims& d, const Grid1D& g, co
   const float two = 2.0f;
   const float dx1i = c.dx1(i), dx1ip = c.dx1(i + 1), dy1j = c.dy1(j), dy1jp = c.dy1(j + 1);
   const float dznk = c.dzn(k), dznkp = c.dzn(k + 1);
+  const double2 rznk = c.rzn(k);
   float fv, gv, hv;
   {  // velFG.f95:59-72
     float covx1 = xdiv<P2>(dx1ip * xc1 + dx1i * xc1n, dx1i + dx1ip);
@@ -444,7 +446,7 @@ __device__ __forceinline__ void velfg_xy_body(const Dims& d, const Grid1D& g, co
     float covz1 = (zc3 + zc3n) / two;
     float covc = covx1 + covy1 + covz1;
     float dfu1 = xdiv<P2>(two * (-x1 + x1n), dx1i + dx1ip) + xdiv<P2>(-y2 + y2n, dy1j) +
-                 (-z3 + z3n) / dznk;
+                 zdiv(-z3 + z3n, dznk, rznk);
     float df = ph.vn * dfu1;
     fv = (-covc + df);
   }
@@ -454,7 +456,7 @@ __device__ __forceinline__ void velfg_xy_body(const Dims& d, const Grid1D& g, co
     float covz1 = (zc6 + zc6n) / two;
     float covc = covx1 + covy1 + covz1;
     float dfv1 = xdiv<P2>(-x4 + x4n, dx1i) + xdiv<P2>(two * (-y5 + y5n), dy1j + dy1jp) +
-                 (-z6 + z6n) / dznk;
+                 zdiv(-z6 + z6n, dznk, rznk);
     float df = ph.vn * dfv1;
     gv = (-covc + df);
   }
@@ -466,9 +468,10 @@ __device__ __forceinline__ void velfg_xy_body(const Dims& d, const Grid1D& g, co
     const float z9n = c.diu9(i, j, k + 1), zc9 = c.cov9(i, j, k), zc9n = c.cov9(i, j, k + 1);
     float covx1 = (xc7 + xc7n) / two;
     float covy1 = (yc8 + yc8n) / two;
-    float covz1 = (dznkp * zc9 + dznk * zc9n) / (dznk + dznkp);
+    float covz1 = zdiv(dznkp * zc9 + dznk * zc9n, dznk + dznkp, c.rzp(k));
     float covc = covx1 + covy1 + covz1;
-    float dfw1 = xdiv<P2>(-x7 + x7n, dx1i) + xdiv<P2>(-y8 + y8n, dy1j) + (-z9 + z9n) / c.dzs(k);
+    float dfw1 = xdiv<P2>(-x7 + x7n, dx1i) + xdiv<P2>(-y8 + y8n, dy1j) +
+                 zdiv(-z9 + z9n, c.dzs(k), c.rzs(k));
     float df = ph.vn * dfw1;
     hv = (-covc + df);
   } else {
@@ -745,6 +748,7 @@ __device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const
   const float dx1i = g.DX1(i), dx1ip = g.DX1(i + 1), dx1im = g.DX1(i - 1);
   const float dy1j = g.DY1(j), dy1jp = g.DY1(j + 1), dy1jm = g.DY1(j - 1);
   const float dznk = g.DZN(k), dznkp = g.DZN(k + 1), dznkm = g.DZN(k - 1);
+  const double2 rznk = g.RZN(k), rzpk = g.RZP(k), rzpkm = g.RZP(k - 1);
   const float s000 = sm(i, j, k), s100 = sm(i + 1, j, k);
   // x-momentum (les.f95:78-109)
   {
@@ -756,10 +760,10 @@ __device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const
     float evsy1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1ip * sm(i, j - 1, k) + dx1i * sm(i + 1, j - 1, k), dx1i + dx1ip)) +
                        dy1j * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)),
                        dy1j + dy1jp);
-    float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
-    float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
-                       dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
+    float evsz2 = zdiv(dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip)), dznk + dznkp, rzpk);
+    float evsz1 = zdiv(dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
+                       dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)), dznkm + dznk, rzpkm);
     float visux2 = (evsx2) * two * D1(i + 1, j, k);
     float visux1 = (evsx1) * two * D1(i, j, k);
     float visuy2 = (evsy2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
@@ -768,7 +772,7 @@ __device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const
     float visuz1 = (evsz1) * (D3(i, j, k) + D7(i + 1, j, k - 1));
     float vfu = xdiv<P2>(visux2 - visux1, dx1i) +
                 xdiv<P2>(visuy2 - visuy1, dy1j) +
-                (visuz2 - visuz1) / dznk;
+                zdiv(visuz2 - visuz1, dznk, rznk);
     fv = (fv + vfu);
   }
   // y-momentum (les.f95:112-143)
@@ -781,10 +785,10 @@ __device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const
     float evsx1 = xdiv<P2>(dy1jp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
                        dy1j * (xdiv<P2>(dx1i * sm(i - 1, j + 1, k) + dx1im * sm(i, j + 1, k), dx1im + dx1i)),
                        dy1j + dy1jp);
-    float evsz2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
-    float evsz1 = (dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
-                       dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip))) / (dznkm + dznk);
+    float evsz2 = zdiv(dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip)), dznk + dznkp, rzpk);
+    float evsz1 = zdiv(dznk * (xdiv<P2>(dx1ip * sm(i, j, k - 1) + dx1i * sm(i + 1, j, k - 1), dx1i + dx1ip)) +
+                       dznkm * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)), dznkm + dznk, rzpkm);
     float visvx2 = (evsx2) * (D2(i, j + 1, k) + D4(i + 1, j, k));
     float visvx1 = (evsx1) * (D2(i - 1, j + 1, k) + D4(i, j, k));
     float visvy2 = (evsy2) * two * D5(i, j + 1, k);
@@ -793,21 +797,21 @@ __device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const
     float visvz1 = (evsz1) * (D6(i, j, k) + D8(i, j + 1, k - 1));
     float vfv = xdiv<P2>(visvx2 - visvx1, dx1i) +
                 xdiv<P2>(visvy2 - visvy1, dy1j) +
-                (visvz2 - visvz1) / dznk;
+                zdiv(visvz2 - visvz1, dznk, rznk);
     gv = (gv + vfv);
   }
   // z-momentum (les.f95:146-180)
   {
     float evsz2 = sm(i, j, k + 1);
     float evsz1 = s000;
-    float evsx2 = (dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
-                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip))) / (dznk + dznkp);
-    float evsx1 = (dznkp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
-                       dznk * (xdiv<P2>(dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1), dx1im + dx1i))) / (dznk + dznkp);
-    float evsy2 = (dznkp * (xdiv<P2>(dy1jp * s000 + dy1j * sm(i, j + 1, k), dy1j + dy1jp)) +
-                       dznk * (xdiv<P2>(dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1), dy1j + dy1jp))) / (dznk + dznkp);
-    float evsy1 = (dznkp * (xdiv<P2>(dy1j * sm(i, j - 1, k) + dy1jm * s000, dy1jm + dy1j)) +
-                       dznk * (xdiv<P2>(dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1), dy1jm + dy1j))) / (dznk + dznkp);
+    float evsx2 = zdiv(dznkp * (xdiv<P2>(dx1ip * s000 + dx1i * s100, dx1i + dx1ip)) +
+                       dznk * (xdiv<P2>(dx1ip * sm(i, j, k + 1) + dx1i * sm(i + 1, j, k + 1), dx1i + dx1ip)), dznk + dznkp, rzpk);
+    float evsx1 = zdiv(dznkp * (xdiv<P2>(dx1i * sm(i - 1, j, k) + dx1im * s000, dx1im + dx1i)) +
+                       dznk * (xdiv<P2>(dx1i * sm(i - 1, j, k + 1) + dx1im * sm(i, j, k + 1), dx1im + dx1i)), dznk + dznkp, rzpk);
+    float evsy2 = zdiv(dznkp * (xdiv<P2>(dy1jp * s000 + dy1j * sm(i, j + 1, k), dy1j + dy1jp)) +
+                       dznk * (xdiv<P2>(dy1jp * sm(i, j, k + 1) + dy1j * sm(i, j + 1, k + 1), dy1j + dy1jp)), dznk + dznkp, rzpk);
+    float evsy1 = zdiv(dznkp * (xdiv<P2>(dy1j * sm(i, j - 1, k) + dy1jm * s000, dy1jm + dy1j)) +
+                       dznk * (xdiv<P2>(dy1j * sm(i, j - 1, k + 1) + dy1jm * sm(i, j, k + 1), dy1jm + dy1j)), dznk + dznkp, rzpk);
     float viswx2 = (evsx2) * (D3(i, j, k + 1) + D7(i + 1, j, k));
     float viswx1 = (evsx1) * (D3(i - 1, j, k + 1) + D7(i, j, k));
     float viswy2 = (evsy2) * (D6(i, j, k + 1) + D8(i, j + 1, k));
@@ -816,7 +820,7 @@ __device__ __forceinline__ void visc_terms(const Dims& d, const Grid1D& g, const
     float viswz1 = (evsz1) * two * D9(i, j, k);
     float vfw = xdiv<P2>(viswx2 - viswx1, dx1i) +
                 xdiv<P2>(viswy2 - viswy1, dy1j) +
-                (viswz2 - viswz1) / dznk;
+                zdiv(viswz2 - viswz1, dznk, rznk);
     hv = (hv + vfw);
   }
 }
@@ -882,7 +886,7 @@ __device__ __forceinline__ void visc_adam_body(const Dims& d, const Grid1D& g, c
 }
 
 template <bool P2>
-__global__ __launch_bounds__(256) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_visc_adam(Dims d, Grid1D g, Fields F, int do_visc,
                                                    int do_adam, StripGrid sg) {
   visc_adam_body<P2>(d, g, F, do_visc, do_adam, sg, xcd_tile((int)blockIdx.x, (int)gridDim.x));
 }
@@ -992,10 +996,11 @@ __global__ __launch_bounds__(256) void k_rhs(Dims d, Grid1D g, Fields F, Physics
       const In c = nxt;
       if (k < k1) nxt = load(k + 1);
       const float dznk = g.DZN(k);
+      const double2 rznk = g.RZN(k);
       float r = xdiv<P2>(-c.um + c.uc, dx1i) + xdiv<P2>(-c.vm + c.vc, dy1j) +
-                (-c.wm + c.wc) / dznk;
-      r = xdiv<P2>(c.fc - c.fm, dx1i) + xdiv<P2>(c.gc - c.gm, dy1j) + (c.hc - c.hm) / dznk +
-          (r) / dt;
+                zdiv(-c.wm + c.wc, dznk, rznk);
+      r = xdiv<P2>(c.fc - c.fm, dx1i) + xdiv<P2>(c.gc - c.gm, dy1j) +
+          zdiv(c.hc - c.hm, dznk, rznk) + (r) / dt;
       if (rs0)
         (((i + j + k) & 1) ? rs1 : rs0)[((i - 1) >> 1) + nr * ((j - 1) + jm * (k - 1))] = r;
       else
@@ -1584,7 +1589,7 @@ __global__ __launch_bounds__(256) void k_calc_sm_batch(const BatchStencil* __res
 }
 
 template <bool P2>
-__global__ __launch_bounds__(256) void k_visc_adam_batch(const BatchStencil* __restrict__ tab,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_visc_adam_batch(const BatchStencil* __restrict__ tab,
                                                          const uint8_t* __restrict__ wgm,
                                                          int total) {
   const int w = xcd_tile((int)blockIdx.x, (int)gridDim.x);

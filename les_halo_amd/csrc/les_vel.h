@@ -19,6 +19,9 @@ struct VelCtx {
   __device__ float dy1(int j) const { return g.DY1(j); }
   __device__ float dzn(int k) const { return g.DZN(k); }
   __device__ float dzs(int k) const { return g.DZS(k); }
+  __device__ double2 rzn(int k) const { return g.RZN(k); }
+  __device__ double2 rzs(int k) const { return g.RZS(k); }
+  __device__ double2 rzp(int k) const { return g.RZP(k); }
 
   // vel2.f95:51-60
   __device__ float diu1(int i, int j, int k) const { return xdiv<P2>(-U(i - 1, j, k) + U(i, j, k), dx1(i)); }
@@ -29,7 +32,7 @@ struct VelCtx {
   __device__ float cov5(int i, int j, int k) const {
     return ((V(i, j - 1, k) + V(i, j, k)) / 2.0f) * diu5(i, j, k);
   }
-  __device__ float diu9(int i, int j, int k) const { return (-W(i, j, k - 1) + W(i, j, k)) / dzn(k); }
+  __device__ float diu9(int i, int j, int k) const { return zdiv(-W(i, j, k - 1) + W(i, j, k), dzn(k), rzn(k)); }
   __device__ float cov9(int i, int j, int k) const {
     return ((W(i, j, k - 1) + W(i, j, k)) / 2.0f) * diu9(i, j, k);
   }
@@ -43,7 +46,7 @@ struct VelCtx {
   }
   // vel2.f95:78-80
   __device__ float diu3(int i, int j, int k) const {
-    return (-U(i, j, k - 1) + U(i, j, k)) / dzs(k - 1);
+    return zdiv(-U(i, j, k - 1) + U(i, j, k), dzs(k - 1), rzs(k - 1));
   }
   __device__ float cov3(int i, int j, int k) const {
     float nou = xdiv<P2>(dx1(i + 1) * W(i, j, k - 1) + dx1(i) * W(i + 1, j, k - 1), dx1(i) + dx1(i + 1));
@@ -59,7 +62,7 @@ struct VelCtx {
   }
   // vel2.f95:98-100
   __device__ float diu6(int i, int j, int k) const {
-    return (-V(i, j, k - 1) + V(i, j, k)) / dzs(k - 1);
+    return zdiv(-V(i, j, k - 1) + V(i, j, k), dzs(k - 1), rzs(k - 1));
   }
   __device__ float cov6(int i, int j, int k) const {
     float nou = xdiv<P2>(dy1(j + 1) * W(i, j, k - 1) + dy1(j) * W(i, j + 1, k - 1), dy1(j) + dy1(j + 1));
@@ -70,7 +73,8 @@ struct VelCtx {
     return xdiv<P2>(2.0f * (-W(i - 1, j, k) + W(i, j, k)), dx1(i - 1) + dx1(i));
   }
   __device__ float cov7(int i, int j, int k) const {
-    float nou = (dzn(k + 1) * U(i - 1, j, k) + dzn(k) * U(i - 1, j, k + 1)) / (dzn(k) + dzn(k + 1));
+    float nou = zdiv(dzn(k + 1) * U(i - 1, j, k) + dzn(k) * U(i - 1, j, k + 1), dzn(k) + dzn(k + 1),
+                     rzp(k));
     return nou * diu7(i, j, k);
   }
   // vel2.f95:118-120
@@ -78,7 +82,8 @@ struct VelCtx {
     return xdiv<P2>(2.0f * (-W(i, j - 1, k) + W(i, j, k)), dy1(j - 1) + dy1(j));
   }
   __device__ float cov8(int i, int j, int k) const {
-    float nou = (dzn(k + 1) * V(i, j - 1, k) + dzn(k) * V(i, j - 1, k + 1)) / (dzn(k) + dzn(k + 1));
+    float nou = zdiv(dzn(k + 1) * V(i, j - 1, k) + dzn(k) * V(i, j - 1, k + 1), dzn(k) + dzn(k + 1),
+                     rzp(k));
     return nou * diu8(i, j, k);
   }
 };
