@@ -164,6 +164,45 @@ def test_trajectory_odd_km(sor):
     _compare(L, o, "odd km step 4")
 
 
+def _stretched(case, fx=1.03, fy=0.97):
+    """The case on a horizontal grid with varying spacings (dx1, dy1 scaled by powers of fx, fy
+    over a 5-cell cycle) and dxs, dys from them as grid.f95:67-82 forms them."""
+    c = dict(case)
+    ip, jp, kp = c["dims"]
+    dx1 = c["dx1"].copy()  # dx1(-1:ip+1)
+    dy1 = c["dy1"].copy()  # dy1(0:jp+1)
+    for q in range(dx1.size):
+        dx1[q] = np.float32(dx1[q] * np.float32(fx ** (q % 5)))
+    for q in range(dy1.size):
+        dy1[q] = np.float32(dy1[q] * np.float32(fy ** (q % 5)))
+    dxs = np.array([dx1[i + 1] / np.float32(2) + dx1[i + 2] / np.float32(2) for i in range(ip + 1)],
+                   np.float32)
+    dys = np.array([dy1[j] / np.float32(2) + dy1[j + 1] / np.float32(2) for j in range(jp + 1)],
+                   np.float32)
+    c.update(dx1=dx1, dy1=dy1, dxs=dxs, dys=dys)
+    return c
+
+
+@pytest.mark.parametrize("grid", ["uniform5x3", "stretched"])
+@pytest.mark.parametrize("sor", ["ref", "cfg"])
+def test_trajectory_horizontal_spacings(grid, sor):
+    """Horizontal spacings that are not powers of two (5 m by 3 m: true divisions along i and j
+    in every stencil kernel) and spacings that vary along i and j (also the SOR's general
+    coefficients instead of the uniform-grid form)."""
+    from les_halo_amd import inputs
+    case = inputs.make_case(16, 12, 10, seed=17, hmax=12.0, coverage=0.3, dx=5.0, dy=3.0)
+    if grid == "stretched":
+        case = _stretched(case)
+    sorp = common.REF_SOR if sor == "ref" else common.CFG_SOR
+    L, o = _pair(case, **sorp)
+    L.ifdata()
+    o.ifdata()
+    for n in range(1, 5):
+        L.step(n)
+        o.step(n)
+    _compare(L, o, "%s grid step 4" % grid)
+
+
 def test_graph_replay_equals_eager(case):
     """les_run replays a captured HIP graph; several replays with eager work in between."""
     A, o = _pair(case, **common.CFG_SOR)
