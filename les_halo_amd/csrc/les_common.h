@@ -122,9 +122,10 @@ __device__ __forceinline__ float xdiv(float a, float b) {
 //    a = A 2^s, b = B 2^t, m = M 2^u with 24-bit A, B and 25-bit odd M: a - m b is a nonzero
 //    multiple of 2^min(s, u+t)), so RN32(q0) = RN32(a/b) unless a/b is a midpoint, which it
 //    can only be when it is subnormal.
-//  * q1 = q0 + (a - b q0) y (Markstein's correction; the FMA remainder is exact) is within
-//    2^-104 of a/b, so on a midpoint it is that midpoint exactly and the final rounding ties to
-//    even like the division's.
+//  * q1 = RN64(q0 + (a - b q0) y) (Markstein's correction; the FMA remainder is exact): the
+//    sum before its rounding is within about 2^-104 (relative) of a/b, so where a/b is a
+//    midpoint (a double) q1 is that midpoint exactly, and elsewhere q1 is as close to a/b as
+//    q0 is; the final rounding then ties to even like the division's.
 // v_div_fixup_f32, the last instruction of the division sequence, then gives the division's
 // results for zero, infinite and NaN operands (and the sign); for other operands it passes
 // its first operand with the quotient's sign, which is RN32(a/b) here as in the division.
